@@ -1,0 +1,117 @@
+// Shared definitions of the gfx950 library: the per-entry arithmetic of the lazy
+// transform (identical on host and device), launch helpers and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace crest {
+
+constexpr int kWave = 64;
+constexpr int kRowGroup = 8;      // rows interleaved per (row-group, gene) unit of a tile
+constexpr int kGramBlock = 128;   // Gram output block (per workgroup)
+constexpr int kStatsWindow = 2048;  // vars accumulated in LDS per gene_stats window
+
+// Element formats of a densified tile.
+enum TileFmt : int { TILE_F64 = 0, TILE_F32 = 1, TILE_BF16X2 = 2 };
+
+// normalize_total + log1p of one entry (kernels.rs:77-82).  `ns` is the
+// per-cell factor target_sum/total (1 when total == 0).  log1p is the correctly
+// rounded float of the double log1p, identical on host and device.
+__host__ __device__ inline float lazy_transform(float x, double ns, double target_sum, int log1p) {
+    if (!(target_sum > 0.0 || log1p)) return x;
+    float y = target_sum > 0.0 ? (float)((double)x * ns) : x;
+    return log1p ? (float)::log1p((double)y) : y;
+}
+
+__host__ __device__ inline double norm_factor(double total, double target_sum) {
+    return (target_sum > 0.0 && total > 0.0) ? target_sum / total : 1.0;
+}
+
+// scanpy-style scale with clipping, sparse form (kernels.rs:209-218):
+// s = clip((x - mu)/sd) - clip(-mu/sd); the identity when scaling is off.
+__host__ __device__ inline double clip_mv(double z, double mv) {
+    if (mv > 0.0) {
+        if (z < -mv) return -mv;
+        if (z > mv) return mv;
+    }
+    return z;
+}
+__host__ __device__ inline double scale_value(float v, double mu, double sd, double mv, int scale) {
+    if (!scale) return (double)v;
+    return clip_mv(((double)v - mu) / sd, mv) - clip_mv(-mu / sd, mv);
+}
+
+// Chunk geometry on the device: spans of entries per cell.
+struct DevChunk {
+    const int64_t *span_off;   // [n_spans+1] absolute offsets; entries are genes[off-base]
+    const uint32_t *span_cell; // [n_spans] store cell id, or nullptr -> first_cell + i
+    uint32_t first_cell;
+    int64_t n_spans;
+    const uint32_t *genes;
+    const float *values;
+    const int64_t *cell_map;   // store cell -> row
+    int64_t n_cell_map;
+    const int32_t *gene_var;   // store gene -> var (-1 filtered)
+    int64_t n_gene_map;
+    double target_sum;
+    int log1p;
+    int64_t base;              // absolute offset of genes[0] / values[0]
+};
+
+__device__ inline int64_t chunk_row(const DevChunk &c, int64_t i) {
+    uint64_t cell = c.span_cell ? (uint64_t)c.span_cell[i] : (uint64_t)c.first_cell + (uint64_t)i;
+    if (cell >= (uint64_t)c.n_cell_map) return -1;
+    return c.cell_map[cell];
+}
+__device__ inline int32_t chunk_var(const DevChunk &c, uint32_t g) {
+    return (int64_t)g < c.n_gene_map ? c.gene_var[g] : -1;
+}
+
+__device__ inline double wave_sum_f64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Scale parameters on the device.
+struct DevScale {
+    const int32_t *sub;  // var -> hvg index (-1 = not in the Gram)
+    const double *mean;  // [d]
+    const double *sd;    // [d]
+    double max_value;
+    int scale;
+    int d;
+};
+
+// ---- kernel launchers (each defined next to its kernel) ----------------------
+void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s);
+void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b,
+                    int64_t rows_pad, int d_pad, double *colsum, hipStream_t s);
+void launch_materialize_count(const DevChunk &c, int64_t *row_kept, hipStream_t s);
+void launch_materialize(const DevChunk &c, const int64_t *out_off, uint32_t *rows, uint32_t *vars,
+                        float *vals, hipStream_t s);
+void launch_project(const DevChunk &c, const DevScale &sc, const double *V, int k, float *out,
+                    int64_t row0, hipStream_t s);
+void launch_gram(int fmt, const void *tile_a, const void *tile_b, int64_t rows_pad, int d_pad,
+                 int n_split, double *partial, hipStream_t s);
+void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *gram, hipStream_t s);
+void launch_cov(const double *gram, const double *colsum, int d, int d_pad, double n, double *cov,
+                double *total, hipStream_t s);
+void launch_eig_select(const double *evecs, const double *evals, int d, int k, double *variance,
+                       double *loadings, hipStream_t s);
+void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
+                       float *x, float *shift_scratch, hipStream_t s);
+void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, float *norms, void *xa,
+                     void *xb, hipStream_t s);
+int knn_kpad(int prec, int p);
+bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int kpad, int L,
+                       float *cand_d, uint32_t *cand_i, hipStream_t s);
+void launch_knn_rerank(const float *x, int64_t n, int p, const float *cand_d, const uint32_t *cand_i,
+                       int n_cand, int k, uint32_t *idx_out, float *dist_out, hipStream_t s);
+void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
+void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+}  // namespace crest

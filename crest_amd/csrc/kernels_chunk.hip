@@ -1,0 +1,326 @@
+// Streaming chunk kernels: per-gene statistics, CSR -> dense scaled tile (with the
+// column sums of the scaled matrix), the FP64 sparse projection and materialize.
+//
+// One wavefront owns one cell (span of CSR entries) at a time: the 64 lanes
+// stride over the cell's entries, so reads of `genes`/`values` are coalesced, and
+// the per-cell total used by normalize_total is a wave reduction.
+#include "common.hpp"
+
+namespace crest {
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f32(float *p, int64_t n, float v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_f64, dim3(g), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s) {
+    if (n <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_f32, dim3(g), dim3(256), 0, s, p, n, v);
+}
+
+// Sum over the kept genes of one cell (kernels.rs:69-76), identical in every lane.
+__device__ inline double cell_total(const DevChunk &c, int64_t lo, int64_t hi, int lane) {
+    double t = 0.0;
+    for (int64_t e = lo + lane; e < hi; e += kWave)
+        if (chunk_var(c, c.genes[e]) >= 0) t += (double)c.values[e];
+    return wave_sum_f64(t);
+}
+
+// ---------------------------------------------------------------- gene_stats
+// kernels.rs:141-160.  Each workgroup owns a contiguous block of cells and
+// accumulates a window of vars in LDS (FP64), then adds it to `out` once.
+__global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int n_vars) {
+    __shared__ double acc[5 * kStatsWindow];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t base = c.base;
+    const int64_t per = (c.n_spans + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r1 = r0 + per < c.n_spans ? r0 + per : c.n_spans;
+    for (int w0 = 0; w0 < n_vars; w0 += kStatsWindow) {
+        for (int i = threadIdx.x; i < 5 * kStatsWindow; i += blockDim.x) acc[i] = 0.0;
+        __syncthreads();
+        for (int64_t r = r0 + wave; r < r1; r += blockDim.x / kWave) {
+            if (chunk_row(c, r) < 0) continue;
+            const int64_t lo = c.span_off[r] - base, hi = c.span_off[r + 1] - base;
+            const double ns = norm_factor(cell_total(c, lo, hi, lane), c.target_sum);
+            for (int64_t e = lo + lane; e < hi; e += kWave) {
+                const int32_t m = chunk_var(c, c.genes[e]);
+                if (m >= w0 && m < w0 + kStatsWindow) {
+                    const double x = (double)lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+                    const double ex = expm1(x);
+                    double *a = acc + (m - w0);
+                    atomicAdd(a, x);
+                    atomicAdd(a + kStatsWindow, x * x);
+                    atomicAdd(a + 2 * kStatsWindow, ex);
+                    atomicAdd(a + 3 * kStatsWindow, ex * ex);
+                    atomicAdd(a + 4 * kStatsWindow, 1.0);
+                }
+            }
+        }
+        __syncthreads();
+        const int wn = n_vars - w0 < kStatsWindow ? n_vars - w0 : kStatsWindow;
+        for (int i = threadIdx.x; i < wn; i += blockDim.x)
+            for (int s = 0; s < 5; ++s) {
+                const double v = acc[s * kStatsWindow + i];
+                if (v != 0.0) atomicAdd(&out[(int64_t)(w0 + i) * 5 + s], v);
+            }
+        __syncthreads();
+    }
+}
+
+void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s) {
+    if (c.n_spans <= 0 || n_vars <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 16), 1024);
+    hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(256), 0, s, c, out, n_vars);
+}
+
+// ---------------------------------------------------------------- densify
+// Dense scaled tile S of a chunk (the sparse S of Z = 1 b^T + S, kernels.rs:197-219),
+// written in the blocked layout tile[(group*d_pad + h)*8 + r] (row = 8*group + r), so
+// that one 16-byte load gives an MFMA lane its K-run of consecutive rows.  Row i of
+// the tile is span i of the chunk; filtered cells and padding rows are zero.
+// colsum[h] += s over every non-zero (kernels.rs:258-260).
+__device__ inline uint16_t bf16_bits_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+template <int FMT> struct TileElem;
+template <> struct TileElem<TILE_F64> { typedef double T; static constexpr int DW = 1024; };
+template <> struct TileElem<TILE_F32> { typedef float T; static constexpr int DW = 2048; };
+template <> struct TileElem<TILE_BF16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *tile_a, void *tile_b,
+                                                 int64_t n_groups, int d_pad, double *colsum) {
+    typedef typename TileElem<FMT>::T T;
+    constexpr int DW = TileElem<FMT>::DW;
+    constexpr int NST = FMT == TILE_BF16X2 ? 2 : 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double *cs = reinterpret_cast<double *>(lds_raw);              // [DW]
+    T *stage = reinterpret_cast<T *>(lds_raw + DW * sizeof(double)); // [NST][DW][8]
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int nwave = blockDim.x / kWave;
+    const int64_t base = c.base;
+    for (int h0 = 0; h0 < d_pad; h0 += DW) {
+        const int dw = d_pad - h0 < DW ? d_pad - h0 : DW;
+        for (int i = threadIdx.x; i < DW; i += blockDim.x) cs[i] = 0.0;
+        for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+            // zero the stage (vectorised)
+            uint4 *st4 = reinterpret_cast<uint4 *>(stage);
+            const int n4 = (int)((int64_t)NST * DW * 8 * sizeof(T) / 16);
+            for (int i = threadIdx.x; i < n4; i += blockDim.x) st4[i] = make_uint4(0, 0, 0, 0);
+            __syncthreads();
+            for (int r = wave; r < kRowGroup; r += nwave) {
+                const int64_t span = g * kRowGroup + r;
+                if (span >= c.n_spans) break;
+                if (chunk_row(c, span) < 0) continue;
+                const int64_t lo = c.span_off[span] - base, hi = c.span_off[span + 1] - base;
+                const double ns = norm_factor(cell_total(c, lo, hi, lane), c.target_sum);
+                for (int64_t e = lo + lane; e < hi; e += kWave) {
+                    const int32_t m = chunk_var(c, c.genes[e]);
+                    if (m < 0) continue;
+                    const int32_t h = sc.sub[m];
+                    if (h < h0 || h >= h0 + dw) continue;
+                    const float v = lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+                    const double s = sc.scale ? scale_value(v, sc.mean[h], sc.sd[h], sc.max_value, 1) : (double)v;
+                    const int off = (h - h0) * kRowGroup + r;
+                    if constexpr (FMT == TILE_F64) {
+                        stage[off] = s;
+                    } else if constexpr (FMT == TILE_F32) {
+                        stage[off] = (float)s;
+                    } else {
+                        const uint16_t hb = bf16_bits_rn((float)s);
+                        const uint16_t lb = bf16_bits_rn((float)(s - (double)bf16_to_float(hb)));
+                        stage[off] = hb;
+                        stage[DW * kRowGroup + off] = lb;
+                    }
+                    atomicAdd(&cs[h - h0], s);
+                }
+            }
+            __syncthreads();
+            // stage -> tile (contiguous run of dw*8 elements per stage)
+            for (int t = 0; t < NST; ++t) {
+                T *dst = reinterpret_cast<T *>(t == 0 ? tile_a : tile_b) + ((int64_t)g * d_pad + h0) * kRowGroup;
+                const uint4 *src = reinterpret_cast<const uint4 *>(stage + (int64_t)t * DW * kRowGroup);
+                uint4 *dst4 = reinterpret_cast<uint4 *>(dst);
+                const int m4 = (int)((int64_t)dw * kRowGroup * sizeof(T) / 16);
+                for (int i = threadIdx.x; i < m4; i += blockDim.x) dst4[i] = src[i];
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < dw; i += blockDim.x)
+            if (cs[i] != 0.0) atomicAdd(&colsum[h0 + i], cs[i]);
+        __syncthreads();
+    }
+}
+
+void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b,
+                    int64_t rows_pad, int d_pad, double *colsum, hipStream_t s) {
+    const int64_t n_groups = rows_pad / kRowGroup;
+    if (n_groups <= 0) return;
+    size_t lds;
+    int g;
+    switch (fmt) {
+    case TILE_F64:
+        lds = TileElem<TILE_F64>::DW * (sizeof(double) + 8 * sizeof(double));
+        g = (int)std::min<int64_t>(n_groups, 512);
+        hipFuncSetAttribute((const void *)k_densify<TILE_F64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_densify<TILE_F64>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        break;
+    case TILE_F32:
+        lds = TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float));
+        g = (int)std::min<int64_t>(n_groups, 1024);
+        hipFuncSetAttribute((const void *)k_densify<TILE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_densify<TILE_F32>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        break;
+    default:
+        lds = TileElem<TILE_BF16X2>::DW * (sizeof(double) + 16 * sizeof(uint16_t));
+        g = (int)std::min<int64_t>(n_groups, 1024);
+        hipFuncSetAttribute((const void *)k_densify<TILE_BF16X2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_densify<TILE_BF16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        break;
+    }
+}
+
+// ---------------------------------------------------------------- project
+// kernels.rs:350-371: per kept cell, acc[j] = Σ_e s_e * V[h_e, j] in FP64 over the
+// cell's HVG entries in chunk order (multiply and add rounded separately, as the
+// reference does), then out[row, j] += (float)acc[j] in FP32.  Lane j owns column
+// j (and j+64, ...); the (h, s) of 64 entries at a time are computed one per lane
+// and broadcast with shuffles.
+template <int NC>
+__global__ __launch_bounds__(256) void k_project(DevChunk c, DevScale sc, const double *V, int k, float *out,
+                                                  int64_t row0) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    const int64_t base = c.base;
+    for (int64_t span = wave_id; span < c.n_spans; span += n_waves) {
+        const int64_t row = chunk_row(c, span);
+        if (row < 0) continue;
+        const int64_t lo = c.span_off[span] - base, hi = c.span_off[span + 1] - base;
+        const double ns = norm_factor(cell_total(c, lo, hi, lane), c.target_sum);
+        double acc[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q] = 0.0;
+        for (int64_t e0 = lo; e0 < hi; e0 += kWave) {
+            const int64_t e = e0 + lane;
+            int32_t h = -1;
+            double s = 0.0;
+            if (e < hi) {
+                const int32_t m = chunk_var(c, c.genes[e]);
+                if (m >= 0) {
+                    h = sc.sub[m];
+                    if (h >= 0) {
+                        const float v = lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+                        s = sc.scale ? scale_value(v, sc.mean[h], sc.sd[h], sc.max_value, 1) : (double)v;
+                    }
+                }
+            }
+            const int cnt = hi - e0 < kWave ? (int)(hi - e0) : kWave;
+            for (int j = 0; j < cnt; ++j) {
+                const int32_t hj = __shfl(h, j, kWave);
+                if (hj < 0) continue;
+                const double sj = __shfl(s, j, kWave);
+                const double *vr = V + (int64_t)hj * k;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const int col = lane + q * kWave;
+                    if (col < k) {
+                        const double prod = sj * vr[col];
+                        acc[q] = acc[q] + prod;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const int col = lane + q * kWave;
+            if (col < k) {
+                float *o = out + (row - row0) * (int64_t)k + col;
+                *o = *o + (float)acc[q];
+            }
+        }
+    }
+}
+
+void launch_project(const DevChunk &c, const DevScale &sc, const double *V, int k, float *out, int64_t row0,
+                    hipStream_t s) {
+    if (c.n_spans <= 0 || k <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 4), 8192);
+    if (k <= 64)
+        hipLaunchKernelGGL(k_project<1>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+    else if (k <= 128)
+        hipLaunchKernelGGL(k_project<2>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+    else
+        hipLaunchKernelGGL(k_project<4>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+}
+
+// ---------------------------------------------------------------- materialize
+// kernels.rs:122-139: kept entries of kept cells, transformed, in chunk order.
+__global__ __launch_bounds__(256) void k_mat_count(DevChunk c, int64_t *kept) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    const int64_t base = c.base;
+    for (int64_t span = wave_id; span < c.n_spans; span += n_waves) {
+        int64_t cnt = 0;
+        if (chunk_row(c, span) >= 0) {
+            const int64_t lo = c.span_off[span] - base, hi = c.span_off[span + 1] - base;
+            for (int64_t e = lo + lane; e < hi; e += kWave) cnt += chunk_var(c, c.genes[e]) >= 0;
+            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
+        }
+        if (lane == 0) kept[span] = cnt;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mat_write(DevChunk c, const int64_t *out_off, uint32_t *rows, uint32_t *vars, float *vals) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    const int64_t base = c.base;
+    for (int64_t span = wave_id; span < c.n_spans; span += n_waves) {
+        const int64_t row = chunk_row(c, span);
+        if (row < 0) continue;
+        const int64_t lo = c.span_off[span] - base, hi = c.span_off[span + 1] - base;
+        const double ns = norm_factor(cell_total(c, lo, hi, lane), c.target_sum);
+        int64_t pos = out_off[span];
+        for (int64_t e0 = lo; e0 < hi; e0 += kWave) {
+            const int64_t e = e0 + lane;
+            const int32_t m = e < hi ? chunk_var(c, c.genes[e]) : -1;
+            const bool keep = m >= 0;
+            const uint64_t mask = __ballot(keep);
+            const int before = __popcll(mask & ((1ull << lane) - 1ull));
+            if (keep) {
+                rows[pos + before] = (uint32_t)row;
+                vars[pos + before] = (uint32_t)m;
+                vals[pos + before] = lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+            }
+            pos += __popcll(mask);
+        }
+    }
+}
+
+void launch_materialize_count(const DevChunk &c, int64_t *kept, hipStream_t s) {
+    if (c.n_spans <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 4), 8192);
+    hipLaunchKernelGGL(k_mat_count, dim3(g), dim3(256), 0, s, c, kept);
+}
+void launch_materialize(const DevChunk &c, const int64_t *out_off, uint32_t *rows, uint32_t *vars, float *vals, hipStream_t s) {
+    if (c.n_spans <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 4), 8192);
+    hipLaunchKernelGGL(k_mat_write, dim3(g), dim3(256), 0, s, c, out_off, rows, vars, vals);
+}
+
+}  // namespace crest

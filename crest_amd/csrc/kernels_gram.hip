@@ -1,0 +1,378 @@
+// Gram SYRK G += S^T S on MFMA over a densified chunk tile, deterministic split-K
+// reduction into the FP64 Gram, FP64 centring into the covariance, and the
+// post-processing of the eigensolve (descending order, floor at 0, sign rule).
+//
+// Tile layout (written by k_densify): tile[(g*d_pad + h)*8 + r] holds S[8g + r, h].
+// G is d_pad x d_pad row-major FP64; only blocks bi <= bj (of 128 x 128) are
+// produced, which covers every a <= b entry the reference reads (kernels.rs:319-322).
+#include "common.hpp"
+
+namespace crest {
+
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) double f64x4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) {
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, 16);
+    return r;
+}
+
+// blk -> (bi, bj) over the upper triangle of nb x nb blocks, row by row.
+__device__ inline void tri_block(int blk, int nb, int &bi, int &bj) {
+    int r = 0;
+    while (blk >= nb - r) {
+        blk -= nb - r;
+        ++r;
+    }
+    bi = r;
+    bj = r + blk;
+}
+
+// Split-K partition: workgroup w handles split w % n_split of block w / n_split, so
+// with n_split a multiple of 8 every split (one range of tile rows) lives on one XCD
+// and the row slab it streams is shared through that XCD's L2 by all its blocks.
+__device__ inline void gram_work(int n_split, int nb, int &blk, int &split, int &bi, int &bj) {
+    split = blockIdx.x % n_split;
+    blk = blockIdx.x / n_split;
+    tri_block(blk, nb, bi, bj);
+}
+
+// Output of a 32x32 MFMA tile: register r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l%32].
+__device__ inline void store_partial32(double *dst, const double (&v)[16], int m0, int n0, int lane) {
+    const int col = lane & 31, half = lane >> 5;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = m0 + 8 * (r / 4) + 4 * half + (r % 4);
+        dst[m * kGramBlock + n0 + col] = v[r];
+    }
+}
+
+// ---------------------------------------------------------------- split-BF16
+// S = H + L (bf16 hi/lo); S^T S ~ H^T H + H^T L + L^T H on v_mfma_f32_32x32x16_bf16.
+// FP32 accumulators are added into FP64 every FLUSH K-steps (16 rows each).
+template <int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
+                                                     int64_t n_groups, int d_pad, int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
+    const int col = lane & 31, half = lane >> 5;
+    const int ga = bi * kGramBlock + wm * 64 + col, gb = bj * kGramBlock + wn * 64 + col;
+    const int64_t n_pairs = n_groups / 2;
+    const int64_t per = (n_pairs + n_split - 1) / n_split;
+    const int64_t p0 = (int64_t)split * per;
+    const int64_t p1 = p0 + per < n_pairs ? p0 + per : n_pairs;
+    f32x16 acc[2][2];
+    double dacc[2][2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (f32x16){};
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] = 0.0;
+        }
+    int cnt = 0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const int64_t u = (2 * p + half) * (int64_t)d_pad;
+        const bf16x8 ah0 = as_bf16x8(th[u + ga]), ah1 = as_bf16x8(th[u + ga + 32]);
+        const bf16x8 al0 = as_bf16x8(tl[u + ga]), al1 = as_bf16x8(tl[u + ga + 32]);
+        const bf16x8 bh0 = as_bf16x8(th[u + gb]), bh1 = as_bf16x8(th[u + gb + 32]);
+        const bf16x8 bl0 = as_bf16x8(tl[u + gb]), bl1 = as_bf16x8(tl[u + gb + 32]);
+        const bf16x8 ah[2] = {ah0, ah1}, al[2] = {al0, al1}, bh[2] = {bh0, bh1}, bl[2] = {bl0, bl1};
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (++cnt == FLUSH) {
+            cnt = 0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = (f32x16){};
+                }
+        }
+    }
+    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane);
+        }
+}
+
+// ---------------------------------------------------------------- FP32
+// v_mfma_f32_32x32x2_f32; lane half h takes rows 4h..4h+3 of each row group, so the
+// four MFMAs of a group cover its eight rows.
+template <int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, int64_t n_groups, int d_pad,
+                                                  int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
+    const int col = lane & 31, half = lane >> 5;
+    const int ga = bi * kGramBlock + wm * 64 + col, gb = bj * kGramBlock + wn * 64 + col;
+    const int64_t per = (n_groups + n_split - 1) / n_split;
+    const int64_t g0 = (int64_t)split * per;
+    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
+    f32x16 acc[2][2];
+    double dacc[2][2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (f32x16){};
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] = 0.0;
+        }
+    int cnt = 0;
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t u = g * (int64_t)d_pad;
+        const float4 a[2] = {t[(u + ga) * 2 + half], t[(u + ga + 32) * 2 + half]};
+        const float4 b[2] = {t[(u + gb) * 2 + half], t[(u + gb + 32) * 2 + half]};
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].x, b[j].x, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].z, b[j].z, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].w, b[j].w, acc[i][j], 0, 0, 0);
+            }
+        if (++cnt == FLUSH) {
+            cnt = 0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = (f32x16){};
+                }
+        }
+    }
+    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane);
+        }
+}
+
+// ---------------------------------------------------------------- FP64
+// v_mfma_f64_16x16x4_f64; lane quarter q = l/16 takes rows 2q, 2q+1 of each row
+// group.  Output: register r of lane l is C[4*(l/16) + r][l%16].
+__global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t, int64_t n_groups, int d_pad,
+                                                  int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
+    const int c16 = lane & 15, q = lane >> 4;
+    const int ga = bi * kGramBlock + wm * 64 + c16, gb = bj * kGramBlock + wn * 64 + c16;
+    const int64_t per = (n_groups + n_split - 1) / n_split;
+    const int64_t g0 = (int64_t)split * per;
+    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){};
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t u = g * (int64_t)d_pad;
+        double2 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = t[(u + ga + 16 * i) * 4 + q];
+            b[i] = t[(u + gb + 16 * i) * 4 + q];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
+            }
+    }
+    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = wm * 64 + i * 16 + 4 * q + r;
+                const int n = wn * 64 + j * 16 + c16;
+                dst[m * kGramBlock + n] = acc[i][j][r];
+            }
+}
+
+void launch_gram(int fmt, const void *tile_a, const void *tile_b, int64_t rows_pad, int d_pad, int n_split,
+                 double *partial, hipStream_t s) {
+    const int nb = d_pad / kGramBlock;
+    const int nblk = nb * (nb + 1) / 2;
+    const int64_t n_groups = rows_pad / kRowGroup;
+    dim3 grid(nblk * n_split), block(256);
+    switch (fmt) {
+    case TILE_F64:
+        hipLaunchKernelGGL(k_gram_f64, grid, block, 0, s, (const double2 *)tile_a, n_groups, d_pad, n_split, partial);
+        break;
+    case TILE_F32:
+        hipLaunchKernelGGL(k_gram_f32<32>, grid, block, 0, s, (const float4 *)tile_a, n_groups, d_pad, n_split, partial);
+        break;
+    default:
+        hipLaunchKernelGGL(k_gram_bf16x3<16>, grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b, n_groups,
+                           d_pad, n_split, partial);
+        break;
+    }
+}
+
+// Deterministic split-K sum into the FP64 Gram, splits in order.
+__global__ void k_gram_reduce(const double *partial, int n_split, int nb, int d_pad, double *gram) {
+    const int nblk = nb * (nb + 1) / 2;
+    const int64_t total = (int64_t)nblk * kGramBlock * kGramBlock;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int blk = (int)(e / (kGramBlock * kGramBlock));
+        const int rem = (int)(e % (kGramBlock * kGramBlock));
+        int bi, bj;
+        tri_block(blk, nb, bi, bj);
+        const double *src = partial + (int64_t)blk * n_split * kGramBlock * kGramBlock + rem;
+        double s = 0.0;
+        for (int sp = 0; sp < n_split; ++sp) s += src[(int64_t)sp * kGramBlock * kGramBlock];
+        const int m = rem / kGramBlock, n = rem % kGramBlock;
+        gram[(int64_t)(bi * kGramBlock + m) * d_pad + bj * kGramBlock + n] += s;
+    }
+}
+
+void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *gram, hipStream_t s) {
+    const int nb = d_pad / kGramBlock;
+    const int64_t total = (int64_t)nb * (nb + 1) / 2 * kGramBlock * kGramBlock;
+    const int g = (int)std::min<int64_t>(ceil_div(total, 256), 8192);
+    hipLaunchKernelGGL(k_gram_reduce, dim3(g), dim3(256), 0, s, partial, n_split, nb, d_pad, gram);
+}
+
+// ---------------------------------------------------------------- covariance
+// cov[i, j] = (G[a, b] - c_a c_b / n) / (n - 1) with a = min, b = max (kernels.rs:318-322),
+// in the reference's operation order; written full and column-major (symmetric).
+__global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov) {
+    const int64_t total = (int64_t)d * d;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e / d), i = (int)(e % d);
+        const int a = i <= j ? i : j, b = i <= j ? j : i;
+        const double cc = colsum[a] * colsum[b];
+        const double t = cc / n;
+        const double u = gram[(int64_t)a * ld_g + b] - t;
+        cov[e] = u / (n - 1.0);
+    }
+}
+// total = Σ_i cov[i, i] in index order (kernels.rs:323).
+__global__ void k_trace(const double *cov, int d, double *total) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < d; ++i) s += cov[(int64_t)i * d + i];
+        *total = s;
+    }
+}
+
+void launch_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov, double *total,
+                hipStream_t s) {
+    const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 8192);
+    hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov);
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(64), 0, s, cov, d, total);
+}
+
+// ---------------------------------------------------------------- eigen selection
+// Component c takes eigenpair d-1-c of the ascending solver output; variance is
+// floored at 0; the sign makes the first largest-|loading| entry positive
+// (kernels.rs:328-346).  evecs is column-major (column j = eigenvector j).
+__global__ __launch_bounds__(256) void k_eig_select(const double *evecs, const double *evals, int d, int k,
+                                                    double *variance, double *loadings) {
+    __shared__ double sbest[256];
+    __shared__ int sidx[256];
+    const int c = blockIdx.x;
+    const int src = d - 1 - c;
+    const double *u = evecs + (int64_t)src * d;
+    double best = 0.0;
+    int bidx = -1;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        const double a = fabs(u[i]);
+        if (a > best) {
+            best = a;
+            bidx = i;
+        }
+    }
+    sbest[threadIdx.x] = best;
+    sidx[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double b2 = sbest[threadIdx.x + o];
+            const int i2 = sidx[threadIdx.x + o];
+            const double b1 = sbest[threadIdx.x];
+            const int i1 = sidx[threadIdx.x];
+            if (i2 >= 0 && (i1 < 0 || b2 > b1 || (b2 == b1 && i2 < i1))) {
+                sbest[threadIdx.x] = b2;
+                sidx[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    double sign = 1.0;
+    if (sidx[0] >= 0) sign = signbit(u[sidx[0]]) ? -1.0 : 1.0;
+    if (threadIdx.x == 0) {
+        const double v = evals[src];
+        variance[c] = v > 0.0 ? v : 0.0;
+    }
+    for (int i = threadIdx.x; i < d; i += blockDim.x) loadings[(int64_t)i * k + c] = u[i] * sign;
+}
+
+void launch_eig_select(const double *evecs, const double *evals, int d, int k, double *variance, double *loadings,
+                       hipStream_t s) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_eig_select, dim3(k), dim3(256), 0, s, evecs, evals, d, k, variance, loadings);
+}
+
+// ---------------------------------------------------------------- shift prefill
+// shift = (colsum / n) @ V (tl.py:76), then X[:] = -(f32)shift (tl.py:77-78).
+__global__ void k_shift(const double *colsum, const double *V, int d, int k, double n, float *shift_f) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (int a = 0; a < d; ++a) {
+        const double m = colsum[a] / n;
+        const double p = m * V[(int64_t)a * k + c];
+        s = s + p;
+    }
+    shift_f[c] = -(float)s;
+}
+__global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x) {
+    const int64_t total = n_rows * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = shift_f[e % k];
+}
+
+void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows, float *x,
+                       float *shift_f, hipStream_t s) {
+    hipLaunchKernelGGL(k_shift, dim3((k + 63) / 64), dim3(64), 0, s, colsum, V, d, k, n, shift_f);
+    const int g = (int)std::min<int64_t>(ceil_div(n_rows * k, 256), 8192);
+    hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x);
+}
+
+}  // namespace crest

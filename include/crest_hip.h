@@ -1,0 +1,169 @@
+/*
+ * crest_hip.h — C ABI of the MI355X (gfx950) implementation of CREST's streamed
+ * counts -> scale -> PCA -> kNN path.
+ *
+ * Every operator entry point below replaces one function of the reference's PyO3
+ * module `crest.crest` (src/py.rs) and keeps its argument meaning, accumulation
+ * semantics (outputs marked "+=" are accumulated in place, as the reference does)
+ * and error classes.  Pointers are host pointers unless the name says `dev`.
+ * All functions are thread-safe (one mutex per device context) and return
+ * CREST_OK (0) or a negative status; the message of the last failure on the
+ * calling thread is returned by crest_last_error().
+ *
+ * Status classes map onto the reference's Python exceptions:
+ *   CREST_E_VALUE   (-1)  -> ValueError   (py.rs:11-27,162-189,226-228,250-268,279-281,295-304)
+ *   CREST_E_RUNTIME (-2)  -> RuntimeError (py.rs:396, kernels.rs:315-317)
+ *   CREST_E_DEVICE  (-3)  -> RuntimeError (HIP / rocSOLVER failure; no reference counterpart)
+ */
+#ifndef CREST_HIP_H
+#define CREST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CREST_OK 0
+#define CREST_E_VALUE (-1)
+#define CREST_E_RUNTIME (-2)
+#define CREST_E_DEVICE (-3)
+
+/* Arithmetic of the dense contractions (Gram SYRK and the kNN distance screen).
+ * The transform, gene statistics, centring, eigensolve, projection and the kNN
+ * rerank are FP64 in every mode.                                                  */
+#define CREST_PREC_FP64 0   /* Gram on FP64 MFMA; kNN screen on FP32 MFMA          */
+#define CREST_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation flushed to FP64          */
+#define CREST_PREC_BF16X3 2 /* split-BF16 hi+lo, three MFMA products, FP32 accum.   */
+
+/* One chunk of raw counts plus the recorded filters and lazy transform: the
+ * 9-tuple `(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr,
+ * first_cell)` of crest/core.py:265-278 and py.rs:162-189,195-211.
+ * Exactly one of `cells` / `indptr` is non-NULL.                                  */
+typedef struct crest_chunk {
+    const uint32_t *genes;    /* [nnz] store gene id of each entry                  */
+    const float *values;      /* [nnz] raw value of each entry                      */
+    size_t nnz;
+    const int64_t *cell_map;  /* [n_cell_map] store cell -> row (-1 = filtered)     */
+    size_t n_cell_map;
+    const int32_t *gene_map;  /* [n_gene_map] store gene -> var (-1 = filtered)     */
+    size_t n_gene_map;
+    double target_sum;        /* normalize_total target (<= 0: off)                 */
+    int32_t log1p;            /* apply log1p after normalisation                    */
+    const uint32_t *cells;    /* [nnz] per-entry store cell id, non-decreasing      */
+    const int64_t *indptr;    /* [n_indptr] absolute CSR offsets of cells first..   */
+    size_t n_indptr;
+    uint32_t first_cell;
+} crest_chunk;
+
+/* Library / device state ------------------------------------------------------ */
+
+/* Select the device used by this thread's calls (default 0). */
+int crest_set_device(int32_t device);
+/* Precision used by crest_gram_accumulate / crest_knn_graph (default FP64). */
+int crest_set_precision(int32_t prec);
+int32_t crest_get_precision(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *crest_last_error(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int32_t crest_device_count(void);
+
+/* Operator entry points (host buffers) ------------------------------------------ */
+
+/* Replaces py.rs:221-232 `gene_stats(*ctx, out)` (kernels.rs:141-160).
+ * out (+=) is [n_vars, 5] row-major f64: Σx, Σx², Σexpm1(x), Σexpm1(x)², nnz
+ * over the transformed values of kept cells and genes; n_vars = max(gene_map)+1. */
+int crest_gene_stats(const crest_chunk *chunk, double *out, size_t out_len);
+
+/* Replaces py.rs:270-287 `gram_accumulate(*ctx, sub_map, gram, colsum, scale,
+ * mean, std, max_value)` (kernels.rs:197-309).  gram (+=) is [d, d] row-major
+ * f64, of which the upper triangle a <= b is written; colsum (+=) is [d].
+ * sub_map is [n_sub_map] var -> 0..d-1 or -1; mean/std are [d] (may be NULL when
+ * scale == 0).                                                                    */
+int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub_map,
+                          double *gram, size_t gram_len, double *colsum, size_t d, int32_t scale,
+                          const double *mean, size_t n_mean, const double *std, size_t n_std,
+                          double max_value);
+
+/* Replaces py.rs:386-399 `pca_from_gram(gram, colsum, n, k)` (kernels.rs:311-348).
+ * Writes variance [k'] (descending, floored at 0), loadings [d, k'] row-major with
+ * the sklearn svd_flip sign rule, *total = trace of the covariance and *k_out = k'
+ * = min(k, d).  The caller sizes variance/loadings for min(k, d).                */
+int crest_pca_from_gram(const double *gram, size_t gram_len, const double *colsum, size_t d,
+                        size_t n, size_t k, double *variance, double *loadings, double *total,
+                        size_t *k_out);
+
+/* Replaces py.rs:289-309 `project(*ctx, sub_map, loadings, out, scale, mean, std,
+ * max_value)` (kernels.rs:350-371).  out (+=) is [n_rows, k] f32 with n_rows =
+ * max(cell_map)+1; out_len must be >= n_rows*k.  loadings is [d, k] row-major.    */
+int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub_map,
+                  const double *loadings, size_t d, size_t k, float *out, size_t out_len,
+                  int32_t scale, const double *mean, size_t n_mean, const double *std,
+                  size_t n_std, double max_value);
+
+/* Replaces py.rs:52-70 `knn_graph(data, k, exact=None, seed=0)` (knn.rs:335-369).
+ * data is [n, dim] row-major f32.  k' = min(k, n-1) (>= 1 when n > 1) is written to
+ * *k_out; indices [n, k'] u32 and distances [n, k'] f32 (Euclidean, ascending, ties
+ * by index, self excluded) must hold n*min(k, max(n-1,1)) entries.  The search is
+ * exact for every n: `exact` (-1 = None, 0, 1) and `seed` are accepted and ignored. */
+int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact,
+                    uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
+
+/* Replaces py.rs:213-219 `materialize(*ctx)` (kernels.rs:122-139) for parity
+ * checks of the lazy transform.  Writes the kept, transformed entries of the chunk
+ * in chunk order: rows/vars u32 and values f32, each of capacity `cap` (>= nnz);
+ * *n_out is the count written.                                                    */
+int crest_materialize(const crest_chunk *chunk, uint32_t *rows, uint32_t *vars, float *values,
+                      size_t cap, size_t *n_out);
+
+/* Device-resident pipeline (no reference counterpart: the fused form of tl.pca +
+ * pp.neighbors over a CSR already resident in HBM) -------------------------------- */
+
+typedef struct crest_pipeline_args {
+    /* CSR of raw counts in device memory: cells x genes */
+    const int64_t *dev_indptr; /* [n_cells+1], indptr[0] == 0                       */
+    const uint32_t *dev_genes; /* [nnz]                                             */
+    const float *dev_values;   /* [nnz]                                             */
+    size_t n_cells;
+    size_t n_genes;
+    /* host: store gene -> HVG index 0..d-1, or -1 (kept, not HVG), length n_genes  */
+    const int32_t *hvg_of_gene;
+    size_t d;
+    double target_sum; /* normalize_total target (<= 0: off)                       */
+    int32_t log1p;
+    int32_t scale;      /* pp.scale recorded                                         */
+    double max_value;   /* clip (<= 0: off)                                          */
+    size_t n_comps;     /* requested PCs                                             */
+    size_t k;           /* neighbours per cell (knn_graph k)                         */
+    int32_t prec_gram;  /* CREST_PREC_* for the Gram                                 */
+    int32_t prec_knn;   /* CREST_PREC_FP32 or CREST_PREC_BF16X3 for the screen       */
+    size_t chunk_rows;  /* cells per densified tile (0 = default)                    */
+    /* outputs in device memory */
+    float *dev_x_pca;        /* [n_cells, k_pcs]                                     */
+    uint32_t *dev_knn_idx;   /* [n_cells, k']                                        */
+    float *dev_knn_dist;     /* [n_cells, k']                                        */
+} crest_pipeline_args;
+
+typedef struct crest_pipeline_result {
+    size_t k_pcs;          /* min(n_comps, d, n-1)                                   */
+    size_t k_nn;           /* k'                                                     */
+    double total_variance;
+    /* per-stage device times in ms (HIP events on the pipeline stream)            */
+    double ms_stats, ms_gram, ms_eig, ms_project, ms_knn_prep, ms_knn_screen, ms_knn_rerank;
+    double ms_total;
+} crest_pipeline_result;
+
+/* Size the host buffers: variance [n_comps], loadings [d, n_comps] (row-major, k_pcs
+ * columns used).  Runs gene_stats -> mean/std -> Gram -> eigensolve -> project ->
+ * kNN on the current device's pipeline stream and waits for it to finish.         */
+int crest_pipeline_run(const crest_pipeline_args *args, double *variance, double *loadings,
+                       crest_pipeline_result *result);
+
+/* Release every cached device buffer of the current device. */
+int crest_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CREST_HIP_H */
