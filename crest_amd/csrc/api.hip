@@ -1,0 +1,670 @@
+// C ABI of the gfx950 library (include/crest_hip.h): argument validation with the
+// reference's error classes (py.rs), a per-device context (stream, rocBLAS handle,
+// grow-only device buffers) and the orchestration of the kernels.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/crest_hip.h"
+#include "common.hpp"
+
+namespace crest {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_err;
+static thread_local int g_device = 0;
+static std::atomic<int> g_prec{CREST_PREC_FP64};
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+[[noreturn]] static void fail(int code, const std::string &msg) { throw Fail{code, msg}; }
+static void value_error(const std::string &msg) { fail(CREST_E_VALUE, msg); }
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) fail(CREST_E_DEVICE, std::string(#x ": ") + hipGetErrorString(e_));     \
+    } while (0)
+#define BLASCHK(x)                                                                                    \
+    do {                                                                                              \
+        rocblas_status s_ = (x);                                                                      \
+        if (s_ != rocblas_status_success)                                                             \
+            fail(CREST_E_DEVICE, std::string(#x ": ") + rocblas_status_to_string(s_));               \
+    } while (0)
+
+template <class F> static int guarded(F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return CREST_OK;
+    } catch (const Fail &e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return CREST_E_DEVICE;
+    }
+}
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+    std::mutex mu;
+    int dev = 0;
+    hipStream_t st = nullptr;
+    rocblas_handle blas = nullptr;
+    std::map<std::string, std::pair<void *, size_t>> bufs;
+    hipEvent_t ev[12] = {};
+
+    void init(int d) {
+        dev = d;
+        HIPCHK(hipSetDevice(dev));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        BLASCHK(rocblas_create_handle(&blas));
+        BLASCHK(rocblas_set_stream(blas, st));
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    // Device buffer `name` of at least `bytes` (grow-only; contents undefined).
+    template <class T> T *get(const std::string &name, size_t count) {
+        size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        auto &b = bufs[name];
+        if (b.second < bytes) {
+            if (b.first) HIPCHK(hipFree(b.first));
+            b.first = nullptr;
+            b.second = 0;
+            HIPCHK(hipMalloc(&b.first, bytes));
+            b.second = bytes;
+        }
+        return static_cast<T *>(b.first);
+    }
+    template <class T> T *upload(const std::string &name, const T *host, size_t count) {
+        T *d = get<T>(name, count);
+        if (count) HIPCHK(hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, st));
+        return d;
+    }
+    template <class T> void download(T *host, const T *dev, size_t count) {
+        if (count) HIPCHK(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, st));
+    }
+    void sync() { HIPCHK(hipStreamSynchronize(st)); }
+    void check_launch() { HIPCHK(hipGetLastError()); }
+    void release() {
+        for (auto &kv : bufs)
+            if (kv.second.first) hipFree(kv.second.first);
+        bufs.clear();
+    }
+};
+
+static std::mutex g_ctx_mu;
+static Ctx *g_ctx[64] = {};
+
+static Ctx &ctx() {
+    const int d = g_device;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (d < 0 || d >= 64) value_error("device index out of range");
+    if (!g_ctx[d]) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) fail(CREST_E_DEVICE, "no HIP device is visible");
+        if (d >= n) value_error("device index out of range");
+        Ctx *c = new Ctx();
+        c->init(d);
+        g_ctx[d] = c;
+    }
+    HIPCHK(hipSetDevice(d));
+    return *g_ctx[d];
+}
+
+// ---------------------------------------------------------------- chunk validation
+// py.rs:162-189 (view) and :191-193 (n_var).
+struct HostChunk {
+    const crest_chunk *c;
+    std::vector<int64_t> span_off;  // only for per-entry cell ids
+    std::vector<uint32_t> span_cell;
+    int64_t n_spans = 0;
+    int64_t base = 0;
+    int64_t n_rows = 0;  // max(cell_map) + 1
+    int64_t n_vars = 0;  // max(gene_map) + 1
+};
+
+static HostChunk check_chunk(const crest_chunk *c) {
+    if (!c) value_error("chunk is NULL");
+    HostChunk h;
+    h.c = c;
+    if ((c->nnz && (!c->genes || !c->values)) || (c->n_cell_map && !c->cell_map) || (c->n_gene_map && !c->gene_map))
+        value_error("chunk arrays must not be NULL");
+    if ((c->cells != nullptr) == (c->indptr != nullptr)) value_error("pass exactly one of cells / indptr");
+    if (c->cells) {
+        for (size_t i = 1; i < c->nnz; ++i)
+            if (c->cells[i - 1] > c->cells[i]) value_error("chunk must be sorted by cell");
+        size_t p = 0;
+        while (p < c->nnz) {
+            size_t q = p + 1;
+            while (q < c->nnz && c->cells[q] == c->cells[p]) ++q;
+            h.span_off.push_back((int64_t)p);
+            h.span_cell.push_back(c->cells[p]);
+            p = q;
+        }
+        h.span_off.push_back((int64_t)c->nnz);
+        h.n_spans = (int64_t)h.span_cell.size();
+        h.base = 0;
+    } else {
+        const int64_t *p = c->indptr;
+        const size_t m = c->n_indptr;
+        bool ok = m > 0;
+        for (size_t i = 1; ok && i < m; ++i) ok = p[i - 1] <= p[i];
+        if (ok) ok = (uint64_t)(p[m - 1] - p[0]) == (uint64_t)c->nnz;
+        if (!ok) value_error("indptr must be non-decreasing and span the chunk");
+        h.n_spans = (int64_t)m - 1;
+        h.base = p[0];
+    }
+    int64_t mr = -1;
+    for (size_t i = 0; i < c->n_cell_map; ++i) mr = std::max<int64_t>(mr, c->cell_map[i]);
+    h.n_rows = mr + 1 > 0 ? mr + 1 : 0;
+    int64_t mv = -1;
+    for (size_t i = 0; i < c->n_gene_map; ++i) mv = std::max<int64_t>(mv, c->gene_map[i]);
+    h.n_vars = mv + 1 > 0 ? mv + 1 : 0;
+    return h;
+}
+
+// Row of each span on the host (for the rows a chunk touches).
+static int64_t host_row(const HostChunk &h, int64_t i) {
+    const crest_chunk *c = h.c;
+    const uint64_t cell = c->cells ? (uint64_t)h.span_cell[i] : (uint64_t)c->first_cell + (uint64_t)i;
+    if (cell >= c->n_cell_map) return -1;
+    return c->cell_map[cell];
+}
+
+static DevChunk upload_chunk(Ctx &x, const HostChunk &h) {
+    const crest_chunk *c = h.c;
+    DevChunk d{};
+    d.genes = x.upload("genes", c->genes, c->nnz);
+    d.values = x.upload("values", c->values, c->nnz);
+    d.cell_map = x.upload("cell_map", c->cell_map, c->n_cell_map);
+    d.n_cell_map = (int64_t)c->n_cell_map;
+    d.gene_var = x.upload("gene_map", c->gene_map, c->n_gene_map);
+    d.n_gene_map = (int64_t)c->n_gene_map;
+    d.target_sum = c->target_sum;
+    d.log1p = c->log1p ? 1 : 0;
+    d.n_spans = h.n_spans;
+    d.base = h.base;
+    if (c->cells) {
+        d.span_off = x.upload("span_off", h.span_off.data(), h.span_off.size());
+        d.span_cell = x.upload("span_cell", h.span_cell.data(), h.span_cell.size());
+        d.first_cell = 0;
+    } else {
+        d.span_off = x.upload("span_off", c->indptr, c->n_indptr);
+        d.span_cell = nullptr;
+        d.first_cell = c->first_cell;
+    }
+    return d;
+}
+
+// Rows [r0, r1) of a device chunk (entry offsets stay absolute).
+static DevChunk sub_chunk(const DevChunk &c, int64_t r0, int64_t r1) {
+    DevChunk s = c;
+    s.span_off = c.span_off + r0;
+    if (c.span_cell)
+        s.span_cell = c.span_cell + r0;
+    else
+        s.first_cell = c.first_cell + (uint32_t)r0;
+    s.n_spans = r1 - r0;
+    return s;
+}
+
+// py.rs:263-268
+static void check_sub(const int32_t *sub, size_t n_sub, int64_t nv, size_t d) {
+    if ((int64_t)n_sub != nv) value_error("sub_map must have length n_vars and values < d");
+    for (size_t i = 0; i < n_sub; ++i)
+        if (sub[i] >= (int64_t)d) value_error("sub_map must have length n_vars and values < d");
+}
+// py.rs:250-261
+static void check_scale(int32_t scale, const double *mean, size_t n_mean, const double *sd, size_t n_sd, size_t d) {
+    if (!scale) return;
+    if (!mean || !sd) value_error("scale=True needs mean and std");
+    if (n_mean != d || n_sd != d) value_error("mean/std must have length d");
+}
+
+static int tile_fmt(int prec) { return prec == CREST_PREC_FP32 ? TILE_F32 : prec == CREST_PREC_BF16X3 ? TILE_BF16X2 : TILE_F64; }
+static size_t tile_elem_bytes(int fmt) { return fmt == TILE_F64 ? 8 : fmt == TILE_F32 ? 4 : 2; }
+
+static int choose_split(int nblk, int64_t rows_pad) {
+    const int64_t ksteps = rows_pad / 16;
+    int s = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, nblk), ksteps / 8));
+    if (s >= 8) s = s / 8 * 8;
+    return std::max(s, 1);
+}
+
+// Gram + colsum of the rows of a device chunk, accumulated into dev_gram/dev_colsum.
+static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, int d_pad, int64_t tile_rows,
+                      double *dev_gram, double *dev_colsum) {
+    const int fmt = tile_fmt(prec);
+    const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
+    for (int64_t r0 = 0; r0 < c.n_spans; r0 += tile_rows) {
+        const int64_t r1 = std::min<int64_t>(c.n_spans, r0 + tile_rows);
+        const int64_t rows_pad = round_up(r1 - r0, 16);
+        const size_t tile_elems = (size_t)rows_pad * d_pad;
+        void *ta = x.get<uint8_t>("tile_a", tile_elems * tile_elem_bytes(fmt));
+        void *tb = fmt == TILE_BF16X2 ? x.get<uint8_t>("tile_b", tile_elems * 2) : nullptr;
+        const int n_split = choose_split(nblk, rows_pad);
+        double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
+        launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, x.st);
+        launch_gram(fmt, ta, tb, rows_pad, d_pad, n_split, partial, x.st);
+        launch_gram_reduce(partial, n_split, d_pad, dev_gram, x.st);
+        x.check_launch();
+    }
+}
+
+// Eigen-decomposition of the covariance from (gram [ld_g], colsum, n): variance and
+// loadings (device), total (device scalar).
+static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *dev_colsum, int d, double n, int k,
+                       double *dev_var, double *dev_load, double *dev_total) {
+    double *cov = x.get<double>("cov", (size_t)d * d);
+    double *w = x.get<double>("eig_w", d);
+    double *e = x.get<double>("eig_e", d);
+    rocblas_int *info = x.get<rocblas_int>("eig_info", 1);
+    launch_cov(dev_gram, dev_colsum, d, ld_g, n, cov, dev_total, x.st);
+    x.check_launch();
+    BLASCHK(rocsolver_dsyevd(x.blas, rocblas_evect_original, rocblas_fill_upper, d, cov, d, w, e, info));
+    launch_eig_select(cov, w, d, k, dev_var, dev_load, x.st);
+    x.check_launch();
+    rocblas_int hinfo = 0;
+    x.download(&hinfo, info, 1);
+    x.sync();
+    if (hinfo != 0) fail(CREST_E_RUNTIME, "eigensolver did not converge (info " + std::to_string(hinfo) + ")");
+}
+
+// kNN of the rows of dev_x [n, p] into dev_idx/dev_dist [n, k].
+static void knn_device(Ctx &x, const float *dev_x, int64_t n, int p, int k, int prec, uint32_t *dev_idx,
+                       float *dev_dist, hipEvent_t *ev3) {
+    const int kp = prec == CREST_PREC_BF16X3 ? CREST_PREC_BF16X3 : CREST_PREC_FP32;
+    const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
+    if (k > 64) value_error("k > 64 is not supported by the exact GPU kNN");
+    const int kpad = knn_kpad(kp, p);
+    if (kpad < 0) value_error("data has too many columns for the GPU kNN (at most 128)");
+    const int64_t n_pad = round_up(n, 256);
+    const size_t eb = kp == CREST_PREC_BF16X3 ? 2 : 4;
+    void *xa = x.get<uint8_t>("knn_xa", (size_t)n_pad * kpad * eb);
+    void *xb = x.get<uint8_t>("knn_xb", (size_t)n_pad * kpad * eb);
+    float *cd = x.get<float>("knn_cd", (size_t)n_pad * 2 * L);
+    uint32_t *ci = x.get<uint32_t>("knn_ci", (size_t)n_pad * 2 * L);
+    if (ev3) HIPCHK(hipEventRecord(ev3[0], x.st));
+    launch_knn_prep(kp, dev_x, n, p, kpad, nullptr, xa, xb, x.st);
+    if (ev3) HIPCHK(hipEventRecord(ev3[1], x.st));
+    if (!launch_knn_screen(kp, xa, xb, n, kpad, L, cd, ci, x.st)) value_error("no kNN kernel for this width");
+    if (ev3) HIPCHK(hipEventRecord(ev3[2], x.st));
+    launch_knn_rerank(dev_x, n, p, cd, ci, 2 * L, k, dev_idx, dev_dist, x.st);
+    if (ev3) HIPCHK(hipEventRecord(ev3[3], x.st));
+    x.check_launch();
+}
+
+// mean/std of the HVGs from per-gene stats (pp.py:123-126, tl.py:31-34).
+__global__ void k_mean_std(const double *stats, const int32_t *gene_of_hvg, int d, double n, double *mean,
+                           double *sd) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= d) return;
+    const int g = gene_of_hvg[h];
+    const double m = stats[(int64_t)g * 5] / n;
+    double var = 0.0;
+    if (n > 1.0) {
+        const double nm = n * m;
+        const double nmm = nm * m;
+        var = (stats[(int64_t)g * 5 + 1] - nmm) / (n - 1.0);
+    }
+    var = var > 0.0 ? var : 0.0;
+    double s = sqrt(var);
+    if (s == 0.0) s = 1.0;
+    mean[h] = m;
+    sd[h] = s;
+}
+
+__global__ void k_iota_i64(int64_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = i;
+}
+__global__ void k_iota_i32(int32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (int32_t)i;
+}
+
+}  // namespace crest
+
+using namespace crest;
+
+// ================================================================ C ABI
+extern "C" {
+
+int crest_set_device(int32_t device) {
+    return guarded([&] {
+        if (device < 0 || device >= 64) value_error("device index out of range");
+        g_device = device;
+    });
+}
+
+int crest_set_precision(int32_t prec) {
+    return guarded([&] {
+        if (prec != CREST_PREC_FP64 && prec != CREST_PREC_FP32 && prec != CREST_PREC_BF16X3)
+            value_error("unknown precision");
+        g_prec = prec;
+    });
+}
+
+int32_t crest_get_precision(void) { return g_prec; }
+
+const char *crest_last_error(void) { return g_err.c_str(); }
+
+int32_t crest_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int crest_release(void) {
+    return guarded([&] {
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.sync();
+        x.release();
+    });
+}
+
+int crest_gene_stats(const crest_chunk *chunk, double *out, size_t out_len) {
+    return guarded([&] {
+        HostChunk h = check_chunk(chunk);
+        if ((int64_t)out_len != h.n_vars * 5 || (out_len && !out)) value_error("out must be (n_vars, 5)");
+        if (h.n_vars == 0 || h.n_spans == 0) return;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        DevChunk c = upload_chunk(x, h);
+        double *dev = x.get<double>("stats", out_len);
+        launch_fill_f64(dev, (int64_t)out_len, 0.0, x.st);
+        launch_gene_stats(c, dev, (int)h.n_vars, x.st);
+        x.check_launch();
+        std::vector<double> tmp(out_len);
+        x.download(tmp.data(), dev, out_len);
+        x.sync();
+        for (size_t i = 0; i < out_len; ++i) out[i] += tmp[i];
+    });
+}
+
+int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub_map, double *gram,
+                          size_t gram_len, double *colsum, size_t d, int32_t scale, const double *mean, size_t n_mean,
+                          const double *std_, size_t n_std, double max_value) {
+    return guarded([&] {
+        HostChunk h = check_chunk(chunk);
+        if (gram_len != d * d || (d && (!gram || !colsum))) value_error("gram must be (d, d)");
+        check_sub(sub_map, n_sub_map, h.n_vars, d);
+        check_scale(scale, mean, n_mean, std_, n_std, d);
+        if (d == 0 || h.n_spans == 0) return;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const int prec = g_prec;
+        const int d_pad = (int)round_up((int64_t)d, kGramBlock);
+        DevChunk c = upload_chunk(x, h);
+        DevScale sc{};
+        sc.sub = x.upload("sub_map", sub_map, n_sub_map);
+        sc.scale = scale ? 1 : 0;
+        sc.max_value = max_value;
+        sc.d = (int)d;
+        if (scale) {
+            sc.mean = x.upload("mean", mean, d);
+            sc.sd = x.upload("std", std_, d);
+        }
+        double *dg = x.get<double>("gram", (size_t)d_pad * d_pad);
+        double *dc = x.get<double>("colsum", d_pad);
+        launch_fill_f64(dg, (int64_t)d_pad * d_pad, 0.0, x.st);
+        launch_fill_f64(dc, d_pad, 0.0, x.st);
+        gram_rows(x, c, sc, prec, d_pad, 65536, dg, dc);
+        std::vector<double> hg((size_t)d_pad * d_pad), hc(d);
+        x.download(hg.data(), dg, hg.size());
+        x.download(hc.data(), dc, d);
+        x.sync();
+        for (size_t a = 0; a < d; ++a) {
+            colsum[a] += hc[a];
+            for (size_t b = a; b < d; ++b) gram[a * d + b] += hg[a * d_pad + b];
+        }
+    });
+}
+
+int crest_pca_from_gram(const double *gram, size_t gram_len, const double *colsum, size_t d, size_t n, size_t k,
+                        double *variance, double *loadings, double *total, size_t *k_out) {
+    return guarded([&] {
+        if (gram_len != d * d) value_error("gram must be (d, d)");
+        if (n < 2 || d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
+        const size_t kk = std::min(k, d);
+        if (k_out) *k_out = kk;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const double *dg = x.upload("pca_gram", gram, d * d);
+        const double *dc = x.upload("pca_colsum", colsum, d);
+        double *dv = x.get<double>("pca_var", std::max<size_t>(kk, 1));
+        double *dl = x.get<double>("pca_load", std::max<size_t>(d * kk, 1));
+        double *dt = x.get<double>("pca_total", 1);
+        pca_device(x, dg, (int)d, dc, (int)d, (double)n, (int)kk, dv, dl, dt);
+        x.download(variance, dv, kk);
+        x.download(loadings, dl, d * kk);
+        x.download(total, dt, 1);
+        x.sync();
+    });
+}
+
+int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub_map, const double *loadings, size_t d,
+                  size_t k, float *out, size_t out_len, int32_t scale, const double *mean, size_t n_mean,
+                  const double *std_, size_t n_std, double max_value) {
+    return guarded([&] {
+        HostChunk h = check_chunk(chunk);
+        check_sub(sub_map, n_sub_map, h.n_vars, d);
+        if ((int64_t)out_len < h.n_rows * (int64_t)k) value_error("out must be (n_rows, k)");
+        check_scale(scale, mean, n_mean, std_, n_std, d);
+        if (k > 256) value_error("k > 256 is not supported by the GPU projection");
+        if (k == 0 || h.n_spans == 0) return;
+        // rows this chunk writes
+        int64_t rmin = INT64_MAX, rmax = -1;
+        for (int64_t i = 0; i < h.n_spans; ++i) {
+            const int64_t r = host_row(h, i);
+            if (r >= 0) {
+                rmin = std::min(rmin, r);
+                rmax = std::max(rmax, r);
+            }
+        }
+        if (rmax < 0) return;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        DevChunk c = upload_chunk(x, h);
+        DevScale sc{};
+        sc.sub = x.upload("sub_map", sub_map, n_sub_map);
+        sc.scale = scale ? 1 : 0;
+        sc.max_value = max_value;
+        sc.d = (int)d;
+        if (scale) {
+            sc.mean = x.upload("mean", mean, d);
+            sc.sd = x.upload("std", std_, d);
+        }
+        const double *V = x.upload("loadings", loadings, d * k);
+        const size_t nrow = (size_t)(rmax - rmin + 1);
+        float *dout = x.upload("proj_out", out + rmin * (int64_t)k, nrow * k);
+        launch_project(c, sc, V, (int)k, dout, rmin, x.st);
+        x.check_launch();
+        x.download(out + rmin * (int64_t)k, dout, nrow * k);
+        x.sync();
+    });
+}
+
+int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact, uint64_t seed,
+                    uint32_t *indices, float *distances, size_t *k_out) {
+    (void)exact;
+    (void)seed;
+    return guarded([&] {
+        // knn.rs:338-345
+        size_t kk = 0;
+        if (n > 1) kk = std::max<size_t>(std::min(k, n - 1), 1);
+        if (k_out) *k_out = kk;
+        if (n == 0 || kk == 0) return;
+        if (dim == 0) value_error("data must have at least one column");
+        if (n >= 0xFFFFFFFFull) value_error("too many points");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const float *dx = x.upload("knn_x", data, n * dim);
+        uint32_t *di = x.get<uint32_t>("knn_idx", n * kk);
+        float *dd = x.get<float>("knn_dist", n * kk);
+        knn_device(x, dx, (int64_t)n, (int)dim, (int)kk, g_prec, di, dd, nullptr);
+        x.download(indices, di, n * kk);
+        x.download(distances, dd, n * kk);
+        x.sync();
+    });
+}
+
+int crest_materialize(const crest_chunk *chunk, uint32_t *rows, uint32_t *vars, float *values, size_t cap,
+                      size_t *n_out) {
+    return guarded([&] {
+        HostChunk h = check_chunk(chunk);
+        if (n_out) *n_out = 0;
+        if (h.n_spans == 0) return;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        DevChunk c = upload_chunk(x, h);
+        int64_t *kept = x.get<int64_t>("mat_kept", h.n_spans);
+        launch_materialize_count(c, kept, x.st);
+        x.check_launch();
+        std::vector<int64_t> hk(h.n_spans), off(h.n_spans);
+        x.download(hk.data(), kept, hk.size());
+        x.sync();
+        int64_t tot = 0;
+        for (int64_t i = 0; i < h.n_spans; ++i) {
+            off[i] = tot;
+            tot += hk[i];
+        }
+        if ((size_t)tot > cap) value_error("output capacity too small");
+        if (n_out) *n_out = (size_t)tot;
+        if (tot == 0) return;
+        const int64_t *doff = x.upload("mat_off", off.data(), off.size());
+        uint32_t *dr = x.get<uint32_t>("mat_rows", tot);
+        uint32_t *dv = x.get<uint32_t>("mat_vars", tot);
+        float *dx = x.get<float>("mat_vals", tot);
+        launch_materialize(c, doff, dr, dv, dx, x.st);
+        x.check_launch();
+        x.download(rows, dr, tot);
+        x.download(vars, dv, tot);
+        x.download(values, dx, tot);
+        x.sync();
+    });
+}
+
+int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *loadings, crest_pipeline_result *res) {
+    return guarded([&] {
+        if (!a || !res || !variance || !loadings) value_error("NULL argument");
+        const int64_t n = (int64_t)a->n_cells;
+        const int d = (int)a->d;
+        if (n < 2 || d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
+        if (!a->dev_indptr || !a->dev_genes || !a->dev_values || !a->hvg_of_gene || !a->dev_x_pca ||
+            !a->dev_knn_idx || !a->dev_knn_dist)
+            value_error("NULL argument");
+        const int kp = (int)std::min<int64_t>({(int64_t)a->n_comps, (int64_t)d, n - 1});
+        if (kp > 256) value_error("n_comps > 256 is not supported");
+        const int kn = (int)std::max<int64_t>(std::min<int64_t>((int64_t)a->k, n - 1), 1);
+        std::vector<int32_t> gene_of_hvg(d, -1);
+        for (size_t g = 0; g < a->n_genes; ++g) {
+            const int32_t h = a->hvg_of_gene[g];
+            if (h >= d) value_error("hvg_of_gene values must be < d");
+            if (h >= 0) gene_of_hvg[h] = (int32_t)g;
+        }
+        for (int h = 0; h < d; ++h)
+            if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const int d_pad = (int)round_up(d, kGramBlock);
+        const int64_t tile_rows = a->chunk_rows ? (int64_t)round_up((int64_t)a->chunk_rows, 16) : 65536;
+        // constant maps: identity cell map and gene map (every cell and gene kept)
+        int64_t *cmap = x.get<int64_t>("pl_cmap", n);
+        int32_t *gmap = x.get<int32_t>("pl_gmap", a->n_genes);
+        hipLaunchKernelGGL(k_iota_i64, dim3(1024), dim3(256), 0, x.st, cmap, n);
+        hipLaunchKernelGGL(k_iota_i32, dim3(256), dim3(256), 0, x.st, gmap, (int64_t)a->n_genes);
+        const int32_t *sub = x.upload("pl_sub", a->hvg_of_gene, a->n_genes);
+        const int32_t *goh = x.upload("pl_goh", gene_of_hvg.data(), d);
+        double *stats = x.get<double>("pl_stats", a->n_genes * 5);
+        double *mean = x.get<double>("pl_mean", d);
+        double *sd = x.get<double>("pl_sd", d);
+        double *dg = x.get<double>("pl_gram", (size_t)d_pad * d_pad);
+        double *dc = x.get<double>("pl_colsum", d_pad);
+        double *dv = x.get<double>("pl_var", kp);
+        double *dl = x.get<double>("pl_load", (size_t)d * kp);
+        double *dt = x.get<double>("pl_total", 1);
+        float *shift = x.get<float>("pl_shift", kp);
+        x.sync();
+
+        DevChunk c{};
+        c.span_off = a->dev_indptr;
+        c.span_cell = nullptr;
+        c.first_cell = 0;
+        c.n_spans = n;
+        c.genes = a->dev_genes;
+        c.values = a->dev_values;
+        c.cell_map = cmap;
+        c.n_cell_map = n;
+        c.gene_var = gmap;
+        c.n_gene_map = (int64_t)a->n_genes;
+        c.target_sum = a->target_sum;
+        c.log1p = a->log1p ? 1 : 0;
+        c.base = 0;
+        DevScale sc{};
+        sc.sub = sub;
+        sc.scale = a->scale ? 1 : 0;
+        sc.max_value = a->max_value;
+        sc.d = d;
+        sc.mean = mean;
+        sc.sd = sd;
+
+        hipEvent_t *ev = x.ev;
+        HIPCHK(hipEventRecord(ev[0], x.st));
+        // PASS 0: gene statistics -> mean/std (tl.py:24-34)
+        if (a->scale) {
+            launch_fill_f64(stats, (int64_t)a->n_genes * 5, 0.0, x.st);
+            launch_gene_stats(c, stats, (int)a->n_genes, x.st);
+            hipLaunchKernelGGL(k_mean_std, dim3((d + 255) / 256), dim3(256), 0, x.st, stats, goh, d, (double)n, mean, sd);
+        }
+        HIPCHK(hipEventRecord(ev[1], x.st));
+        // PASS 1: Gram + colsum (tl.py:68-71)
+        launch_fill_f64(dg, (int64_t)d_pad * d_pad, 0.0, x.st);
+        launch_fill_f64(dc, d_pad, 0.0, x.st);
+        gram_rows(x, c, sc, a->prec_gram, d_pad, tile_rows, dg, dc);
+        HIPCHK(hipEventRecord(ev[2], x.st));
+        // eigensolve (tl.py:72-73)
+        pca_device(x, dg, d_pad, dc, d, (double)n, kp, dv, dl, dt);
+        HIPCHK(hipEventRecord(ev[3], x.st));
+        // PASS 2: shift prefill + projection (tl.py:76-80)
+        launch_shift_fill(dc, dl, d, kp, (double)n, n, a->dev_x_pca, shift, x.st);
+        launch_project(c, sc, dl, kp, a->dev_x_pca, 0, x.st);
+        x.check_launch();
+        HIPCHK(hipEventRecord(ev[4], x.st));
+        // kNN (pp.py:309)
+        knn_device(x, a->dev_x_pca, n, kp, kn, a->prec_knn, a->dev_knn_idx, a->dev_knn_dist, ev + 5);
+        x.sync();
+        float ms[8] = {};
+        for (int i = 0; i < 8; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        res->k_pcs = (size_t)kp;
+        res->k_nn = (size_t)kn;
+        res->ms_stats = ms[0];
+        res->ms_gram = ms[1];
+        res->ms_eig = ms[2];
+        res->ms_project = ms[3];
+        res->ms_knn_prep = ms[5];
+        res->ms_knn_screen = ms[6];
+        res->ms_knn_rerank = ms[7];
+        float tot = 0.f;
+        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[8]));
+        res->ms_total = tot;
+        x.download(variance, dv, kp);
+        std::vector<double> hl((size_t)d * kp);
+        x.download(hl.data(), dl, hl.size());
+        x.download(&res->total_variance, dt, 1);
+        x.sync();
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < kp; ++j) loadings[(size_t)i * a->n_comps + j] = hl[(size_t)i * kp + j];
+    });
+}
+
+}  // extern "C"

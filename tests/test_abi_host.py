@@ -1,0 +1,167 @@
+"""CPU checks of the boundary and the host logic: the C ABI library loads and exports
+every function include/crest_hip.h declares; argument validation raises the
+reference's exception classes before any device work; the chunk protocol and the
+lazy-transform record behave as crest/core.py and crest/pp.py do."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import crest_amd as ca
+from crest_amd import _lib, _native as N
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    text = (ROOT / "include" / "crest_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(crest_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib._SIGS), "ctypes signatures cover exactly the header"
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if _lib.lib().crest_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        N.knn_graph(np.zeros((4, 2), np.float32), 2)
+
+
+def _ctx(nnz=5):
+    genes = np.array([0, 1, 0, 1, 2], np.uint32)[:nnz]
+    vals = np.array([2, 6, 5, 1, 3], np.float32)[:nnz]
+    return genes, vals, np.array([0, 1, 2], np.int64), np.array([0, 1, 2], np.int32)
+
+
+def test_validation_errors_match_py_rs():
+    g, v, cm, gm = _ctx()
+    out = np.zeros((3, 5))
+    ip = np.array([0, 2, 3, 5], np.int64)
+    with pytest.raises(ValueError, match="pass exactly one of cells / indptr"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, None, None, 0, out)
+    with pytest.raises(ValueError, match="indptr must be non-decreasing"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, None, np.array([0, 3, 2, 5], np.int64), 0, out)
+    with pytest.raises(ValueError, match="indptr must be non-decreasing"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, None, np.array([0, 2, 4], np.int64), 0, out)
+    with pytest.raises(ValueError, match="chunk must be sorted by cell"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, np.array([0, 1, 0, 1, 2], np.uint32), None, 0, out)
+    with pytest.raises(ValueError, match="equal length"):
+        N.gene_stats(g, v[:4], cm, gm, 0.0, False, None, ip, 0, out)
+    with pytest.raises(ValueError, match=r"out must be \(n_vars, 5\)"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, None, ip, 0, np.zeros((2, 5)))
+    with pytest.raises(ValueError, match="C-contiguous"):
+        N.gene_stats(g, v, cm, gm, 0.0, False, None, ip, 0, np.zeros((5, 3)).T)
+    with pytest.raises(TypeError):
+        N.gene_stats(g.astype(np.int64), v, cm, gm, 0.0, False, None, ip, 0, out)
+    gram, cs = np.zeros((2, 2)), np.zeros(2)
+    with pytest.raises(ValueError, match="sub_map must have length n_vars"):
+        N.gram_accumulate(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1], np.int32), gram, cs,
+                          False, None, None, 0.0)
+    with pytest.raises(ValueError, match="sub_map must have length n_vars"):
+        N.gram_accumulate(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, 2], np.int32), gram, cs,
+                          False, None, None, 0.0)
+    with pytest.raises(ValueError, match=r"gram must be \(d, d\)"):
+        N.gram_accumulate(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, -1], np.int32),
+                          np.zeros((2, 3)), cs, False, None, None, 0.0)
+    with pytest.raises(ValueError, match="scale=True needs mean and std"):
+        N.gram_accumulate(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, -1], np.int32), gram, cs,
+                          True, None, None, 0.0)
+    with pytest.raises(ValueError, match="mean/std must have length d"):
+        N.gram_accumulate(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, -1], np.int32), gram, cs,
+                          True, np.zeros(3), np.ones(3), 0.0)
+    with pytest.raises(ValueError, match=r"out must be \(n_rows, k\)"):
+        N.project(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, -1], np.int32), np.zeros((2, 4)),
+                  np.zeros((2, 4), np.float32), False, None, None, 0.0)
+    with pytest.raises(ValueError, match=r"loadings must be \(d, k\)"):
+        N.project(g, v, cm, gm, 0.0, False, None, ip, 0, np.array([0, 1, -1], np.int32), np.zeros(2),
+                  np.zeros((3, 4), np.float32), False, None, None, 0.0)
+    with pytest.raises(RuntimeError, match="need at least 2 cells"):
+        N.pca_from_gram(np.zeros((2, 2)), np.zeros(2), 1, 2)
+    with pytest.raises(ValueError, match=r"gram must be \(d, d\)"):
+        N.pca_from_gram(np.zeros((2, 3)), np.zeros(2), 5, 2)
+
+
+def test_knn_trivial_sizes_need_no_device():
+    i, d = N.knn_graph(np.zeros((0, 3), np.float32), 5)
+    assert i.shape == (0, 0)
+    i, d = N.knn_graph(np.zeros((1, 3), np.float32), 5)
+    assert i.shape == (1, 0) and i.dtype == np.uint32 and d.dtype == np.float32
+
+
+def test_precision_switch():
+    assert ca.get_precision() == "fp64"
+    ca.set_precision("bf16x3")
+    assert ca.get_precision() == "bf16x3"
+    ca.set_precision("fp64")
+    with pytest.raises(KeyError):
+        ca.set_precision("int4")
+
+
+def _csr(n=50, g=30, seed=0):
+    rng = np.random.default_rng(seed)
+    D = (rng.random((n, g)) < 0.3) * rng.integers(1, 9, (n, g))
+    D[7] = 0  # an empty cell
+    indptr = np.r_[0, np.cumsum((D != 0).sum(1))].astype(np.int64)
+    r, c = np.nonzero(D)
+    return indptr, c.astype(np.uint32), D[r, c].astype(np.float32), D
+
+
+def test_csr_chunks_cover_whole_cells():
+    indptr, idx, val, D = _csr()
+    st = ca.CSRStore(indptr, idx, val, D.shape[1])
+    seen = []
+    for genes, values, cells, ip, first in st.chunks(40):
+        assert cells is None and ip[0] == indptr[first] and len(genes) == ip[-1] - ip[0]
+        assert len(genes) <= 40 or len(ip) == 2
+        seen.extend(range(first, first + len(ip) - 1))
+    assert seen == list(range(D.shape[0]))
+
+
+def test_triplet_chunks_keep_cells_whole():
+    indptr, idx, val, D = _csr()
+    cells = np.repeat(np.arange(D.shape[0]), np.diff(indptr)).astype(np.uint32)
+    st = ca.TripletStore(cells[::-1], idx[::-1], val[::-1], D.shape[0], D.shape[1])
+    tot = 0
+    last = -1
+    for genes, values, c, ip, first in st.chunks(33):
+        assert ip is None and (np.diff(c.astype(np.int64)) >= 0).all() and c[0] > last
+        last = c[-1]
+        tot += len(genes)
+    assert tot == len(idx)
+
+
+def test_iter_ctx_and_ops_record():
+    indptr, idx, val, D = _csr()
+    bf = ca.BioFrame.from_csr(indptr, idx, val, D.shape[1])
+    bf = bf.filter_cells(np.arange(bf.n_obs) % 3 != 0)
+    ca.pp.normalize_total(bf, 1e4)
+    ca.pp.log1p(bf)
+    ca.pp.scale(bf, None)
+    assert bf.uns["scale"] == {"max_value": 0.0}
+    ctx = next(bf.iter_ctx())
+    assert len(ctx) == 9 and ctx[4] == 1e4 and ctx[5] is True
+    assert ctx[2].dtype == np.int64 and (ctx[2] == -1).sum() == (np.arange(50) % 3 == 0).sum()
+    with pytest.raises(ValueError):
+        ca.pp.normalize_total(bf, 1e4)  # after log1p
+    with pytest.raises(ValueError):
+        ca.pp.normalize_total(bf, 0)
+    assert next(bf.iter_ctx(transform=False))[4:6] == (0.0, False)
+
+
+def test_pca_argument_errors():
+    indptr, idx, val, D = _csr()
+    bf = ca.BioFrame.from_csr(indptr, idx, val, D.shape[1])
+    bf.var["highly_variable"] = np.arange(D.shape[1]) == 0
+    with pytest.raises(ValueError, match="at least 2 genes"):
+        ca.tl.pca(bf)
+    with pytest.raises(KeyError):
+        ca.pp.neighbors(bf)

@@ -1,0 +1,228 @@
+"""GPU parity: libcrest_hip.so (through the C ABI via crest_amd._native) against the
+oracle's restatement of the reference on the same seeded inputs.
+
+Bars: the lazy transform and the projection are bit-exact; FP64 sums whose order
+differs (gene stats, FP64 Gram) agree to 1e-12 relative; FP32 / split-BF16 Grams to
+the stated relative tolerance; eigenvalues to 1e-9 relative; kNN index sets equal
+except documented distance ties.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import crest_amd as ca  # noqa: E402
+from crest_amd import _native as N  # noqa: E402
+from crest_amd import synth  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def counts():
+    indptr, indices, data, _ = synth.make_counts(1500, 900, 6, density=0.07, seed=3)
+    return indptr, indices, data, 900
+
+
+def _frame(counts, chunk_nnz=None, drop=True):
+    indptr, indices, data, g = counts
+    bf = ca.BioFrame.from_csr(indptr, indices, data, g)
+    if chunk_nnz:
+        bf.chunk_nnz = chunk_nnz
+    if drop:  # filtered cells and genes exercise the maps
+        rng = np.random.default_rng(0)
+        bf = bf.filter_cells(rng.random(bf.n_obs) > 0.05).filter_genes(rng.random(bf.n_vars) > 0.1)
+    ca.pp.normalize_total(bf, 1e4)
+    ca.pp.log1p(bf)
+    return bf
+
+
+def test_materialize_hand_values():
+    """kernels.rs:564-578: normalised values 25/75/100, CSR == COO."""
+    cells = np.array([0, 0, 1, 2, 2], np.uint32)
+    genes = np.array([0, 1, 0, 1, 2], np.uint32)
+    vals = np.array([2, 6, 5, 1, 3], np.float32)
+    cm = np.array([0, -1, 1], np.int64)
+    gm = np.array([0, 1, -1], np.int32)
+    r, g, x = N.materialize(genes, vals, cm, gm, 100.0, False, cells, None, 0)
+    assert r.tolist() == [0, 0, 1] and g.tolist() == [0, 1, 1]
+    assert x.tolist() == [25.0, 75.0, 100.0]
+    indptr = np.array([0, 2, 3, 5], np.int64)
+    r2, g2, x2 = N.materialize(genes, vals, cm, gm, 100.0, False, None, indptr, 0)
+    assert r2.tolist() == r.tolist() and g2.tolist() == g.tolist() and x2.tolist() == x.tolist()
+
+
+@pytest.mark.parametrize("chunk_nnz", [None, 3000])
+def test_materialize_bit_exact(counts, oracle, chunk_nnz):
+    bf = _frame(counts, chunk_nnz)
+    for ctx in bf.iter_ctx():
+        got = N.materialize(*ctx)
+        ref = oracle.materialize(*ctx)
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_gene_stats(counts, oracle):
+    bf = _frame(counts, 4000)
+    got = np.zeros((bf.n_vars, 5))
+    ref = np.zeros((bf.n_vars, 5))
+    for ctx in bf.iter_ctx():
+        N.gene_stats(*ctx, got)
+        oracle.gene_stats(*ctx, ref)
+    np.testing.assert_array_equal(got[:, 4], ref[:, 4])
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def _gram_inputs(bf, oracle, d_top=300):
+    st = np.zeros((bf.n_vars, 5))
+    for ctx in bf.iter_ctx():
+        oracle.gene_stats(*ctx, st)
+    n = bf.n_obs
+    mean = st[:, 0] / n
+    var = np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0)
+    sd = np.sqrt(var)
+    sd[sd == 0] = 1
+    mask = np.zeros(bf.n_vars, bool)
+    mask[np.argsort(-var)[:d_top]] = True
+    d = int(mask.sum())
+    gmap = np.full(bf.n_vars, -1, np.int32)
+    gmap[mask] = np.arange(d, dtype=np.int32)
+    return gmap, np.ascontiguousarray(mean[mask]), np.ascontiguousarray(sd[mask]), d
+
+
+@pytest.mark.parametrize("prec,rtol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-5)])
+def test_gram_accumulate(counts, oracle, prec, rtol):
+    bf = _frame(counts, 5000)
+    gmap, mean, sd, d = _gram_inputs(bf, oracle)
+    ca.set_precision(prec)
+    try:
+        g_got, c_got = np.zeros((d, d)), np.zeros(d)
+        g_ref, c_ref = np.zeros((d, d)), np.zeros(d)
+        for ctx in bf.iter_ctx():
+            N.gram_accumulate(*ctx, gmap, g_got, c_got, True, mean, sd, 10.0)
+            oracle.gram_accumulate(*ctx, gmap, g_ref, c_ref, True, mean, sd, 10.0)
+    finally:
+        ca.set_precision("fp64")
+    iu = np.triu_indices(d)
+    scale = np.abs(g_ref[iu]).max()
+    assert np.abs(g_got[iu] - g_ref[iu]).max() <= rtol * scale
+    assert np.all(np.tril(g_got, -1) == 0)  # lower triangle untouched
+    np.testing.assert_allclose(c_got, c_ref, rtol=1e-12, atol=1e-9)
+
+
+def test_pca_from_gram(counts, oracle):
+    bf = _frame(counts, 5000)
+    gmap, mean, sd, d = _gram_inputs(bf, oracle, 200)
+    g, c = np.zeros((d, d)), np.zeros(d)
+    for ctx in bf.iter_ctx():
+        oracle.gram_accumulate(*ctx, gmap, g, c, True, mean, sd, 10.0)
+    var_g, load_g, tot_g = N.pca_from_gram(g, c, bf.n_obs, 20)
+    var_o, load_o, tot_o = oracle.pca_from_gram(g, c, bf.n_obs, 20)
+    np.testing.assert_allclose(var_g, var_o, rtol=1e-9)
+    assert abs(tot_g - tot_o) <= 1e-12 * abs(tot_o)
+    # per-component vectors where the eigenvalue gap allows: same sign rule
+    gaps = np.minimum(np.abs(np.diff(np.r_[np.inf, var_o])), np.abs(np.diff(np.r_[var_o, 0.0])))
+    ok = gaps > 1e-3 * var_o[0]
+    for c_ in np.nonzero(ok)[0]:
+        assert np.abs(load_g[:, c_] - load_o[:, c_]).max() < 1e-6
+    assert oracle.principal_angle(load_g, load_o) < 1e-6
+
+
+def test_pca_from_gram_errors():
+    with pytest.raises(RuntimeError):
+        N.pca_from_gram(np.zeros((3, 3)), np.zeros(3), 1, 2)
+    with pytest.raises(ValueError):
+        N.pca_from_gram(np.zeros((3, 2)), np.zeros(3), 10, 2)
+
+
+def test_project_bit_exact(counts, oracle):
+    bf = _frame(counts, 3000)
+    gmap, mean, sd, d = _gram_inputs(bf, oracle, 150)
+    rng = np.random.default_rng(1)
+    V = rng.normal(size=(d, 17))
+    out_g = np.full((bf.n_obs, 17), 0.25, np.float32)
+    out_o = out_g.copy()
+    for ctx in bf.iter_ctx():
+        N.project(*ctx, gmap, V, out_g, True, mean, sd, 10.0)
+        oracle.project(*ctx, gmap, V, out_o, True, mean, sd, 10.0)
+    np.testing.assert_array_equal(out_g, out_o)
+
+
+def _clustered(n, d, seed):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-10, 10, (20, d))
+    return (centers[np.arange(n) % 20] + rng.uniform(-1.5, 1.5, (n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("n,d,k", [(600, 7, 10), (5000, 50, 15), (2049, 30, 32), (300, 3, 1)])
+def test_knn_graph(oracle, prec, n, d, k):
+    x = _clustered(n, d, n + d)
+    ca.set_precision(prec)
+    try:
+        idx, dist = N.knn_graph(x, k)
+    finally:
+        ca.set_precision("fp64")
+    assert idx.shape == (n, k) and dist.shape == (n, k)
+    assert not (idx == np.arange(n)[:, None]).any()
+    assert (np.diff(dist, axis=1) >= 0).all()
+    ref_i, ref_d2 = oracle.knn_exact_f64(x, k)
+    bad, ties = oracle.knn_mismatch_ties(idx, x, np.arange(n), ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    # distances are the reference's f32 formula on the reported pairs
+    pd = oracle.pair_dist(x, np.repeat(np.arange(n), k), idx.ravel()).reshape(n, k)
+    np.testing.assert_array_equal(dist, pd)
+
+
+def test_knn_small_edge_cases():
+    idx, dist = N.knn_graph(np.zeros((1, 4), np.float32), 5)
+    assert idx.shape == (1, 0)
+    idx, dist = N.knn_graph(np.array([[0, 0], [1, 0], [3, 0]], np.float32), 10)
+    assert idx.tolist() == [[1, 2], [0, 2], [1, 0]]
+    np.testing.assert_array_equal(dist, [[1, 3], [1, 2], [2, 3]])
+
+
+def test_scaled_pca_matches_dense(counts, oracle):
+    """tests/test_crest.py:88-107: implicit scale + Gram PCA == dense scale + SVD."""
+    indptr, indices, data, g = counts
+    bf = ca.BioFrame.from_csr(indptr, indices, data, g)
+    ca.pp.normalize_total(bf, 1e4)
+    ca.pp.log1p(bf)
+    ca.pp.scale(bf, max_value=2.0)
+    ca.tl.pca(bf, n_comps=10, use_highly_variable=False)
+    D = bf.to_dense().astype(np.float64)
+    mu, sd = D.mean(0), D.std(0, ddof=1)
+    sd[sd == 0] = 1
+    Z = np.clip((D - mu) / sd, -2, 2)
+    Z -= Z.mean(0)
+    U, S, _ = np.linalg.svd(Z, full_matrices=False)
+    ref = (S[:10] ** 2) / (len(Z) - 1)
+    np.testing.assert_allclose(bf.uns["pca"]["variance"], ref, rtol=1e-6)
+    ref_scores = U[:, :10] * S[:10]
+    got = bf.obsm["X_pca"].astype(np.float64)
+    cos = np.abs((got * ref_scores).sum(0)) / np.linalg.norm(got, axis=0) / np.linalg.norm(ref_scores, axis=0)
+    assert cos.min() > 0.9999
+
+
+def test_chunking_invariance(counts):
+    """tests/test_crest.py:54-61."""
+    def pipe(chunk):
+        bf = _frame(counts, chunk, drop=False)
+        ca.pp.highly_variable_genes(bf, n_top_genes=300)
+        ca.pp.scale(bf, max_value=10)
+        ca.tl.pca(bf, n_comps=20)
+        return bf
+    a, b = pipe(None), pipe(1000)
+    assert (a.var["highly_variable"] == b.var["highly_variable"]).all()
+    np.testing.assert_allclose(np.abs(a.obsm["X_pca"]), np.abs(b.obsm["X_pca"]), rtol=1e-4, atol=1e-3)
+
+
+def test_hvg_matches_oracle(counts, oracle):
+    bf = _frame(counts, drop=False)
+    ca.pp.highly_variable_genes(bf, n_top_genes=300)
+    st = np.zeros((bf.n_vars, 5))
+    for ctx in bf.iter_ctx():
+        oracle.gene_stats(*ctx, st)
+    n = bf.n_obs
+    m = st[:, 2] / n
+    v = np.maximum((st[:, 3] - n * m * m) / (n - 1), 0)
+    hv, _ = oracle.hvg_seurat_pandas(m, v, n_top_genes=300)
+    np.testing.assert_array_equal(bf.var["highly_variable"], hv)

@@ -1,0 +1,73 @@
+"""The fused device pipeline (crest_pipeline_run) against the oracle and against the
+per-chunk operator path, in each precision."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import crest_amd as ca  # noqa: E402
+from crest_amd import synth  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def data():
+    indptr, indices, data, _ = synth.make_counts(20000, 600, 12, density=0.08, seed=5)
+    return indptr, indices, data, 600
+
+
+@pytest.fixture(scope="module")
+def reference(data, oracle):
+    """Oracle: streamed FP64 Gram PCA of all 600 genes, scaled, max_value 10."""
+    indptr, indices, vals, g = data
+    bf = ca.BioFrame.from_csr(indptr, indices, vals, g)
+    ca.pp.normalize_total(bf, 1e4)
+    ca.pp.log1p(bf)
+    st = np.zeros((g, 5))
+    for ctx in bf.iter_ctx():
+        oracle.gene_stats(*ctx, st)
+    n = bf.n_obs
+    mean = st[:, 0] / n
+    var = np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0)
+    sd = np.sqrt(var)
+    sd[sd == 0] = 1
+    gmap = np.arange(g, dtype=np.int32)
+    gram, colsum = np.zeros((g, g)), np.zeros(g)
+    for ctx in bf.iter_ctx():
+        oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=8)
+    var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, 30)
+    shift = (colsum / n) @ load_o
+    X = np.empty((n, 30), np.float32)
+    X[:] = -shift.astype(np.float32)
+    for ctx in bf.iter_ctx():
+        oracle.project(*ctx, gmap, load_o, X, True, mean, sd, 10.0)
+    return var_o, load_o, tot_o, X
+
+
+@pytest.mark.parametrize("prec_gram,prec_knn,var_rtol", [("fp64", "fp32", 1e-9), ("fp32", "fp32", 1e-5),
+                                                         ("bf16x3", "bf16x3", 1e-4)])
+def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol):
+    from crest_amd.pipeline import DeviceCSR, Pipeline
+    indptr, indices, vals, g = data
+    csr = DeviceCSR.upload(indptr, indices, vals, g)
+    p = Pipeline(csr, np.arange(g, dtype=np.int32), n_comps=30, k=15, prec_gram=prec_gram, prec_knn=prec_knn,
+                 chunk_rows=4096)
+    out = p.run()
+    var_o, load_o, tot_o, X_o = reference
+    np.testing.assert_allclose(out.variance, var_o, rtol=var_rtol)
+    assert abs(out.total_variance - tot_o) <= 1e-9 * tot_o
+    lead = 10  # well separated leading components
+    assert oracle.principal_angle(out.loadings[:, :lead], load_o[:, :lead]) < (1e-7 if prec_gram == "fp64" else 1e-3)
+    X = out.x_pca.cpu().numpy()
+    if prec_gram == "fp64":
+        # same loadings up to eigen-solver rounding: scores agree closely
+        np.testing.assert_allclose(np.abs(X[:, :lead]), np.abs(X_o[:, :lead]), rtol=1e-4, atol=1e-3)
+    idx = out.knn_idx.cpu().numpy().view(np.uint32)
+    dist = out.knn_dist.cpu().numpy()
+    q = np.random.default_rng(0).choice(len(X), 400, replace=False)
+    ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
+    bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    pd = oracle.pair_dist(X, np.repeat(q, 15), idx[q].ravel()).reshape(len(q), 15)
+    np.testing.assert_array_equal(dist[q], pd)
+    for k_, v in out.times_ms.items():
+        assert v >= 0.0, k_

@@ -1,0 +1,276 @@
+"""Pins of the oracle (CPU): the reference's own known answers and tests restated,
+closed forms, dense FP64 references, brute force, invariances, and checks that
+plausible mistakes would fail these pins."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+
+def make_counts(n_cells=400, n_genes=120, n_clusters=4, seed=0):
+    """tests/test_crest.py:13-27 generator (NB counts with cluster programs)."""
+    rng = np.random.default_rng(seed)
+    base = rng.gamma(0.4, 1.0, n_genes)
+    labels = rng.integers(0, n_clusters, n_cells)
+    prog = np.ones((n_clusters, n_genes))
+    for c in range(n_clusters):
+        up = rng.choice(n_genes, 12, replace=False)
+        prog[c, up] *= rng.uniform(3, 10, 12)
+    lib = rng.lognormal(7.5, 0.4, n_cells)
+    mu = prog[labels] * base[None, :]
+    mu = mu / mu.sum(1, keepdims=True) * lib[:, None]
+    counts = rng.negative_binomial(2, 2 / (2 + mu)).astype(np.float32)
+    return sp.csr_matrix(counts)
+
+
+@pytest.fixture(scope="module")
+def X():
+    return make_counts()
+
+
+def ctx_of(X, target=1e4, log=True, cell_map=None, gene_map=None, rows=None):
+    X = X.tocsr()
+    X.sort_indices()
+    n, g = X.shape
+    lo, hi = (0, n) if rows is None else rows
+    ip = X.indptr.astype(np.int64)
+    cm = np.arange(n, dtype=np.int64) if cell_map is None else cell_map
+    gm = np.arange(g, dtype=np.int32) if gene_map is None else gene_map
+    a, b = ip[lo], ip[hi]
+    return (X.indices[a:b].astype(np.uint32), X.data[a:b].astype(np.float32), cm, gm, target, log, None,
+            ip[lo:hi + 1].copy(), lo)
+
+
+def test_materialize_hand_values(oracle):
+    """kernels.rs:564-578."""
+    cells = np.array([0, 0, 1, 2, 2], np.uint32)
+    genes = np.array([0, 1, 0, 1, 2], np.uint32)
+    vals = np.array([2, 6, 5, 1, 3], np.float32)
+    cm = np.array([0, -1, 1], np.int64)
+    gm = np.array([0, 1, -1], np.int32)
+    r, g, x = oracle.materialize(genes, vals, cm, gm, 100.0, False, cells, None, 0)
+    assert r.tolist() == [0, 0, 1] and g.tolist() == [0, 1, 1]
+    assert np.allclose(x, [25, 75, 100], atol=1e-5)
+    ip = np.array([0, 2, 3, 5], np.int64)
+    r2, g2, x2 = oracle.materialize(genes, vals, cm, gm, 100.0, False, None, ip, 0)
+    assert r2.tolist() == r.tolist() and g2.tolist() == g.tolist() and x2.tolist() == x.tolist()
+
+
+def test_lazy_normalisation_closed_form(oracle, X):
+    """tests/test_crest.py:77-85, and bit-exact against the float32 rounding chain
+    (f32)(x * (1e4/total)) -> (f32)log1p(double)."""
+    r, g, x = oracle.materialize(*ctx_of(X))
+    D = X.toarray().astype(np.float64)
+    ref = np.log1p(D / D.sum(1, keepdims=True) * 1e4)
+    np.testing.assert_allclose(x, ref[r, g], rtol=1e-5, atol=1e-5)
+    tot = D.sum(1)
+    y = (D[r, g] * (1e4 / tot[r])).astype(np.float32)
+    exact = np.log1p(y.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(x, exact)
+
+
+def test_log1p_modes_differ_by_at_most_one_ulp(oracle, X):
+    ctx = ctx_of(X)
+    _, _, a = oracle.materialize(*ctx)
+    oracle.set_log1p_mode(1)
+    try:
+        _, _, b = oracle.materialize(*ctx)
+    finally:
+        oracle.set_log1p_mode(0)
+    ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+
+
+def test_filters_and_maps(oracle, X):
+    n, g = X.shape
+    rng = np.random.default_rng(2)
+    keep_c = rng.random(n) > 0.2
+    keep_g = rng.random(g) > 0.3
+    cm = np.full(n, -1, np.int64)
+    cm[keep_c] = np.arange(keep_c.sum())
+    gm = np.full(g, -1, np.int32)
+    gm[keep_g] = np.arange(keep_g.sum())
+    r, v, x = oracle.materialize(*ctx_of(X, cell_map=cm, gene_map=gm))
+    D = X.toarray().astype(np.float64)[keep_c][:, keep_g]
+    ref = np.log1p(D / D.sum(1, keepdims=True) * 1e4)
+    np.testing.assert_allclose(x, ref[r, v], rtol=1e-5, atol=1e-5)
+    assert len(x) == np.count_nonzero(D)
+
+
+def lognorm(X):
+    """The reference's rounding chain: y = (f32)(x * (1e4 / total)), v = (f32)log1p(y)."""
+    D = X.toarray().astype(np.float64)
+    y = (D * (1e4 / D.sum(1, keepdims=True))).astype(np.float32)
+    return np.log1p(y.astype(np.float64)).astype(np.float32).astype(np.float64)
+
+
+def test_gene_stats_dense(oracle, X):
+    st = np.zeros((X.shape[1], 5))
+    oracle.gene_stats(*ctx_of(X), st)
+    L = lognorm(X)
+    E = np.expm1(L)
+    ref = np.stack([L.sum(0), (L * L).sum(0), E.sum(0), (E * E).sum(0), (L != 0).sum(0)], 1)
+    np.testing.assert_allclose(st, ref, rtol=1e-12, atol=1e-12)
+
+
+def _scaled(X, mv):
+    L = lognorm(X)
+    n = len(L)
+    mean = L.mean(0)
+    sd = np.sqrt(np.maximum(((L * L).sum(0) - n * mean * mean) / (n - 1), 0))
+    sd[sd == 0] = 1
+    return L, mean, sd
+
+
+def test_gram_matches_dense(oracle, X):
+    """G = S^T S with S = clip((x-mu)/sd) - clip(-mu/sd) on non-zeros (kernels.rs:197-219)."""
+    mv = 2.0
+    L, mean, sd = _scaled(X, mv)
+    d = X.shape[1]
+    gmap = np.arange(d, dtype=np.int32)
+    gram, colsum = np.zeros((d, d)), np.zeros(d)
+    oracle.gram_accumulate(*ctx_of(X), gmap, gram, colsum, True, mean, sd, mv)
+    Z = np.clip((L - mean) / sd, -mv, mv)
+    b = np.clip(-mean / sd, -mv, mv)
+    S = np.where(L != 0, Z - b, 0.0)
+    ref = S.T @ S
+    iu = np.triu_indices(d)
+    np.testing.assert_allclose(gram[iu], ref[iu], rtol=1e-11, atol=1e-9)
+    assert np.all(np.tril(gram, -1) == 0)
+    np.testing.assert_allclose(colsum, S.sum(0), rtol=1e-12, atol=1e-9)
+    # a plausible mistake -- dropping the clipped-zero offset b -- is caught
+    wrong = np.where(L != 0, Z, 0.0)
+    assert not np.allclose(gram[iu], (wrong.T @ wrong)[iu], rtol=1e-6)
+
+
+def test_gram_chunk_and_thread_invariance(oracle, X):
+    L, mean, sd = _scaled(X, 10.0)
+    d = X.shape[1]
+    gmap = np.arange(d, dtype=np.int32)
+    g1, c1 = np.zeros((d, d)), np.zeros(d)
+    oracle.gram_accumulate(*ctx_of(X), gmap, g1, c1, True, mean, sd, 10.0, threads=1)
+    g4, c4 = np.zeros((d, d)), np.zeros(d)
+    oracle.gram_accumulate(*ctx_of(X), gmap, g4, c4, True, mean, sd, 10.0, threads=4)
+    np.testing.assert_array_equal(g1, g4)
+    gc, cc = np.zeros((d, d)), np.zeros(d)
+    n = X.shape[0]
+    for lo, hi in [(0, 130), (130, 131), (131, n)]:
+        oracle.gram_accumulate(*ctx_of(X, rows=(lo, hi)), gmap, gc, cc, True, mean, sd, 10.0)
+    np.testing.assert_allclose(gc, g1, rtol=1e-12, atol=1e-10)
+
+
+def test_scaled_pca_matches_dense_svd(oracle, X):
+    """tests/test_crest.py:88-107 and kernels.rs:580-633 (eigenvalues of the scaled,
+    clipped, centred dense matrix)."""
+    mv = 2.0
+    L, mean, sd = _scaled(X, mv)
+    n, d = L.shape
+    gmap = np.arange(d, dtype=np.int32)
+    gram, colsum = np.zeros((d, d)), np.zeros(d)
+    oracle.gram_accumulate(*ctx_of(X), gmap, gram, colsum, True, mean, sd, mv)
+    var, load, total = oracle.pca_from_gram(gram, colsum, n, 10)
+    Z = np.clip((L - mean) / sd, -mv, mv)
+    Zc = Z - Z.mean(0)
+    U, S, Vt = np.linalg.svd(Zc, full_matrices=False)
+    np.testing.assert_allclose(var, S[:10] ** 2 / (n - 1), rtol=1e-9)
+    np.testing.assert_allclose(total, (Zc * Zc).sum() / (n - 1), rtol=1e-12)
+    assert oracle.principal_angle(load[:, :5], Vt[:5].T) < 1e-8
+    # projection: X = -shift + Σ s V == Zc V
+    shift = (colsum / n) @ load
+    Xp = np.empty((n, 10), np.float32)
+    Xp[:] = -shift.astype(np.float32)
+    oracle.project(*ctx_of(X), gmap, load, Xp, True, mean, sd, mv)
+    np.testing.assert_allclose(Xp, Zc @ load, rtol=1e-4, atol=1e-4)
+
+
+def test_sign_rule(oracle):
+    """kernels.rs:334-345: the first largest-|loading| entry of each component is
+    positive."""
+    rng = np.random.default_rng(0)
+    Q, _ = np.linalg.qr(rng.normal(size=(6, 6)))
+    Q[:, 0] = [0.1, -0.7, 0.7, 0.1, 0.0, 0.0]
+    Q, _ = np.linalg.qr(Q)
+    lam = np.array([9.0, 5.0, 3.0, 2.0, 1.0, 0.5])
+    cov = (Q * lam) @ Q.T
+    n = 11
+    gram = np.triu(cov * (n - 1))
+    var, load, total = oracle.pca_from_gram(gram, np.zeros(6), n, 6)
+    np.testing.assert_allclose(var, lam, rtol=1e-12)
+    for c in range(6):
+        i = int(np.argmax(np.abs(load[:, c])))
+        assert load[i, c] > 0
+        np.testing.assert_allclose(np.abs(load[:, c]), np.abs(Q[:, c]), atol=1e-10)
+
+
+def test_pca_from_gram_errors(oracle):
+    with pytest.raises(RuntimeError):
+        oracle.pca_from_gram(np.zeros((2, 2)), np.zeros(2), 1, 1)
+
+
+def _clustered(n, d, seed):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-10, 10, (20, d))
+    return (centers[np.arange(n) % 20] + rng.uniform(-1.5, 1.5, (n, d))).astype(np.float32)
+
+
+def test_knn_brute_matches_naive(oracle):
+    """knn.rs:399-414 and tests/test_crest.py:504-511."""
+    x = _clustered(600, 7, 1)
+    idx, dist = oracle.knn(x, 10)
+    D = ((x[:, None, :].astype(np.float64) - x[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(D, np.inf)
+    order = np.argsort(D, axis=1, kind="stable")[:, :10]
+    assert not (idx == np.arange(600)[:, None]).any()
+    np.testing.assert_allclose(dist[:, 9], np.sqrt(np.sort(D, 1)[:, 9]), rtol=1e-4)
+    same = (np.sort(idx, 1) == np.sort(order, 1)).all(1).mean()
+    assert same > 0.99
+    rng = np.random.default_rng(0)
+    y = rng.normal(size=(800, 10)).astype(np.float32)
+    idx, dist = oracle.knn(y, 5)
+    D = ((y[:, None, :].astype(np.float64) - y[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(D, np.inf)
+    np.testing.assert_array_equal(idx, np.argsort(D, axis=1)[:, :5])
+    np.testing.assert_allclose(dist, np.sqrt(np.sort(D, axis=1)[:, :5]), rtol=1e-4)
+
+
+def test_knn_exact_f64_helper(oracle):
+    y = np.random.default_rng(3).normal(size=(500, 6))
+    i, d2 = oracle.knn_exact_f64(y, 4, np.arange(0, 500, 7))
+    D = ((y[::7, None, :] - y[None]) ** 2).sum(-1)
+    D[np.arange(len(D)), np.arange(0, 500, 7)] = np.inf
+    np.testing.assert_array_equal(i, np.argsort(D, 1)[:, :4])
+    np.testing.assert_allclose(d2, np.sort(D, 1)[:, :5], rtol=1e-12)
+
+
+def test_principal_angle(oracle):
+    t = 1e-7
+    a = np.array([[1.0, 0], [0, 1], [0, 0]])
+    b = np.array([[1.0, 0], [0, np.cos(t)], [0, np.sin(t)]])
+    assert abs(oracle.principal_angle(a, b) - t) < 1e-12
+    assert oracle.principal_angle(a, a[:, ::-1]) < 1e-15
+
+
+def test_pd_cut_codes_match_pandas():
+    import pandas as pd
+    from crest_amd.pp import _pd_cut_codes
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        x = rng.gamma(0.5, 1.0, 3000)
+        codes = _pd_cut_codes(x, 20)
+        np.testing.assert_array_equal(codes, pd.cut(x, 20).codes)
+
+
+def test_hvg_restatement_matches_pandas_oracle(oracle):
+    from crest_amd.pp import hvg_from_stats
+    Xb = make_counts(1500, 600, 5, seed=4)
+    st = np.zeros((600, 5))
+    oracle.gene_stats(*ctx_of(Xb), st)
+    n = Xb.shape[0]
+    cols = hvg_from_stats(st, n, n_top_genes=200)
+    m = st[:, 2] / n
+    v = np.maximum((st[:, 3] - n * m * m) / (n - 1), 0)
+    hv, dn = oracle.hvg_seurat_pandas(m, v, n_top_genes=200)
+    np.testing.assert_array_equal(cols["highly_variable"], hv)
+    np.testing.assert_allclose(cols["dispersions_norm"], dn, rtol=1e-10, equal_nan=True)
+    cols2 = hvg_from_stats(st, n)
+    hv2, _ = oracle.hvg_seurat_pandas(m, v)
+    np.testing.assert_array_equal(cols2["highly_variable"], hv2)
