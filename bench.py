@@ -1,0 +1,216 @@
+"""Benchmark: cells/s of scale + PCA + kNN at 1,000,000 cells x 2,000 HVGs on one
+MI355X, split-BF16 against the plain FP32 GPU path, at fixed kNN agreement.
+
+A step is one pass of the hot path over the device-resident CSR: gene statistics ->
+mean/std -> Gram (SYRK on MFMA) -> FP64 eigensolve -> projection -> exact kNN (k=15),
+through crest_pipeline_run.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  After the timed region, the cpu_baseline leg times the
+oracle (the C restatement of the reference's CPU algorithm) on a bounded sample and
+checks the GPU results against FP64 references (checker use only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cells/s scale+PCA+kNN @1M×2k HVG: split-BF16 vs FP32 at fixed kNN agreement"
+PEAK_BF16 = 2.5e15      # MI355X dense BF16 MFMA, FLOP/s
+PEAK_FP32 = 157.3e12    # MI355X FP32 MFMA, FLOP/s
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_steps(pipe, steps: int, warmup: int):
+    import torch
+    for _ in range(warmup):
+        pipe.run()
+    torch.cuda.synchronize()
+    outs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs.append(pipe.run().times_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = {k: float(np.mean([o[k] for o in outs])) for k in outs[0]}
+    return wall / steps * 1e3, stages
+
+
+def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
+    """Oracle (C restatement of the reference's CPU path) on a bounded sample, plus the
+    accuracy checks of the GPU results against FP64 references."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    n = len(indptr) - 1
+    gmap = np.arange(n_genes, dtype=np.int32)
+    cmap = np.arange(n, dtype=np.int64)
+
+    def ctx(lo, hi):
+        a, b = int(indptr[lo]), int(indptr[hi])
+        return (indices[a:b], data[a:b], cmap, gmap, 1e4, True, None, indptr[lo:hi + 1].copy(), lo)
+
+    t = {}
+    # stage 1: gene statistics over every cell (serial in the reference's one part)
+    t0 = time.perf_counter()
+    st = np.zeros((n_genes, 5))
+    step = 1 << 17
+    for lo in range(0, n, step):
+        O.gene_stats(*ctx(lo, min(n, lo + step)), st)
+    t["stats"] = time.perf_counter() - t0
+    mean = st[:, 0] / n
+    var = np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0)
+    sd = np.sqrt(var)
+    sd[sd == 0] = 1
+    # stage 2: the FP64 sparse Gram over every cell (thread-owned rows, as the reference)
+    t0 = time.perf_counter()
+    gram, colsum = np.zeros((n_genes, n_genes)), np.zeros(n_genes)
+    for lo in range(0, n, step):
+        O.gram_accumulate(*ctx(lo, min(n, lo + step)), gmap, gram, colsum, True, mean, sd, 10.0, threads=threads)
+    t["gram"] = time.perf_counter() - t0
+    # stage 3: eigensolve (numpy LAPACK)
+    t0 = time.perf_counter()
+    var_o, load_o, tot_o = O.pca_from_gram(gram, colsum, n, args.pcs)
+    t["eig"] = time.perf_counter() - t0
+    # stage 4: projection on a sample of cells, scaled to n
+    ns = min(n, 100_000)
+    X = np.zeros((ns, load_o.shape[1]), np.float32)
+    t0 = time.perf_counter()
+    O.project(*ctx(0, ns), gmap, load_o, X, True, mean, sd, 10.0)
+    t["project"] = (time.perf_counter() - t0) * n / ns
+    # stage 5: exact kNN of a query sample against all n, scaled to n queries (on the
+    # GPU's own X_pca so the accuracy check below can reuse it)
+    Xg = gpu["x_bf16"]
+    q = args.cpu_knn_queries
+    t0 = time.perf_counter()
+    O.knn(Xg, args.k, 0, q, threads=threads)
+    t["knn"] = (time.perf_counter() - t0) * n / q
+    total = sum(t.values())
+    base = {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": (f"C restatement of kernels.rs/knn.rs: gene_stats and FP64 Gram over all {n} cells, "
+                       f"numpy eigh, projection on {ns} cells and exact kNN of {q} queries x {n} points, "
+                       f"both scaled linearly to {n}"),
+            "stage_s": {k: round(v, 3) for k, v in t.items()}}
+
+    # accuracy against FP64 references (checker)
+    acc = {}
+    rng = np.random.default_rng(0)
+    qs = rng.choice(n, args.acc_queries, replace=False)
+    for tag in ("bf16", "fp32"):
+        Xp = gpu[f"x_{tag}"]
+        ref_i, ref_d2 = O.knn_exact_f64(Xp, args.k, qs)
+        got = gpu[f"idx_{tag}"][qs]
+        bad, ties = O.knn_mismatch_ties(got, Xp, qs, ref_i, ref_d2)
+        acc[f"knn_set_agreement_{tag}"] = O.knn_set_agreement(got, ref_i)
+        acc[f"knn_mismatch_{tag}"] = [int(bad), int(ties)]
+        L = gpu[f"load_{tag}"]
+        r = args.lead
+        acc[f"pca_angle_lead{r}_{tag}"] = O.principal_angle(L[:, :r], load_o[:, :r])
+        acc[f"pca_angle_all_{tag}"] = O.principal_angle(L, load_o)
+        acc[f"variance_ratio_err_{tag}"] = float(np.max(np.abs(gpu[f"var_{tag}"] / gpu[f"tot_{tag}"]
+                                                               - var_o / tot_o)))
+    acc["eig_gap_p"] = float(var_o[args.pcs - 1] - (np.linalg.eigvalsh(O.cov_from_gram(gram, colsum, n)[0])[::-1]
+                                                     [args.pcs]))
+    acc["queries"] = int(args.acc_queries)
+    return base, acc
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--genes", type=int, default=2000)
+    ap.add_argument("--clusters", type=int, default=30)
+    ap.add_argument("--density", type=float, default=0.08)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--pcs", type=int, default=50)
+    ap.add_argument("--k", type=int, default=15)
+    ap.add_argument("--lead", type=int, default=20)
+    ap.add_argument("--chunk-rows", type=int, default=65536)
+    ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--cpu-knn-queries", type=int, default=512)
+    ap.add_argument("--acc-queries", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import torch
+    from crest_amd import synth
+    from crest_amd.pipeline import DeviceCSR, Pipeline
+
+    t0 = time.perf_counter()
+    indptr, indices, data, info = synth.make_counts(args.cells, args.genes, args.clusters, density=args.density,
+                                                    seed=args.seed)
+    log(f"generated {args.cells} x {args.genes}, nnz {info['nnz']} (density {info['density']:.4f}) "
+        f"in {time.perf_counter() - t0:.1f}s")
+    csr = DeviceCSR.upload(indptr, indices, data, args.genes)
+    torch.cuda.synchronize()
+    hvg = np.arange(args.genes, dtype=np.int32)  # the 2,000 genes are the pre-selected HVGs
+    pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram="bf16x3", prec_knn="bf16x3",
+                    chunk_rows=args.chunk_rows)
+
+    ms, stages = timed_steps(pipe, args.steps, args.warmup)
+    log(f"split-BF16: {ms:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
+    gpu = {"x_bf16": pipe.x.cpu().numpy(), "idx_bf16": pipe.idx.cpu().numpy().view(np.uint32),
+           "var_bf16": pipe.variance[:pipe.kp].copy(), "load_bf16": pipe.loadings[:, :pipe.kp].copy()}
+    out = pipe.run()
+    gpu["tot_bf16"] = out.total_variance
+
+    pipe.set_precision("fp32", "fp32")
+    ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
+    log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
+    out32 = pipe.run()
+    gpu.update({"x_fp32": pipe.x.cpu().numpy(), "idx_fp32": pipe.idx.cpu().numpy().view(np.uint32),
+                "var_fp32": out32.variance, "load_fp32": out32.loadings, "tot_fp32": out32.total_variance})
+
+    n = args.cells
+    p = pipe.kp
+    screen_s = stages["knn_screen"] * 1e-3
+    knn_flops = 2.0 * n * n * p
+    achieved = knn_flops / screen_s
+    line = {
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "split-bf16 MFMA (f32 accumulate) + f64",
+        "data": "synthetic (seeded NB counts, crest_amd.synth)",
+        "config": {"workload": (f"{n} cells x {args.genes} pre-selected HVGs, NB counts density "
+                                f"{info['density']:.4f}, {args.clusters} clusters, seed {args.seed}; normalize_total 1e4,"
+                                f" log1p, scale max_value 10, PCA {args.pcs}, exact kNN k={args.k}"),
+                   "n_cells": n, "n_genes": args.genes, "nnz": info["nnz"], "n_comps": args.pcs, "k": args.k,
+                   "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": "bf16x3"},
+        "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "fp32": {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
+                 "stages_ms": {k: round(v, 3) for k, v in stages32.items()}},
+        "speedup_vs_fp32": ms32 / ms,
+        "roofline": {"kernel": "k_knn_screen_bf16", "bound": "mfma", "achieved": achieved / 1e12,
+                     "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16, "traffic": None,
+                     "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
+                     "launch_ms": stages["knn_screen"]},
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        base, acc = cpu_baseline_leg(indptr, indices, data, args.genes, gpu, args)
+        log(f"cpu baseline + accuracy leg {time.perf_counter() - t0:.1f}s")
+        line["cpu_baseline"] = base
+        line["accuracy"] = acc
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

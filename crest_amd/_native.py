@@ -14,7 +14,8 @@ import numpy as np
 
 from ._lib import Chunk, check, lib
 
-__all__ = ["gene_stats", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "materialize"]
+__all__ = ["gene_stats", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "connectivities",
+           "materialize"]
 
 
 def _arr(a, dtype, name: str, writable: bool = False):
@@ -141,6 +142,26 @@ def knn_graph(data, k, exact=None, seed=0):
     ex = -1 if exact is None else int(bool(exact))
     check(lib().crest_knn_graph(_p(data), n, dim, int(k), ex, int(seed), _p(idx), _p(dist), C.byref(ko)))
     return idx[:, :ko.value], dist[:, :ko.value]
+
+
+def connectivities(indices, distances, n_neighbors):
+    """py.rs:72-99: UMAP fuzzy simplicial set of a kNN graph whose rows hold
+    n_neighbors - 1 neighbours; returns the undirected edges (rows < cols, weights)."""
+    if not isinstance(indices, np.ndarray) or not isinstance(distances, np.ndarray):
+        raise TypeError("indices and distances must be numpy arrays")
+    if indices.ndim != 2 or distances.shape != indices.shape:
+        raise ValueError("indices and distances must be (n, k) arrays of equal shape")
+    indices = _arr(indices, np.uint32, "indices")
+    distances = _arr(distances, np.float32, "distances")
+    if n_neighbors < 0:
+        raise OverflowError("can't convert negative int to unsigned")
+    n, k = indices.shape
+    cap = max(n * k, 1)
+    rows, cols, w = np.empty(cap, np.uint32), np.empty(cap, np.uint32), np.empty(cap, np.float32)
+    m = C.c_size_t(0)
+    check(lib().crest_connectivities(_p(indices), _p(distances), n, k, int(n_neighbors), _p(rows), _p(cols), _p(w),
+                                     cap, C.byref(m)))
+    return rows[:m.value].copy(), cols[:m.value].copy(), w[:m.value].copy()
 
 
 def materialize(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, first_cell):

@@ -518,6 +518,42 @@ int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t e
     });
 }
 
+int crest_connectivities(const uint32_t *indices, const float *distances, size_t n, size_t k, size_t n_neighbors,
+                         uint32_t *rows, uint32_t *cols, float *weights, size_t cap, size_t *n_out) {
+    return guarded([&] {
+        if (n_out) *n_out = 0;
+        if (n == 0 || k == 0) return;  // graph.rs:63-65
+        if (!indices || !distances) value_error("indices and distances must be (n, k) arrays of equal shape");
+        if (n * k >= 0xFFFFFFFFull) value_error("graph too large");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const int64_t m = (int64_t)(n * k);
+        const uint32_t *di = x.upload("conn_idx", indices, m);
+        const float *dd = x.upload("conn_dist", distances, m);
+        uint64_t *k1 = x.get<uint64_t>("conn_k1", m), *k2 = x.get<uint64_t>("conn_k2", m);
+        float *w1 = x.get<float>("conn_w1", m), *w2 = x.get<float>("conn_w2", m), *wu = x.get<float>("conn_wu", m);
+        uint32_t *fl = x.get<uint32_t>("conn_flag", m), *ps = x.get<uint32_t>("conn_pos", m);
+        double *part = x.get<double>("conn_part", 257);
+        const size_t tb = conn_temp_bytes(m);
+        void *tmp = x.get<uint8_t>("conn_tmp", tb);
+        uint32_t *orow = x.get<uint32_t>("conn_rows", m), *ocol = x.get<uint32_t>("conn_cols", m);
+        float *ow = x.get<float>("conn_w", m);
+        uint32_t *cnt = x.get<uint32_t>("conn_cnt", 1);
+        launch_connectivities(di, dd, (int64_t)n, (int)k, (int)n_neighbors, k1, k2, w1, w2, wu, fl, ps, part, tmp, tb,
+                              orow, ocol, ow, cnt, x.st);
+        x.check_launch();
+        uint32_t hc = 0;
+        x.download(&hc, cnt, 1);
+        x.sync();
+        if (hc > cap) value_error("output capacity too small");
+        x.download(rows, orow, hc);
+        x.download(cols, ocol, hc);
+        x.download(weights, ow, hc);
+        x.sync();
+        if (n_out) *n_out = hc;
+    });
+}
+
 int crest_materialize(const crest_chunk *chunk, uint32_t *rows, uint32_t *vars, float *values, size_t cap,
                       size_t *n_out) {
     return guarded([&] {

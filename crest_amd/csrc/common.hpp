@@ -108,6 +108,11 @@ bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int 
                        float *cand_d, uint32_t *cand_i, hipStream_t s);
 void launch_knn_rerank(const float *x, int64_t n, int p, const float *cand_d, const uint32_t *cand_i,
                        int n_cand, int k, uint32_t *idx_out, float *dist_out, hipStream_t s);
+size_t conn_temp_bytes(int64_t m);
+void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, int k, int n_neighbors,
+                           uint64_t *keys, uint64_t *keys2, float *w, float *w2, float *wu, uint32_t *flag,
+                           uint32_t *pos, double *part, void *temp, size_t temp_bytes, uint32_t *rows,
+                           uint32_t *cols, float *wout, uint32_t *count, hipStream_t s);
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);
 

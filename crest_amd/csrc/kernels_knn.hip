@@ -39,7 +39,7 @@ __device__ inline float bf16_f(uint16_t b) { return __uint_as_float((uint32_t)b 
 
 // Instantiated operand widths (16-byte chunks per row).
 static const int kBf16Chunks[] = {4, 8, 12, 16, 20, 24, 32, 52};  // KC = 2*KSTEPS
-static const int kF32Chunks[] = {2, 4, 8, 13, 16, 24, 33};         // KC, 4 floats each
+static const int kF32Chunks[] = {2, 4, 8, 14, 16, 24, 34};         // KC (even), 4 floats each
 
 int knn_kpad(int prec, int p) {
     if (prec == kPrecBF16X3) {
@@ -231,7 +231,8 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
                                                          int64_t n_pad, float *cand_d, uint32_t *cand_i) {
     constexpr int KS = KC / 2;  // MFMA K-steps of 16
     constexpr int TU = KC * kTC;  // 16-byte units per tile
-    constexpr int PER = (TU + 255) / 256;
+    static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
+    constexpr int PER = TU / 256;
     __shared__ uint4 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
     const int64_t q0 = (int64_t)blockIdx.x * kQWG + wave * kQW;
@@ -255,15 +256,14 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
-        const bool has_next = t + 1 < n_tiles;
+        // prefetch the next tile (the last iteration re-reads the final tile; its LDS
+        // copy is never read), unconditionally so the registers stay registers
+        const int64_t tn = t + 1 < n_tiles ? t + 1 : t;
         uint4 pre[PER];
-        if (has_next) {
-            const uint4 *src = xa + (t + 1) * TU;
+        {
+            const uint4 *src = xa + tn * TU;
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int i = threadIdx.x + u * 256;
-                if (i < TU) pre[u] = src[i];
-            }
+            for (int u = 0; u < PER; ++u) pre[u] = src[threadIdx.x + u * 256];
         }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
@@ -282,13 +282,8 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-        if (has_next) {
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int i = threadIdx.x + u * 256;
-                if (i < TU) lds[buf ^ 1][i] = pre[u];
-            }
-        }
+        for (int u = 0; u < PER; ++u) lds[buf ^ 1][threadIdx.x + u * 256] = pre[u];
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -302,7 +297,8 @@ template <int KC, int L>
 __global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict__ xa, const float2 *__restrict__ xb,
                                                         int64_t n_pad, float *cand_d, uint32_t *cand_i) {
     constexpr int TU = KC * 2 * kTC;  // float2 units per tile
-    constexpr int PER = (TU + 255) / 256;
+    static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
+    constexpr int PER = TU / 256;
     __shared__ float2 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
     const int64_t q0 = (int64_t)blockIdx.x * kQWG + wave * kQW;
@@ -326,15 +322,14 @@ __global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
-        const bool has_next = t + 1 < n_tiles;
+        // prefetch the next tile (the last iteration re-reads the final tile; its LDS
+        // copy is never read), unconditionally so the registers stay registers
+        const int64_t tn = t + 1 < n_tiles ? t + 1 : t;
         float2 pre[PER];
-        if (has_next) {
-            const float2 *src = xa + (t + 1) * TU;
+        {
+            const float2 *src = xa + tn * TU;
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int i = threadIdx.x + u * 256;
-                if (i < TU) pre[u] = src[i];
-            }
+            for (int u = 0; u < PER; ++u) pre[u] = src[threadIdx.x + u * 256];
         }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
@@ -355,13 +350,8 @@ __global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-        if (has_next) {
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int i = threadIdx.x + u * 256;
-                if (i < TU) lds[buf ^ 1][i] = pre[u];
-            }
-        }
+        for (int u = 0; u < PER; ++u) lds[buf ^ 1][threadIdx.x + u * 256] = pre[u];
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -387,7 +377,7 @@ static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t n_
             CREST_BF(4) CREST_BF(8) CREST_BF(12) CREST_BF(16) CREST_BF(20) CREST_BF(24) CREST_BF(32) CREST_BF(52)
         }
     } else {
-        switch (kpad / 4) { CREST_F(2) CREST_F(4) CREST_F(8) CREST_F(13) CREST_F(16) CREST_F(24) CREST_F(33) }
+        switch (kpad / 4) { CREST_F(2) CREST_F(4) CREST_F(8) CREST_F(14) CREST_F(16) CREST_F(24) CREST_F(34) }
     }
 #undef CREST_BF
 #undef CREST_F

@@ -11,7 +11,8 @@ import numpy as np
 from . import _native
 from .core import BioFrame
 
-__all__ = ["normalize_total", "log1p", "scale", "gene_stats", "highly_variable_genes", "neighbors"]
+__all__ = ["normalize_total", "log1p", "scale", "gene_stats", "highly_variable_genes", "neighbors",
+           "connectivities_matrix"]
 
 
 def normalize_total(bf: BioFrame, target_sum: float = 1e4) -> BioFrame:
@@ -153,9 +154,9 @@ def hvg_from_stats(st: np.ndarray, n: int, n_top_genes: Optional[int] = None, fl
 
 def neighbors(bf: BioFrame, n_neighbors: int = 15, n_pcs: Optional[int] = None, use_rep: str = "X_pca",
               exact: Optional[bool] = None, random_state: int = 0) -> BioFrame:
-    """kNN graph of obsm[use_rep] (crest/pp.py:296-315).  n_neighbors counts the cell
-    itself, as in scanpy, so n_neighbors - 1 neighbours are stored.  The search is
-    exact for every n."""
+    """kNN graph + UMAP fuzzy connectivities of obsm[use_rep] (crest/pp.py:296-315).
+    n_neighbors counts the cell itself, as in scanpy, so n_neighbors - 1 neighbours
+    are stored.  The search is exact for every n."""
     if use_rep not in bf.obsm:
         raise KeyError(f"{use_rep!r} not in obsm; run crest_amd.tl.pca first")
     X = bf.obsm[use_rep]
@@ -163,8 +164,18 @@ def neighbors(bf: BioFrame, n_neighbors: int = 15, n_pcs: Optional[int] = None, 
         X = X[:, :n_pcs]
     X = np.ascontiguousarray(X, dtype=np.float32)
     idx, dist = _native.knn_graph(X, n_neighbors - 1, exact=exact, seed=random_state)
+    rows, cols, w = _native.connectivities(idx, dist, n_neighbors)
     bf.uns["neighbors"] = {
         "params": {"n_neighbors": n_neighbors, "n_pcs": n_pcs, "use_rep": use_rep, "method": "umap"},
-        "indices": idx, "distances": dist,
+        "indices": idx, "distances": dist, "rows": rows, "cols": cols, "weights": w,
     }
     return bf
+
+
+def connectivities_matrix(bf: BioFrame):
+    """Symmetric scipy.sparse CSR connectivities (crest/pp.py:318-324)."""
+    import scipy.sparse as sp
+    nb = bf.uns["neighbors"]
+    n = bf.n_obs
+    C = sp.coo_matrix((nb["weights"], (nb["rows"], nb["cols"])), shape=(n, n)).tocsr()
+    return (C + C.T).tocsr()

@@ -110,6 +110,15 @@ int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub
 int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact,
                     uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
 
+/* Replaces py.rs:72-99 `connectivities(indices, distances, n_neighbors)`
+ * (umap/graph.rs:29-107, scanpy's fuzzy simplicial set).  indices/distances are
+ * [n, k] as knn_graph returns them; writes the undirected edges with source <
+ * target in (source, target) order into rows/cols/weights of capacity `cap`
+ * (>= n*k suffices); *n_out is the edge count.                                    */
+int crest_connectivities(const uint32_t *indices, const float *distances, size_t n, size_t k,
+                         size_t n_neighbors, uint32_t *rows, uint32_t *cols, float *weights,
+                         size_t cap, size_t *n_out);
+
 /* Replaces py.rs:213-219 `materialize(*ctx)` (kernels.rs:122-139) for parity
  * checks of the lazy transform.  Writes the kept, transformed entries of the chunk
  * in chunk order: rows/vars u32 and values f32, each of capacity `cap` (>= nnz);

@@ -409,3 +409,95 @@ void oc_knn(const float *x, size_t n, size_t p, size_t k, size_t q0, size_t q1, 
 void oc_pair_dist(const float *x, size_t p, const uint32_t *a, const uint32_t *b, size_t m, float *out) {
     for (size_t i = 0; i < m; ++i) out[i] = sqrtf(sq_dist(x + (size_t)a[i] * p, x + (size_t)b[i] * p, p));
 }
+
+/* ------------------------------------------------------------------ connectivities */
+/* umap/graph.rs:29-58 smooth_knn_dist (local_connectivity = 1). */
+static void smooth_knn_dist(const float *dists, size_t k, size_t n_neighbors, double mean_all, double *rho_out,
+                            double *sigma_out) {
+    const double target = log2((double)n_neighbors);
+    double rho = 0.0;
+    for (size_t i = 0; i < k; ++i)
+        if (dists[i] > 0.0f) {
+            rho = (double)dists[i];
+            break;
+        }
+    double lo = 0.0, hi = INFINITY, mid = 1.0;
+    for (int it = 0; it < 64; ++it) {
+        double psum = 0.0;
+        for (size_t i = 0; i < k; ++i) {
+            const double v = (double)dists[i] - rho;
+            psum += v > 0.0 ? exp(-v / mid) : 1.0;
+        }
+        if (fabs(psum - target) < 1e-5) break;
+        if (psum > target) {
+            hi = mid;
+            mid = (lo + hi) / 2.0;
+        } else {
+            lo = mid;
+            if (hi == INFINITY)
+                mid *= 2.0;
+            else
+                mid = (lo + hi) / 2.0;
+        }
+    }
+    double s = 0.0;
+    for (size_t i = 0; i < k; ++i) s += (double)dists[i];
+    const double mean_i = s / (double)(k + 1);
+    const double floor_ = rho > 0.0 ? 1e-3 * mean_i : 1e-3 * mean_all;
+    *rho_out = rho;
+    *sigma_out = mid > floor_ ? mid : floor_;
+}
+
+typedef struct kentry { uint64_t key; float w; } kentry;
+static int cmp_key(const void *a, const void *b) {
+    const kentry *p = a, *q = b;
+    return p->key < q->key ? -1 : (p->key > q->key ? 1 : 0);
+}
+
+/* umap/graph.rs:61-107 fuzzy_simplicial_set + py.rs:92-97 (edges with source < target,
+ * in key order).  Returns the number of edges written. */
+size_t oc_connectivities(const uint32_t *idx, const float *dist, size_t n, size_t k, size_t n_neighbors,
+                         uint32_t *rows, uint32_t *cols, float *w) {
+    if (n == 0 || k == 0) return 0;
+    double sa = 0.0;
+    for (size_t i = 0; i < n * k; ++i) sa += (double)dist[i];
+    const double mean_all = sa / (double)(n * (k + 1));
+    kentry *e = malloc(n * k * sizeof(kentry));
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        double rho, sigma;
+        smooth_knn_dist(dist + i * k, k, n_neighbors, mean_all, &rho, &sigma);
+        for (size_t t = 0; t < k; ++t) {
+            const uint32_t j = idx[i * k + t];
+            if (j == UINT32_MAX || (size_t)j == i) continue;
+            const double v = (double)dist[i * k + t] - rho;
+            const double ww = (v <= 0.0 || sigma == 0.0) ? 1.0 : exp(-v / sigma);
+            const uint64_t a = (uint32_t)i < j ? (uint64_t)i : (uint64_t)j;
+            const uint64_t b = (uint32_t)i < j ? (uint64_t)j : (uint64_t)i;
+            e[m].key = (a << 32) | b;
+            e[m].w = (float)ww;
+            ++m;
+        }
+    }
+    qsort(e, m, sizeof(kentry), cmp_key);
+    size_t out = 0, p = 0;
+    while (p < m) {
+        const uint64_t key = e[p].key;
+        float ww = e[p].w;
+        size_t q = p + 1;
+        while (q < m && e[q].key == key) {
+            const float b = e[q].w;
+            ww = ww + b - ww * b;
+            ++q;
+        }
+        if (ww > 0.0f) {
+            rows[out] = (uint32_t)(key >> 32);
+            cols[out] = (uint32_t)(key & 0xffffffffu);
+            w[out] = ww;
+            ++out;
+        }
+        p = q;
+    }
+    free(e);
+    return out;
+}

@@ -64,6 +64,9 @@ def lib():
         _LIB.oc_materialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB.oc_gene_stats.argtypes = [C.c_void_p, C.c_void_p]
         _LIB.oc_set_log1p_mode.argtypes = [C.c_int]
+        _LIB.oc_connectivities.restype = C.c_size_t
+        _LIB.oc_connectivities.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
     return _LIB
 
 
@@ -152,6 +155,17 @@ def knn(x, k, q0=0, q1=None, threads=1):
     if kk:
         lib().oc_knn(_p(x), n, p, kk, q0, q1, _p(idx), _p(dist), int(threads))
     return idx, dist
+
+
+def connectivities(idx, dist, n_neighbors):
+    """umap/graph.rs:61-107 + py.rs:92-97: undirected edges (rows < cols, weights)."""
+    idx = np.ascontiguousarray(idx, np.uint32)
+    dist = np.ascontiguousarray(dist, np.float32)
+    n, k = idx.shape
+    cap = max(n * k, 1)
+    r, c, w = np.empty(cap, np.uint32), np.empty(cap, np.uint32), np.empty(cap, np.float32)
+    m = lib().oc_connectivities(_p(idx), _p(dist), n, k, int(n_neighbors), _p(r), _p(c), _p(w))
+    return r[:m], c[:m], w[:m]
 
 
 def pair_dist(x, a, b):

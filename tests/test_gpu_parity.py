@@ -226,3 +226,26 @@ def test_hvg_matches_oracle(counts, oracle):
     v = np.maximum((st[:, 3] - n * m * m) / (n - 1), 0)
     hv, _ = oracle.hvg_seurat_pandas(m, v, n_top_genes=300)
     np.testing.assert_array_equal(bf.var["highly_variable"], hv)
+
+
+@pytest.mark.parametrize("n,d,k", [(3000, 20, 14), (500, 5, 4)])
+def test_connectivities(oracle, n, d, k):
+    x = _clustered(n, d, 7 + n)
+    idx, dist = N.knn_graph(x, k)
+    r, c, w = N.connectivities(idx, dist, k + 1)
+    ro, co, wo = oracle.connectivities(idx, dist, k + 1)
+    np.testing.assert_array_equal(r, ro)
+    np.testing.assert_array_equal(c, co)
+    np.testing.assert_allclose(w, wo, rtol=2e-6, atol=1e-7)
+
+
+def test_neighbors_api(counts):
+    bf = _frame(counts, drop=False)
+    ca.pp.highly_variable_genes(bf, n_top_genes=200)
+    ca.pp.scale(bf)
+    ca.tl.pca(bf, n_comps=15)
+    ca.pp.neighbors(bf, n_neighbors=10)
+    nb = bf.uns["neighbors"]
+    assert nb["indices"].shape == (bf.n_obs, 9) and len(nb["rows"]) > 0
+    C = ca.pp.connectivities_matrix(bf)
+    assert (abs(C - C.T) > 0).nnz == 0

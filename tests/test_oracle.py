@@ -274,3 +274,31 @@ def test_hvg_restatement_matches_pandas_oracle(oracle):
     cols2 = hvg_from_stats(st, n)
     hv2, _ = oracle.hvg_seurat_pandas(m, v)
     np.testing.assert_array_equal(cols2["highly_variable"], hv2)
+
+
+def test_connectivities_pins(oracle):
+    """umap/graph.rs:127-160: the 4-point toy graph is symmetric and links the two
+    pairs; each row's membership strengths sum to log2(n_neighbors)."""
+    x = np.array([[0, 0], [0.1, 0.1], [1, 1], [1.1, 1.1]], np.float32)
+    idx, dist = oracle.knn(x, 1)
+    r, c, w = oracle.connectivities(idx, dist, 2)
+    pairs = set(zip(r.tolist(), c.tolist()))
+    assert (0, 1) in pairs and (2, 3) in pairs and np.all(r < c)
+    # sigma hits the target: rebuild the per-row strengths from the edge list
+    rng = np.random.default_rng(0)
+    y = rng.normal(size=(300, 5)).astype(np.float32)
+    idx, dist = oracle.knn(y, 9)
+    r, c, w = oracle.connectivities(idx, dist, 10)
+    assert np.all(r < c) and np.all(np.diff(r.astype(np.int64) * 300 + c) > 0)
+    assert np.all((w > 0) & (w <= 1))
+    # directed strengths of row 0 from its own (rho, sigma): Σ = log2(10) within 1e-4
+    d = dist[0].astype(np.float64)
+    rho = d[d > 0][0]
+    lo, hi = 1e-6, 100.0
+    for _ in range(200):
+        mid = (lo + hi) / 2
+        s = np.where(d - rho > 0, np.exp(-(d - rho) / mid), 1.0).sum()
+        lo, hi = (lo, mid) if s > np.log2(10) else (mid, hi)
+    assert abs(np.where(d - rho > 0, np.exp(-(d - rho) / lo), 1.0).sum() - np.log2(10)) < 1e-4
+    r0, c0, w0 = oracle.connectivities(np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.float32), 4)
+    assert len(r0) == 0
