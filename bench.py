@@ -33,15 +33,17 @@ def log(msg: str) -> None:
 
 
 def timed_steps(pipe, steps: int, warmup: int):
-    import torch
+    """W untimed steps, then K timed steps; each run ends with the library's stream
+    synchronized, and the timed region is bracketed by a device synchronization."""
+    from crest_amd._lib import check, lib
     for _ in range(warmup):
         pipe.run()
-    torch.cuda.synchronize()
+    check(lib().crest_synchronize())
     outs = []
     t0 = time.perf_counter()
     for _ in range(steps):
         outs.append(pipe.run().times_ms)
-    torch.cuda.synchronize()
+    check(lib().crest_synchronize())
     wall = time.perf_counter() - t0
     stages = {k: float(np.mean([o[k] for o in outs])) for k in outs[0]}
     return wall / steps * 1e3, stages
@@ -149,7 +151,6 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
 
-    import torch
     from crest_amd import synth
     from crest_amd.pipeline import DeviceCSR, Pipeline
 
@@ -159,14 +160,13 @@ def main() -> int:
     log(f"generated {args.cells} x {args.genes}, nnz {info['nnz']} (density {info['density']:.4f}) "
         f"in {time.perf_counter() - t0:.1f}s")
     csr = DeviceCSR.upload(indptr, indices, data, args.genes)
-    torch.cuda.synchronize()
     hvg = np.arange(args.genes, dtype=np.int32)  # the 2,000 genes are the pre-selected HVGs
     pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram="bf16x3", prec_knn="bf16x3",
                     chunk_rows=args.chunk_rows)
 
     ms, stages = timed_steps(pipe, args.steps, args.warmup)
     log(f"split-BF16: {ms:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
-    gpu = {"x_bf16": pipe.x.cpu().numpy(), "idx_bf16": pipe.idx.cpu().numpy().view(np.uint32),
+    gpu = {"x_bf16": pipe.x_pca(), "idx_bf16": pipe.knn()[0],
            "var_bf16": pipe.variance[:pipe.kp].copy(), "load_bf16": pipe.loadings[:, :pipe.kp].copy()}
     out = pipe.run()
     gpu["tot_bf16"] = out.total_variance
@@ -175,7 +175,7 @@ def main() -> int:
     ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
     log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
     out32 = pipe.run()
-    gpu.update({"x_fp32": pipe.x.cpu().numpy(), "idx_fp32": pipe.idx.cpu().numpy().view(np.uint32),
+    gpu.update({"x_fp32": pipe.x_pca(), "idx_fp32": pipe.knn()[0],
                 "var_fp32": out32.variance, "load_fp32": out32.loadings, "tot_fp32": out32.total_variance})
 
     n = args.cells

@@ -61,6 +61,11 @@ _SIGS = {
     "crest_last_error": (C.c_char_p, []),
     "crest_device_count": (C.c_int32, []),
     "crest_release": (C.c_int, []),
+    "crest_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "crest_dev_free": (C.c_int, [C.c_void_p]),
+    "crest_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "crest_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "crest_synchronize": (C.c_int, []),
     "crest_gene_stats": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t]),
     "crest_gram_accumulate": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                         C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,

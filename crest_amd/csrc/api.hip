@@ -372,6 +372,45 @@ int crest_release(void) {
     });
 }
 
+int crest_dev_alloc(size_t bytes, void **ptr) {
+    return guarded([&] {
+        if (!ptr) value_error("ptr is NULL");
+        *ptr = nullptr;
+        ctx();
+        HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+
+int crest_dev_free(void *ptr) {
+    return guarded([&] {
+        ctx();
+        if (ptr) HIPCHK(hipFree(ptr));
+    });
+}
+
+int crest_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        Ctx &x = ctx();
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, x.st));
+        x.sync();
+    });
+}
+
+int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        Ctx &x = ctx();
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, x.st));
+        x.sync();
+    });
+}
+
+int crest_synchronize(void) {
+    return guarded([&] {
+        Ctx &x = ctx();
+        x.sync();
+    });
+}
+
 int crest_gene_stats(const crest_chunk *chunk, double *out, size_t out_len) {
     return guarded([&] {
         HostChunk h = check_chunk(chunk);

@@ -50,7 +50,9 @@ __device__ inline void store_partial32(double *dst, const double (&v)[16], int m
 }
 
 // ---------------------------------------------------------------- split-BF16
-// S = H + L (bf16 hi/lo); S^T S ~ H^T H + H^T L + L^T H on v_mfma_f32_32x32x16_bf16.
+// S = H + L (bf16 hi/lo); S^T S = H^T H + H^T L + L^T H + L^T L on
+// v_mfma_f32_32x32x16_bf16.  The L^T L product is kept: on the diagonal it is a
+// positive bias of up to 2^-18 relative that would otherwise shift every variance.
 // FP32 accumulators are added into FP64 every FLUSH K-steps (16 rows each).
 template <int FLUSH>
 __global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
@@ -90,6 +92,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ t
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bl[j], acc[i][j], 0, 0, 0);
             }
         if (++cnt == FLUSH) {
             cnt = 0;
@@ -178,7 +181,8 @@ __global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, 
 
 // ---------------------------------------------------------------- FP64
 // v_mfma_f64_16x16x4_f64; lane quarter q = l/16 takes rows 2q, 2q+1 of each row
-// group.  Output: register r of lane l is C[4*(l/16) + r][l%16].
+// group.  Output (FP64 layout, rows interleaved across the lane quarters): register
+// r of lane l is C[4*r + l/16][l%16].
 __global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t, int64_t n_groups, int d_pad,
                                                   int n_split, double *partial) {
     const int nb = d_pad / kGramBlock;
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t,
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = wm * 64 + i * 16 + 4 * q + r;
+                const int m = wm * 64 + i * 16 + 4 * r + q;
                 const int n = wn * 64 + j * 16 + c16;
                 dst[m * kGramBlock + n] = acc[i][j][r];
             }

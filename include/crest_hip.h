@@ -172,6 +172,15 @@ int crest_pipeline_run(const crest_pipeline_args *args, double *variance, double
 /* Release every cached device buffer of the current device. */
 int crest_release(void);
 
+/* Device memory for callers of crest_pipeline_run (no reference counterpart):
+ * allocation on the current device, synchronous copies, and a wait for the
+ * library's stream.                                                               */
+int crest_dev_alloc(size_t bytes, void **ptr);
+int crest_dev_free(void *ptr);
+int crest_memcpy_htod(void *dst, const void *src, size_t bytes);
+int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int crest_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,9 +43,10 @@ def reference(data, oracle):
     return var_o, load_o, tot_o, X
 
 
-@pytest.mark.parametrize("prec_gram,prec_knn,var_rtol", [("fp64", "fp32", 1e-9), ("fp32", "fp32", 1e-5),
-                                                         ("bf16x3", "bf16x3", 1e-4)])
-def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol):
+@pytest.mark.parametrize("prec_gram,prec_knn,var_rtol,tot_rtol", [("fp64", "fp32", 1e-9, 1e-9),
+                                                                  ("fp32", "fp32", 1e-5, 1e-7),
+                                                                  ("bf16x3", "bf16x3", 1e-4, 1e-6)])
+def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol, tot_rtol):
     from crest_amd.pipeline import DeviceCSR, Pipeline
     indptr, indices, vals, g = data
     csr = DeviceCSR.upload(indptr, indices, vals, g)
@@ -54,15 +55,14 @@ def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol):
     out = p.run()
     var_o, load_o, tot_o, X_o = reference
     np.testing.assert_allclose(out.variance, var_o, rtol=var_rtol)
-    assert abs(out.total_variance - tot_o) <= 1e-9 * tot_o
+    assert abs(out.total_variance - tot_o) <= tot_rtol * tot_o
     lead = 10  # well separated leading components
     assert oracle.principal_angle(out.loadings[:, :lead], load_o[:, :lead]) < (1e-7 if prec_gram == "fp64" else 1e-3)
-    X = out.x_pca.cpu().numpy()
+    X = p.x_pca()
     if prec_gram == "fp64":
         # same loadings up to eigen-solver rounding: scores agree closely
         np.testing.assert_allclose(np.abs(X[:, :lead]), np.abs(X_o[:, :lead]), rtol=1e-4, atol=1e-3)
-    idx = out.knn_idx.cpu().numpy().view(np.uint32)
-    dist = out.knn_dist.cpu().numpy()
+    idx, dist = p.knn()
     q = np.random.default_rng(0).choice(len(X), 400, replace=False)
     ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
     bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
