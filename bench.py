@@ -109,7 +109,7 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     acc = {}
     rng = np.random.default_rng(0)
     qs = rng.choice(n, args.acc_queries, replace=False)
-    for tag in ("bf16", "fp32"):
+    for tag in [t for t in ("bf16", "fp32") if f"x_{t}" in gpu]:
         Xp = gpu[f"x_{tag}"]
         ref_i, ref_d2 = O.knn_exact_f64(Xp, args.k, qs)
         got = gpu[f"idx_{tag}"][qs]
@@ -126,6 +126,31 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
                                                      [args.pcs]))
     acc["queries"] = int(args.acc_queries)
     return base, acc
+
+
+def knn_only(args) -> int:
+    """The kNN stage alone (knn_graph through the C ABI) on clustered synthetic scores of
+    the workload's shape, plus the optional copy calibration: a short program for
+    hardware-counter runs of the screen kernel."""
+    import crest_amd as ca
+    from crest_amd import _native as N
+    rng = np.random.default_rng(args.seed)
+    n, p = args.cells, args.pcs
+    centers = rng.normal(0.0, 6.0, (args.clusters, p))
+    x = (centers[rng.integers(0, args.clusters, n)] + rng.normal(0.0, 1.0, (n, p))).astype(np.float32)
+    ca.set_precision(args.knn_prec)
+    t0 = time.perf_counter()
+    idx, dist = N.knn_graph(x, args.k)
+    log(f"knn_graph {n} x {p} k={args.k} ({args.knn_prec}) in {time.perf_counter() - t0:.2f}s")
+    if args.calibrate_copy:
+        from crest_amd._lib import check, lib
+        from crest_amd.pipeline import DeviceBuffer
+        nb = args.calibrate_copy << 20
+        a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+        check(lib().crest_dev_copy(b.ptr, a.ptr, nb))
+        log(f"copy kernel: {nb} bytes read, {nb} bytes written")
+    print(json.dumps({"knn_only": True, "n": n, "p": p, "k": args.k, "prec": args.knn_prec}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -146,6 +171,10 @@ def main() -> int:
     ap.add_argument("--cpu-knn-queries", type=int, default=512)
     ap.add_argument("--acc-queries", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
+    ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
+    ap.add_argument("--knn-prec", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="also run one plain device copy of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -153,6 +182,9 @@ def main() -> int:
 
     from crest_amd import synth
     from crest_amd.pipeline import DeviceCSR, Pipeline
+
+    if args.knn_only:
+        return knn_only(args)
 
     t0 = time.perf_counter()
     indptr, indices, data, info = synth.make_counts(args.cells, args.genes, args.clusters, density=args.density,
@@ -171,12 +203,21 @@ def main() -> int:
     out = pipe.run()
     gpu["tot_bf16"] = out.total_variance
 
-    pipe.set_precision("fp32", "fp32")
-    ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
-    log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
-    out32 = pipe.run()
-    gpu.update({"x_fp32": pipe.x_pca(), "idx_fp32": pipe.knn()[0],
-                "var_fp32": out32.variance, "load_fp32": out32.loadings, "tot_fp32": out32.total_variance})
+    if args.fp32_steps > 0:
+        pipe.set_precision("fp32", "fp32")
+        ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
+        log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
+        out32 = pipe.run()
+        gpu.update({"x_fp32": pipe.x_pca(), "idx_fp32": pipe.knn()[0],
+                    "var_fp32": out32.variance, "load_fp32": out32.loadings, "tot_fp32": out32.total_variance})
+    if args.calibrate_copy:
+        # a plain copy of known size: the reference reading for the HBM byte counters
+        from crest_amd._lib import check, lib
+        from crest_amd.pipeline import DeviceBuffer
+        nb = args.calibrate_copy << 20
+        a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+        check(lib().crest_dev_copy(b.ptr, a.ptr, nb))
+        log(f"copy kernel: {nb} bytes read, {nb} bytes written")
 
     n = args.cells
     p = pipe.kp
@@ -194,14 +235,15 @@ def main() -> int:
                    "n_cells": n, "n_genes": args.genes, "nnz": info["nnz"], "n_comps": args.pcs, "k": args.k,
                    "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": "bf16x3"},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-        "fp32": {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
-                 "stages_ms": {k: round(v, 3) for k, v in stages32.items()}},
-        "speedup_vs_fp32": ms32 / ms,
         "roofline": {"kernel": "k_knn_screen_bf16", "bound": "mfma", "achieved": achieved / 1e12,
                      "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16, "traffic": None,
                      "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
                      "launch_ms": stages["knn_screen"]},
     }
+    if args.fp32_steps > 0:
+        line["fp32"] = {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
+                        "stages_ms": {k: round(v, 3) for k, v in stages32.items()}}
+        line["speedup_vs_fp32"] = ms32 / ms
     if not args.no_cpu:
         t0 = time.perf_counter()
         base, acc = cpu_baseline_leg(indptr, indices, data, args.genes, gpu, args)

@@ -324,6 +324,11 @@ __global__ void k_mean_std(const double *stats, const int32_t *gene_of_hvg, int 
     sd[h] = s;
 }
 
+__global__ void k_copy16(uint4 *dst, const uint4 *src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_iota_i64(int64_t *p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = i;
 }
@@ -400,6 +405,17 @@ int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     return guarded([&] {
         Ctx &x = ctx();
         if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, x.st));
+        x.sync();
+    });
+}
+
+int crest_dev_copy(void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        if (bytes % 16) value_error("bytes must be a multiple of 16");
+        Ctx &x = ctx();
+        hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, x.st, (uint4 *)dst, (const uint4 *)src,
+                           (int64_t)(bytes / 16));
+        x.check_launch();
         x.sync();
     });
 }

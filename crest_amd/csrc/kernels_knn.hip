@@ -19,17 +19,14 @@ namespace crest {
 
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 constexpr int kTC = 64;    // candidates per LDS tile
 constexpr int kQW = 64;    // queries per wave (two 32-column MFMA groups)
 constexpr int kQWG = 256;  // queries per workgroup
 constexpr int kPrecFP32 = 1, kPrecBF16X3 = 2;
 
-__device__ inline bf16x8 as_bf16x8(uint4 v) {
-    bf16x8 r;
-    __builtin_memcpy(&r, &v, 16);
-    return r;
-}
+__device__ inline bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ inline uint16_t bf16_rn(float f) {
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
@@ -167,6 +164,24 @@ void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, float
                            (float2 *)xb);
 }
 
+// ---------------------------------------------------------------- tile staging
+// N values carried in registers across a loop iteration: a recursive struct instead
+// of an array, so no indexing ever asks for a stack slot.
+template <class T, int N> struct Regs {
+    T head;
+    Regs<T, N - 1> tail;
+};
+template <class T> struct Regs<T, 0> {};
+
+template <class T, int N> __device__ inline void regs_load(Regs<T, N> &r, const T *src, int i) {
+    r.head = src[i];
+    if constexpr (N > 1) regs_load<T, N - 1>(r.tail, src, i + 256);
+}
+template <class T, int N> __device__ inline void regs_store(const Regs<T, N> &r, T *dst, int i) {
+    dst[i] = r.head;
+    if constexpr (N > 1) regs_store<T, N - 1>(r.tail, dst, i + 256);
+}
+
 // ---------------------------------------------------------------- top-L lists
 // Sorted ascending; a new entry goes after equal ones, so among equal distances the
 // earlier (lower-index) candidate stays ahead, as push_topk does (knn.rs:68-87).
@@ -227,17 +242,17 @@ __device__ inline void write_lists(const float (&ld)[L], const uint32_t (&li)[L]
 
 // ---------------------------------------------------------------- screen, split-BF16
 template <int KC, int L>
-__global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict__ xa, const uint4 *__restrict__ xb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 && KC <= 24) ? 2 : 1))) void k_knn_screen_bf16(const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb,
                                                          int64_t n_pad, float *cand_d, uint32_t *cand_i) {
     constexpr int KS = KC / 2;  // MFMA K-steps of 16
     constexpr int TU = KC * kTC;  // 16-byte units per tile
     static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
     constexpr int PER = TU / 256;
-    __shared__ uint4 lds[2][TU];
+    __shared__ u32x4 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
     const int64_t q0 = (int64_t)blockIdx.x * kQWG + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
-    const uint4 *qbase = xb + (q0 / 64) * KC * 64;
+    const u32x4 *qbase = xb + (q0 / 64) * KC * 64;
     bf16x8 b0[KS], b1[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -252,19 +267,16 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
     const int64_t n_tiles = n_pad / kTC;
+    // Pipeline: LDS holds tile t (being computed) and receives tile t+1 from
+    // registers loaded one iteration earlier; the loads for tile t+2 are issued after
+    // that store, so their latency overlaps the whole next iteration.  (The final
+    // iterations re-load the last tile; those copies are never read.)
     for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
+    Regs<u32x4, PER> pre;
+    regs_load<u32x4, PER>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
-        // prefetch the next tile (the last iteration re-reads the final tile; its LDS
-        // copy is never read), unconditionally so the registers stay registers
-        const int64_t tn = t + 1 < n_tiles ? t + 1 : t;
-        uint4 pre[PER];
-        {
-            const uint4 *src = xa + tn * TU;
-#pragma unroll
-            for (int u = 0; u < PER; ++u) pre[u] = src[threadIdx.x + u * 256];
-        }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             f32x16 a0 = {}, a1 = {};
@@ -282,8 +294,9 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) lds[buf ^ 1][threadIdx.x + u * 256] = pre[u];
+        regs_store<u32x4, PER>(pre, lds[buf ^ 1], threadIdx.x);
+        const int64_t tn = t + 2 < n_tiles ? t + 2 : n_tiles - 1;
+        regs_load<u32x4, PER>(pre, xa + tn * TU, threadIdx.x);
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -294,7 +307,7 @@ __global__ __launch_bounds__(256) void k_knn_screen_bf16(const uint4 *__restrict
 // v_mfma_f32_32x32x2_f32: per 16-byte chunk, lane half h supplies floats 2h, 2h+1 in
 // two MFMAs, so the chunk's four K values are covered by the two half-waves.
 template <int KC, int L>
-__global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict__ xa, const float2 *__restrict__ xb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 && KC <= 16) ? 2 : 1))) void k_knn_screen_f32(const float2 *__restrict__ xa, const float2 *__restrict__ xb,
                                                         int64_t n_pad, float *cand_d, uint32_t *cand_i) {
     constexpr int TU = KC * 2 * kTC;  // float2 units per tile
     static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
@@ -318,19 +331,13 @@ __global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
     const int64_t n_tiles = n_pad / kTC;
+    // same loop-carried prefetch as the split-BF16 screen
     for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
+    Regs<float2, PER> pre;
+    regs_load<float2, PER>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
-        // prefetch the next tile (the last iteration re-reads the final tile; its LDS
-        // copy is never read), unconditionally so the registers stay registers
-        const int64_t tn = t + 1 < n_tiles ? t + 1 : t;
-        float2 pre[PER];
-        {
-            const float2 *src = xa + tn * TU;
-#pragma unroll
-            for (int u = 0; u < PER; ++u) pre[u] = src[threadIdx.x + u * 256];
-        }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             f32x16 a0 = {}, a1 = {};
@@ -350,8 +357,9 @@ __global__ __launch_bounds__(256) void k_knn_screen_f32(const float2 *__restrict
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) lds[buf ^ 1][threadIdx.x + u * 256] = pre[u];
+        regs_store<float2, PER>(pre, lds[buf ^ 1], threadIdx.x);
+        const int64_t tn = t + 2 < n_tiles ? t + 2 : n_tiles - 1;
+        regs_load<float2, PER>(pre, xa + tn * TU, threadIdx.x);
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -364,7 +372,7 @@ static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t n_
     const dim3 grid((unsigned)(n_pad / kQWG)), block(256);
 #define CREST_BF(KCV)                                                                                             \
     case KCV:                                                                                                     \
-        hipLaunchKernelGGL((k_knn_screen_bf16<KCV, L>), grid, block, 0, s, (const uint4 *)xa, (const uint4 *)xb, \
+        hipLaunchKernelGGL((k_knn_screen_bf16<KCV, L>), grid, block, 0, s, (const u32x4 *)xa, (const u32x4 *)xb, \
                            n_pad, cd, ci);                                                                        \
         return true;
 #define CREST_F(KCV)                                                                                                \

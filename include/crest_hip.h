@@ -180,6 +180,9 @@ int crest_dev_free(void *ptr);
 int crest_memcpy_htod(void *dst, const void *src, size_t bytes);
 int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int crest_synchronize(void);
+/* Device-to-device copy by a plain vectorised kernel (k_copy16): the reference point
+ * for calibrating HBM byte counters.  bytes must be a multiple of 16.            */
+int crest_dev_copy(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }
