@@ -228,20 +228,33 @@ __global__ __launch_bounds__(256) void k_project(DevChunk c, DevScale sc, const 
                     }
                 }
             }
+            // Four entries at a time: their V rows are loaded before any of the four
+            // multiply-adds.  An entry outside the Gram contributes an explicit +0.0,
+            // which leaves the sum unchanged (it starts at +0.0 and, rounding to
+            // nearest, never becomes -0.0), so the result equals skipping it.
             const int cnt = hi - e0 < kWave ? (int)(hi - e0) : kWave;
-            for (int j = 0; j < cnt; ++j) {
-                const int32_t hj = __shfl(h, j, kWave);
-                if (hj < 0) continue;
-                const double sj = __shfl(s, j, kWave);
-                const double *vr = V + (int64_t)hj * k;
+            for (int j0 = 0; j0 < cnt; j0 += 4) {
+                int32_t hj[4];
+                double sj[4], vj[4][NC];
 #pragma unroll
-                for (int q = 0; q < NC; ++q) {
-                    const int col = lane + q * kWave;
-                    if (col < k) {
-                        const double prod = sj * vr[col];
-                        acc[q] = acc[q] + prod;
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u < cnt ? j0 + u : cnt - 1;
+                    hj[u] = j0 + u < cnt ? __shfl(h, j, kWave) : -1;
+                    sj[u] = __shfl(s, j, kWave);
+                    const double *vr = V + (int64_t)(hj[u] >= 0 ? hj[u] : 0) * k;
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) {
+                        const int col = lane + q * kWave;
+                        vj[u][q] = col < k ? vr[col] : 0.0;
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) {
+                        const double prod = hj[u] >= 0 ? sj[u] * vj[u][q] : 0.0;
+                        acc[q] = acc[q] + prod;
+                    }
             }
         }
 #pragma unroll

@@ -286,20 +286,27 @@ __global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g,
         cov[e] = u / (n - 1.0);
     }
 }
-// total = Σ_i cov[i, i] in index order (kernels.rs:323).
-__global__ void k_trace(const double *cov, int d, double *total) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < d; ++i) s += cov[(int64_t)i * d + i];
-        *total = s;
+// total = Σ_i cov[i, i] in index order (kernels.rs:323): the strided diagonal is
+// gathered into LDS by the whole block, then summed sequentially by one thread.
+__global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double *total) {
+    __shared__ double diag[4096];
+    double s = 0.0;
+    for (int i0 = 0; i0 < d; i0 += 4096) {
+        const int m = d - i0 < 4096 ? d - i0 : 4096;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) diag[i] = cov[(int64_t)(i0 + i) * d + i0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; ++i) s += diag[i];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *total = s;
 }
 
 void launch_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov, double *total,
                 hipStream_t s) {
     const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 8192);
     hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov);
-    hipLaunchKernelGGL(k_trace, dim3(1), dim3(64), 0, s, cov, d, total);
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, cov, d, total);
 }
 
 // ---------------------------------------------------------------- eigen selection
@@ -354,17 +361,26 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
 }
 
 // ---------------------------------------------------------------- shift prefill
-// shift = (colsum / n) @ V (tl.py:76), then X[:] = -(f32)shift (tl.py:77-78).
-__global__ void k_shift(const double *colsum, const double *V, int d, int k, double n, float *shift_f) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
+// shift = (colsum / n) @ V (tl.py:76), one block per component (a matrix-vector
+// product, so the summation order is the block reduction's), then
+// X[:] = -(f32)shift (tl.py:77-78).
+__global__ __launch_bounds__(256) void k_shift(const double *colsum, const double *V, int d, int k, double n,
+                                               float *shift_f) {
+    __shared__ double part[256];
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int a = 0; a < d; ++a) {
+    for (int a = threadIdx.x; a < d; a += blockDim.x) {
         const double m = colsum[a] / n;
         const double p = m * V[(int64_t)a * k + c];
         s = s + p;
     }
-    shift_f[c] = -(float)s;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) shift_f[c] = -(float)part[0];
 }
 __global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x) {
     const int64_t total = n_rows * k;
@@ -374,7 +390,7 @@ __global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *
 
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows, float *x,
                        float *shift_f, hipStream_t s) {
-    hipLaunchKernelGGL(k_shift, dim3((k + 63) / 64), dim3(64), 0, s, colsum, V, d, k, n, shift_f);
+    hipLaunchKernelGGL(k_shift, dim3(k), dim3(256), 0, s, colsum, V, d, k, n, shift_f);
     const int g = (int)std::min<int64_t>(ceil_div(n_rows * k, 256), 8192);
     hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x);
 }
