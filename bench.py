@@ -24,7 +24,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "cells/s scale+PCA+kNN @1M×2k HVG: split-BF16 vs FP32 at fixed kNN agreement"
-PEAK_BF16 = 2.5e15      # MI355X dense BF16 MFMA, FLOP/s
+PEAK_BF16 = 2.5e15      # MI355X dense BF16 / FP16 MFMA, FLOP/s
+PEAK_FP32 = 157.3e12    # MI355X dense FP32 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32 MFMA, FLOP/s
 
 
@@ -122,6 +123,11 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
         acc[f"pca_angle_all_{tag}"] = O.principal_angle(L, load_o)
         acc[f"variance_ratio_err_{tag}"] = float(np.max(np.abs(gpu[f"var_{tag}"] / gpu[f"tot_{tag}"]
                                                                - var_o / tot_o)))
+        if tag == "bf16" and "idx_alt" in gpu:  # the alternate screen on the same scores
+            got = gpu["idx_alt"][qs]
+            bad, ties = O.knn_mismatch_ties(got, Xp, qs, ref_i, ref_d2)
+            acc["knn_set_agreement_alt_screen"] = O.knn_set_agreement(got, ref_i)
+            acc["knn_mismatch_alt_screen"] = [int(bad), int(ties)]
     acc["eig_gap_p"] = float(var_o[args.pcs - 1] - (np.linalg.eigvalsh(O.cov_from_gram(gram, colsum, n)[0])[::-1]
                                                      [args.pcs]))
     acc["queries"] = int(args.acc_queries)
@@ -138,7 +144,7 @@ def knn_only(args) -> int:
     n, p = args.cells, args.pcs
     centers = rng.normal(0.0, 6.0, (args.clusters, p))
     x = (centers[rng.integers(0, args.clusters, n)] + rng.normal(0.0, 1.0, (n, p))).astype(np.float32)
-    ca.set_precision(args.knn_prec)
+    ca.set_knn_precision(args.knn_prec)
     t0 = time.perf_counter()
     idx, dist = N.knn_graph(x, args.k)
     log(f"knn_graph {n} x {p} k={args.k} ({args.knn_prec}) in {time.perf_counter() - t0:.2f}s")
@@ -172,7 +178,10 @@ def main() -> int:
     ap.add_argument("--acc-queries", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
-    ap.add_argument("--knn-prec", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
+                    help="kNN screen of the headline leg (the Gram is split-BF16)")
+    ap.add_argument("--alt-screen", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
+                    help="also time one step with this screen (reported, not the headline)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -193,15 +202,25 @@ def main() -> int:
         f"in {time.perf_counter() - t0:.1f}s")
     csr = DeviceCSR.upload(indptr, indices, data, args.genes)
     hvg = np.arange(args.genes, dtype=np.int32)  # the 2,000 genes are the pre-selected HVGs
-    pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram="bf16x3", prec_knn="bf16x3",
+    pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram="bf16x3", prec_knn=args.knn_prec,
                     chunk_rows=args.chunk_rows)
 
     ms, stages = timed_steps(pipe, args.steps, args.warmup)
-    log(f"split-BF16: {ms:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
+    log(f"split-BF16 Gram + {args.knn_prec} screen: {ms:.1f} ms/step, "
+        f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
     gpu = {"x_bf16": pipe.x_pca(), "idx_bf16": pipe.knn()[0],
            "var_bf16": pipe.variance[:pipe.kp].copy(), "load_bf16": pipe.loadings[:, :pipe.kp].copy()}
     out = pipe.run()
     gpu["tot_bf16"] = out.total_variance
+    alt = None
+    if args.alt_screen not in ("none", args.knn_prec):
+        pipe.set_precision("bf16x3", args.alt_screen)
+        ms_alt, stages_alt = timed_steps(pipe, 1, 0)
+        log(f"split-BF16 Gram + {args.alt_screen} screen: {ms_alt:.1f} ms/step, "
+            f"stages {json.dumps({k: round(v, 2) for k, v in stages_alt.items()})}")
+        gpu["idx_alt"] = pipe.knn()[0]
+        alt = {"prec_knn": args.alt_screen, "ms_per_step": ms_alt,
+               "stages_ms": {k: round(v, 3) for k, v in stages_alt.items()}}
 
     if args.fp32_steps > 0:
         pipe.set_precision("fp32", "fp32")
@@ -224,22 +243,29 @@ def main() -> int:
     screen_s = stages["knn_screen"] * 1e-3
     knn_flops = 2.0 * n * n * p
     achieved = knn_flops / screen_s
+    peak = PEAK_FP32 if args.knn_prec == "fp32" else PEAK_BF16
     line = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "split-bf16 MFMA (f32 accumulate) + f64",
+        "vs_baseline": None,
+        "dtype": f"split-bf16 Gram + {args.knn_prec} kNN screen (MFMA, f32 accumulate); f64 centring, eigensolve, "
+                 "projection and kNN rerank",
         "data": "synthetic (seeded NB counts, crest_amd.synth)",
         "config": {"workload": (f"{n} cells x {args.genes} pre-selected HVGs, NB counts density "
                                 f"{info['density']:.4f}, {args.clusters} clusters, seed {args.seed}; normalize_total 1e4,"
                                 f" log1p, scale max_value 10, PCA {args.pcs}, exact kNN k={args.k}"),
                    "n_cells": n, "n_genes": args.genes, "nnz": info["nnz"], "n_comps": args.pcs, "k": args.k,
-                   "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": "bf16x3"},
+                   "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": args.knn_prec},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-        "roofline": {"kernel": "k_knn_screen_bf16", "bound": "mfma", "achieved": achieved / 1e12,
-                     "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16, "traffic": None,
+        "roofline": {"kernel": {"fp16": "k_knn_screen16<1,...>", "bf16x3": "k_knn_screen16<0,...>",
+                                "fp32": "k_knn_screen_f32"}[args.knn_prec],
+                     "bound": "mfma", "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
                      "launch_ms": stages["knn_screen"]},
     }
+    if alt is not None:
+        line["alt_screen"] = alt
     if args.fp32_steps > 0:
         line["fp32"] = {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
                         "stages_ms": {k: round(v, 3) for k, v in stages32.items()}}

@@ -5,8 +5,8 @@ reference; the per-entry and dense work runs in hand-written HIP kernels of
 libcrest_hip.so, reached through a C ABI (include/crest_hip.h).
 """
 from . import pp, tl
-from ._lib import get_precision, set_precision
+from ._lib import get_knn_precision, get_precision, set_knn_precision, set_precision
 from .core import DEFAULT_CHUNK_NNZ, BioFrame, CSRStore, Frame, TripletStore
 
 __all__ = ["pp", "tl", "BioFrame", "CSRStore", "TripletStore", "Frame", "DEFAULT_CHUNK_NNZ",
-           "set_precision", "get_precision"]
+           "set_precision", "get_precision", "set_knn_precision", "get_knn_precision"]

@@ -12,8 +12,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libcrest_hip.so"
 
 CREST_OK, CREST_E_VALUE, CREST_E_RUNTIME, CREST_E_DEVICE = 0, -1, -2, -3
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
-PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2, 3
+PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
 
 
 class Chunk(C.Structure):
@@ -58,6 +58,8 @@ _SIGS = {
     "crest_set_device": (C.c_int, [C.c_int32]),
     "crest_set_precision": (C.c_int, [C.c_int32]),
     "crest_get_precision": (C.c_int32, []),
+    "crest_set_knn_precision": (C.c_int, [C.c_int32]),
+    "crest_get_knn_precision": (C.c_int32, []),
     "crest_last_error": (C.c_char_p, []),
     "crest_device_count": (C.c_int32, []),
     "crest_release": (C.c_int, []),
@@ -120,4 +122,16 @@ def set_precision(prec) -> None:
 
 def get_precision() -> str:
     p = lib().crest_get_precision()
+    return {v: k for k, v in PREC_NAMES.items()}[p]
+
+
+def set_knn_precision(prec) -> None:
+    """Arithmetic of the kNN distance screen alone: "fp64"/"fp32" (FP32 screen),
+    "bf16x3", "fp16", or None to follow set_precision."""
+    p = -1 if prec is None else (PREC_NAMES[prec] if isinstance(prec, str) else int(prec))
+    check(lib().crest_set_knn_precision(p))
+
+
+def get_knn_precision() -> str:
+    p = lib().crest_get_knn_precision()
     return {v: k for k, v in PREC_NAMES.items()}[p]

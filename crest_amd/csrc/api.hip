@@ -21,6 +21,7 @@ namespace crest {
 static thread_local std::string g_err;
 static thread_local int g_device = 0;
 static std::atomic<int> g_prec{CREST_PREC_FP64};
+static std::atomic<int> g_knn_prec{-1};  // -1: follow g_prec
 
 struct Fail {
     int code;
@@ -283,19 +284,22 @@ static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *d
 // kNN of the rows of dev_x [n, p] into dev_idx/dev_dist [n, k].
 static void knn_device(Ctx &x, const float *dev_x, int64_t n, int p, int k, int prec, uint32_t *dev_idx,
                        float *dev_dist, hipEvent_t *ev3) {
-    const int kp = prec == CREST_PREC_BF16X3 ? CREST_PREC_BF16X3 : CREST_PREC_FP32;
+    const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
     const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
     if (k > 64) value_error("k > 64 is not supported by the exact GPU kNN");
     const int kpad = knn_kpad(kp, p);
-    if (kpad < 0) value_error("data has too many columns for the GPU kNN (at most 128)");
-    const int64_t n_pad = round_up(n, 256);
-    const size_t eb = kp == CREST_PREC_BF16X3 ? 2 : 4;
+    if (kpad < 0)
+        value_error("data has too many columns for the GPU kNN (at most 134 for fp32/fp64, 137 for bf16x3, "
+                    "252 for fp16)");
+    const int64_t n_pad = knn_rows_padded(n);
+    const size_t eb = kp == CREST_PREC_FP32 ? 4 : 2;
     void *xa = x.get<uint8_t>("knn_xa", (size_t)n_pad * kpad * eb);
     void *xb = x.get<uint8_t>("knn_xb", (size_t)n_pad * kpad * eb);
+    void *work = x.get<uint8_t>("knn_work", knn_work_bytes(p));
     float *cd = x.get<float>("knn_cd", (size_t)n_pad * 2 * L);
     uint32_t *ci = x.get<uint32_t>("knn_ci", (size_t)n_pad * 2 * L);
     if (ev3) HIPCHK(hipEventRecord(ev3[0], x.st));
-    launch_knn_prep(kp, dev_x, n, p, kpad, nullptr, xa, xb, x.st);
+    launch_knn_prep(kp, dev_x, n, p, kpad, work, xa, xb, x.st);
     if (ev3) HIPCHK(hipEventRecord(ev3[1], x.st));
     if (!launch_knn_screen(kp, xa, xb, n, kpad, L, cd, ci, x.st)) value_error("no kNN kernel for this width");
     if (ev3) HIPCHK(hipEventRecord(ev3[2], x.st));
@@ -352,6 +356,7 @@ int crest_set_device(int32_t device) {
 
 int crest_set_precision(int32_t prec) {
     return guarded([&] {
+        if (prec == CREST_PREC_FP16) value_error("FP16 applies to the kNN screen only (crest_set_knn_precision)");
         if (prec != CREST_PREC_FP64 && prec != CREST_PREC_FP32 && prec != CREST_PREC_BF16X3)
             value_error("unknown precision");
         g_prec = prec;
@@ -359,6 +364,20 @@ int crest_set_precision(int32_t prec) {
 }
 
 int32_t crest_get_precision(void) { return g_prec; }
+
+int crest_set_knn_precision(int32_t prec) {
+    return guarded([&] {
+        if (prec != -1 && prec != CREST_PREC_FP64 && prec != CREST_PREC_FP32 && prec != CREST_PREC_BF16X3 &&
+            prec != CREST_PREC_FP16)
+            value_error("unknown kNN screen precision");
+        g_knn_prec = prec;
+    });
+}
+
+int32_t crest_get_knn_precision(void) {
+    const int p = g_knn_prec;
+    return p < 0 ? (int32_t)g_prec : p;
+}
 
 const char *crest_last_error(void) { return g_err.c_str(); }
 
@@ -566,7 +585,7 @@ int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t e
         const float *dx = x.upload("knn_x", data, n * dim);
         uint32_t *di = x.get<uint32_t>("knn_idx", n * kk);
         float *dd = x.get<float>("knn_dist", n * kk);
-        knn_device(x, dx, (int64_t)n, (int)dim, (int)kk, g_prec, di, dd, nullptr);
+        knn_device(x, dx, (int64_t)n, (int)dim, (int)kk, crest_get_knn_precision(), di, dd, nullptr);
         x.download(indices, di, n * kk);
         x.download(distances, dd, n * kk);
         x.sync();
@@ -654,6 +673,11 @@ int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *l
         if (!a->dev_indptr || !a->dev_genes || !a->dev_values || !a->hvg_of_gene || !a->dev_x_pca ||
             !a->dev_knn_idx || !a->dev_knn_dist)
             value_error("NULL argument");
+        if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3)
+            value_error("unknown Gram precision");
+        if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
+            a->prec_knn != CREST_PREC_FP16)
+            value_error("unknown kNN screen precision");
         const int kp = (int)std::min<int64_t>({(int64_t)a->n_comps, (int64_t)d, n - 1});
         if (kp > 256) value_error("n_comps > 256 is not supported");
         const int kn = (int)std::max<int64_t>(std::min<int64_t>((int64_t)a->k, n - 1), 1);

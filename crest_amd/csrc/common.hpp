@@ -101,7 +101,11 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
                        double *loadings, hipStream_t s);
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
                        float *x, float *shift_scratch, hipStream_t s);
-void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, float *norms, void *xa,
+// kNN screen operands: rows are padded to knn_rows_padded(n); `work` holds
+// knn_work_bytes(p) bytes (column means of the centring and the FP16 prescale).
+int64_t knn_rows_padded(int64_t n);
+size_t knn_work_bytes(int p);
+void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, void *work, void *xa,
                      void *xb, hipStream_t s);
 int knn_kpad(int prec, int p);
 bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int kpad, int L,

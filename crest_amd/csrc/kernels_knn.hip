@@ -2,31 +2,36 @@
 // best L candidates seen by each of its two lanes, then an FP64 rerank picks the k
 // nearest and reports the reference's FP32 Euclidean distances.
 //
-// Distances come straight out of the MFMA: operands are augmented with the squared
-// norms so that A_c . B_q = |c|^2 + |q|^2 - 2 c.q (the GEMM expansion of knn.rs:126):
-//   FP32      A_c = [x_c | |c|^2, 1 | 0..]          B_q = [-2 x_q | 1, |q|^2 | 0..]
+// The screen works on centred rows x' = x - mean (distances are translation
+// invariant; centring keeps the GEMM expansion well conditioned).  Distances come
+// straight out of the MFMA: operands are augmented with the squared norms so that
+// A_c . B_q = |c|^2 + |q|^2 - 2 c.q (the GEMM expansion of knn.rs:126):
+//   FP32       A_c = [x_c | |c|^2, 1 | 0..]          B_q = [-2 x_q | 1, |q|^2 | 0..]
 //   split-BF16 A_c = [h | h | l | nh, nl, 1, 1 | 0..]  B_q = [-2h | -2l | -2h | 1, 1, qh, ql | 0..]
-// with x = h + l (bf16 hi/lo) so that A.B = -2(hh + hl + lh) + norms.  Padding
-// candidates carry |c|^2 = +inf and never enter a list.
+//   FP16       A_c = [s x_c | nh, nl, 1, 1 | 0..]      B_q = [-2 s x_q | 1, 1, qh, ql | 0..]
+// with x = h + l (bf16 hi/lo) so that A.B = -2(hh + hl + lh) + norms, and for FP16
+// a power-of-two prescale s that brings the largest squared norm to at most 2^14
+// (exact, and ranking-neutral), norms split into fp16 hi + lo.  Padding candidates
+// carry |c|^2 = +inf and never enter a list.
 //
 // Operands are stored "fragment-major": 16-byte units of row i, chunk j at
-// ((i/64)*KC + j)*64 + i%64 (bf16; FP32 splits each chunk into two float2 halves),
-// so one 64-row tile is one contiguous block that a workgroup copies into LDS and
-// each MFMA fragment read from LDS is a contiguous 512-byte run per half-wave.
+// ((i/64)*KC + j)*64 + i%64 (16-bit types; FP32 splits each chunk into two float2
+// halves), so one 64-row tile is one contiguous block that a workgroup copies into
+// LDS and each MFMA fragment read from LDS is a contiguous 512-byte run per half-wave.
 #include "common.hpp"
 
 namespace crest {
 
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 constexpr int kTC = 64;    // candidates per LDS tile
 constexpr int kQW = 64;    // queries per wave (two 32-column MFMA groups)
-constexpr int kQWG = 256;  // queries per workgroup
-constexpr int kPrecFP32 = 1, kPrecBF16X3 = 2;
+constexpr int kPadRows = 512;  // n is padded to a multiple of the largest workgroup's queries
+constexpr int kPrecFP32 = 1, kPrecBF16X3 = 2, kPrecFP16 = 3;
 
-__device__ inline bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ inline uint16_t bf16_rn(float f) {
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
@@ -37,10 +42,16 @@ __device__ inline float bf16_f(uint16_t b) { return __uint_as_float((uint32_t)b 
 // Instantiated operand widths (16-byte chunks per row).
 static const int kBf16Chunks[] = {4, 8, 12, 16, 20, 24, 32, 52};  // KC = 2*KSTEPS
 static const int kF32Chunks[] = {2, 4, 8, 14, 16, 24, 34};         // KC (even), 4 floats each
+static const int kF16Chunks[] = {4, 8, 16, 24, 32};                 // KC = 2*KSTEPS
 
 int knn_kpad(int prec, int p) {
-    if (prec == kPrecBF16X3) {
-        const int need = (3 * p + 4 + 7) / 8;
+    if (prec == kPrecBF16X3 || prec == kPrecFP16) {
+        const int need = prec == kPrecFP16 ? (p + 4 + 7) / 8 : (3 * p + 4 + 7) / 8;
+        if (prec == kPrecFP16) {
+            for (int kc : kF16Chunks)
+                if (kc >= need) return kc * 8;
+            return -1;
+        }
         for (int kc : kBf16Chunks)
             if (kc >= need) return kc * 8;
         return -1;
@@ -52,24 +63,97 @@ int knn_kpad(int prec, int p) {
 }
 
 // ---------------------------------------------------------------- prep
-// norms[i] = Σ x² in f32, sequentially (knn.rs:346); then the augmented operands,
-// one 16-byte chunk at a time so that a wave (64 consecutive rows) stores 1 KB runs.
-__device__ inline float row_norm(const float *r, int p) {
+// Column means (deterministic two-stage FP64 sum), then the centred rows' squared
+// norms Σ x'² in f32, summed sequentially as knn.rs:346 does, and the augmented
+// operands, one 16-byte chunk at a time so that a wave (64 consecutive rows) stores
+// 1 KB runs.  The work buffer holds kColParts*p partial sums, the p means and the
+// largest centred norm (as its bit pattern; norms are >= 0, so integer order is
+// float order).
+constexpr int kColParts = 512;
+
+size_t knn_work_bytes(int p) { return (size_t)kColParts * p * sizeof(double) + (size_t)p * sizeof(float) + 64; }
+
+struct KnnWork {
+    double *part;
+    float *mean;
+    uint32_t *maxbits;
+};
+static KnnWork knn_work(void *w, int p) {
+    KnnWork k;
+    k.part = (double *)w;
+    k.mean = (float *)(k.part + (size_t)kColParts * p);
+    k.maxbits = (uint32_t *)(k.mean + p);
+    return k;
+}
+
+// Block b sums rows [b*rows_per, (b+1)*rows_per): thread t takes column t % p of
+// every (256/p)-th row, and the row groups are combined in order through LDS.
+__global__ __launch_bounds__(256) void k_knn_colsum(const float *x, int64_t n, int p, int64_t rows_per,
+                                                    double *part) {
+    __shared__ double red[256];
+    const int g = 256 / p, t = threadIdx.x, f = t % p, j = t / p;
+    const int64_t r0 = blockIdx.x * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    double acc = 0.0;
+    if (j < g)
+        for (int64_t r = r0 + j; r < r1; r += g) acc += (double)x[r * p + f];
+    red[t] = acc;
+    __syncthreads();
+    if (t < p) {
+        double s = 0.0;
+        for (int jj = 0; jj < g; ++jj) s += red[jj * p + t];
+        part[(int64_t)blockIdx.x * p + t] = s;
+    }
+}
+
+__global__ void k_knn_colmean(const double *part, int nparts, int64_t n, int p, float *mean, uint32_t *maxbits) {
+    for (int f = threadIdx.x; f < p; f += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * p + f];
+        mean[f] = (float)(s / (double)n);
+    }
+    if (threadIdx.x == 0) *maxbits = 0u;
+}
+
+__device__ inline float centred_norm(const float *r, const float *mean, int p, float s) {
     float nn = 0.0f;
     for (int f = 0; f < p; ++f) {
-        const float sq = r[f] * r[f];
+        const float v = (r[f] - mean[f]) * s;
+        const float sq = v * v;
         nn = nn + sq;
     }
     return nn;
 }
 
-// Element k of the split-BF16 operands of a row with values r[0..p) and norm nn.
-__device__ inline void bf16_elem(const float *r, int p, float nn, int k, uint16_t &a, uint16_t &b) {
+__global__ __launch_bounds__(256) void k_knn_maxnorm(const float *x, int64_t n, int p, const float *mean,
+                                                     uint32_t *maxbits) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    float nn = i < n ? centred_norm(x + i * p, mean, p, 1.0f) : 0.0f;
+    for (int o = 32; o > 0; o >>= 1) nn = fmaxf(nn, __shfl_xor(nn, o, 64));
+    if ((threadIdx.x & 63) == 0 && nn > 0.0f) atomicMax(maxbits, __float_as_uint(nn));
+}
+
+// Power-of-two s with s^2 * max|x'|^2 <= 2^14, so every |s x'| <= 128 and every
+// fp16 norm stays far below the fp16 maximum (65504).
+__device__ inline float fp16_prescale(const uint32_t *maxbits) {
+    const float mx = __uint_as_float(*maxbits);
+    if (!(mx > 0.0f) || !(mx <= 3.0e38f)) return 1.0f;
+    int ex;
+    (void)frexpf(mx, &ex);  // mx < 2^ex
+    int e = 14 - ex;
+    e = e >= 0 ? e / 2 : -((1 - e) / 2);  // floor(e / 2)
+    e = e < -60 ? -60 : (e > 60 ? 60 : e);
+    return ldexpf(1.0f, e);
+}
+
+// Element k of the split-BF16 operands of a centred row r' (values r - mean) with
+// norm nn.
+__device__ inline void bf16_elem(const float *r, const float *mean, int p, float nn, int k, uint16_t &a,
+                                 uint16_t &b) {
     a = 0;
     b = 0;
     if (k < 3 * p) {
         const int seg = k / p, f = k - seg * p;
-        const float v = r[f];
+        const float v = r[f] - mean[f];
         const uint16_t h = bf16_rn(v);
         const uint16_t l = bf16_rn(v - bf16_f(h));
         a = seg == 2 ? l : h;
@@ -83,20 +167,19 @@ __device__ inline void bf16_elem(const float *r, int p, float nn, int k, uint16_
     }
 }
 
-__global__ void k_knn_prep_bf16(const float *x, int64_t n, int64_t n_pad, int p, int kc, float *norms, uint4 *xa,
-                                uint4 *xb) {
+__global__ void k_knn_prep_bf16(const float *x, const float *mean, int64_t n, int64_t n_pad, int p, int kc,
+                                uint4 *xa, uint4 *xb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
     const bool real = i < n;
     const float *r = x + (real ? i : 0) * (int64_t)p;
-    const float nn = real ? row_norm(r, p) : INFINITY;
-    if (norms) norms[i] = nn;
+    const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j) {
         uint16_t ea[8], eb[8];
         for (int e = 0; e < 8; ++e) {
             const int k = 8 * j + e;
             if (real) {
-                bf16_elem(r, p, nn, k, ea[e], eb[e]);
+                bf16_elem(r, mean, p, nn, k, ea[e], eb[e]);
             } else {  // padding candidate: +inf norm, never a neighbour
                 const int t = k - 3 * p;
                 ea[e] = t == 0 ? (uint16_t)0x7F80 : ((t == 2 || t == 3) ? (uint16_t)0x3F80 : (uint16_t)0);
@@ -112,12 +195,51 @@ __global__ void k_knn_prep_bf16(const float *x, int64_t n, int64_t n_pad, int p,
     }
 }
 
-__device__ inline void f32_elem(const float *r, int p, float nn, int k, float &a, float &b) {
+// FP16 operands of a centred, prescaled row; fp16 conversions round to nearest even.
+__global__ void k_knn_prep_f16(const float *x, const float *mean, const uint32_t *maxbits, int64_t n, int64_t n_pad,
+                               int p, int kc, uint4 *xa, uint4 *xb) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    const bool real = i < n;
+    const float s = fp16_prescale(maxbits);
+    const float *r = x + (real ? i : 0) * (int64_t)p;
+    const float nn = real ? centred_norm(r, mean, p, s) : INFINITY;
+    const _Float16 nh = (_Float16)nn;
+    const _Float16 nl = real ? (_Float16)(nn - (float)nh) : (_Float16)0.0f;
+    const _Float16 one = (_Float16)1.0f, zero = (_Float16)0.0f;
+    for (int j = 0; j < kc; ++j) {
+        _Float16 ea[8], eb[8];
+        for (int e = 0; e < 8; ++e) {
+            const int k = 8 * j + e;
+            ea[e] = eb[e] = zero;
+            if (k < p) {
+                if (real) {
+                    const _Float16 h = (_Float16)((r[k] - mean[k]) * s);
+                    ea[e] = h;
+                    eb[e] = (_Float16)(-2.0f * (float)h);
+                }
+            } else if (k < p + 4) {
+                const int t = k - p;
+                ea[e] = t == 0 ? nh : (t == 1 ? nl : one);
+                eb[e] = real ? (t == 2 ? nh : (t == 3 ? nl : one)) : zero;
+            }
+        }
+        uint4 va, vb;
+        __builtin_memcpy(&va, ea, 16);
+        __builtin_memcpy(&vb, eb, 16);
+        const int64_t u = ((i / 64) * kc + j) * 64 + (i % 64);
+        xa[u] = va;
+        xb[u] = vb;
+    }
+}
+
+__device__ inline void f32_elem(const float *r, const float *mean, int p, float nn, int k, float &a, float &b) {
     a = 0.0f;
     b = 0.0f;
     if (k < p) {
-        a = r[k];
-        b = -2.0f * r[k];
+        const float v = r[k] - mean[k];
+        a = v;
+        b = -2.0f * v;
     } else if (k == p) {
         a = nn;
         b = 1.0f;
@@ -127,20 +249,19 @@ __device__ inline void f32_elem(const float *r, int p, float nn, int k, float &a
     }
 }
 
-__global__ void k_knn_prep_f32(const float *x, int64_t n, int64_t n_pad, int p, int kc, float *norms, float2 *xa,
-                               float2 *xb) {
+__global__ void k_knn_prep_f32(const float *x, const float *mean, int64_t n, int64_t n_pad, int p, int kc,
+                               float2 *xa, float2 *xb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
     const bool real = i < n;
     const float *r = x + (real ? i : 0) * (int64_t)p;
-    const float nn = real ? row_norm(r, p) : INFINITY;
-    if (norms) norms[i] = nn;
+    const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j)
         for (int h = 0; h < 2; ++h) {
             float a0, a1, b0, b1;
             if (real) {
-                f32_elem(r, p, nn, 4 * j + 2 * h, a0, b0);
-                f32_elem(r, p, nn, 4 * j + 2 * h + 1, a1, b1);
+                f32_elem(r, mean, p, nn, 4 * j + 2 * h, a0, b0);
+                f32_elem(r, mean, p, nn, 4 * j + 2 * h + 1, a1, b1);
             } else {
                 a0 = (4 * j + 2 * h == p) ? INFINITY : 0.0f;
                 a1 = (4 * j + 2 * h + 1 == p) ? INFINITY : 0.0f;
@@ -152,34 +273,48 @@ __global__ void k_knn_prep_f32(const float *x, int64_t n, int64_t n_pad, int p, 
         }
 }
 
-void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, float *norms, void *xa, void *xb,
+int64_t knn_rows_padded(int64_t n) { return round_up(n, kPadRows); }
+
+void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, void *work, void *xa, void *xb,
                      hipStream_t s) {
-    const int64_t n_pad = round_up(n, kQWG);
+    const KnnWork w = knn_work(work, p);
+    const int64_t n_pad = knn_rows_padded(n);
+    const int64_t rows_per = ceil_div(n, kColParts);
+    const int nparts = (int)ceil_div(n, rows_per);
+    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, x, n, p, rows_per, w.part);
+    hipLaunchKernelGGL(k_knn_colmean, dim3(1), dim3(256), 0, s, w.part, nparts, n, p, w.mean, w.maxbits);
     const int g = (int)ceil_div(n_pad, 256);
-    if (prec == kPrecBF16X3)
-        hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, n, n_pad, p, kpad / 8, norms, (uint4 *)xa,
+    if (prec == kPrecFP16) {
+        hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, n, p, w.mean,
+                           w.maxbits);
+        hipLaunchKernelGGL(k_knn_prep_f16, dim3(g), dim3(256), 0, s, x, w.mean, w.maxbits, n, n_pad, p, kpad / 8,
+                           (uint4 *)xa, (uint4 *)xb);
+    } else if (prec == kPrecBF16X3) {
+        hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, w.mean, n, n_pad, p, kpad / 8, (uint4 *)xa,
                            (uint4 *)xb);
-    else
-        hipLaunchKernelGGL(k_knn_prep_f32, dim3(g), dim3(256), 0, s, x, n, n_pad, p, kpad / 4, norms, (float2 *)xa,
+    } else {
+        hipLaunchKernelGGL(k_knn_prep_f32, dim3(g), dim3(256), 0, s, x, w.mean, n, n_pad, p, kpad / 4, (float2 *)xa,
                            (float2 *)xb);
+    }
 }
 
 // ---------------------------------------------------------------- tile staging
 // N values carried in registers across a loop iteration: a recursive struct instead
-// of an array, so no indexing ever asks for a stack slot.
+// of an array, so no indexing ever asks for a stack slot.  Element m of thread t is
+// unit t + m*STRIDE of the tile.
 template <class T, int N> struct Regs {
     T head;
     Regs<T, N - 1> tail;
 };
 template <class T> struct Regs<T, 0> {};
 
-template <class T, int N> __device__ inline void regs_load(Regs<T, N> &r, const T *src, int i) {
+template <class T, int N, int STRIDE> __device__ inline void regs_load(Regs<T, N> &r, const T *src, int i) {
     r.head = src[i];
-    if constexpr (N > 1) regs_load<T, N - 1>(r.tail, src, i + 256);
+    if constexpr (N > 1) regs_load<T, N - 1, STRIDE>(r.tail, src, i + STRIDE);
 }
-template <class T, int N> __device__ inline void regs_store(const Regs<T, N> &r, T *dst, int i) {
+template <class T, int N, int STRIDE> __device__ inline void regs_store(const Regs<T, N> &r, T *dst, int i) {
     dst[i] = r.head;
-    if constexpr (N > 1) regs_store<T, N - 1>(r.tail, dst, i + 256);
+    if constexpr (N > 1) regs_store<T, N - 1, STRIDE>(r.tail, dst, i + STRIDE);
 }
 
 // ---------------------------------------------------------------- top-L lists
@@ -240,24 +375,42 @@ __device__ inline void write_lists(const float (&ld)[L], const uint32_t (&li)[L]
     }
 }
 
-// ---------------------------------------------------------------- screen, split-BF16
-template <int KC, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 && KC <= 24) ? 2 : 1))) void k_knn_screen_bf16(const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb,
-                                                         int64_t n_pad, float *cand_d, uint32_t *cand_i) {
-    constexpr int KS = KC / 2;  // MFMA K-steps of 16
+// ---------------------------------------------------------------- screen, 16-bit operands
+// One MFMA K-step of 16: v_mfma_f32_32x32x16_{bf16,f16}.  TY 0 = bf16, 1 = fp16.
+template <int TY> __device__ inline f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
+    if constexpr (TY == 0)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
+}
+
+// Workgroup of NW waves, 64 queries per wave.  Two waves per SIMD fit when the
+// B fragments and lists leave room in 256 VGPRs (checked against the compiler's
+// register report: no spills).  Eight-wave workgroups (half the tile traffic per
+// query) are used where that register budget holds.
+template <int KC, int L> constexpr int screen_waves() { return (L <= 16 && KC <= 20) ? 2 : 1; }
+template <int TY, int KC, int L> constexpr int screen_nw() { return TY == 1 && L <= 16 && KC % 8 == 0 && KC <= 24 ? 8 : 4; }
+
+template <int TY, int KC, int L, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_waves<KC, L>()))) void k_knn_screen16(
+    const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t n_pad, float *cand_d, uint32_t *cand_i) {
+    constexpr int KS = KC / 2;    // MFMA K-steps of 16
     constexpr int TU = KC * kTC;  // 16-byte units per tile
-    static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
-    constexpr int PER = TU / 256;
+    constexpr int NT = NW * 64;
+    static_assert(TU % NT == 0, "tile must split evenly over the workgroup");
+    constexpr int PER = TU / NT;
     __shared__ u32x4 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
-    const int64_t q0 = (int64_t)blockIdx.x * kQWG + wave * kQW;
+    const int64_t q0 = (int64_t)blockIdx.x * NT + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
     const u32x4 *qbase = xb + (q0 / 64) * KC * 64;
-    bf16x8 b0[KS], b1[KS];
+    u32x4 b0[KS], b1[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        b0[s] = as_bf16x8(qbase[(2 * s + half) * 64 + col]);
-        b1[s] = as_bf16x8(qbase[(2 * s + half) * 64 + 32 + col]);
+        b0[s] = qbase[(2 * s + half) * 64 + col];
+        b1[s] = qbase[(2 * s + half) * 64 + 32 + col];
     }
     float d0[L], d1[L];
     uint32_t i0[L], i1[L];
@@ -271,9 +424,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
     // registers loaded one iteration earlier; the loads for tile t+2 are issued after
     // that store, so their latency overlaps the whole next iteration.  (The final
     // iterations re-load the last tile; those copies are never read.)
-    for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
+    for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = xa[i];
     Regs<u32x4, PER> pre;
-    regs_load<u32x4, PER>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
+    regs_load<u32x4, PER, NT>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
@@ -282,9 +435,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
             f32x16 a0 = {}, a1 = {};
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const bf16x8 a = as_bf16x8(lds[buf][(2 * s + half) * kTC + mt * 32 + col]);
-                a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[s], a0, 0, 0, 0);
-                a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[s], a1, 0, 0, 0);
+                const u32x4 a = lds[buf][(2 * s + half) * kTC + mt * 32 + col];
+                a0 = mfma16<TY>(a, b0[s], a0);
+                a1 = mfma16<TY>(a, b1[s], a1);
             }
             const uint32_t cb = (uint32_t)(t * kTC + mt * 32);
             if (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32) {
@@ -294,9 +447,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-        regs_store<u32x4, PER>(pre, lds[buf ^ 1], threadIdx.x);
+        regs_store<u32x4, PER, NT>(pre, lds[buf ^ 1], threadIdx.x);
         const int64_t tn = t + 2 < n_tiles ? t + 2 : n_tiles - 1;
-        regs_load<u32x4, PER>(pre, xa + tn * TU, threadIdx.x);
+        regs_load<u32x4, PER, NT>(pre, xa + tn * TU, threadIdx.x);
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -314,7 +467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
     constexpr int PER = TU / 256;
     __shared__ float2 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
-    const int64_t q0 = (int64_t)blockIdx.x * kQWG + wave * kQW;
+    const int64_t q0 = (int64_t)blockIdx.x * 256 + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
     const float2 *qbase = xb + (q0 / 64) * KC * 2 * 64;
     float2 b0[KC], b1[KC];
@@ -331,10 +484,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
     const int64_t n_tiles = n_pad / kTC;
-    // same loop-carried prefetch as the split-BF16 screen
+    // same loop-carried prefetch as the 16-bit screen
     for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
     Regs<float2, PER> pre;
-    regs_load<float2, PER>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
+    regs_load<float2, PER, 256>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
     __syncthreads();
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int buf = (int)(t & 1);
@@ -357,9 +510,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
             topk_update<L>(d0, i0, a0, cb, half);
             topk_update<L>(d1, i1, a1, cb, half);
         }
-        regs_store<float2, PER>(pre, lds[buf ^ 1], threadIdx.x);
+        regs_store<float2, PER, 256>(pre, lds[buf ^ 1], threadIdx.x);
         const int64_t tn = t + 2 < n_tiles ? t + 2 : n_tiles - 1;
-        regs_load<float2, PER>(pre, xa + tn * TU, threadIdx.x);
+        regs_load<float2, PER, 256>(pre, xa + tn * TU, threadIdx.x);
         __syncthreads();
     }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
@@ -369,34 +522,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
 template <int L>
 static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t n_pad, int kpad, float *cd, uint32_t *ci,
                             hipStream_t s) {
-    const dim3 grid((unsigned)(n_pad / kQWG)), block(256);
-#define CREST_BF(KCV)                                                                                             \
+#define CREST_X16(TY, KCV)                                                                                       \
+    case KCV: {                                                                                                 \
+        constexpr int NW = screen_nw<TY, KCV, L>();                                                             \
+        hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW>), dim3((unsigned)(n_pad / (NW * 64))), dim3(NW * 64), \
+                           0, s, (const u32x4 *)xa, (const u32x4 *)xb, n_pad, cd, ci);                          \
+        return true;                                                                                            \
+    }
+#define CREST_F(KCV)                                                                                              \
     case KCV:                                                                                                     \
-        hipLaunchKernelGGL((k_knn_screen_bf16<KCV, L>), grid, block, 0, s, (const u32x4 *)xa, (const u32x4 *)xb, \
-                           n_pad, cd, ci);                                                                        \
-        return true;
-#define CREST_F(KCV)                                                                                                \
-    case KCV:                                                                                                       \
-        hipLaunchKernelGGL((k_knn_screen_f32<KCV, L>), grid, block, 0, s, (const float2 *)xa, (const float2 *)xb, \
-                           n_pad, cd, ci);                                                                          \
+        hipLaunchKernelGGL((k_knn_screen_f32<KCV, L>), dim3((unsigned)(n_pad / 256)), dim3(256), 0, s,          \
+                           (const float2 *)xa, (const float2 *)xb, n_pad, cd, ci);                               \
         return true;
     if (prec == kPrecBF16X3) {
         switch (kpad / 8) {
-            CREST_BF(4) CREST_BF(8) CREST_BF(12) CREST_BF(16) CREST_BF(20) CREST_BF(24) CREST_BF(32) CREST_BF(52)
+            CREST_X16(0, 4) CREST_X16(0, 8) CREST_X16(0, 12) CREST_X16(0, 16) CREST_X16(0, 20) CREST_X16(0, 24)
+            CREST_X16(0, 32) CREST_X16(0, 52)
+        }
+    } else if (prec == kPrecFP16) {
+        switch (kpad / 8) {
+            CREST_X16(1, 4) CREST_X16(1, 8) CREST_X16(1, 16) CREST_X16(1, 24) CREST_X16(1, 32)
         }
     } else {
         switch (kpad / 4) { CREST_F(2) CREST_F(4) CREST_F(8) CREST_F(14) CREST_F(16) CREST_F(24) CREST_F(34) }
     }
-#undef CREST_BF
+#undef CREST_X16
 #undef CREST_F
     return false;
 }
 
 // Screen with lists of L per lane (L in {16, 32, 64}, L >= k); cand arrays hold 2L per
-// query.  Returns false when no kernel is instantiated for (prec, kpad, L).
+// query for knn_rows_padded(n) rows.  Returns false when no kernel is instantiated for
+// (prec, kpad, L).
 bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int kpad, int L, float *cand_d,
                        uint32_t *cand_i, hipStream_t s) {
-    const int64_t n_pad = round_up(n, kQWG);
+    const int64_t n_pad = knn_rows_padded(n);
     if (L == 16) return launch_screen_L<16>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);
     if (L == 32) return launch_screen_L<32>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);
     if (L == 64) return launch_screen_L<64>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);

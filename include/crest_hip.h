@@ -32,10 +32,13 @@ extern "C" {
 
 /* Arithmetic of the dense contractions (Gram SYRK and the kNN distance screen).
  * The transform, gene statistics, centring, eigensolve, projection and the kNN
- * rerank are FP64 in every mode.                                                  */
+ * rerank are FP64 in every mode.  The screen only chooses candidates: the reported
+ * neighbours and distances always come from the exact rerank.                    */
 #define CREST_PREC_FP64 0   /* Gram on FP64 MFMA; kNN screen on FP32 MFMA          */
 #define CREST_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation flushed to FP64          */
 #define CREST_PREC_BF16X3 2 /* split-BF16 hi+lo, three MFMA products, FP32 accum.   */
+#define CREST_PREC_FP16 3   /* kNN screen only: FP16 operands (power-of-two
+                               prescaled), one MFMA product, FP32 accumulation       */
 
 /* One chunk of raw counts plus the recorded filters and lazy transform: the
  * 9-tuple `(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr,
@@ -64,6 +67,12 @@ int crest_set_device(int32_t device);
 /* Precision used by crest_gram_accumulate / crest_knn_graph (default FP64). */
 int crest_set_precision(int32_t prec);
 int32_t crest_get_precision(void);
+/* Precision of the kNN distance screen alone (crest_knn_graph): CREST_PREC_FP64 /
+ * FP32 / BF16X3 / FP16, or -1 (default) to follow crest_set_precision.  The getter
+ * returns the precision in effect.  No reference counterpart (the reference's
+ * screen is FP32).                                                               */
+int crest_set_knn_precision(int32_t prec);
+int32_t crest_get_knn_precision(void);
 /* Message of the last failing call on this thread ("" if none). */
 const char *crest_last_error(void);
 /* Number of visible HIP devices (0 when none; never fails). */
@@ -146,7 +155,7 @@ typedef struct crest_pipeline_args {
     size_t n_comps;     /* requested PCs                                             */
     size_t k;           /* neighbours per cell (knn_graph k)                         */
     int32_t prec_gram;  /* CREST_PREC_* for the Gram                                 */
-    int32_t prec_knn;   /* CREST_PREC_FP32 or CREST_PREC_BF16X3 for the screen       */
+    int32_t prec_knn;   /* CREST_PREC_FP64/FP32 (FP32 screen), BF16X3 or FP16         */
     size_t chunk_rows;  /* cells per densified tile (0 = default)                    */
     /* outputs in device memory */
     float *dev_x_pca;        /* [n_cells, k_pcs]                                     */

@@ -152,15 +152,23 @@ def _clustered(n, d, seed):
     return (centers[np.arange(n) % 20] + rng.uniform(-1.5, 1.5, (n, d))).astype(np.float32)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
-@pytest.mark.parametrize("n,d,k", [(600, 7, 10), (5000, 50, 15), (2049, 30, 32), (300, 3, 1)])
-def test_knn_graph(oracle, prec, n, d, k):
-    x = _clustered(n, d, n + d)
-    ca.set_precision(prec)
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
+@pytest.mark.parametrize("n,d,k,scale,offset", [(600, 7, 10, 1.0, 0.0), (5000, 50, 15, 1.0, 0.0),
+                                                (2049, 30, 32, 1.0, 0.0), (300, 3, 1, 1.0, 0.0),
+                                                (1500, 200, 15, 1.0, 0.0), (2000, 50, 15, 3e4, 0.0),
+                                                (2000, 20, 15, 1e-3, 50.0), (1000, 12, 64, 1.0, 0.0)])
+def test_knn_graph(oracle, prec, n, d, k, scale, offset):
+    if prec == "bf16x3" and d > 137:
+        pytest.skip("split-BF16 screen holds at most 137 columns")
+    if prec == "fp64" and d > 134:
+        pytest.skip("FP32 screen holds at most 134 columns")
+    x = (_clustered(n, d, n + d) * np.float32(scale) + np.float32(offset)).astype(np.float32)
+    ca.set_knn_precision(prec)
     try:
+        assert ca.get_knn_precision() == prec
         idx, dist = N.knn_graph(x, k)
     finally:
-        ca.set_precision("fp64")
+        ca.set_knn_precision(None)
     assert idx.shape == (n, k) and dist.shape == (n, k)
     assert not (idx == np.arange(n)[:, None]).any()
     assert (np.diff(dist, axis=1) >= 0).all()
@@ -170,6 +178,31 @@ def test_knn_graph(oracle, prec, n, d, k):
     # distances are the reference's f32 formula on the reported pairs
     pd = oracle.pair_dist(x, np.repeat(np.arange(n), k), idx.ravel()).reshape(n, k)
     np.testing.assert_array_equal(dist, pd)
+
+
+def test_knn_precision_knob():
+    ca.set_precision("bf16x3")
+    try:
+        assert ca.get_knn_precision() == "bf16x3"  # follows set_precision by default
+        ca.set_knn_precision("fp16")
+        assert ca.get_knn_precision() == "fp16" and ca.get_precision() == "bf16x3"
+        with pytest.raises(ValueError):
+            ca.set_knn_precision(9)
+        with pytest.raises(ValueError):
+            ca.set_precision("fp16")  # FP16 is a screen-only precision
+    finally:
+        ca.set_knn_precision(None)
+        ca.set_precision("fp64")
+    assert ca.get_knn_precision() == "fp64"
+
+
+def test_knn_too_wide():
+    ca.set_knn_precision("fp16")
+    try:
+        with pytest.raises(ValueError):
+            N.knn_graph(np.zeros((10, 300), np.float32), 3)
+    finally:
+        ca.set_knn_precision(None)
 
 
 def test_knn_small_edge_cases():
