@@ -80,6 +80,8 @@ _SIGS = {
                                 C.c_double]),
     "crest_knn_graph": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int32, C.c_uint64,
                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]),
+    "crest_knn_query": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                  C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]),
     "crest_connectivities": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "crest_materialize": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,

@@ -14,8 +14,8 @@ import numpy as np
 
 from ._lib import Chunk, check, lib
 
-__all__ = ["gene_stats", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "connectivities",
-           "materialize"]
+__all__ = ["gene_stats", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "knn_query",
+           "connectivities", "materialize"]
 
 
 def _arr(a, dtype, name: str, writable: bool = False):
@@ -141,6 +141,33 @@ def knn_graph(data, k, exact=None, seed=0):
     ko = C.c_size_t(0)
     ex = -1 if exact is None else int(bool(exact))
     check(lib().crest_knn_graph(_p(data), n, dim, int(k), ex, int(seed), _p(idx), _p(dist), C.byref(ko)))
+    return idx[:, :ko.value], dist[:, :ko.value]
+
+
+def knn_query(reference, query, k, exact=None, seed=0):
+    """py.rs:635-653: the k' = min(k, n_ref) nearest reference rows of each query row
+    (knn.rs:485-546); returns (indices uint32 (n_query, k'), distances float32
+    (n_query, k')) ordered by (distance, index).  The GPU search is exact for every
+    size, so `exact` and `seed` do not change the result."""
+    if not isinstance(reference, np.ndarray) or not isinstance(query, np.ndarray):
+        raise TypeError("reference and query must be numpy arrays")
+    if reference.ndim != 2 or query.ndim != 2:
+        raise ValueError("reference and query must be 2-D arrays")
+    if query.shape[1] != reference.shape[1]:
+        raise ValueError("reference and query must have the same number of columns")
+    reference = _arr(reference, np.float32, "reference")
+    query = _arr(query, np.float32, "query")
+    if k < 0 or seed < 0:
+        raise OverflowError("can't convert negative int to unsigned")
+    nr, dim = reference.shape
+    nq = query.shape[0]
+    kk = min(int(k), nr) if nq > 0 else 0
+    idx = np.empty((nq, kk), np.uint32)
+    dist = np.empty((nq, kk), np.float32)
+    ko = C.c_size_t(0)
+    ex = -1 if exact is None else int(bool(exact))
+    check(lib().crest_knn_query(_p(reference), nr, _p(query), nq, dim, int(k), ex, int(seed), _p(idx), _p(dist),
+                                C.byref(ko)))
     return idx[:, :ko.value], dist[:, :ko.value]
 
 

@@ -281,9 +281,13 @@ static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *d
     if (hinfo != 0) fail(CREST_E_RUNTIME, "eigensolver did not converge (info " + std::to_string(hinfo) + ")");
 }
 
-// kNN of the rows of dev_x [n, p] into dev_idx/dev_dist [n, k].
-static void knn_device(Ctx &x, const float *dev_x, int64_t n, int p, int k, int prec, uint32_t *dev_idx,
-                       float *dev_dist, hipEvent_t *ev3) {
+// k nearest rows of dev_xc [nc, p] for each row of dev_xq [nq, p] into
+// dev_idx/dev_dist [nq, k]; dev_xq == nullptr gives the kNN graph of dev_xc (the
+// queries are the candidates, each row's own entry excluded).
+static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
+                       int prec, uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
+    const bool graph = dev_xq == nullptr;
+    if (graph) nq = nc;
     const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
     const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
     if (k > 64) value_error("k > 64 is not supported by the exact GPU kNN");
@@ -291,19 +295,21 @@ static void knn_device(Ctx &x, const float *dev_x, int64_t n, int p, int k, int 
     if (kpad < 0)
         value_error("data has too many columns for the GPU kNN (at most 134 for fp32/fp64, 137 for bf16x3, "
                     "252 for fp16)");
-    const int64_t n_pad = knn_rows_padded(n);
+    const int64_t nc_pad = knn_rows_padded(nc), nq_pad = knn_rows_padded(nq);
     const size_t eb = kp == CREST_PREC_FP32 ? 4 : 2;
-    void *xa = x.get<uint8_t>("knn_xa", (size_t)n_pad * kpad * eb);
-    void *xb = x.get<uint8_t>("knn_xb", (size_t)n_pad * kpad * eb);
-    void *work = x.get<uint8_t>("knn_work", knn_work_bytes(p));
-    float *cd = x.get<float>("knn_cd", (size_t)n_pad * 2 * L);
-    uint32_t *ci = x.get<uint32_t>("knn_ci", (size_t)n_pad * 2 * L);
+    void *xa = x.get<uint8_t>("knn_xa", (size_t)nc_pad * kpad * eb);
+    void *xb = x.get<uint8_t>("knn_xb", (size_t)nq_pad * kpad * eb);
+    void *work = x.get<uint8_t>("knn_work", knn_work_bytes(nc, graph ? 0 : nq, p));
+    float *cd = x.get<float>("knn_cd", (size_t)nq_pad * 2 * L);
+    uint32_t *ci = x.get<uint32_t>("knn_ci", (size_t)nq_pad * 2 * L);
     if (ev3) HIPCHK(hipEventRecord(ev3[0], x.st));
-    launch_knn_prep(kp, dev_x, n, p, kpad, work, xa, xb, x.st);
+    const KnnOrder o = launch_knn_prep(kp, dev_xc, nc, dev_xq, nq, p, kpad, work, xa, xb, x.st);
     if (ev3) HIPCHK(hipEventRecord(ev3[1], x.st));
-    if (!launch_knn_screen(kp, xa, xb, n, kpad, L, cd, ci, x.st)) value_error("no kNN kernel for this width");
+    if (!launch_knn_screen(kp, xa, xb, nc, nq, kpad, L, graph ? 1 : 0, o, cd, ci, x.st))
+        value_error("no kNN kernel for this width");
     if (ev3) HIPCHK(hipEventRecord(ev3[2], x.st));
-    launch_knn_rerank(dev_x, n, p, cd, ci, 2 * L, k, dev_idx, dev_dist, x.st);
+    launch_knn_rerank(graph ? dev_xc : dev_xq, nq, dev_xc, nc, p, o, graph ? 1 : 0, ci, 2 * L, k, dev_idx, dev_dist,
+                      x.st);
     if (ev3) HIPCHK(hipEventRecord(ev3[3], x.st));
     x.check_launch();
 }
@@ -585,10 +591,37 @@ int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t e
         const float *dx = x.upload("knn_x", data, n * dim);
         uint32_t *di = x.get<uint32_t>("knn_idx", n * kk);
         float *dd = x.get<float>("knn_dist", n * kk);
-        knn_device(x, dx, (int64_t)n, (int)dim, (int)kk, crest_get_knn_precision(), di, dd, nullptr);
+        knn_device(x, dx, (int64_t)n, nullptr, 0, (int)dim, (int)kk, crest_get_knn_precision(), di, dd, nullptr);
         x.download(indices, di, n * kk);
         x.download(distances, dd, n * kk);
         x.sync();
+    });
+}
+
+int crest_knn_query(const float *reference, size_t n_ref, const float *query, size_t n_query, size_t dim, size_t k,
+                    int32_t exact, uint64_t seed, uint32_t *indices, float *distances, size_t *k_out) {
+    (void)exact;
+    (void)seed;
+    return guarded([&] {
+        // knn.rs:485-490
+        const size_t kk = std::min(k, n_ref);
+        if (k_out) *k_out = 0;
+        if (n_query == 0 || kk == 0) return;
+        if (dim == 0) value_error("data must have at least one column");
+        if (n_ref >= 0xFFFFFFFFull || n_query >= 0xFFFFFFFFull) value_error("too many points");
+        if (!reference || !query || !indices || !distances) value_error("NULL argument");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        const float *dr = x.upload("knnq_ref", reference, n_ref * dim);
+        const float *dq = x.upload("knnq_query", query, n_query * dim);
+        uint32_t *di = x.get<uint32_t>("knn_idx", n_query * kk);
+        float *dd = x.get<float>("knn_dist", n_query * kk);
+        knn_device(x, dr, (int64_t)n_ref, dq, (int64_t)n_query, (int)dim, (int)kk, crest_get_knn_precision(), di, dd,
+                   nullptr);
+        x.download(indices, di, n_query * kk);
+        x.download(distances, dd, n_query * kk);
+        x.sync();
+        if (k_out) *k_out = kk;
     });
 }
 
@@ -756,7 +789,7 @@ int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *l
         x.check_launch();
         HIPCHK(hipEventRecord(ev[4], x.st));
         // kNN (pp.py:309)
-        knn_device(x, a->dev_x_pca, n, kp, kn, a->prec_knn, a->dev_knn_idx, a->dev_knn_dist, ev + 5);
+        knn_device(x, a->dev_x_pca, n, nullptr, 0, kp, kn, a->prec_knn, a->dev_knn_idx, a->dev_knn_dist, ev + 5);
         x.sync();
         float ms[8] = {};
         for (int i = 0; i < 8; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));

@@ -101,17 +101,25 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
                        double *loadings, hipStream_t s);
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
                        float *x, float *shift_scratch, hipStream_t s);
-// kNN screen operands: rows are padded to knn_rows_padded(n); `work` holds
-// knn_work_bytes(p) bytes (column means of the centring and the FP16 prescale).
+// kNN screen operands: rows are padded to knn_rows_padded(n) and stored in a
+// spatial order; `work` holds knn_work_bytes(nc, nq, p) bytes (column statistics,
+// sort keys, the FP16 prescale and the permutations; nq = 0 when the queries are the
+// candidates).  Candidate position i of the screen is row perm_c[i], query
+// position j is query row perm_q[j]; keys_c / keys_q are the sorted keys of a
+// separate query set (null for a kNN graph).
+struct KnnOrder {
+    const uint32_t *perm_c, *perm_q, *keys_c, *keys_q;
+};
 int64_t knn_rows_padded(int64_t n);
-size_t knn_work_bytes(int p);
-void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, void *work, void *xa,
-                     void *xb, hipStream_t s);
+size_t knn_work_bytes(int64_t nc, int64_t nq, int p);
+KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq, int64_t nq, int p, int kpad,
+                         void *work, void *xa, void *xb, hipStream_t s);
 int knn_kpad(int prec, int p);
-bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int kpad, int L,
-                       float *cand_d, uint32_t *cand_i, hipStream_t s);
-void launch_knn_rerank(const float *x, int64_t n, int p, const float *cand_d, const uint32_t *cand_i,
-                       int n_cand, int k, uint32_t *idx_out, float *dist_out, hipStream_t s);
+bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int L,
+                       int self, const KnnOrder &o, float *cand_d, uint32_t *cand_i, hipStream_t s);
+void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o,
+                       int self, const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
+                       hipStream_t s);
 size_t conn_temp_bytes(int64_t m);
 void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, int k, int n_neighbors,
                            uint64_t *keys, uint64_t *keys2, float *w, float *w2, float *wu, uint32_t *flag,

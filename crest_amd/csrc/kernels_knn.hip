@@ -18,6 +18,11 @@
 // ((i/64)*KC + j)*64 + i%64 (16-bit types; FP32 splits each chunk into two float2
 // halves), so one 64-row tile is one contiguous block that a workgroup copies into
 // LDS and each MFMA fragment read from LDS is a contiguous 512-byte run per half-wave.
+#include <cstring>
+#include <stdexcept>
+
+#include <rocprim/rocprim.hpp>
+
 #include "common.hpp"
 
 namespace crest {
@@ -63,39 +68,89 @@ int knn_kpad(int prec, int p) {
 }
 
 // ---------------------------------------------------------------- prep
-// Column means (deterministic two-stage FP64 sum), then the centred rows' squared
-// norms Σ x'² in f32, summed sequentially as knn.rs:346 does, and the augmented
-// operands, one 16-byte chunk at a time so that a wave (64 consecutive rows) stores
-// 1 KB runs.  The work buffer holds kColParts*p partial sums, the p means and the
-// largest centred norm (as its bit pattern; norms are >= 0, so integer order is
-// float order).
+// 1. Column means and standard deviations (deterministic two-stage FP64 sums).
+// 2. A spatial order: a Morton key of the (up to) four highest-variance columns,
+//    quantised to +-4 sd, and a stable radix sort of the keys.  perm[i] is the row at
+//    sorted position i.  Nearby rows end up in nearby positions, so a query's
+//    neighbours sit in the candidate stages its workgroup visits first.
+// 3. The augmented operands of the rows in sorted order, centred (x' = x - mean):
+//    squared norms Σ x'² in f32, summed sequentially as knn.rs:346 does, written one
+//    16-byte chunk at a time so that a wave (64 consecutive rows) stores 1 KB runs.
 constexpr int kColParts = 512;
-
-size_t knn_work_bytes(int p) { return (size_t)kColParts * p * sizeof(double) + (size_t)p * sizeof(float) + 64; }
+constexpr int kKeyCols = 4;
 
 struct KnnWork {
-    double *part;
-    float *mean;
+    double *part;    // [kColParts * p]
+    float *mean;     // [p]
+    float *sd;       // [p]
+    int32_t *cols;   // [kKeyCols] key columns (-1 = unused)
     uint32_t *maxbits;
+    uint32_t *keys_c, *perm_c;   // [nc_pad] sorted keys and permutation of the candidate rows
+    uint32_t *keys_q, *perm_q;   // [nq_pad] the same for separate query rows
+    uint32_t *keys_in, *vals_in; // unsorted keys and row ids
+    void *temp;
+    size_t temp_bytes;
 };
-static KnnWork knn_work(void *w, int p) {
+
+static size_t knn_sort_temp(int64_t n) {
+    size_t b = 0;
+    (void)rocprim::radix_sort_pairs((void *)nullptr, b, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0, 32);
+    return b;
+}
+
+// nq == 0: the queries are the candidates (kNN graph).
+static KnnWork knn_work(void *w, int64_t nc, int64_t nq, int p, size_t *total) {
+    const int64_t nc_pad = round_up(nc, kPadRows), nq_pad = round_up(nq, kPadRows);
+    const int64_t nmax = nc_pad > nq_pad ? nc_pad : nq_pad;
+    uint8_t *b = (uint8_t *)w;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *r = b ? b + o : nullptr;
+        o = round_up((int64_t)(o + bytes), 256);
+        return (void *)r;
+    };
     KnnWork k;
-    k.part = (double *)w;
-    k.mean = (float *)(k.part + (size_t)kColParts * p);
-    k.maxbits = (uint32_t *)(k.mean + p);
+    k.part = (double *)take((size_t)kColParts * p * sizeof(double));
+    k.mean = (float *)take((size_t)p * sizeof(float));
+    k.sd = (float *)take((size_t)p * sizeof(float));
+    k.cols = (int32_t *)take(kKeyCols * sizeof(int32_t));
+    k.maxbits = (uint32_t *)take(sizeof(uint32_t));
+    k.keys_c = (uint32_t *)take((size_t)nc_pad * 4);
+    k.perm_c = (uint32_t *)take((size_t)nc_pad * 4);
+    k.keys_q = nq ? (uint32_t *)take((size_t)nq_pad * 4) : k.keys_c;
+    k.perm_q = nq ? (uint32_t *)take((size_t)nq_pad * 4) : k.perm_c;
+    k.keys_in = (uint32_t *)take((size_t)nmax * 4);
+    k.vals_in = (uint32_t *)take((size_t)nmax * 4);
+    const size_t t1 = knn_sort_temp(nc), t2 = nq ? knn_sort_temp(nq) : 0;
+    k.temp_bytes = t1 > t2 ? t1 : t2;
+    k.temp = take(k.temp_bytes);
+    if (total) *total = o;
     return k;
 }
 
-// Block b sums rows [b*rows_per, (b+1)*rows_per): thread t takes column t % p of
-// every (256/p)-th row, and the row groups are combined in order through LDS.
-__global__ __launch_bounds__(256) void k_knn_colsum(const float *x, int64_t n, int p, int64_t rows_per,
-                                                    double *part) {
+size_t knn_work_bytes(int64_t nc, int64_t nq, int p) {
+    size_t t = 0;
+    (void)knn_work(nullptr, nc, nq, p, &t);
+    return t;
+}
+
+// Block b sums rows [b*rows_per, (b+1)*rows_per) of x (mode 0) or of (x - mean)^2
+// (mode 1): thread t takes column t % p of every (256/p)-th row, and the row groups
+// are combined in order through LDS.
+__global__ __launch_bounds__(256) void k_knn_colsum(const float *x, const float *mean, int mode, int64_t n, int p,
+                                                    int64_t rows_per, double *part) {
     __shared__ double red[256];
     const int g = 256 / p, t = threadIdx.x, f = t % p, j = t / p;
     const int64_t r0 = blockIdx.x * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
     double acc = 0.0;
-    if (j < g)
-        for (int64_t r = r0 + j; r < r1; r += g) acc += (double)x[r * p + f];
+    if (j < g) {
+        const double mu = mode ? (double)mean[f] : 0.0;
+        for (int64_t r = r0 + j; r < r1; r += g) {
+            const double v = (double)x[r * p + f] - mu;
+            acc += mode ? v * v : v;
+        }
+    }
     red[t] = acc;
     __syncthreads();
     if (t < p) {
@@ -105,13 +160,63 @@ __global__ __launch_bounds__(256) void k_knn_colsum(const float *x, int64_t n, i
     }
 }
 
-__global__ void k_knn_colmean(const double *part, int nparts, int64_t n, int p, float *mean, uint32_t *maxbits) {
+// mode 0: mean[f] = Σ/n and the running maximum norm is reset; mode 1: sd[f] and the
+// key columns (largest variance first, ties to the lower column).
+__global__ void k_knn_colfinal(const double *part, int nparts, int64_t n, int p, int mode, float *mean, float *sd,
+                               int32_t *cols, uint32_t *maxbits) {
+    __shared__ float var[256];
     for (int f = threadIdx.x; f < p; f += blockDim.x) {
         double s = 0.0;
         for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * p + f];
-        mean[f] = (float)(s / (double)n);
+        if (mode == 0) {
+            mean[f] = (float)(s / (double)n);
+        } else {
+            const float v = (float)(s / (double)n);
+            var[f] = v;
+            sd[f] = sqrtf(v);
+        }
     }
-    if (threadIdx.x == 0) *maxbits = 0u;
+    if (mode == 0) {
+        if (threadIdx.x == 0) *maxbits = 0u;
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t used[4] = {0, 0, 0, 0};
+        for (int c = 0; c < kKeyCols; ++c) {
+            int best = -1;
+            for (int f = 0; f < p; ++f)
+                if (!((used[f >> 6] >> (f & 63)) & 1) && (best < 0 || var[f] > var[best])) best = f;
+            cols[c] = best;
+            if (best >= 0) used[best >> 6] |= 1ull << (best & 63);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_knn_keys(const float *x, int64_t n, int p, const float *mean,
+                                                  const float *sd, const int32_t *cols, uint32_t *keys,
+                                                  uint32_t *vals) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int m = 0;
+    uint32_t q[kKeyCols];
+    for (int c = 0; c < kKeyCols; ++c)
+        if (cols[c] >= 0) ++m;
+    const int bits = 32 / m < 10 ? 32 / m : 10;
+    for (int c = 0; c < m; ++c) {
+        const int f = cols[c];
+        const float s = sd[f];
+        const float z = s > 0.0f ? (x[i * p + f] - mean[f]) / s : 0.0f;
+        float v = floorf(z * (float)(1 << (bits - 3)) + (float)(1 << (bits - 1)));
+        v = v < 0.0f ? 0.0f : v;  // NaN rows end at 0 as well
+        const float top = (float)((1 << bits) - 1);
+        q[c] = (uint32_t)(v < top ? v : top);
+    }
+    uint32_t key = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int c = 0; c < m; ++c) key = (key << 1) | ((q[c] >> b) & 1u);
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
 }
 
 __device__ inline float centred_norm(const float *r, const float *mean, int p, float s) {
@@ -167,12 +272,13 @@ __device__ inline void bf16_elem(const float *r, const float *mean, int p, float
     }
 }
 
-__global__ void k_knn_prep_bf16(const float *x, const float *mean, int64_t n, int64_t n_pad, int p, int kc,
-                                uint4 *xa, uint4 *xb) {
+// Operand row i (sorted position) holds data row perm[i]; rows i >= n are padding.
+__global__ void k_knn_prep_bf16(const float *x, const uint32_t *perm, const float *mean, int64_t n, int64_t n_pad,
+                                int p, int kc, uint4 *xa, uint4 *xb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
     const bool real = i < n;
-    const float *r = x + (real ? i : 0) * (int64_t)p;
+    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
     const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j) {
         uint16_t ea[8], eb[8];
@@ -190,19 +296,19 @@ __global__ void k_knn_prep_bf16(const float *x, const float *mean, int64_t n, in
         __builtin_memcpy(&va, ea, 16);
         __builtin_memcpy(&vb, eb, 16);
         const int64_t u = ((i / 64) * kc + j) * 64 + (i % 64);
-        xa[u] = va;
-        xb[u] = vb;
+        if (xa) xa[u] = va;
+        if (xb) xb[u] = vb;
     }
 }
 
 // FP16 operands of a centred, prescaled row; fp16 conversions round to nearest even.
-__global__ void k_knn_prep_f16(const float *x, const float *mean, const uint32_t *maxbits, int64_t n, int64_t n_pad,
-                               int p, int kc, uint4 *xa, uint4 *xb) {
+__global__ void k_knn_prep_f16(const float *x, const uint32_t *perm, const float *mean, const uint32_t *maxbits,
+                               int64_t n, int64_t n_pad, int p, int kc, uint4 *xa, uint4 *xb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
     const bool real = i < n;
     const float s = fp16_prescale(maxbits);
-    const float *r = x + (real ? i : 0) * (int64_t)p;
+    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
     const float nn = real ? centred_norm(r, mean, p, s) : INFINITY;
     const _Float16 nh = (_Float16)nn;
     const _Float16 nl = real ? (_Float16)(nn - (float)nh) : (_Float16)0.0f;
@@ -228,8 +334,8 @@ __global__ void k_knn_prep_f16(const float *x, const float *mean, const uint32_t
         __builtin_memcpy(&va, ea, 16);
         __builtin_memcpy(&vb, eb, 16);
         const int64_t u = ((i / 64) * kc + j) * 64 + (i % 64);
-        xa[u] = va;
-        xb[u] = vb;
+        if (xa) xa[u] = va;
+        if (xb) xb[u] = vb;
     }
 }
 
@@ -249,12 +355,12 @@ __device__ inline void f32_elem(const float *r, const float *mean, int p, float 
     }
 }
 
-__global__ void k_knn_prep_f32(const float *x, const float *mean, int64_t n, int64_t n_pad, int p, int kc,
-                               float2 *xa, float2 *xb) {
+__global__ void k_knn_prep_f32(const float *x, const uint32_t *perm, const float *mean, int64_t n, int64_t n_pad,
+                               int p, int kc, float2 *xa, float2 *xb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
     const bool real = i < n;
-    const float *r = x + (real ? i : 0) * (int64_t)p;
+    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
     const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j)
         for (int h = 0; h < 2; ++h) {
@@ -268,34 +374,71 @@ __global__ void k_knn_prep_f32(const float *x, const float *mean, int64_t n, int
                 b0 = b1 = 0.0f;
             }
             const int64_t u = (((i / 64) * kc + j) * 2 + h) * 64 + (i % 64);
-            xa[u] = make_float2(a0, a1);
-            xb[u] = make_float2(b0, b1);
+            if (xa) xa[u] = make_float2(a0, a1);
+            if (xb) xb[u] = make_float2(b0, b1);
         }
 }
 
 int64_t knn_rows_padded(int64_t n) { return round_up(n, kPadRows); }
 
-void launch_knn_prep(int prec, const float *x, int64_t n, int p, int kpad, void *work, void *xa, void *xb,
-                     hipStream_t s) {
-    const KnnWork w = knn_work(work, p);
+static void sort_rows(const KnnWork &w, const float *x, int64_t n, int p, uint32_t *keys_out, uint32_t *perm,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_knn_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, n, p, w.mean, w.sd, w.cols,
+                       w.keys_in, w.vals_in);
+    size_t tb = w.temp_bytes;
+    if (rocprim::radix_sort_pairs(w.temp, tb, w.keys_in, keys_out, w.vals_in, perm, (size_t)n, 0, 32, s) !=
+        hipSuccess)
+        throw std::runtime_error("rocprim radix_sort_pairs failed");
+}
+
+static void write_operands(int prec, const float *x, const uint32_t *perm, const KnnWork &w, int64_t n, int p,
+                           int kpad, void *xa, void *xb, hipStream_t s) {
     const int64_t n_pad = knn_rows_padded(n);
-    const int64_t rows_per = ceil_div(n, kColParts);
-    const int nparts = (int)ceil_div(n, rows_per);
-    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, x, n, p, rows_per, w.part);
-    hipLaunchKernelGGL(k_knn_colmean, dim3(1), dim3(256), 0, s, w.part, nparts, n, p, w.mean, w.maxbits);
     const int g = (int)ceil_div(n_pad, 256);
-    if (prec == kPrecFP16) {
-        hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, n, p, w.mean,
-                           w.maxbits);
-        hipLaunchKernelGGL(k_knn_prep_f16, dim3(g), dim3(256), 0, s, x, w.mean, w.maxbits, n, n_pad, p, kpad / 8,
+    if (prec == kPrecFP16)
+        hipLaunchKernelGGL(k_knn_prep_f16, dim3(g), dim3(256), 0, s, x, perm, w.mean, w.maxbits, n, n_pad, p,
+                           kpad / 8, (uint4 *)xa, (uint4 *)xb);
+    else if (prec == kPrecBF16X3)
+        hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, perm, w.mean, n, n_pad, p, kpad / 8,
                            (uint4 *)xa, (uint4 *)xb);
-    } else if (prec == kPrecBF16X3) {
-        hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, w.mean, n, n_pad, p, kpad / 8, (uint4 *)xa,
-                           (uint4 *)xb);
-    } else {
-        hipLaunchKernelGGL(k_knn_prep_f32, dim3(g), dim3(256), 0, s, x, w.mean, n, n_pad, p, kpad / 4, (float2 *)xa,
-                           (float2 *)xb);
+    else
+        hipLaunchKernelGGL(k_knn_prep_f32, dim3(g), dim3(256), 0, s, x, perm, w.mean, n, n_pad, p, kpad / 4,
+                           (float2 *)xa, (float2 *)xb);
+}
+
+KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq, int64_t nq, int p, int kpad,
+                         void *work, void *xa, void *xb, hipStream_t s) {
+    const bool graph = xq == nullptr;
+    const KnnWork w = knn_work(work, nc, graph ? 0 : nq, p, nullptr);
+    const int64_t rows_per = ceil_div(nc, kColParts);
+    const int nparts = (int)ceil_div(nc, rows_per);
+    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 0, nc, p, rows_per, w.part);
+    hipLaunchKernelGGL(k_knn_colfinal, dim3(1), dim3(256), 0, s, w.part, nparts, nc, p, 0, w.mean, w.sd, w.cols,
+                       w.maxbits);
+    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 1, nc, p, rows_per, w.part);
+    hipLaunchKernelGGL(k_knn_colfinal, dim3(1), dim3(256), 0, s, w.part, nparts, nc, p, 1, w.mean, w.sd, w.cols,
+                       w.maxbits);
+    sort_rows(w, xc, nc, p, w.keys_c, w.perm_c, s);
+    if (!graph) sort_rows(w, xq, nq, p, w.keys_q, w.perm_q, s);
+    if (prec == kPrecFP16) {
+        hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, s, xc, nc, p, w.mean,
+                           w.maxbits);
+        if (!graph)
+            hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, xq, nq, p, w.mean,
+                               w.maxbits);
     }
+    if (graph) {
+        write_operands(prec, xc, w.perm_c, w, nc, p, kpad, xa, xb, s);
+    } else {
+        write_operands(prec, xc, w.perm_c, w, nc, p, kpad, xa, nullptr, s);
+        write_operands(prec, xq, w.perm_q, w, nq, p, kpad, nullptr, xb, s);
+    }
+    KnnOrder o;
+    o.perm_c = w.perm_c;
+    o.perm_q = w.perm_q;
+    o.keys_c = graph ? nullptr : w.keys_c;
+    o.keys_q = graph ? nullptr : w.keys_q;
+    return o;
 }
 
 // ---------------------------------------------------------------- tile staging
@@ -319,40 +462,47 @@ template <class T, int N, int STRIDE> __device__ inline void regs_store(const Re
 
 // ---------------------------------------------------------------- top-L lists
 // Sorted ascending; a new entry goes after equal ones, so among equal distances the
-// earlier (lower-index) candidate stays ahead, as push_topk does (knn.rs:68-87).
+// entry offered first stays ahead, as push_topk does (knn.rs:68-87).  With the
+// comparisons c_j = d < ld[j] taken on the old list (monotone in j), slot j keeps
+// its entry, takes d, or takes slot j-1's entry; the distances need only
+// ld[j] = min(ld[j], max(d, ld[j-1])), and no value is carried from slot to slot.
+// Entries are +0.0 or positive (or +inf), so their bit patterns order as unsigned
+// integers and the min/max run on the integer unit without NaN canonicalisation.
 template <int L>
 __device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, uint32_t id) {
+    const uint32_t db = __float_as_uint(d);
+    bool c[L];
 #pragma unroll
-    for (int j = 0; j < L; ++j) {
-        const bool sw = d < ld[j];
-        const float td = sw ? d : ld[j];
-        const uint32_t ti = sw ? id : li[j];
-        d = sw ? ld[j] : d;
-        id = sw ? li[j] : id;
-        ld[j] = td;
-        li[j] = ti;
+    for (int j = 0; j < L; ++j) c[j] = db < __float_as_uint(ld[j]);
+#pragma unroll
+    for (int j = L - 1; j > 0; --j) {
+        li[j] = c[j - 1] ? li[j - 1] : (c[j] ? id : li[j]);
+        ld[j] = __uint_as_float(min(__float_as_uint(ld[j]), max(db, __float_as_uint(ld[j - 1]))));
     }
+    li[0] = c[0] ? id : li[0];
+    ld[0] = __uint_as_float(min(__float_as_uint(ld[0]), db));
 }
 
 // Register r of lane l holds the pair (candidate cb + 8(r/4) + 4(l/32) + r%4, query
-// of column l%32); candidates are offered in increasing index order.
+// of column l%32).  The registers that hold an entry below some lane's threshold
+// are visited in a wave-uniform loop, so a[r] is a uniform register index; each
+// visit inserts into the lists of the lanes whose value is still below their
+// threshold.
 template <int L>
 __device__ inline void topk_update(float (&ld)[L], uint32_t (&li)[L], const f32x16 &a, uint32_t cb, int half) {
     float m = a[0];
 #pragma unroll
     for (int r = 1; r < 16; ++r) m = fminf(m, a[r]);
-    if (m < ld[L - 1]) {
-        uint32_t mask = 0;
+    const float thr = ld[L - 1];
+    if (__builtin_amdgcn_ballot_w64(m < thr) == 0) return;
+    uint32_t u = 0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mask |= (a[r] < ld[L - 1]) ? (1u << r) : 0u;
-        while (mask) {
-            const int r = __builtin_ctz(mask);
-            mask &= mask - 1;
-            float v = a[0];
-#pragma unroll
-            for (int j = 1; j < 16; ++j) v = (r == j) ? a[j] : v;
-            if (v < ld[L - 1]) topk_insert<L>(ld, li, fmaxf(v, 0.0f), cb + 8 * (r >> 2) + 4 * half + (r & 3));
-        }
+    for (int r = 0; r < 16; ++r) u |= (__builtin_amdgcn_ballot_w64(a[r] < thr) != 0 ? 1u : 0u) << r;
+    while (u) {
+        const int r = __builtin_ctz(u);
+        u &= u - 1;
+        const float v = a[r];
+        if (v < ld[L - 1]) topk_insert<L>(ld, li, v > 0.0f ? v : 0.0f, cb + 8 * (r >> 2) + 4 * half + (r & 3));
     }
 }
 
@@ -387,19 +537,29 @@ template <int TY> __device__ inline f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
 }
 
 // Workgroup of NW waves, 64 queries per wave.  Two waves per SIMD fit when the
-// B fragments and lists leave room in 256 VGPRs (checked against the compiler's
-// register report: no spills).  Eight-wave workgroups (half the tile traffic per
-// query) are used where that register budget holds.
-template <int KC, int L> constexpr int screen_waves() { return (L <= 16 && KC <= 20) ? 2 : 1; }
-template <int TY, int KC, int L> constexpr int screen_nw() { return TY == 1 && L <= 16 && KC % 8 == 0 && KC <= 24 ? 8 : 4; }
+// B fragments, both accumulator pairs and the lists leave room in 256 VGPRs
+// (checked against the compiler's register report: no spills).  Eight-wave
+// workgroups (half the tile traffic per query) are used where that budget holds.
+template <int KC, int L> constexpr int screen_waves() { return (L <= 16 && KC <= 16) ? 2 : 1; }
+template <int TY, int KC, int L> constexpr int screen_nw() { return TY == 1 && L <= 16 && KC % 8 == 0 && KC <= 16 ? 8 : 4; }
+// 64-candidate blocks per LDS stage: two while the double-buffered stages stay
+// within 64 KB of LDS.
+template <int KC> constexpr int screen_tb() { return KC <= 16 ? 2 : 1; }
 
-template <int TY, int KC, int L, int NW>
+// A stage holds TB blocks of 64 candidates = 2*TB fragments of 32.  Fragments
+// alternate between two accumulator pairs, and each fragment's list update is
+// issued after the MFMAs of the next one, so the matrix core works while the
+// wave's VALU merges the previous results.
+template <int TY, int KC, int L, int NW, int TB>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_waves<KC, L>()))) void k_knn_screen16(
-    const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t n_pad, float *cand_d, uint32_t *cand_i) {
-    constexpr int KS = KC / 2;    // MFMA K-steps of 16
-    constexpr int TU = KC * kTC;  // 16-byte units per tile
+    const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t nc, int64_t nq, const uint32_t *keys_c,
+    const uint32_t *keys_q, int self, float *cand_d, uint32_t *cand_i) {
+    constexpr int KS = KC / 2;     // MFMA K-steps of 16
+    constexpr int BU = KC * kTC;   // 16-byte units per 64-candidate block
+    constexpr int TU = TB * BU;    // units per stage
     constexpr int NT = NW * 64;
-    static_assert(TU % NT == 0, "tile must split evenly over the workgroup");
+    constexpr int NF = 2 * TB;     // fragments per stage
+    static_assert(TU % NT == 0, "stage must split evenly over the workgroup");
     constexpr int PER = TU / NT;
     __shared__ u32x4 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
@@ -419,39 +579,80 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
         d0[j] = d1[j] = INFINITY;
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
-    const int64_t n_tiles = n_pad / kTC;
-    // Pipeline: LDS holds tile t (being computed) and receives tile t+1 from
-    // registers loaded one iteration earlier; the loads for tile t+2 are issued after
-    // that store, so their latency overlaps the whole next iteration.  (The final
-    // iterations re-load the last tile; those copies are never read.)
-    for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = xa[i];
-    Regs<u32x4, PER> pre;
-    regs_load<u32x4, PER, NT>(pre, xa + (n_tiles > 1 ? 1 : 0) * TU, threadIdx.x);
-    __syncthreads();
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        const int buf = (int)(t & 1);
+    auto mma = [&](const u32x4 *stage, int f, f32x16 &c0, f32x16 &c1) {
+        const u32x4 *base = stage + (f >> 1) * BU + (f & 1) * 32 + col;
+        c0 = f32x16{};
+        c1 = f32x16{};
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            f32x16 a0 = {}, a1 = {};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const u32x4 a = lds[buf][(2 * s + half) * kTC + mt * 32 + col];
-                a0 = mfma16<TY>(a, b0[s], a0);
-                a1 = mfma16<TY>(a, b1[s], a1);
-            }
-            const uint32_t cb = (uint32_t)(t * kTC + mt * 32);
-            if (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32) {
-                self_mask<L>(a0, cb, half, qa);
-                self_mask<L>(a1, cb, half, qb);
-            }
-            topk_update<L>(d0, i0, a0, cb, half);
-            topk_update<L>(d1, i1, a1, cb, half);
+        for (int s = 0; s < KS; ++s) {
+            const u32x4 a = base[(2 * s + half) * kTC];
+            c0 = mfma16<TY>(a, b0[s], c0);
+            c1 = mfma16<TY>(a, b1[s], c1);
         }
-        regs_store<u32x4, PER, NT>(pre, lds[buf ^ 1], threadIdx.x);
-        const int64_t tn = t + 2 < n_tiles ? t + 2 : n_tiles - 1;
-        regs_load<u32x4, PER, NT>(pre, xa + tn * TU, threadIdx.x);
+    };
+    auto update = [&](f32x16 &c0, f32x16 &c1, uint32_t cb) {
+        if (self && (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32)) {
+            self_mask<L>(c0, cb, half, qa);
+            self_mask<L>(c1, cb, half, qb);
+        }
+        topk_update<L>(d0, i0, c0, cb, half);
+        topk_update<L>(d1, i1, c1, cb, half);
+    };
+    const int64_t n_stages = (nc + kPadRows - 1) / kPadRows * (kPadRows / (kTC * TB));
+    // Stages are visited outward from the one holding the middle of this
+    // workgroup's queries (h, h+1, h-1, h+2, ...), so in the spatial order the
+    // nearest candidates come first and the lists tighten early.  For a separate
+    // query set, that is where the middle query's key falls among the sorted
+    // candidate keys.
+    int64_t mid = (int64_t)blockIdx.x * NT + NT / 2;
+    if (keys_q) {
+        mid = mid < nq ? mid : nq - 1;
+        const uint32_t key = keys_q[mid];
+        int64_t lo = 0, hi = nc;
+        while (lo < hi) {
+            const int64_t m2 = (lo + hi) >> 1;
+            if (keys_c[m2] < key) lo = m2 + 1;
+            else hi = m2;
+        }
+        mid = lo;
+    }
+    const int64_t home = mid / (kTC * TB);
+    const int64_t h = home < n_stages ? home : n_stages - 1;
+    const int64_t below = h, above = n_stages - 1 - h, m = below < above ? below : above;
+    auto stage_at = [&](int64_t k) -> int64_t {
+        if (k <= 2 * m) return (k & 1) ? h + (k + 1) / 2 : h - k / 2;
+        return above > below ? h + (k - m) : h - (k - m);
+    };
+    // Pipeline: LDS holds visit t (being computed) and receives visit t+1 from
+    // registers loaded one iteration earlier; the loads for visit t+2 are issued after
+    // that store, so their latency overlaps the whole next iteration.  (The final
+    // iterations re-load the last stage; those copies are never read.)
+    {
+        const u32x4 *first = xa + stage_at(0) * TU;
+        for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = first[i];
+    }
+    Regs<u32x4, PER> pre;
+    regs_load<u32x4, PER, NT>(pre, xa + stage_at(n_stages > 1 ? 1 : 0) * TU, threadIdx.x);
+    __syncthreads();
+    f32x16 x0, x1, y0, y1;  // even fragments in x, odd in y
+    uint32_t cy = 0;        // first candidate of the fragment held in y
+    for (int64_t t = 0; t < n_stages; ++t) {
+        const u32x4 *stage = lds[t & 1];
+        const uint32_t c0 = (uint32_t)(stage_at(t) * kTC * TB);
+#pragma unroll
+        for (int f = 0; f < NF; f += 2) {
+            mma(stage, f, x0, x1);
+            if (f > 0 || t > 0) update(y0, y1, cy);
+            mma(stage, f + 1, y0, y1);
+            cy = c0 + (uint32_t)((f + 1) * 32);
+            update(x0, x1, c0 + (uint32_t)(f * 32));
+        }
+        regs_store<u32x4, PER, NT>(pre, lds[(t & 1) ^ 1], threadIdx.x);
+        const int64_t tn = t + 2 < n_stages ? t + 2 : n_stages - 1;
+        regs_load<u32x4, PER, NT>(pre, xa + stage_at(tn) * TU, threadIdx.x);
         __syncthreads();
     }
+    update(y0, y1, cy);
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
     write_lists<L>(d1, i1, qb, half, cand_d, cand_i);
 }
@@ -461,7 +662,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
 // two MFMAs, so the chunk's four K values are covered by the two half-waves.
 template <int KC, int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 && KC <= 16) ? 2 : 1))) void k_knn_screen_f32(const float2 *__restrict__ xa, const float2 *__restrict__ xb,
-                                                        int64_t n_pad, float *cand_d, uint32_t *cand_i) {
+                                                        int64_t nc_pad, int self, float *cand_d, uint32_t *cand_i) {
     constexpr int TU = KC * 2 * kTC;  // float2 units per tile
     static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
     constexpr int PER = TU / 256;
@@ -483,7 +684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
         d0[j] = d1[j] = INFINITY;
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
-    const int64_t n_tiles = n_pad / kTC;
+    const int64_t n_tiles = nc_pad / kTC;
     // same loop-carried prefetch as the 16-bit screen
     for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
     Regs<float2, PER> pre;
@@ -503,7 +704,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
                 a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b1[j].y, a1, 0, 0, 0);
             }
             const uint32_t cb = (uint32_t)(t * kTC + mt * 32);
-            if (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32) {
+            if (self && (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32)) {
                 self_mask<L>(a0, cb, half, qa);
                 self_mask<L>(a1, cb, half, qb);
             }
@@ -520,19 +721,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
 }
 
 template <int L>
-static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t n_pad, int kpad, float *cd, uint32_t *ci,
+static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int self,
+                            const uint32_t *keys_c, const uint32_t *keys_q, float *cd, uint32_t *ci,
                             hipStream_t s) {
-#define CREST_X16(TY, KCV)                                                                                       \
-    case KCV: {                                                                                                 \
-        constexpr int NW = screen_nw<TY, KCV, L>();                                                             \
-        hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW>), dim3((unsigned)(n_pad / (NW * 64))), dim3(NW * 64), \
-                           0, s, (const u32x4 *)xa, (const u32x4 *)xb, n_pad, cd, ci);                          \
-        return true;                                                                                            \
+    const int64_t nq_pad = knn_rows_padded(nq), nc_pad = knn_rows_padded(nc);
+#define CREST_X16(TY, KCV)                                                                                   \
+    case KCV: {                                                                                             \
+        constexpr int NW = screen_nw<TY, KCV, L>();                                                         \
+        hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW, screen_tb<KCV>()>), dim3((unsigned)(nq_pad / (NW * 64))), \
+                           dim3(NW * 64), 0, s, (const u32x4 *)xa, (const u32x4 *)xb, nc, nq, keys_c, keys_q, \
+                           self, cd, ci);                                                                   \
+        return true;                                                                                        \
     }
-#define CREST_F(KCV)                                                                                              \
-    case KCV:                                                                                                     \
-        hipLaunchKernelGGL((k_knn_screen_f32<KCV, L>), dim3((unsigned)(n_pad / 256)), dim3(256), 0, s,          \
-                           (const float2 *)xa, (const float2 *)xb, n_pad, cd, ci);                               \
+#define CREST_F(KCV)                                                                                          \
+    case KCV:                                                                                                 \
+        hipLaunchKernelGGL((k_knn_screen_f32<KCV, L>), dim3((unsigned)(nq_pad / 256)), dim3(256), 0, s,      \
+                           (const float2 *)xa, (const float2 *)xb, nc_pad, self, cd, ci);                    \
         return true;
     if (prec == kPrecBF16X3) {
         switch (kpad / 8) {
@@ -551,15 +755,15 @@ static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t n_
     return false;
 }
 
-// Screen with lists of L per lane (L in {16, 32, 64}, L >= k); cand arrays hold 2L per
-// query for knn_rows_padded(n) rows.  Returns false when no kernel is instantiated for
-// (prec, kpad, L).
-bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t n, int kpad, int L, float *cand_d,
-                       uint32_t *cand_i, hipStream_t s) {
-    const int64_t n_pad = knn_rows_padded(n);
-    if (L == 16) return launch_screen_L<16>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);
-    if (L == 32) return launch_screen_L<32>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);
-    if (L == 64) return launch_screen_L<64>(prec, xa, xb, n_pad, kpad, cand_d, cand_i, s);
+// Screen of nq queries against nc candidates (self != 0: the same rows, each
+// query's own row excluded) with lists of L per lane (L in {16, 32, 64}, L >= k);
+// cand arrays hold 2L per query for knn_rows_padded(nq) rows.  Returns false when no
+// kernel is instantiated for (prec, kpad, L).
+bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int L, int self,
+                       const KnnOrder &o, float *cand_d, uint32_t *cand_i, hipStream_t s) {
+    if (L == 16) return launch_screen_L<16>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
+    if (L == 32) return launch_screen_L<32>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
+    if (L == 64) return launch_screen_L<64>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
     return false;
 }
 
@@ -591,14 +795,18 @@ __device__ inline float ref_sq_dist(const float *a, const float *b, int p) {
 // (ties by index); the k best are reported with the reference's FP32 distance and
 // ordered by (distance, index) as knn.rs:356-367 does.
 template <int NCL>
-__global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ x, int64_t n, int p,
+__global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xqs, int64_t nq,
+                                                    const float *__restrict__ x, int64_t n, int p,
+                                                    const uint32_t *__restrict__ perm_q,
+                                                    const uint32_t *__restrict__ perm, int self,
                                                     const uint32_t *__restrict__ cand_i, int n_cand, int k,
                                                     uint32_t *idx_out, float *dist_out) {
     const int lane = threadIdx.x & 63;
-    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (q >= n) return;
-    const float *xq = x + q * (int64_t)p;
-    const uint32_t *ci = cand_i + q * (int64_t)n_cand;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;  // sorted query position
+    if (i >= nq) return;
+    const int64_t q = perm_q[i];
+    const float *xq = xqs + q * (int64_t)p;
+    const uint32_t *ci = cand_i + i * (int64_t)n_cand;
     double dd[NCL];
     uint32_t cc[NCL];
     bool vv[NCL];
@@ -609,8 +817,9 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ x,
         vv[s] = false;
     }
     for (int j = 0; j < n_cand; ++j) {
-        const uint32_t c = ci[j];
-        const bool valid = (int64_t)c < n && (int64_t)c != q;
+        const uint32_t cpos = ci[j];
+        const bool valid = (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+        const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
         double part = 0.0;
         if (valid) {
             const float *xc = x + (int64_t)c * p;
@@ -664,14 +873,16 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ x,
         }
 }
 
-void launch_knn_rerank(const float *x, int64_t n, int p, const float *cand_d, const uint32_t *cand_i, int n_cand,
-                       int k, uint32_t *idx_out, float *dist_out, hipStream_t s) {
-    (void)cand_d;
-    const int g = (int)ceil_div(n, 4);
+void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o, int self,
+                       const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
+                       hipStream_t s) {
+    const int g = (int)ceil_div(nq, 4);
     if (n_cand <= 64)
-        hipLaunchKernelGGL(k_knn_rerank<1>, dim3(g), dim3(256), 0, s, x, n, p, cand_i, n_cand, k, idx_out, dist_out);
+        hipLaunchKernelGGL(k_knn_rerank<1>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
+                           cand_i, n_cand, k, idx_out, dist_out);
     else
-        hipLaunchKernelGGL(k_knn_rerank<2>, dim3(g), dim3(256), 0, s, x, n, p, cand_i, n_cand, k, idx_out, dist_out);
+        hipLaunchKernelGGL(k_knn_rerank<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
+                           cand_i, n_cand, k, idx_out, dist_out);
 }
 
 }  // namespace crest

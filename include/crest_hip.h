@@ -119,6 +119,16 @@ int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub
 int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact,
                     uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
 
+/* Replaces py.rs:635-653 `knn_query(reference, query, k, exact=None, seed=0)`
+ * (knn.rs:485-546), the reference-to-query search behind crest.ingest.  reference
+ * is [n_ref, dim] and query [n_query, dim], both row-major f32.  k' = min(k, n_ref)
+ * (0 when n_query == 0) is written to *k_out; indices [n_query, k'] u32 and
+ * distances [n_query, k'] f32 (Euclidean, ascending, ties by index) must hold
+ * n_query*min(k, n_ref) entries.  Exact for every size: `exact` and `seed` are
+ * accepted and ignored.                                                          */
+int crest_knn_query(const float *reference, size_t n_ref, const float *query, size_t n_query, size_t dim,
+                    size_t k, int32_t exact, uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
+
 /* Replaces py.rs:72-99 `connectivities(indices, distances, n_neighbors)`
  * (umap/graph.rs:29-107, scanpy's fuzzy simplicial set).  indices/distances are
  * [n, k] as knn_graph returns them; writes the undirected edges with source <

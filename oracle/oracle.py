@@ -224,6 +224,46 @@ def knn_exact_f64(x, k, queries=None):
     return out_i, out_d
 
 
+def knn_query_exact_f64(ref, query, k):
+    """FP64 exact search of each query row among the reference rows (knn.rs:485-546
+    with exact distances): returns (indices [nq, k], squared distances [nq, k+1]
+    -- the extra column, +inf when n_ref == k, is the (k+1)-th for tie accounting),
+    ordered by (distance, index)."""
+    r = np.asarray(ref, np.float64)
+    qx = np.asarray(query, np.float64)
+    nr = len(r)
+    m = min(k + 1, nr)
+    nrm = (r * r).sum(1)
+    out_i = np.empty((len(qx), k), np.int64)
+    out_d = np.full((len(qx), k + 1), np.inf)
+    for s in range(0, len(qx), 256):
+        xq = qx[s:s + 256]
+        d2 = (xq * xq).sum(1)[:, None] + nrm[None, :] - 2.0 * (xq @ r.T)
+        part = np.argpartition(d2, m - 1, axis=1)[:, :m] if m < nr else np.tile(np.arange(nr), (len(xq), 1))
+        diff = xq[:, None, :] - r[part]
+        pd = (diff * diff).sum(-1)
+        o = np.lexsort((part, pd), axis=1)
+        part = np.take_along_axis(part, o, 1)
+        pd = np.take_along_axis(pd, o, 1)
+        out_i[s:s + 256] = part[:, :k]
+        out_d[s:s + 256, :m] = pd
+    return out_i, out_d
+
+
+def knn_query_mismatch_ties(idx_got, ref, query, ref_idx, ref_d2, rtol=1e-5):
+    """knn_mismatch_ties for a separate query set."""
+    r = np.asarray(ref, np.float64)
+    qx = np.asarray(query, np.float64)
+    k = ref_idx.shape[1]
+    bad = np.nonzero(~(np.sort(idx_got, 1) == np.sort(ref_idx, 1)).all(1))[0]
+    ties = 0
+    for b in bad:
+        d = ((r[idx_got[b].astype(np.int64)] - qx[b]) ** 2).sum(1)
+        if np.all(d <= ref_d2[b, k - 1] * (1 + rtol) + 1e-12):
+            ties += 1
+    return len(bad), ties
+
+
 # ------------------------------------------------------------------ seurat HVG (pandas)
 def hvg_seurat_pandas(mean_expm1, var_expm1, n_top_genes=None, n_bins=20, min_mean=0.0125, max_mean=3.0,
                       min_disp=0.5, max_disp=np.inf):

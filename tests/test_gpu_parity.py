@@ -205,6 +205,57 @@ def test_knn_too_wide():
         ca.set_knn_precision(None)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
+def test_knn_query_exact(prec):
+    """tests/test_crest.py:504-511: indices equal numpy argsort, distances rtol 1e-4."""
+    rng = np.random.default_rng(0)
+    R = rng.normal(size=(3000, 10)).astype(np.float32)
+    Q = rng.normal(size=(200, 10)).astype(np.float32)
+    ca.set_knn_precision(prec)
+    try:
+        idx, dist = N.knn_query(R, Q, 5)
+    finally:
+        ca.set_knn_precision(None)
+    D = ((Q[:, None, :] - R[None, :, :]) ** 2).sum(-1)
+    np.testing.assert_array_equal(idx, np.argsort(D, axis=1)[:, :5])
+    np.testing.assert_allclose(dist, np.sqrt(np.sort(D, axis=1)[:, :5]), rtol=1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
+@pytest.mark.parametrize("nr,nq,d,k,shift", [(5000, 3000, 50, 15, 0.0), (700, 1300, 7, 30, 0.0),
+                                             (40, 600, 3, 64, 0.0), (2000, 900, 20, 15, 4.0)])
+def test_knn_query_clustered(oracle, prec, nr, nq, d, k, shift):
+    R = _clustered(nr, d, nr + d)
+    Q = (_clustered(nq, d, nq + 7 * d) + np.float32(shift)).astype(np.float32)  # shift: query set offset
+    ca.set_knn_precision(prec)
+    try:
+        idx, dist = N.knn_query(R, Q, k)
+    finally:
+        ca.set_knn_precision(None)
+    kk = min(k, nr)
+    assert idx.shape == (nq, kk) and dist.shape == (nq, kk)
+    assert (np.diff(dist, axis=1) >= 0).all()
+    ref_i, ref_d2 = oracle.knn_query_exact_f64(R, Q, kk)
+    bad, ties = oracle.knn_query_mismatch_ties(idx, R, Q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    X = np.vstack([Q, R])
+    pd = oracle.pair_dist(X, np.repeat(np.arange(nq), kk), nq + idx.ravel()).reshape(nq, kk)
+    np.testing.assert_array_equal(dist, pd)
+
+
+def test_knn_query_edges():
+    R = np.arange(12, dtype=np.float32).reshape(6, 2)
+    idx, dist = N.knn_query(R, np.zeros((0, 2), np.float32), 3)
+    assert idx.shape == (0, 0) and dist.shape == (0, 0)
+    idx, dist = N.knn_query(R, R[:2].copy(), 10)  # k > n_ref: all reference rows
+    assert idx.shape == (2, 6)
+    assert idx[0].tolist() == [0, 1, 2, 3, 4, 5] and dist[0, 0] == 0.0
+    with pytest.raises(ValueError):
+        N.knn_query(R, np.zeros((3, 3), np.float32), 2)
+    with pytest.raises(TypeError):
+        N.knn_query(R.astype(np.float64), R, 2)
+
+
 def test_knn_small_edge_cases():
     idx, dist = N.knn_graph(np.zeros((1, 4), np.float32), 5)
     assert idx.shape == (1, 0)

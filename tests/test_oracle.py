@@ -302,3 +302,19 @@ def test_connectivities_pins(oracle):
     assert abs(np.where(d - rho > 0, np.exp(-(d - rho) / lo), 1.0).sum() - np.log2(10)) < 1e-4
     r0, c0, w0 = oracle.connectivities(np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.float32), 4)
     assert len(r0) == 0
+
+
+def test_knn_query_oracle_pins(oracle):
+    """tests/test_crest.py:504-511 restated on the oracle's exact query search: the
+    indices equal numpy's argsort of the full distance matrix; an off-by-one in the
+    partition or a transposed operand would change them."""
+    rng = np.random.default_rng(0)
+    R = rng.normal(size=(3000, 10)).astype(np.float32)
+    Q = rng.normal(size=(200, 10)).astype(np.float32)
+    idx, d2 = oracle.knn_query_exact_f64(R, Q, 5)
+    D = ((Q[:, None, :].astype(np.float64) - R[None, :, :]) ** 2).sum(-1)
+    np.testing.assert_array_equal(idx, np.argsort(D, axis=1, kind="stable")[:, :5])
+    np.testing.assert_allclose(d2, np.sort(D, axis=1)[:, :6], rtol=1e-12)
+    # k == n_ref: every reference row, the (k+1)-th slot is +inf
+    idx, d2 = oracle.knn_query_exact_f64(R[:4], Q[:3], 4)
+    assert sorted(idx[0].tolist()) == [0, 1, 2, 3] and np.isinf(d2[:, 4]).all()
