@@ -540,16 +540,21 @@ template <int TY> __device__ inline f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
 // B fragments, both accumulator pairs and the lists leave room in 256 VGPRs
 // (checked against the compiler's register report: no spills).  Eight-wave
 // workgroups (half the tile traffic per query) are used where that budget holds.
-template <int KC, int L> constexpr int screen_waves() { return (L <= 16 && KC <= 16) ? 2 : 1; }
+// The software pipeline (two accumulator pairs, below) is used where it keeps two
+// waves per SIMD, or where one wave per SIMD is all that fits anyway.
+template <int KC, int L> constexpr bool screen_pipe() { return (L <= 16 && KC <= 16) || !(L <= 16 && KC <= 20); }
+template <int KC, int L> constexpr int screen_waves() {
+    return screen_pipe<KC, L>() ? ((L <= 16 && KC <= 16) ? 2 : 1) : 2;
+}
 template <int TY, int KC, int L> constexpr int screen_nw() { return TY == 1 && L <= 16 && KC % 8 == 0 && KC <= 16 ? 8 : 4; }
-// 64-candidate blocks per LDS stage: two while the double-buffered stages stay
-// within 64 KB of LDS.
-template <int KC> constexpr int screen_tb() { return KC <= 16 ? 2 : 1; }
+// 64-candidate blocks per LDS stage: as many (up to four) as keep the
+// double-buffered stages within 64 KB of LDS.
+template <int KC> constexpr int screen_tb() { return KC <= 8 ? 4 : KC <= 16 ? 2 : 1; }
 
-// A stage holds TB blocks of 64 candidates = 2*TB fragments of 32.  Fragments
-// alternate between two accumulator pairs, and each fragment's list update is
-// issued after the MFMAs of the next one, so the matrix core works while the
-// wave's VALU merges the previous results.
+// A stage holds TB blocks of 64 candidates = 2*TB fragments of 32.  Pipelined,
+// fragments alternate between two accumulator pairs, and each fragment's list
+// update is issued after the MFMAs of the next one, so the matrix core works while
+// the wave's VALU merges the previous results.
 template <int TY, int KC, int L, int NW, int TB>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_waves<KC, L>()))) void k_knn_screen16(
     const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t nc, int64_t nq, const uint32_t *keys_c,
@@ -563,7 +568,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     constexpr int PER = TU / NT;
     __shared__ u32x4 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
-    const int64_t q0 = (int64_t)blockIdx.x * NT + wave * kQW;
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of query blocks, so the workgroups sharing an XCD's L2 have
+    // neighbouring home stages and walk overlapping candidate streams.
+    const int64_t nb = gridDim.x, xcd = blockIdx.x % 8, per = nb / 8, rem = nb % 8;
+    const int64_t wg = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / 8;
+    const int64_t q0 = wg * NT + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
     const u32x4 *qbase = xb + (q0 / 64) * KC * 64;
     u32x4 b0[KS], b1[KS];
@@ -603,8 +613,13 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     // workgroup's queries (h, h+1, h-1, h+2, ...), so in the spatial order the
     // nearest candidates come first and the lists tighten early.  For a separate
     // query set, that is where the middle query's key falls among the sorted
-    // candidate keys.
-    int64_t mid = (int64_t)blockIdx.x * NT + NT / 2;
+    // candidate keys.  The workgroups that run together on one XCD (32 CUs, G at a
+    // time, dispatched in order) share one home: the middle of their joint query
+    // range, so they walk the same stages in step and read them through the L2 once.
+    constexpr int64_t G = 32 * 4 * screen_waves<KC, L>() / NW;
+    const int64_t jx = blockIdx.x / 8, cnt = per + (xcd < rem ? 1 : 0), g0 = (jx / G) * G;
+    const int64_t gn = cnt - g0 < G ? cnt - g0 : G;
+    int64_t mid = (wg - (jx - g0)) * NT + gn * NT / 2;
     if (keys_q) {
         mid = mid < nq ? mid : nq - 1;
         const uint32_t key = keys_q[mid];
@@ -639,20 +654,28 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     for (int64_t t = 0; t < n_stages; ++t) {
         const u32x4 *stage = lds[t & 1];
         const uint32_t c0 = (uint32_t)(stage_at(t) * kTC * TB);
+        if constexpr (screen_pipe<KC, L>()) {
 #pragma unroll
-        for (int f = 0; f < NF; f += 2) {
-            mma(stage, f, x0, x1);
-            if (f > 0 || t > 0) update(y0, y1, cy);
-            mma(stage, f + 1, y0, y1);
-            cy = c0 + (uint32_t)((f + 1) * 32);
-            update(x0, x1, c0 + (uint32_t)(f * 32));
+            for (int f = 0; f < NF; f += 2) {
+                mma(stage, f, x0, x1);
+                if (f > 0 || t > 0) update(y0, y1, cy);
+                mma(stage, f + 1, y0, y1);
+                cy = c0 + (uint32_t)((f + 1) * 32);
+                update(x0, x1, c0 + (uint32_t)(f * 32));
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                mma(stage, f, x0, x1);
+                update(x0, x1, c0 + (uint32_t)(f * 32));
+            }
         }
         regs_store<u32x4, PER, NT>(pre, lds[(t & 1) ^ 1], threadIdx.x);
         const int64_t tn = t + 2 < n_stages ? t + 2 : n_stages - 1;
         regs_load<u32x4, PER, NT>(pre, xa + stage_at(tn) * TU, threadIdx.x);
         __syncthreads();
     }
-    update(y0, y1, cy);
+    if constexpr (screen_pipe<KC, L>()) update(y0, y1, cy);
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
     write_lists<L>(d1, i1, qb, half, cand_d, cand_i);
 }
