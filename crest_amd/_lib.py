@@ -70,6 +70,8 @@ _SIGS = {
     "crest_synchronize": (C.c_int, []),
     "crest_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "crest_gene_stats": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t]),
+    "crest_qc": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
     "crest_gram_accumulate": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                         C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,
                                         C.c_size_t, C.c_double]),

@@ -14,7 +14,7 @@ import numpy as np
 
 from ._lib import Chunk, check, lib
 
-__all__ = ["gene_stats", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "knn_query",
+__all__ = ["gene_stats", "qc", "gram_accumulate", "pca_from_gram", "project", "knn_graph", "knn_query",
            "connectivities", "materialize"]
 
 
@@ -75,6 +75,22 @@ def gene_stats(genes, values, cell_map, gene_map, target_sum, log1p, cells, indp
     ch, _keep = _chunk(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, first_cell)
     out = _arr(out, np.float64, "out", writable=True)
     check(lib().crest_gene_stats(C.byref(ch), _p(out), out.size))
+
+
+def qc(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, first_cell,
+       flag, cell_total, cell_ngenes, cell_flag, gene_ncells, gene_total):
+    """py.rs:234-248: QC accumulation on raw counts (pass target_sum=0, log1p=False):
+    per row total / genes detected / total over flagged genes, per gene cells detected /
+    total, all accumulated (+=)."""
+    ch, _keep = _chunk(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, first_cell)
+    flag = _arr(flag, np.bool_, "flag")
+    ct = _arr(cell_total, np.float64, "cell_total", writable=True)
+    cn = _arr(cell_ngenes, np.uint32, "cell_ngenes", writable=True)
+    cf = _arr(cell_flag, np.float64, "cell_flag", writable=True)
+    gn = _arr(gene_ncells, np.uint32, "gene_ncells", writable=True)
+    gt = _arr(gene_total, np.float64, "gene_total", writable=True)
+    check(lib().crest_qc(C.byref(ch), _p(flag), flag.size, _p(ct), ct.size, _p(cn), cn.size, _p(cf), cf.size,
+                         _p(gn), gn.size, _p(gt), gt.size))
 
 
 def gram_accumulate(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, first_cell,

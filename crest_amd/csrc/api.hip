@@ -471,6 +471,49 @@ int crest_gene_stats(const crest_chunk *chunk, double *out, size_t out_len) {
     });
 }
 
+int crest_qc(const crest_chunk *chunk, const uint8_t *flag, size_t n_flag, double *cell_total, size_t n_cell_total,
+             uint32_t *cell_ngenes, size_t n_cell_ngenes, double *cell_flag, size_t n_cell_flag, uint32_t *gene_ncells,
+             size_t n_gene_ncells, double *gene_total, size_t n_gene_total) {
+    return guarded([&] {
+        HostChunk h = check_chunk(chunk);
+        const int64_t nr = h.n_rows, nv = h.n_vars;
+        // py.rs:242-244
+        if ((int64_t)n_cell_total < nr || (int64_t)n_cell_ngenes < nr || (int64_t)n_cell_flag < nr ||
+            (int64_t)n_gene_ncells != nv || (int64_t)n_gene_total != nv || (int64_t)n_flag != nv)
+            value_error("output sizes do not match the maps");
+        if ((nr && (!cell_total || !cell_ngenes || !cell_flag)) || (nv && (!gene_ncells || !gene_total || !flag)))
+            value_error("NULL argument");
+        if (h.n_spans == 0 || nr == 0 || nv == 0) return;
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        DevChunk c = upload_chunk(x, h);
+        const uint8_t *df = x.upload("qc_flag", flag, (size_t)nv);
+        double *rows = x.get<double>("qc_rows", (size_t)nr * 3);
+        double *gt = x.get<double>("qc_gene_total", (size_t)nv);
+        uint32_t *gn = x.get<uint32_t>("qc_gene_ncells", (size_t)nv);
+        launch_fill_f64(rows, nr * 3, 0.0, x.st);
+        launch_fill_f64(gt, nv, 0.0, x.st);
+        launch_fill_u32(gn, nv, 0u, x.st);
+        launch_qc(c, df, rows, rows + nr, rows + 2 * nr, gn, gt, x.st);
+        x.check_launch();
+        std::vector<double> hr((size_t)nr * 3), hgt((size_t)nv);
+        std::vector<uint32_t> hgn((size_t)nv);
+        x.download(hr.data(), rows, hr.size());
+        x.download(hgt.data(), gt, hgt.size());
+        x.download(hgn.data(), gn, hgn.size());
+        x.sync();
+        for (int64_t i = 0; i < nr; ++i) {
+            cell_total[i] += hr[i];
+            cell_ngenes[i] += (uint32_t)hr[nr + i];
+            cell_flag[i] += hr[2 * nr + i];
+        }
+        for (int64_t g = 0; g < nv; ++g) {
+            gene_ncells[g] += hgn[g];
+            gene_total[g] += hgt[g];
+        }
+    });
+}
+
 int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub_map, double *gram,
                           size_t gram_len, double *colsum, size_t d, int32_t scale, const double *mean, size_t n_mean,
                           const double *std_, size_t n_std, double max_value) {

@@ -85,6 +85,8 @@ struct DevScale {
 
 // ---- kernel launchers (each defined next to its kernel) ----------------------
 void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s);
+void launch_qc(const DevChunk &c, const uint8_t *flag, double *row_total, double *row_ngenes, double *row_flag,
+               uint32_t *gene_ncells, double *gene_total, hipStream_t s);
 void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b,
                     int64_t rows_pad, int d_pad, double *colsum, hipStream_t s);
 void launch_materialize_count(const DevChunk &c, int64_t *row_kept, hipStream_t s);
@@ -126,6 +128,7 @@ void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, in
                            uint32_t *pos, double *part, void *temp, size_t temp_bytes, uint32_t *rows,
                            uint32_t *cols, float *wout, uint32_t *count, hipStream_t s);
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
+void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

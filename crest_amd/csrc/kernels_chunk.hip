@@ -15,6 +15,14 @@ __global__ void k_fill_f64(double *p, int64_t n, double v) {
 __global__ void k_fill_f32(float *p, int64_t n, float v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    int g = (int)std::min<int64_t>(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, n, v);
+}
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return;
     int g = (int)std::min<int64_t>(ceil_div(n, 256), 4096);
@@ -80,6 +88,53 @@ void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s
     if (c.n_spans <= 0 || n_vars <= 0) return;
     int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 16), 1024);
     hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(256), 0, s, c, out, n_vars);
+}
+
+// ---------------------------------------------------------------- qc
+// kernels.rs:162-195 with one wave per cell: for every kept entry whose (transformed)
+// value is non-zero, the row gets its total, its count of such genes and its total
+// over flagged genes, and the gene its cell count and total.  FP64 lane partial sums
+// and atomics: exact for integer counts, whose FP64 sums below 2^53 do not depend on
+// the order of addition (the reference's own order depends on its thread count).
+__global__ __launch_bounds__(256) void k_qc(DevChunk c, const uint8_t *flag, double *row_total, double *row_ngenes,
+                                            double *row_flag, uint32_t *gene_ncells, double *gene_total) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int nwave = blockDim.x / kWave;
+    const int64_t base = c.base;
+    for (int64_t r = (int64_t)blockIdx.x * nwave + wave; r < c.n_spans; r += (int64_t)gridDim.x * nwave) {
+        const int64_t row = chunk_row(c, r);
+        if (row < 0) continue;
+        const int64_t lo = c.span_off[r] - base, hi = c.span_off[r + 1] - base;
+        const double ns = norm_factor(cell_total(c, lo, hi, lane), c.target_sum);
+        double t = 0.0, f = 0.0, k = 0.0;
+        for (int64_t e = lo + lane; e < hi; e += kWave) {
+            const int32_t m = chunk_var(c, c.genes[e]);
+            if (m < 0) continue;
+            const float x = lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+            if (x == 0.0f) continue;
+            t += (double)x;
+            k += 1.0;
+            if (flag[m]) f += (double)x;
+            atomicAdd(&gene_ncells[m], 1u);
+            atomicAdd(&gene_total[m], (double)x);
+        }
+        t = wave_sum_f64(t);
+        f = wave_sum_f64(f);
+        k = wave_sum_f64(k);
+        if (lane == 0) {
+            atomicAdd(&row_total[row], t);
+            atomicAdd(&row_ngenes[row], k);
+            atomicAdd(&row_flag[row], f);
+        }
+    }
+}
+
+void launch_qc(const DevChunk &c, const uint8_t *flag, double *row_total, double *row_ngenes, double *row_flag,
+               uint32_t *gene_ncells, double *gene_total, hipStream_t s) {
+    if (c.n_spans <= 0) return;
+    const int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 4), 4096);
+    hipLaunchKernelGGL(k_qc, dim3(g), dim3(256), 0, s, c, flag, row_total, row_ngenes, row_flag, gene_ncells,
+                       gene_total);
 }
 
 // ---------------------------------------------------------------- densify

@@ -9,10 +9,35 @@ from typing import Optional
 import numpy as np
 
 from . import _native
-from .core import BioFrame
+from .core import BioFrame, Frame
 
-__all__ = ["normalize_total", "log1p", "scale", "gene_stats", "highly_variable_genes", "neighbors",
+__all__ = ["calculate_qc_metrics", "normalize_total", "log1p", "scale", "gene_stats", "highly_variable_genes", "neighbors",
            "connectivities_matrix"]
+
+
+def calculate_qc_metrics(bf: BioFrame, mito_prefix: Optional[str] = "MT-", inplace: bool = True):
+    """pp.py:25-47: per-cell ``total_counts``, ``n_genes_by_counts``, ``pct_counts_mt``
+    and per-gene ``n_cells_by_counts``, ``total_counts`` on the raw counts of the kept
+    cells and genes (the accumulation runs in the GPU qc kernel)."""
+    names = bf.var_names
+    flag = np.array([bool(mito_prefix) and n.upper().startswith(mito_prefix.upper()) for n in names], dtype=bool)
+    ct = np.zeros(bf.n_obs, np.float64)
+    cn = np.zeros(bf.n_obs, np.uint32)
+    cf = np.zeros(bf.n_obs, np.float64)
+    gn = np.zeros(bf.n_vars, np.uint32)
+    gt = np.zeros(bf.n_vars, np.float64)
+    for ctx in bf.iter_ctx(transform=False):
+        _native.qc(*ctx, flag, ct, cn, cf, gn, gt)
+    pct = np.divide(cf, ct, out=np.zeros_like(ct), where=ct > 0) * 100.0
+    obs = Frame({"n_genes_by_counts": cn.astype(np.int64), "total_counts": ct, "pct_counts_mt": pct})
+    var = Frame({"n_cells_by_counts": gn.astype(np.int64), "total_counts": gt})
+    if not inplace:
+        return obs, var
+    for k in obs.columns:
+        bf.obs[k] = obs[k]
+    for k in var.columns:
+        bf.var[k] = var[k]
+    return None
 
 
 def normalize_total(bf: BioFrame, target_sum: float = 1e4) -> BioFrame:

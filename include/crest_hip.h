@@ -85,6 +85,18 @@ int32_t crest_device_count(void);
  * over the transformed values of kept cells and genes; n_vars = max(gene_map)+1. */
 int crest_gene_stats(const crest_chunk *chunk, double *out, size_t out_len);
 
+/* Replaces py.rs:234-248 `qc(*ctx, flag, cell_total, cell_ngenes, cell_flag,
+ * gene_ncells, gene_total)` (kernels.rs:162-195), the accumulation behind
+ * crest.pp.calculate_qc_metrics (call with an un-transformed chunk).  For every kept
+ * entry with a non-zero value: cell_total[row] (+=) Σx, cell_ngenes[row] (+=) count,
+ * cell_flag[row] (+=) Σx over genes with flag[var] != 0; gene_ncells[var] (+=) count,
+ * gene_total[var] (+=) Σx.  The row arrays hold at least n_rows = max(cell_map)+1
+ * entries; flag and the gene arrays exactly n_vars.  Sums are FP64 and exact for
+ * integer counts.                                                                  */
+int crest_qc(const crest_chunk *chunk, const uint8_t *flag, size_t n_flag, double *cell_total, size_t n_cell_total,
+             uint32_t *cell_ngenes, size_t n_cell_ngenes, double *cell_flag, size_t n_cell_flag,
+             uint32_t *gene_ncells, size_t n_gene_ncells, double *gene_total, size_t n_gene_total);
+
 /* Replaces py.rs:270-287 `gram_accumulate(*ctx, sub_map, gram, colsum, scale,
  * mean, std, max_value)` (kernels.rs:197-309).  gram (+=) is [d, d] row-major
  * f64, of which the upper triangle a <= b is written; colsum (+=) is [d].

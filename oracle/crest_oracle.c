@@ -159,6 +159,40 @@ void oc_gene_stats(const oc_chunk *v, double *out) {
     free(s);
 }
 
+/* kernels.rs:162-195 qc: per row (cell_map) the total, the count and the flagged
+ * total of the kept non-zero values; per gene the count and the total.  Sequential,
+ * in chunk order (the reference's order inside a cell). */
+void oc_qc(const oc_chunk *v, const uint8_t *flag, double *cell_total, uint32_t *cell_ngenes, double *cell_flag,
+           uint32_t *gene_ncells, double *gene_total) {
+    span *s;
+    const size_t ns = oc_spans(v, &s);
+    const size_t mx = max_span(s, ns);
+    uint32_t *bv = malloc(mx * sizeof(uint32_t));
+    float *bx = malloc(mx * sizeof(float));
+    for (size_t i = 0; i < ns; ++i) {
+        size_t bn;
+        const int64_t row = oc_cell(v, &s[i], bv, bx, &bn);
+        if (row < 0) continue;
+        double t = 0.0, f = 0.0;
+        uint32_t n = 0;
+        for (size_t j = 0; j < bn; ++j) {
+            const float x = bx[j];
+            if (x == 0.0f) continue;
+            t += (double)x;
+            n += 1;
+            if (flag[bv[j]]) f += (double)x;
+            gene_ncells[bv[j]] += 1;
+            gene_total[bv[j]] += (double)x;
+        }
+        cell_total[row] += t;
+        cell_ngenes[row] += n;
+        cell_flag[row] += f;
+    }
+    free(bv);
+    free(bx);
+    free(s);
+}
+
 /* kernels.rs:209-218 Transform::apply */
 static double clip_mv(double z, double mv) {
     if (mv > 0.0) {

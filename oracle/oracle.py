@@ -51,7 +51,7 @@ def lib():
         _LIB = C.CDLL(str(path))
         _LIB.oc_materialize.restype = C.c_size_t
         for f in ("oc_set_log1p_mode", "oc_gene_stats", "oc_gram_accumulate", "oc_cov_from_gram", "oc_project",
-                  "oc_knn", "oc_pair_dist"):
+                  "oc_knn", "oc_pair_dist", "oc_qc"):
             getattr(_LIB, f).restype = None
         _LIB.oc_gram_accumulate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                             C.c_void_p, C.c_void_p, C.c_double, C.c_int]
@@ -63,6 +63,7 @@ def lib():
         _LIB.oc_pair_dist.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
         _LIB.oc_materialize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB.oc_gene_stats.argtypes = [C.c_void_p, C.c_void_p]
+        _LIB.oc_qc.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         _LIB.oc_set_log1p_mode.argtypes = [C.c_int]
         _LIB.oc_connectivities.restype = C.c_size_t
         _LIB.oc_connectivities.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
@@ -100,6 +101,15 @@ def gene_stats(*ctx_out):
     *ctx, out = ctx_out
     ch, _k = _chunk(*ctx)
     lib().oc_gene_stats(C.byref(ch), _p(out))
+
+
+def qc(*args):
+    """kernels.rs:162-195 (crest_oracle.c oc_qc): (*ctx, flag, cell_total, cell_ngenes,
+    cell_flag, gene_ncells, gene_total), all accumulated in place."""
+    ctx, (flag, ct, cn, cf, gn, gt) = args[:9], args[9:]
+    ch, _k = _chunk(*ctx)
+    flag = np.ascontiguousarray(flag, np.uint8)
+    lib().oc_qc(C.byref(ch), _p(flag), _p(ct), _p(cn), _p(cf), _p(gn), _p(gt))
 
 
 def gram_accumulate(*args, threads: int = 1):

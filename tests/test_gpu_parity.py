@@ -71,6 +71,46 @@ def test_gene_stats(counts, oracle):
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("chunk_nnz", [None, 5000])
+def test_qc_bit_exact(counts, oracle, chunk_nnz):
+    """kernels.rs:162-195: integer counts, so every sum is exact and the GPU result
+    equals the oracle's bit for bit (with filtered cells/genes and chunking)."""
+    bf = _frame(counts, chunk_nnz)
+    rng = np.random.default_rng(1)
+    flag = rng.random(bf.n_vars) < 0.05
+    got = [np.zeros(bf.n_obs), np.zeros(bf.n_obs, np.uint32), np.zeros(bf.n_obs), np.zeros(bf.n_vars, np.uint32),
+           np.zeros(bf.n_vars)]
+    ref = [a.copy() for a in got]
+    for ctx in bf.iter_ctx(transform=False):
+        N.qc(*ctx, flag, *got)
+        oracle.qc(*ctx, flag, *ref)
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)
+    assert got[0].sum() > 0 and got[3].sum() > 0
+
+
+def test_calculate_qc_metrics(counts):
+    """pp.py:25-47 against dense numpy on the kept cells and genes."""
+    indptr, indices, data, g = counts
+    bf = ca.BioFrame.from_csr(indptr, indices, data, g)
+    rng = np.random.default_rng(2)
+    bf = bf.filter_cells(rng.random(bf.n_obs) > 0.1).filter_genes(rng.random(bf.n_vars) > 0.2)
+    bf.var["gene_name"] = np.array([("MT-%d" % i) if i % 11 == 0 else ("G%d" % i) for i in range(bf.n_vars)])
+    obs, var = ca.pp.calculate_qc_metrics(bf, inplace=False)
+    D = bf.to_dense(transform=False).astype(np.float64)
+    mt = np.array([n.startswith("MT-") for n in bf.var_names])
+    np.testing.assert_array_equal(obs["total_counts"], D.sum(1))
+    np.testing.assert_array_equal(obs["n_genes_by_counts"], (D != 0).sum(1))
+    pct = np.divide(D[:, mt].sum(1), D.sum(1), out=np.zeros(len(D)), where=D.sum(1) > 0) * 100.0
+    np.testing.assert_array_equal(obs["pct_counts_mt"], pct)
+    np.testing.assert_array_equal(var["n_cells_by_counts"], (D != 0).sum(0))
+    np.testing.assert_array_equal(var["total_counts"], D.sum(0))
+    with pytest.raises(ValueError):
+        ctx = next(bf.iter_ctx(transform=False))
+        N.qc(*ctx, np.zeros(3, bool), np.zeros(bf.n_obs), np.zeros(bf.n_obs, np.uint32), np.zeros(bf.n_obs),
+             np.zeros(bf.n_vars, np.uint32), np.zeros(bf.n_vars))
+
+
 def _gram_inputs(bf, oracle, d_top=300):
     st = np.zeros((bf.n_vars, 5))
     for ctx in bf.iter_ctx():

@@ -318,3 +318,46 @@ def test_knn_query_oracle_pins(oracle):
     # k == n_ref: every reference row, the (k+1)-th slot is +inf
     idx, d2 = oracle.knn_query_exact_f64(R[:4], Q[:3], 4)
     assert sorted(idx[0].tolist()) == [0, 1, 2, 3] and np.isinf(d2[:, 4]).all()
+
+
+def _qc_ctx(indptr, indices, data, n_cells, n_genes, cmap=None, gmap=None, lo=0, hi=None):
+    hi = n_cells if hi is None else hi
+    a, b = int(indptr[lo]), int(indptr[hi])
+    cmap = np.arange(n_cells, dtype=np.int64) if cmap is None else cmap
+    gmap = np.arange(n_genes, dtype=np.int32) if gmap is None else gmap
+    return (np.ascontiguousarray(indices[a:b], np.uint32), np.ascontiguousarray(data[a:b], np.float32), cmap, gmap,
+            0.0, False, None, np.ascontiguousarray(indptr[lo:hi + 1], np.int64), lo)
+
+
+def test_qc_oracle_pins(oracle):
+    """kernels.rs:162-195 on a hand-checked toy: explicit zeros are skipped, a filtered
+    gene and a filtered cell contribute nothing, flagged genes feed cell_flag."""
+    # cells x genes (gene 0 is "MT-"); cell 1 is filtered, gene 3 is filtered
+    indptr = np.array([0, 3, 5, 9], np.int64)
+    indices = np.array([0, 1, 3, 0, 2, 0, 1, 2, 3], np.uint32)
+    data = np.array([2, 0, 7, 5, 1, 1, 4, 3, 9], np.float32)
+    cmap = np.array([0, -1, 1], np.int64)
+    gmap = np.array([0, 1, 2, -1], np.int32)
+    flag = np.array([1, 0, 0], np.uint8)
+    ct, cn, cf = np.zeros(2), np.zeros(2, np.uint32), np.zeros(2)
+    gn, gt = np.zeros(3, np.uint32), np.zeros(3)
+    oracle.qc(*_qc_ctx(indptr, indices, data, 3, 4, cmap, gmap), flag, ct, cn, cf, gn, gt)
+    assert ct.tolist() == [2.0, 8.0] and cn.tolist() == [1, 3] and cf.tolist() == [2.0, 1.0]
+    assert gn.tolist() == [2, 1, 1] and gt.tolist() == [3.0, 4.0, 3.0]
+    # random integer counts: equals the dense numpy reductions, and chunking adds up
+    rng = np.random.default_rng(5)
+    D = rng.poisson(0.7, size=(300, 40)).astype(np.float32)
+    D[rng.random(D.shape) < 0.3] = 0
+    nz = D != 0
+    indptr = np.concatenate([[0], np.cumsum(nz.sum(1))]).astype(np.int64)
+    indices = np.nonzero(nz)[1].astype(np.uint32)
+    data = D[nz]
+    flag = (np.arange(40) % 7 == 0).astype(np.uint8)
+    out = [np.zeros(300), np.zeros(300, np.uint32), np.zeros(300), np.zeros(40, np.uint32), np.zeros(40)]
+    for lo in range(0, 300, 70):
+        oracle.qc(*_qc_ctx(indptr, indices, data, 300, 40, lo=lo, hi=min(300, lo + 70)), flag, *out)
+    np.testing.assert_array_equal(out[0], D.sum(1))
+    np.testing.assert_array_equal(out[1], nz.sum(1))
+    np.testing.assert_array_equal(out[2], D[:, flag.astype(bool)].sum(1))
+    np.testing.assert_array_equal(out[3], nz.sum(0))
+    np.testing.assert_array_equal(out[4], D.sum(0))
