@@ -89,6 +89,11 @@ _SIGS = {
     "crest_materialize": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                     C.POINTER(C.c_size_t)]),
     "crest_pipeline_run": (C.c_int, [C.POINTER(PipelineArgs), C.c_void_p, C.c_void_p, C.POINTER(PipelineResult)]),
+    "crest_pipeline_run_streamed": (C.c_int, [C.POINTER(PipelineArgs), C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_int32, C.c_size_t, C.c_void_p, C.c_void_p,
+                                              C.POINTER(PipelineResult)]),
+    "crest_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "crest_host_free": (C.c_int, [C.c_void_p]),
 }
 
 

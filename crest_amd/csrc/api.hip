@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/crest_hip.h"
@@ -63,15 +64,36 @@ struct Ctx {
     hipStream_t st = nullptr;
     rocblas_handle blas = nullptr;
     std::map<std::string, std::pair<void *, size_t>> bufs;
+    std::map<std::string, std::pair<void *, size_t>> pinned;  // host staging (hipHostMalloc)
     hipEvent_t ev[12] = {};
+    hipStream_t copy = nullptr;        // host -> device uploads of streamed chunks
+    hipEvent_t copied[2] = {}, consumed[2] = {};
 
     void init(int d) {
         dev = d;
         HIPCHK(hipSetDevice(dev));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
         BLASCHK(rocblas_create_handle(&blas));
         BLASCHK(rocblas_set_stream(blas, st));
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        for (int i = 0; i < 2; ++i) {
+            HIPCHK(hipEventCreateWithFlags(&copied[i], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&consumed[i], hipEventDisableTiming));
+        }
+    }
+    // Pinned host buffer `name` of at least `bytes` (grow-only).
+    void *host(const std::string &name, size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        auto &b = pinned[name];
+        if (b.second < bytes) {
+            if (b.first) HIPCHK(hipHostFree(b.first));
+            b.first = nullptr;
+            b.second = 0;
+            HIPCHK(hipHostMalloc(&b.first, bytes, hipHostMallocDefault));
+            b.second = bytes;
+        }
+        return b.first;
     }
     // Device buffer `name` of at least `bytes` (grow-only; contents undefined).
     template <class T> T *get(const std::string &name, size_t count) {
@@ -97,9 +119,13 @@ struct Ctx {
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     void check_launch() { HIPCHK(hipGetLastError()); }
     void release() {
+        HIPCHK(hipStreamSynchronize(copy));
         for (auto &kv : bufs)
             if (kv.second.first) hipFree(kv.second.first);
         bufs.clear();
+        for (auto &kv : pinned)
+            if (kv.second.first) hipHostFree(kv.second.first);
+        pinned.clear();
     }
 };
 
@@ -351,6 +377,140 @@ __global__ void k_iota_i32(int32_t *p, int64_t n) {
 using namespace crest;
 
 // ================================================================ C ABI
+// ---------------------------------------------------------------- fused pipeline
+static void check_pipeline_args(const crest_pipeline_args *a, double *variance, double *loadings,
+                                crest_pipeline_result *res) {
+    if (!a || !res || !variance || !loadings) value_error("NULL argument");
+    if (a->n_cells < 2 || a->d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
+    if (!a->hvg_of_gene || !a->dev_x_pca || !a->dev_knn_idx || !a->dev_knn_dist) value_error("NULL argument");
+    if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3)
+        value_error("unknown Gram precision");
+    if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
+        a->prec_knn != CREST_PREC_FP16)
+        value_error("unknown kNN screen precision");
+}
+
+// tl.pca + pp.neighbors over a chunk source: each(tmpl, fn) calls fn(chunk) on the
+// library stream for every chunk of the CSR in cell order, where chunk is tmpl with
+// its span/entry fields filled in (cells are rows through the identity cell map).
+template <class Each>
+static void run_pipeline(Ctx &x, const crest_pipeline_args *a, Each &&each, double *variance, double *loadings,
+                         crest_pipeline_result *res) {
+    const int64_t n = (int64_t)a->n_cells;
+    const int d = (int)a->d;
+    const int kp = (int)std::min<int64_t>({(int64_t)a->n_comps, (int64_t)d, n - 1});
+    if (kp > 256) value_error("n_comps > 256 is not supported");
+    const int kn = (int)std::max<int64_t>(std::min<int64_t>((int64_t)a->k, n - 1), 1);
+    std::vector<int32_t> gene_of_hvg(d, -1);
+    for (size_t g = 0; g < a->n_genes; ++g) {
+        const int32_t h = a->hvg_of_gene[g];
+        if (h >= d) value_error("hvg_of_gene values must be < d");
+        if (h >= 0) gene_of_hvg[h] = (int32_t)g;
+    }
+    for (int h = 0; h < d; ++h)
+        if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+    const int d_pad = (int)round_up(d, kGramBlock);
+    const int64_t tile_rows = a->chunk_rows ? (int64_t)round_up((int64_t)a->chunk_rows, 16) : 65536;
+    // constant maps: identity cell map and gene map (every cell and gene kept)
+    int64_t *cmap = x.get<int64_t>("pl_cmap", n);
+    int32_t *gmap = x.get<int32_t>("pl_gmap", a->n_genes);
+    hipLaunchKernelGGL(k_iota_i64, dim3(1024), dim3(256), 0, x.st, cmap, n);
+    hipLaunchKernelGGL(k_iota_i32, dim3(256), dim3(256), 0, x.st, gmap, (int64_t)a->n_genes);
+    const int32_t *sub = x.upload("pl_sub", a->hvg_of_gene, a->n_genes);
+    const int32_t *goh = x.upload("pl_goh", gene_of_hvg.data(), d);
+    double *stats = x.get<double>("pl_stats", a->n_genes * 5);
+    double *mean = x.get<double>("pl_mean", d);
+    double *sd = x.get<double>("pl_sd", d);
+    double *dg = x.get<double>("pl_gram", (size_t)d_pad * d_pad);
+    double *dc = x.get<double>("pl_colsum", d_pad);
+    double *dv = x.get<double>("pl_var", kp);
+    double *dl = x.get<double>("pl_load", (size_t)d * kp);
+    double *dt = x.get<double>("pl_total", 1);
+    float *shift = x.get<float>("pl_shift", kp);
+    x.sync();
+
+    DevChunk tmpl{};
+    tmpl.cell_map = cmap;
+    tmpl.n_cell_map = n;
+    tmpl.gene_var = gmap;
+    tmpl.n_gene_map = (int64_t)a->n_genes;
+    tmpl.target_sum = a->target_sum;
+    tmpl.log1p = a->log1p ? 1 : 0;
+    DevScale sc{};
+    sc.sub = sub;
+    sc.scale = a->scale ? 1 : 0;
+    sc.max_value = a->max_value;
+    sc.d = d;
+    sc.mean = mean;
+    sc.sd = sd;
+
+    hipEvent_t *ev = x.ev;
+    HIPCHK(hipEventRecord(ev[0], x.st));
+    // PASS 0: gene statistics -> mean/std (tl.py:24-34)
+    if (a->scale) {
+        launch_fill_f64(stats, (int64_t)a->n_genes * 5, 0.0, x.st);
+        each(tmpl, [&](const DevChunk &c) { launch_gene_stats(c, stats, (int)a->n_genes, x.st); });
+        hipLaunchKernelGGL(k_mean_std, dim3((d + 255) / 256), dim3(256), 0, x.st, stats, goh, d, (double)n, mean, sd);
+    }
+    HIPCHK(hipEventRecord(ev[1], x.st));
+    // PASS 1: Gram + colsum (tl.py:68-71)
+    launch_fill_f64(dg, (int64_t)d_pad * d_pad, 0.0, x.st);
+    launch_fill_f64(dc, d_pad, 0.0, x.st);
+    each(tmpl, [&](const DevChunk &c) { gram_rows(x, c, sc, a->prec_gram, d_pad, tile_rows, dg, dc); });
+    HIPCHK(hipEventRecord(ev[2], x.st));
+    // eigensolve (tl.py:72-73)
+    pca_device(x, dg, d_pad, dc, d, (double)n, kp, dv, dl, dt);
+    HIPCHK(hipEventRecord(ev[3], x.st));
+    // PASS 2: shift prefill + projection (tl.py:76-80)
+    launch_shift_fill(dc, dl, d, kp, (double)n, n, a->dev_x_pca, shift, x.st);
+    each(tmpl, [&](const DevChunk &c) { launch_project(c, sc, dl, kp, a->dev_x_pca, 0, x.st); });
+    x.check_launch();
+    HIPCHK(hipEventRecord(ev[4], x.st));
+    // kNN (pp.py:309)
+    knn_device(x, a->dev_x_pca, n, nullptr, 0, kp, kn, a->prec_knn, a->dev_knn_idx, a->dev_knn_dist, ev + 5);
+    x.sync();
+    float ms[8] = {};
+    for (int i = 0; i < 8; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    res->k_pcs = (size_t)kp;
+    res->k_nn = (size_t)kn;
+    res->ms_stats = ms[0];
+    res->ms_gram = ms[1];
+    res->ms_eig = ms[2];
+    res->ms_project = ms[3];
+    res->ms_knn_prep = ms[5];
+    res->ms_knn_screen = ms[6];
+    res->ms_knn_rerank = ms[7];
+    float tot = 0.f;
+    HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[8]));
+    res->ms_total = tot;
+    x.download(variance, dv, kp);
+    std::vector<double> hl((size_t)d * kp);
+    x.download(hl.data(), dl, hl.size());
+    x.download(&res->total_variance, dt, 1);
+    x.sync();
+    for (int i = 0; i < d; ++i)
+        for (int j = 0; j < kp; ++j) loadings[(size_t)i * a->n_comps + j] = hl[(size_t)i * kp + j];
+}
+
+// memcpy split over up to 8 threads (host staging of streamed chunks).
+static void par_copy(void *dst, const void *src, size_t bytes) {
+    const size_t part_min = (size_t)8 << 20;
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes / part_min));
+    if (nt <= 1) {
+        if (bytes) std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        const size_t o = (size_t)t * per;
+        if (o >= bytes) break;
+        const size_t len = std::min(per, bytes - o);
+        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, len); });
+    }
+    for (auto &t : th) t.join();
+}
+
 extern "C" {
 
 int crest_set_device(int32_t device) {
@@ -742,119 +902,115 @@ int crest_materialize(const crest_chunk *chunk, uint32_t *rows, uint32_t *vars, 
 
 int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *loadings, crest_pipeline_result *res) {
     return guarded([&] {
-        if (!a || !res || !variance || !loadings) value_error("NULL argument");
-        const int64_t n = (int64_t)a->n_cells;
-        const int d = (int)a->d;
-        if (n < 2 || d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
-        if (!a->dev_indptr || !a->dev_genes || !a->dev_values || !a->hvg_of_gene || !a->dev_x_pca ||
-            !a->dev_knn_idx || !a->dev_knn_dist)
-            value_error("NULL argument");
-        if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3)
-            value_error("unknown Gram precision");
-        if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
-            a->prec_knn != CREST_PREC_FP16)
-            value_error("unknown kNN screen precision");
-        const int kp = (int)std::min<int64_t>({(int64_t)a->n_comps, (int64_t)d, n - 1});
-        if (kp > 256) value_error("n_comps > 256 is not supported");
-        const int kn = (int)std::max<int64_t>(std::min<int64_t>((int64_t)a->k, n - 1), 1);
-        std::vector<int32_t> gene_of_hvg(d, -1);
-        for (size_t g = 0; g < a->n_genes; ++g) {
-            const int32_t h = a->hvg_of_gene[g];
-            if (h >= d) value_error("hvg_of_gene values must be < d");
-            if (h >= 0) gene_of_hvg[h] = (int32_t)g;
-        }
-        for (int h = 0; h < d; ++h)
-            if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+        check_pipeline_args(a, variance, loadings, res);
+        if (!a->dev_indptr || !a->dev_genes || !a->dev_values) value_error("NULL argument");
         Ctx &x = ctx();
         std::lock_guard<std::mutex> lk(x.mu);
-        const int d_pad = (int)round_up(d, kGramBlock);
-        const int64_t tile_rows = a->chunk_rows ? (int64_t)round_up((int64_t)a->chunk_rows, 16) : 65536;
-        // constant maps: identity cell map and gene map (every cell and gene kept)
-        int64_t *cmap = x.get<int64_t>("pl_cmap", n);
-        int32_t *gmap = x.get<int32_t>("pl_gmap", a->n_genes);
-        hipLaunchKernelGGL(k_iota_i64, dim3(1024), dim3(256), 0, x.st, cmap, n);
-        hipLaunchKernelGGL(k_iota_i32, dim3(256), dim3(256), 0, x.st, gmap, (int64_t)a->n_genes);
-        const int32_t *sub = x.upload("pl_sub", a->hvg_of_gene, a->n_genes);
-        const int32_t *goh = x.upload("pl_goh", gene_of_hvg.data(), d);
-        double *stats = x.get<double>("pl_stats", a->n_genes * 5);
-        double *mean = x.get<double>("pl_mean", d);
-        double *sd = x.get<double>("pl_sd", d);
-        double *dg = x.get<double>("pl_gram", (size_t)d_pad * d_pad);
-        double *dc = x.get<double>("pl_colsum", d_pad);
-        double *dv = x.get<double>("pl_var", kp);
-        double *dl = x.get<double>("pl_load", (size_t)d * kp);
-        double *dt = x.get<double>("pl_total", 1);
-        float *shift = x.get<float>("pl_shift", kp);
-        x.sync();
+        const int64_t n = (int64_t)a->n_cells;
+        auto resident = [&](const DevChunk &tmpl, auto &&fn) {
+            DevChunk c = tmpl;
+            c.span_off = a->dev_indptr;
+            c.span_cell = nullptr;
+            c.first_cell = 0;
+            c.n_spans = n;
+            c.genes = a->dev_genes;
+            c.values = a->dev_values;
+            c.base = 0;
+            fn(c);
+        };
+        run_pipeline(x, a, resident, variance, loadings, res);
+    });
+}
 
-        DevChunk c{};
-        c.span_off = a->dev_indptr;
-        c.span_cell = nullptr;
-        c.first_cell = 0;
-        c.n_spans = n;
-        c.genes = a->dev_genes;
-        c.values = a->dev_values;
-        c.cell_map = cmap;
-        c.n_cell_map = n;
-        c.gene_var = gmap;
-        c.n_gene_map = (int64_t)a->n_genes;
-        c.target_sum = a->target_sum;
-        c.log1p = a->log1p ? 1 : 0;
-        c.base = 0;
-        DevScale sc{};
-        sc.sub = sub;
-        sc.scale = a->scale ? 1 : 0;
-        sc.max_value = a->max_value;
-        sc.d = d;
-        sc.mean = mean;
-        sc.sd = sd;
-
-        hipEvent_t *ev = x.ev;
-        HIPCHK(hipEventRecord(ev[0], x.st));
-        // PASS 0: gene statistics -> mean/std (tl.py:24-34)
-        if (a->scale) {
-            launch_fill_f64(stats, (int64_t)a->n_genes * 5, 0.0, x.st);
-            launch_gene_stats(c, stats, (int)a->n_genes, x.st);
-            hipLaunchKernelGGL(k_mean_std, dim3((d + 255) / 256), dim3(256), 0, x.st, stats, goh, d, (double)n, mean, sd);
+int crest_pipeline_run_streamed(const crest_pipeline_args *a, const int64_t *host_indptr,
+                                const uint32_t *host_genes, const float *host_values, int32_t host_pinned,
+                                size_t stream_rows, double *variance, double *loadings, crest_pipeline_result *res) {
+    return guarded([&] {
+        check_pipeline_args(a, variance, loadings, res);
+        if (!host_indptr || !host_genes || !host_values) value_error("NULL argument");
+        const int64_t n = (int64_t)a->n_cells;
+        if (host_indptr[0] != 0) value_error("indptr must start at 0");
+        const int64_t rows = stream_rows ? (int64_t)stream_rows : 32768;
+        // largest chunk, and the CSR's own consistency (non-decreasing offsets)
+        int64_t max_ne = 0;
+        for (int64_t lo = 0; lo < n; lo += rows) {
+            const int64_t hi = std::min(n, lo + rows);
+            if (host_indptr[hi] < host_indptr[lo]) value_error("indptr must be non-decreasing");
+            max_ne = std::max(max_ne, host_indptr[hi] - host_indptr[lo]);
         }
-        HIPCHK(hipEventRecord(ev[1], x.st));
-        // PASS 1: Gram + colsum (tl.py:68-71)
-        launch_fill_f64(dg, (int64_t)d_pad * d_pad, 0.0, x.st);
-        launch_fill_f64(dc, d_pad, 0.0, x.st);
-        gram_rows(x, c, sc, a->prec_gram, d_pad, tile_rows, dg, dc);
-        HIPCHK(hipEventRecord(ev[2], x.st));
-        // eigensolve (tl.py:72-73)
-        pca_device(x, dg, d_pad, dc, d, (double)n, kp, dv, dl, dt);
-        HIPCHK(hipEventRecord(ev[3], x.st));
-        // PASS 2: shift prefill + projection (tl.py:76-80)
-        launch_shift_fill(dc, dl, d, kp, (double)n, n, a->dev_x_pca, shift, x.st);
-        launch_project(c, sc, dl, kp, a->dev_x_pca, 0, x.st);
-        x.check_launch();
-        HIPCHK(hipEventRecord(ev[4], x.st));
-        // kNN (pp.py:309)
-        knn_device(x, a->dev_x_pca, n, nullptr, 0, kp, kn, a->prec_knn, a->dev_knn_idx, a->dev_knn_dist, ev + 5);
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        // both slots sized up front, so nothing is reallocated while a copy is in flight
+        int64_t *d_ip[2];
+        uint32_t *d_g[2];
+        float *d_v[2];
+        uint8_t *h_stage[2] = {nullptr, nullptr};
+        const size_t ip_bytes = (size_t)(rows + 1) * 8, g_bytes = (size_t)max_ne * 4;
+        for (int b = 0; b < 2; ++b) {
+            const std::string sfx = b ? "1" : "0";
+            d_ip[b] = x.get<int64_t>("st_ip" + sfx, rows + 1);
+            d_g[b] = x.get<uint32_t>("st_g" + sfx, max_ne);
+            d_v[b] = x.get<float>("st_v" + sfx, max_ne);
+            if (!host_pinned) h_stage[b] = (uint8_t *)x.host("stage" + sfx, ip_bytes + 2 * g_bytes);
+        }
         x.sync();
-        float ms[8] = {};
-        for (int i = 0; i < 8; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-        res->k_pcs = (size_t)kp;
-        res->k_nn = (size_t)kn;
-        res->ms_stats = ms[0];
-        res->ms_gram = ms[1];
-        res->ms_eig = ms[2];
-        res->ms_project = ms[3];
-        res->ms_knn_prep = ms[5];
-        res->ms_knn_screen = ms[6];
-        res->ms_knn_rerank = ms[7];
-        float tot = 0.f;
-        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[8]));
-        res->ms_total = tot;
-        x.download(variance, dv, kp);
-        std::vector<double> hl((size_t)d * kp);
-        x.download(hl.data(), dl, hl.size());
-        x.download(&res->total_variance, dt, 1);
-        x.sync();
-        for (int i = 0; i < d; ++i)
-            for (int j = 0; j < kp; ++j) loadings[(size_t)i * a->n_comps + j] = hl[(size_t)i * kp + j];
+        HIPCHK(hipStreamSynchronize(x.copy));
+        auto streamed = [&](const DevChunk &tmpl, auto &&fn) {
+            int64_t i = 0;
+            for (int64_t lo = 0; lo < n; lo += rows, ++i) {
+                const int64_t hi = std::min(n, lo + rows), nr = hi - lo;
+                const int b = (int)(i & 1);
+                const int64_t e0 = host_indptr[lo], ne = host_indptr[hi] - e0;
+                const void *sip = host_indptr + lo, *sg = host_genes + e0, *sv = host_values + e0;
+                if (!host_pinned) {  // wait until the slot's previous upload has left the staging buffer
+                    HIPCHK(hipEventSynchronize(x.copied[b]));
+                    uint8_t *st = h_stage[b];
+                    par_copy(st, sip, (size_t)(nr + 1) * 8);
+                    par_copy(st + ip_bytes, sg, (size_t)ne * 4);
+                    par_copy(st + ip_bytes + g_bytes, sv, (size_t)ne * 4);
+                    sip = st;
+                    sg = st + ip_bytes;
+                    sv = st + ip_bytes + g_bytes;
+                }
+                // the device slot is free once the kernels of its previous chunk are done
+                HIPCHK(hipStreamWaitEvent(x.copy, x.consumed[b], 0));
+                HIPCHK(hipMemcpyAsync(d_ip[b], sip, (size_t)(nr + 1) * 8, hipMemcpyHostToDevice, x.copy));
+                if (ne) {
+                    HIPCHK(hipMemcpyAsync(d_g[b], sg, (size_t)ne * 4, hipMemcpyHostToDevice, x.copy));
+                    HIPCHK(hipMemcpyAsync(d_v[b], sv, (size_t)ne * 4, hipMemcpyHostToDevice, x.copy));
+                }
+                HIPCHK(hipEventRecord(x.copied[b], x.copy));
+                HIPCHK(hipStreamWaitEvent(x.st, x.copied[b], 0));
+                DevChunk c = tmpl;
+                c.span_off = d_ip[b];
+                c.span_cell = nullptr;
+                c.first_cell = (uint32_t)lo;
+                c.n_spans = nr;
+                c.genes = d_g[b];
+                c.values = d_v[b];
+                c.base = e0;
+                fn(c);
+                HIPCHK(hipEventRecord(x.consumed[b], x.st));
+            }
+        };
+        run_pipeline(x, a, streamed, variance, loadings, res);
+        HIPCHK(hipStreamSynchronize(x.copy));
+    });
+}
+
+int crest_host_alloc(size_t bytes, void **ptr) {
+    return guarded([&] {
+        if (!ptr) value_error("ptr is NULL");
+        *ptr = nullptr;
+        Ctx &x = ctx();
+        (void)x;
+        HIPCHK(hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    });
+}
+
+int crest_host_free(void *ptr) {
+    return guarded([&] {
+        if (ptr) HIPCHK(hipHostFree(ptr));
     });
 }
 

@@ -1,8 +1,11 @@
-"""Fused tl.pca + pp.neighbors over a CSR resident in device memory: one C-ABI call
-(crest_pipeline_run) runs gene statistics -> mean/std -> Gram -> FP64 eigensolve ->
-projection -> exact kNN -> on the library's stream and reports per-stage HIP-event
-times.  Device buffers come from the library itself (crest_dev_alloc), so no second
-HIP runtime is loaded into the process.
+"""Fused tl.pca + pp.neighbors: one C-ABI call runs gene statistics -> mean/std ->
+Gram -> FP64 eigensolve -> projection -> exact kNN on the library's stream and
+reports per-stage HIP-event times.  The CSR is either resident in device memory
+(DeviceCSR, crest_pipeline_run) or held in host memory and streamed through two
+device slots chunk by chunk (HostCSR, crest_pipeline_run_streamed; pinned host
+arrays are uploaded without a staging copy).  Device and pinned buffers come from
+the library itself (crest_dev_alloc / crest_host_alloc), so no second HIP runtime is
+loaded into the process.
 """
 from __future__ import annotations
 
@@ -65,6 +68,57 @@ class DeviceCSR:
                    int(ip[-1]))
 
 
+class PinnedArray:
+    """A numpy array in pinned host memory from crest_host_alloc, freed with the object."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        check(lib().crest_host_alloc(n, C.byref(p)))
+        self.ptr = p.value
+        buf = (C.c_uint8 * max(n, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            try:
+                lib().crest_host_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+@dataclass
+class HostCSR:
+    indptr: np.ndarray   # int64 [n+1], starts at 0
+    indices: np.ndarray  # uint32 [nnz]
+    data: np.ndarray     # float32 [nnz]
+    n_cells: int
+    n_genes: int
+    nnz: int
+    pinned: bool
+    _keep: tuple = ()
+
+    @classmethod
+    def from_arrays(cls, indptr, indices, data, n_genes: int, pinned: bool = True) -> "HostCSR":
+        ip = np.asarray(indptr, np.int64)
+        ip = ip - ip[0]
+        ix = np.asarray(indices, np.uint32)[:int(ip[-1])]
+        dv = np.asarray(data, np.float32)[:int(ip[-1])]
+        if not pinned:
+            arrs = [np.ascontiguousarray(a) for a in (ip, ix, dv)]
+            return cls(arrs[0], arrs[1], arrs[2], len(ip) - 1, int(n_genes), int(ip[-1]), False)
+        keep = []
+        out = []
+        for a in (ip, ix, dv):
+            pa = PinnedArray(a.shape, a.dtype)
+            pa.array[...] = a
+            keep.append(pa)
+            out.append(pa.array)
+        return cls(out[0], out[1], out[2], len(ip) - 1, int(n_genes), int(ip[-1]), True, tuple(keep))
+
+
 @dataclass
 class PipelineOutput:
     variance: np.ndarray
@@ -74,12 +128,16 @@ class PipelineOutput:
 
 
 class Pipeline:
-    """Reusable device outputs for repeated runs over one device CSR."""
+    """Reusable device outputs for repeated runs over one CSR (DeviceCSR, or HostCSR
+    streamed in chunks of stream_rows cells)."""
 
-    def __init__(self, csr: DeviceCSR, hvg_of_gene: np.ndarray, n_comps: int = 50, k: int = 15,
+    def __init__(self, csr, hvg_of_gene: np.ndarray, n_comps: int = 50, k: int = 15,
                  target_sum: float = 1e4, log1p: bool = True, scale: bool = True, max_value: float = 10.0,
-                 prec_gram: str = "bf16x3", prec_knn: str = "bf16x3", chunk_rows: int = 65536):
+                 prec_gram: str = "bf16x3", prec_knn: str = "bf16x3", chunk_rows: int = 65536,
+                 stream_rows: int = 32768):
         self.csr = csr
+        self.streamed = isinstance(csr, HostCSR)
+        self.stream_rows = int(stream_rows)
         self.hvg = np.ascontiguousarray(hvg_of_gene, np.int32)
         self.d = int(self.hvg.max()) + 1
         n = csr.n_cells
@@ -89,8 +147,9 @@ class Pipeline:
         self.x = DeviceBuffer(4 * n * self.kp)
         self.idx = DeviceBuffer(4 * n * self.kn)
         self.dist = DeviceBuffer(4 * n * self.kn)
+        dev = (None, None, None) if self.streamed else (csr.indptr.ptr, csr.indices.ptr, csr.data.ptr)
         self.args = PipelineArgs(
-            csr.indptr.ptr, csr.indices.ptr, csr.data.ptr, n, csr.n_genes,
+            dev[0], dev[1], dev[2], n, csr.n_genes,
             self.hvg.ctypes.data, self.d, float(target_sum), int(log1p), int(scale), float(max_value or 0.0),
             int(n_comps), int(k), PREC_NAMES[prec_gram], PREC_NAMES[prec_knn], int(chunk_rows),
             self.x.ptr, self.idx.ptr, self.dist.ptr)
@@ -103,8 +162,15 @@ class Pipeline:
 
     def run(self) -> PipelineOutput:
         res = PipelineResult()
-        check(lib().crest_pipeline_run(C.byref(self.args), self.variance.ctypes.data, self.loadings.ctypes.data,
-                                       C.byref(res)))
+        if self.streamed:
+            c = self.csr
+            check(lib().crest_pipeline_run_streamed(
+                C.byref(self.args), c.indptr.ctypes.data, c.indices.ctypes.data, c.data.ctypes.data,
+                int(c.pinned), self.stream_rows, self.variance.ctypes.data, self.loadings.ctypes.data,
+                C.byref(res)))
+        else:
+            check(lib().crest_pipeline_run(C.byref(self.args), self.variance.ctypes.data,
+                                           self.loadings.ctypes.data, C.byref(res)))
         times = {k: getattr(res, "ms_" + k) for k in ("stats", "gram", "eig", "project", "knn_prep",
                                                        "knn_screen", "knn_rerank", "total")}
         kp = res.k_pcs

@@ -200,6 +200,23 @@ typedef struct crest_pipeline_result {
 int crest_pipeline_run(const crest_pipeline_args *args, double *variance, double *loadings,
                        crest_pipeline_result *result);
 
+/* The same pipeline with the CSR of raw counts in host memory (the out-of-core
+ * form; no reference counterpart): args->dev_indptr/dev_genes/dev_values are
+ * ignored.  Chunks of stream_rows cells (0 = 32,768) are uploaded through two
+ * device slots on a copy stream while the previous chunk is computed, three passes
+ * (statistics, Gram, projection), so device memory holds two chunks, the Gram and
+ * the kNN operands, never the whole matrix.  host_pinned != 0 says the three arrays
+ * come from crest_host_alloc and are uploaded straight from there; otherwise each
+ * chunk is first copied into a pinned staging buffer.                             */
+int crest_pipeline_run_streamed(const crest_pipeline_args *args, const int64_t *host_indptr,
+                                const uint32_t *host_genes, const float *host_values, int32_t host_pinned,
+                                size_t stream_rows, double *variance, double *loadings,
+                                crest_pipeline_result *result);
+
+/* Pinned (page-locked) host memory for crest_pipeline_run_streamed's inputs. */
+int crest_host_alloc(size_t bytes, void **ptr);
+int crest_host_free(void *ptr);
+
 /* Release every cached device buffer of the current device. */
 int crest_release(void);
 

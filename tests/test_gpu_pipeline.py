@@ -72,3 +72,45 @@ def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol, tot_rt
     np.testing.assert_array_equal(dist[q], pd)
     for k_, v in out.times_ms.items():
         assert v >= 0.0, k_
+
+
+@pytest.mark.parametrize("pinned,stream_rows", [(True, 4096), (False, 3000), (True, 50000)])
+def test_streamed_matches_resident(data, reference, oracle, pinned, stream_rows):
+    """The out-of-core form (host CSR streamed through two device slots) computes the
+    same PCA and kNN as the device-resident pipeline: the Gram tiles follow the
+    stream's chunk boundaries, so FP64 sums differ only in order."""
+    from crest_amd.pipeline import DeviceCSR, HostCSR, Pipeline
+    indptr, indices, vals, g = data
+    hvg = np.arange(g, dtype=np.int32)
+    res = Pipeline(DeviceCSR.upload(indptr, indices, vals, g), hvg, n_comps=30, k=15, prec_gram="fp64",
+                   prec_knn="fp16", chunk_rows=4096)
+    out_r = res.run()
+    host = HostCSR.from_arrays(indptr, indices, vals, g, pinned=pinned)
+    st = Pipeline(host, hvg, n_comps=30, k=15, prec_gram="fp64", prec_knn="fp16", chunk_rows=4096,
+                  stream_rows=stream_rows)
+    out_s = st.run()
+    np.testing.assert_allclose(out_s.variance, out_r.variance, rtol=1e-11)
+    assert abs(out_s.total_variance - out_r.total_variance) <= 1e-11 * out_r.total_variance
+    assert oracle.principal_angle(out_s.loadings[:, :10], out_r.loadings[:, :10]) < 1e-8
+    var_o, load_o, tot_o, X_o = reference
+    np.testing.assert_allclose(out_s.variance, var_o, rtol=1e-9)
+    X = st.x_pca()
+    np.testing.assert_allclose(np.abs(X[:, :10]), np.abs(res.x_pca()[:, :10]), rtol=1e-4, atol=1e-4)
+    idx, dist = st.knn()
+    q = np.random.default_rng(1).choice(len(X), 300, replace=False)
+    ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
+    bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    # a second run reuses the slots and events
+    out_s2 = st.run()
+    np.testing.assert_allclose(out_s2.variance, out_s.variance, rtol=1e-12)
+
+
+def test_streamed_errors():
+    from crest_amd._lib import PipelineArgs, PipelineResult, lib
+    import ctypes as C
+    a = PipelineArgs()
+    r = PipelineResult()
+    v = np.zeros(4)
+    assert lib().crest_pipeline_run_streamed(C.byref(a), None, None, None, 0, 0, v.ctypes.data, v.ctypes.data,
+                                              C.byref(r)) != 0
