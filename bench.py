@@ -170,6 +170,28 @@ def knn_only(args) -> int:
     return 0
 
 
+def streamed_leg(args) -> dict:
+    """The out-of-core configuration: the CSR stays in pinned host memory and every
+    pass streams it through two device slots (crest_pipeline_run_streamed)."""
+    from crest_amd import synth
+    from crest_amd.pipeline import HostCSR, Pipeline
+    t0 = time.perf_counter()
+    ip, ix, dv, info = synth.make_counts(args.streamed_cells, args.genes, args.clusters, density=args.density,
+                                         seed=args.seed + 1)
+    host = HostCSR.from_arrays(ip, ix, dv, args.genes, pinned=True)
+    del ip, ix, dv
+    log(f"streamed: generated {args.streamed_cells} x {args.genes}, nnz {info['nnz']} "
+        f"in {time.perf_counter() - t0:.1f}s")
+    pipe = Pipeline(host, np.arange(args.genes, dtype=np.int32), n_comps=args.pcs, k=args.k, prec_gram="bf16x3",
+                    prec_knn=args.knn_prec, chunk_rows=args.stream_rows, stream_rows=args.stream_rows)
+    ms, stages = timed_steps(pipe, 1, 1)
+    log(f"streamed ({args.streamed_cells} cells, chunks of {args.stream_rows}): {ms:.1f} ms/step, "
+        f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
+    return {"n_cells": args.streamed_cells, "nnz": info["nnz"], "stream_rows": args.stream_rows,
+            "ms_per_step": ms, "value": args.streamed_cells / (ms * 1e-3), "unit": "cells/s",
+            "stages_ms": {k: round(v, 3) for k, v in stages.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +216,10 @@ def main() -> int:
                     help="kNN screen of the headline leg (the Gram is split-BF16)")
     ap.add_argument("--alt-screen", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
                     help="also time one step with this screen (reported, not the headline)")
+    ap.add_argument("--streamed-cells", type=int, default=0, metavar="N",
+                    help="also time the out-of-core form: N x genes NB counts in pinned host memory, streamed in "
+                         "chunks of --stream-rows cells (reported under 'streamed', not the headline)")
+    ap.add_argument("--stream-rows", type=int, default=32768)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -282,6 +308,8 @@ def main() -> int:
         line["fp32"] = {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
                         "stages_ms": {k: round(v, 3) for k, v in stages32.items()}}
         line["speedup_vs_fp32"] = ms32 / ms
+    if args.streamed_cells:
+        line["streamed"] = streamed_leg(args)
     if not args.no_cpu:
         t0 = time.perf_counter()
         base, acc = cpu_baseline_leg(indptr, indices, data, args.genes, gpu, args)

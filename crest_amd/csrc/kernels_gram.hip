@@ -12,12 +12,14 @@ namespace crest {
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) double f64x4;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 __device__ inline bf16x8 as_bf16x8(uint4 v) {
     bf16x8 r;
     __builtin_memcpy(&r, &v, 16);
     return r;
 }
+__device__ inline bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // blk -> (bi, bj) over the upper triangle of nb x nb blocks, row by row.
 __device__ inline void tri_block(int blk, int nb, int &bi, int &bj) {
@@ -77,22 +79,37 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ t
 #pragma unroll
             for (int r = 0; r < 16; ++r) dacc[i][j][r] = 0.0;
         }
-    int cnt = 0;
-    for (int64_t p = p0; p < p1; ++p) {
+    // Operands of K-step pair p (rows 32p..32p+31): hi and lo of the wave's two A and two
+    // B column runs.  The loads of step p+1 are issued before the MFMAs of step p
+    // (two register sets, alternating), so their latency hides behind matrix work.
+    struct Ops {
+        u32x4 ah[2], al[2], bh[2], bl[2];
+    };
+    const u32x4 *th4 = reinterpret_cast<const u32x4 *>(th), *tl4 = reinterpret_cast<const u32x4 *>(tl);
+    auto load = [&](int64_t p, Ops &o) {
         const int64_t u = (2 * p + half) * (int64_t)d_pad;
-        const bf16x8 ah0 = as_bf16x8(th[u + ga]), ah1 = as_bf16x8(th[u + ga + 32]);
-        const bf16x8 al0 = as_bf16x8(tl[u + ga]), al1 = as_bf16x8(tl[u + ga + 32]);
-        const bf16x8 bh0 = as_bf16x8(th[u + gb]), bh1 = as_bf16x8(th[u + gb + 32]);
-        const bf16x8 bl0 = as_bf16x8(tl[u + gb]), bl1 = as_bf16x8(tl[u + gb + 32]);
-        const bf16x8 ah[2] = {ah0, ah1}, al[2] = {al0, al1}, bh[2] = {bh0, bh1}, bl[2] = {bl0, bl1};
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            o.ah[i] = th4[u + ga + 32 * i];
+            o.al[i] = tl4[u + ga + 32 * i];
+            o.bh[i] = th4[u + gb + 32 * i];
+            o.bl[i] = tl4[u + gb + 32 * i];
+        }
+    };
+    int cnt = 0;
+    auto step = [&](const Ops &o) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.ah[i]), as_bf16x8(o.bh[j]), acc[i][j],
+                                                                    0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.ah[i]), as_bf16x8(o.bl[j]), acc[i][j],
+                                                                    0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.al[i]), as_bf16x8(o.bh[j]), acc[i][j],
+                                                                    0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.al[i]), as_bf16x8(o.bl[j]), acc[i][j],
+                                                                    0, 0, 0);
             }
         if (++cnt == FLUSH) {
             cnt = 0;
@@ -104,6 +121,24 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ t
                     for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
                     acc[i][j] = (f32x16){};
                 }
+        }
+    };
+    // Prefetches are unconditional (clamped to the last step; the extra copy is never
+    // used), so no branch sits between a load and the wait for it; the empty asm
+    // statement with a memory clobber keeps the compiler from sinking the loads down
+    // to their use (it emits no instruction).
+    if (p0 < p1) {
+        Ops oa, ob;
+        load(p0, oa);
+        for (int64_t p = p0;; p += 2) {
+            load(p + 1 < p1 ? p + 1 : p1 - 1, ob);
+            asm volatile("" ::: "memory");
+            step(oa);
+            if (p + 1 >= p1) break;
+            load(p + 2 < p1 ? p + 2 : p1 - 1, oa);
+            asm volatile("" ::: "memory");
+            step(ob);
+            if (p + 2 >= p1) break;
         }
     }
     double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
