@@ -170,6 +170,21 @@ def knn_only(args) -> int:
     return 0
 
 
+def screen_traffic(prec: str, n: int, p: int):
+    """HBM bytes per launch of the screen from the committed copy-calibrated counter
+    summary (profiles/knn_screen_traffic.json), when it was taken on this kernel and
+    shape; otherwise None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "knn_screen_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("prec") == prec and rec.get("n") == n and rec.get("p") == p:
+        return rec.get("hbm_bytes_per_launch")
+    return None
+
+
 def streamed_leg(args) -> dict:
     """The out-of-core configuration: the CSR stays in pinned host memory and every
     pass streams it through two device slots (crest_pipeline_run_streamed)."""
@@ -276,6 +291,25 @@ def main() -> int:
         check(lib().crest_dev_copy(b.ptr, a.ptr, nb))
         log(f"copy kernel: {nb} bytes read, {nb} bytes written")
 
+    # T with the final device-to-host copy of the results (X_pca, kNN indices and
+    # distances), and a plain device copy of 1 GiB as the measured HBM reference
+    from crest_amd._lib import check, lib
+    from crest_amd.pipeline import DeviceBuffer
+    t0 = time.perf_counter()
+    pipe.x_pca()
+    pipe.knn()
+    d2h_ms = (time.perf_counter() - t0) * 1e3
+    nb = 1 << 30
+    ca_, cb_ = DeviceBuffer(nb), DeviceBuffer(nb)
+    check(lib().crest_dev_copy(cb_.ptr, ca_.ptr, nb))
+    check(lib().crest_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(5):
+        check(lib().crest_dev_copy(cb_.ptr, ca_.ptr, nb))
+    check(lib().crest_synchronize())
+    copy_gbps = 5 * 2 * nb / (time.perf_counter() - t0) / 1e9
+    del ca_, cb_
+
     n = args.cells
     p = pipe.kp
     screen_s = stages["knn_screen"] * 1e-3
@@ -295,10 +329,13 @@ def main() -> int:
                    "n_cells": n, "n_genes": args.genes, "nnz": info["nnz"], "n_comps": args.pcs, "k": args.k,
                    "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": args.knn_prec},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "ms_per_step_with_d2h": ms + d2h_ms,
+        "hbm_copy_GBps": copy_gbps,
         "roofline": {"kernel": {"fp16": "k_knn_screen16<1,...>", "bf16x3": "k_knn_screen16<0,...>",
                                 "fp32": "k_knn_screen_f32"}[args.knn_prec],
                      "bound": "mfma", "achieved": achieved / 1e12,
-                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": screen_traffic(args.knn_prec, n, p),
                      "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
                      "launch_ms": stages["knn_screen"]},
     }

@@ -177,20 +177,30 @@ __global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, 
 #pragma unroll
             for (int r = 0; r < 16; ++r) dacc[i][j][r] = 0.0;
         }
-    int cnt = 0;
-    for (int64_t g = g0; g < g1; ++g) {
+    // Same operand prefetch as the split-BF16 kernel: group g+1's loads are issued
+    // (unconditionally, clamped) before group g's MFMAs.
+    typedef __attribute__((ext_vector_type(4))) float f32x4;
+    const f32x4 *t4 = reinterpret_cast<const f32x4 *>(t);
+    struct Ops {
+        f32x4 a[2], b[2];
+    };
+    auto load = [&](int64_t g, Ops &o) {
         const int64_t u = g * (int64_t)d_pad;
-        const float4 a[2] = {t[(u + ga) * 2 + half], t[(u + ga + 32) * 2 + half]};
-        const float4 b[2] = {t[(u + gb) * 2 + half], t[(u + gb + 32) * 2 + half]};
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            o.a[i] = t4[(u + ga + 32 * i) * 2 + half];
+            o.b[i] = t4[(u + gb + 32 * i) * 2 + half];
+        }
+    };
+    int cnt = 0;
+    auto step = [&](const Ops &o) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].x, b[j].x, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].z, b[j].z, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].w, b[j].w, acc[i][j], 0, 0, 0);
-            }
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(o.a[i][e], o.b[j][e], acc[i][j], 0, 0, 0);
         if (++cnt == FLUSH) {
             cnt = 0;
 #pragma unroll
@@ -201,6 +211,20 @@ __global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, 
                     for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
                     acc[i][j] = (f32x16){};
                 }
+        }
+    };
+    if (g0 < g1) {
+        Ops oa, ob;
+        load(g0, oa);
+        for (int64_t g = g0;; g += 2) {
+            load(g + 1 < g1 ? g + 1 : g1 - 1, ob);
+            asm volatile("" ::: "memory");
+            step(oa);
+            if (g + 1 >= g1) break;
+            load(g + 2 < g1 ? g + 2 : g1 - 1, oa);
+            asm volatile("" ::: "memory");
+            step(ob);
+            if (g + 2 >= g1) break;
         }
     }
     double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
