@@ -94,7 +94,7 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     t["project"] = (time.perf_counter() - t0) * n / ns
     # stage 5: exact kNN of a query sample against all n, scaled to n queries (on the
     # GPU's own X_pca so the accuracy check below can reuse it)
-    Xg = gpu["x_bf16"]
+    Xg = gpu["x_head"]
     q = args.cpu_knn_queries
     t0 = time.perf_counter()
     O.knn(Xg, args.k, 0, q, threads=threads)
@@ -117,7 +117,7 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     acc["pca_signal_rel_gap"] = float(rel[r_sig - 1])
     rng = np.random.default_rng(0)
     qs = rng.choice(n, args.acc_queries, replace=False)
-    for tag in [t for t in ("bf16", "fp32") if f"x_{t}" in gpu]:
+    for tag in [t for t in ("head", "alt", "fp32") if f"x_{t}" in gpu]:
         Xp = gpu[f"x_{tag}"]
         ref_i, ref_d2 = O.knn_exact_f64(Xp, args.k, qs)
         got = gpu[f"idx_{tag}"][qs]
@@ -130,11 +130,6 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
         acc[f"pca_angle_all_{tag}"] = O.principal_angle(L, load_o)
         acc[f"variance_ratio_err_{tag}"] = float(np.max(np.abs(gpu[f"var_{tag}"] / gpu[f"tot_{tag}"]
                                                                - var_o / tot_o)))
-        if tag == "bf16" and "idx_alt" in gpu:  # the alternate screen on the same scores
-            got = gpu["idx_alt"][qs]
-            bad, ties = O.knn_mismatch_ties(got, Xp, qs, ref_i, ref_d2)
-            acc["knn_set_agreement_alt_screen"] = O.knn_set_agreement(got, ref_i)
-            acc["knn_mismatch_alt_screen"] = [int(bad), int(ties)]
     acc["eig_gap_p"] = float(var_o[args.pcs - 1] - ev[args.pcs])
     acc["eig_rel_gap_20"] = float(rel[19])
     acc["queries"] = int(args.acc_queries)
@@ -227,10 +222,13 @@ def main() -> int:
     ap.add_argument("--acc-queries", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
+    ap.add_argument("--gram-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
+                    help="Gram of the headline leg (fp16: split-FP16 hi+lo, three MFMA products)")
     ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
-                    help="kNN screen of the headline leg (the Gram is split-BF16)")
-    ap.add_argument("--alt-screen", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
-                    help="also time one step with this screen (reported, not the headline)")
+                    help="kNN screen of the headline leg")
+    ap.add_argument("--alt", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
+                    help="also time one step with this precision for both Gram and screen (reported, not the "
+                         "headline; bf16x3 = the all-split-BF16 configuration)")
     ap.add_argument("--streamed-cells", type=int, default=0, metavar="N",
                     help="also time the out-of-core form: N x genes NB counts in pinned host memory, streamed in "
                          "chunks of --stream-rows cells (reported under 'streamed', not the headline)")
@@ -255,33 +253,34 @@ def main() -> int:
         f"in {time.perf_counter() - t0:.1f}s")
     csr = DeviceCSR.upload(indptr, indices, data, args.genes)
     hvg = np.arange(args.genes, dtype=np.int32)  # the 2,000 genes are the pre-selected HVGs
-    pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram="bf16x3", prec_knn=args.knn_prec,
+    pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram=args.gram_prec, prec_knn=args.knn_prec,
                     chunk_rows=args.chunk_rows)
+    gpu = {}
+
+    def capture(tag):
+        out_ = pipe.run()
+        gpu.update({f"x_{tag}": pipe.x_pca(), f"idx_{tag}": pipe.knn()[0], f"var_{tag}": out_.variance,
+                    f"load_{tag}": out_.loadings, f"tot_{tag}": out_.total_variance})
 
     ms, stages = timed_steps(pipe, args.steps, args.warmup)
-    log(f"split-BF16 Gram + {args.knn_prec} screen: {ms:.1f} ms/step, "
+    log(f"{args.gram_prec} Gram + {args.knn_prec} screen: {ms:.1f} ms/step, "
         f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
-    gpu = {"x_bf16": pipe.x_pca(), "idx_bf16": pipe.knn()[0],
-           "var_bf16": pipe.variance[:pipe.kp].copy(), "load_bf16": pipe.loadings[:, :pipe.kp].copy()}
-    out = pipe.run()
-    gpu["tot_bf16"] = out.total_variance
+    capture("head")
     alt = None
-    if args.alt_screen not in ("none", args.knn_prec):
-        pipe.set_precision("bf16x3", args.alt_screen)
+    if args.alt != "none" and (args.alt, args.alt) != (args.gram_prec, args.knn_prec):
+        pipe.set_precision(args.alt, args.alt)
         ms_alt, stages_alt = timed_steps(pipe, 1, 0)
-        log(f"split-BF16 Gram + {args.alt_screen} screen: {ms_alt:.1f} ms/step, "
+        log(f"{args.alt} Gram + {args.alt} screen: {ms_alt:.1f} ms/step, "
             f"stages {json.dumps({k: round(v, 2) for k, v in stages_alt.items()})}")
-        gpu["idx_alt"] = pipe.knn()[0]
-        alt = {"prec_knn": args.alt_screen, "ms_per_step": ms_alt,
+        capture("alt")
+        alt = {"prec_gram": args.alt, "prec_knn": args.alt, "ms_per_step": ms_alt, "value": args.cells / (ms_alt * 1e-3),
                "stages_ms": {k: round(v, 3) for k, v in stages_alt.items()}}
 
     if args.fp32_steps > 0:
         pipe.set_precision("fp32", "fp32")
         ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
         log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
-        out32 = pipe.run()
-        gpu.update({"x_fp32": pipe.x_pca(), "idx_fp32": pipe.knn()[0],
-                    "var_fp32": out32.variance, "load_fp32": out32.loadings, "tot_fp32": out32.total_variance})
+        capture("fp32")
     if args.calibrate_copy:
         # a plain copy of known size: the reference reading for the HBM byte counters
         from crest_amd._lib import check, lib
@@ -320,14 +319,15 @@ def main() -> int:
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"split-bf16 Gram + {args.knn_prec} kNN screen (MFMA, f32 accumulate); f64 centring, eigensolve, "
-                 "projection and kNN rerank",
+        "dtype": (f"{ {'fp16': 'split-fp16', 'bf16x3': 'split-bf16', 'fp32': 'fp32'}[args.gram_prec] } Gram + "
+                  f"{args.knn_prec} kNN screen (MFMA, f32 accumulate); f64 centring, eigensolve, projection and kNN "
+                  "rerank"),
         "data": "synthetic (seeded NB counts, crest_amd.synth)",
         "config": {"workload": (f"{n} cells x {args.genes} pre-selected HVGs, NB counts density "
                                 f"{info['density']:.4f}, {args.clusters} clusters, seed {args.seed}; normalize_total 1e4,"
                                 f" log1p, scale max_value 10, PCA {args.pcs}, exact kNN k={args.k}"),
                    "n_cells": n, "n_genes": args.genes, "nnz": info["nnz"], "n_comps": args.pcs, "k": args.k,
-                   "chunk_rows": args.chunk_rows, "prec_gram": "bf16x3", "prec_knn": args.knn_prec},
+                   "chunk_rows": args.chunk_rows, "prec_gram": args.gram_prec, "prec_knn": args.knn_prec},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "ms_per_step_with_d2h": ms + d2h_ms,
         "hbm_copy_GBps": copy_gbps,
@@ -340,7 +340,7 @@ def main() -> int:
                      "launch_ms": stages["knn_screen"]},
     }
     if alt is not None:
-        line["alt_screen"] = alt
+        line["alt"] = alt
     if args.fp32_steps > 0:
         line["fp32"] = {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
                         "stages_ms": {k: round(v, 3) for k, v in stages32.items()}}

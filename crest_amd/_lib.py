@@ -124,7 +124,8 @@ def check(rc: int) -> None:
 
 
 def set_precision(prec) -> None:
-    """Arithmetic of the Gram and kNN screen: "fp64" (default), "fp32" or "bf16x3"."""
+    """Arithmetic of the Gram and kNN screen: "fp64" (default), "fp32", "bf16x3" or
+    "fp16" (split-FP16 Gram, which needs clipped scaled values, and FP16 screen)."""
     p = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
     check(lib().crest_set_precision(p))
 

@@ -258,7 +258,15 @@ static void check_scale(int32_t scale, const double *mean, size_t n_mean, const 
     if (n_mean != d || n_sd != d) value_error("mean/std must have length d");
 }
 
-static int tile_fmt(int prec) { return prec == CREST_PREC_FP32 ? TILE_F32 : prec == CREST_PREC_BF16X3 ? TILE_BF16X2 : TILE_F64; }
+static int tile_fmt(int prec) {
+    return prec == CREST_PREC_FP32 ? TILE_F32 : prec == CREST_PREC_BF16X3 ? TILE_BF16X2 : prec == CREST_PREC_FP16 ? TILE_F16X2 : TILE_F64;
+}
+// Split-FP16 tiles hold |s| <= 65504: guaranteed when the scaled values are clipped,
+// |s| = |clip(z) - clip(-mu/sd)| <= 2 max_value.
+static void check_gram_prec(int prec, int scale, double max_value) {
+    if (prec == CREST_PREC_FP16 && !(scale && max_value > 0.0 && max_value <= 30000.0))
+        value_error("the FP16 Gram needs scaled values clipped at 0 < max_value <= 30000");
+}
 static size_t tile_elem_bytes(int fmt) { return fmt == TILE_F64 ? 8 : fmt == TILE_F32 ? 4 : 2; }
 
 static int choose_split(int nblk, int64_t rows_pad) {
@@ -278,10 +286,10 @@ static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, i
         const int64_t rows_pad = round_up(r1 - r0, 16);
         const size_t tile_elems = (size_t)rows_pad * d_pad;
         void *ta = x.get<uint8_t>("tile_a", tile_elems * tile_elem_bytes(fmt));
-        void *tb = fmt == TILE_BF16X2 ? x.get<uint8_t>("tile_b", tile_elems * 2) : nullptr;
+        void *tb = (fmt == TILE_BF16X2 || fmt == TILE_F16X2) ? x.get<uint8_t>("tile_b", tile_elems * 2) : nullptr;
         const int n_split = choose_split(nblk, rows_pad);
         double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
-        launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, x.st);
+        launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, dev_gram, x.st);
         launch_gram(fmt, ta, tb, rows_pad, d_pad, n_split, partial, x.st);
         launch_gram_reduce(partial, n_split, d_pad, dev_gram, x.st);
         x.check_launch();
@@ -383,8 +391,10 @@ static void check_pipeline_args(const crest_pipeline_args *a, double *variance, 
     if (!a || !res || !variance || !loadings) value_error("NULL argument");
     if (a->n_cells < 2 || a->d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
     if (!a->hvg_of_gene || !a->dev_x_pca || !a->dev_knn_idx || !a->dev_knn_dist) value_error("NULL argument");
-    if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3)
+    if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3 &&
+        a->prec_gram != CREST_PREC_FP16)
         value_error("unknown Gram precision");
+    check_gram_prec(a->prec_gram, a->scale, a->max_value);
     if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
         a->prec_knn != CREST_PREC_FP16)
         value_error("unknown kNN screen precision");
@@ -522,8 +532,7 @@ int crest_set_device(int32_t device) {
 
 int crest_set_precision(int32_t prec) {
     return guarded([&] {
-        if (prec == CREST_PREC_FP16) value_error("FP16 applies to the kNN screen only (crest_set_knn_precision)");
-        if (prec != CREST_PREC_FP64 && prec != CREST_PREC_FP32 && prec != CREST_PREC_BF16X3)
+        if (prec != CREST_PREC_FP64 && prec != CREST_PREC_FP32 && prec != CREST_PREC_BF16X3 && prec != CREST_PREC_FP16)
             value_error("unknown precision");
         g_prec = prec;
     });
@@ -686,6 +695,7 @@ int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size
         Ctx &x = ctx();
         std::lock_guard<std::mutex> lk(x.mu);
         const int prec = g_prec;
+        check_gram_prec(prec, scale, max_value);
         const int d_pad = (int)round_up((int64_t)d, kGramBlock);
         DevChunk c = upload_chunk(x, h);
         DevScale sc{};

@@ -13,7 +13,7 @@ constexpr int kGramBlock = 128;   // Gram output block (per workgroup)
 constexpr int kStatsWindow = 2048;  // vars accumulated in LDS per gene_stats window
 
 // Element formats of a densified tile.
-enum TileFmt : int { TILE_F64 = 0, TILE_F32 = 1, TILE_BF16X2 = 2 };
+enum TileFmt : int { TILE_F64 = 0, TILE_F32 = 1, TILE_BF16X2 = 2, TILE_F16X2 = 3 };
 
 // normalize_total + log1p of one entry (kernels.rs:77-82).  `ns` is the
 // per-cell factor target_sum/total (1 when total == 0).  log1p is the correctly
@@ -88,7 +88,7 @@ void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s
 void launch_qc(const DevChunk &c, const uint8_t *flag, double *row_total, double *row_ngenes, double *row_flag,
                uint32_t *gene_ncells, double *gene_total, hipStream_t s);
 void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b,
-                    int64_t rows_pad, int d_pad, double *colsum, hipStream_t s);
+                    int64_t rows_pad, int d_pad, double *colsum, double *gram, hipStream_t s);
 void launch_materialize_count(const DevChunk &c, int64_t *row_kept, hipStream_t s);
 void launch_materialize(const DevChunk &c, const int64_t *out_off, uint32_t *rows, uint32_t *vars,
                         float *vals, hipStream_t s);

@@ -154,22 +154,29 @@ template <int FMT> struct TileElem;
 template <> struct TileElem<TILE_F64> { typedef double T; static constexpr int DW = 1024; };
 template <> struct TileElem<TILE_F32> { typedef float T; static constexpr int DW = 2048; };
 template <> struct TileElem<TILE_BF16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
+template <> struct TileElem<TILE_F16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
 
 template <int FMT>
 __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *tile_a, void *tile_b,
-                                                 int64_t n_groups, int d_pad, double *colsum) {
+                                                 int64_t n_groups, int d_pad, double *colsum, double *gram) {
     typedef typename TileElem<FMT>::T T;
     constexpr int DW = TileElem<FMT>::DW;
-    constexpr int NST = FMT == TILE_BF16X2 ? 2 : 1;
+    constexpr bool X2 = FMT == TILE_BF16X2 || FMT == TILE_F16X2;  // hi/lo pairs
+    constexpr int NST = X2 ? 2 : 1;
+    constexpr bool LL = X2;  // also Σ l² per column, the Gram diagonal's L^T L term
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double *cs = reinterpret_cast<double *>(lds_raw);              // [DW]
-    T *stage = reinterpret_cast<T *>(lds_raw + DW * sizeof(double)); // [NST][DW][8]
+    double *cl = cs + DW;                                          // [DW] when LL
+    T *stage = reinterpret_cast<T *>(lds_raw + (LL ? 2 : 1) * DW * sizeof(double)); // [NST][DW][8]
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int nwave = blockDim.x / kWave;
     const int64_t base = c.base;
     for (int h0 = 0; h0 < d_pad; h0 += DW) {
         const int dw = d_pad - h0 < DW ? d_pad - h0 : DW;
-        for (int i = threadIdx.x; i < DW; i += blockDim.x) cs[i] = 0.0;
+        for (int i = threadIdx.x; i < DW; i += blockDim.x) {
+            cs[i] = 0.0;
+            if constexpr (LL) cl[i] = 0.0;
+        }
         for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
             // zero the stage (vectorised)
             uint4 *st4 = reinterpret_cast<uint4 *>(stage);
@@ -194,11 +201,20 @@ __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *
                         stage[off] = s;
                     } else if constexpr (FMT == TILE_F32) {
                         stage[off] = (float)s;
-                    } else {
+                    } else if constexpr (FMT == TILE_BF16X2) {
                         const uint16_t hb = bf16_bits_rn((float)s);
                         const uint16_t lb = bf16_bits_rn((float)(s - (double)bf16_to_float(hb)));
                         stage[off] = hb;
                         stage[DW * kRowGroup + off] = lb;
+                        const double lf = (double)bf16_to_float(lb);
+                        atomicAdd(&cl[h - h0], lf * lf);
+                    } else {  // split-FP16: the residual from the FP64 value, both halves RNE
+                        const _Float16 hh = (_Float16)(float)s;
+                        const _Float16 ll = (_Float16)(float)(s - (double)(float)hh);
+                        stage[off] = __builtin_bit_cast(uint16_t, hh);
+                        stage[DW * kRowGroup + off] = __builtin_bit_cast(uint16_t, ll);
+                        const double lf = (double)(float)ll;
+                        atomicAdd(&cl[h - h0], lf * lf);
                     }
                     atomicAdd(&cs[h - h0], s);
                 }
@@ -214,14 +230,17 @@ __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *
             }
             __syncthreads();
         }
-        for (int i = threadIdx.x; i < dw; i += blockDim.x)
+        for (int i = threadIdx.x; i < dw; i += blockDim.x) {
             if (cs[i] != 0.0) atomicAdd(&colsum[h0 + i], cs[i]);
+            if constexpr (LL)
+                if (cl[i] != 0.0) atomicAdd(&gram[(int64_t)(h0 + i) * d_pad + h0 + i], cl[i]);
+        }
         __syncthreads();
     }
 }
 
 void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b,
-                    int64_t rows_pad, int d_pad, double *colsum, hipStream_t s) {
+                    int64_t rows_pad, int d_pad, double *colsum, double *gram, hipStream_t s) {
     const int64_t n_groups = rows_pad / kRowGroup;
     if (n_groups <= 0) return;
     size_t lds;
@@ -231,19 +250,29 @@ void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a
         lds = TileElem<TILE_F64>::DW * (sizeof(double) + 8 * sizeof(double));
         g = (int)std::min<int64_t>(n_groups, 512);
         hipFuncSetAttribute((const void *)k_densify<TILE_F64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_densify<TILE_F64>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        hipLaunchKernelGGL(k_densify<TILE_F64>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum,
+                           gram);
         break;
     case TILE_F32:
         lds = TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float));
         g = (int)std::min<int64_t>(n_groups, 1024);
         hipFuncSetAttribute((const void *)k_densify<TILE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_densify<TILE_F32>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        hipLaunchKernelGGL(k_densify<TILE_F32>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum,
+                           gram);
         break;
     default:
-        lds = TileElem<TILE_BF16X2>::DW * (sizeof(double) + 16 * sizeof(uint16_t));
+        lds = TileElem<TILE_BF16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t));
         g = (int)std::min<int64_t>(n_groups, 1024);
         hipFuncSetAttribute((const void *)k_densify<TILE_BF16X2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_densify<TILE_BF16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum);
+        if (fmt == TILE_F16X2) {
+            hipFuncSetAttribute((const void *)k_densify<TILE_F16X2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+            hipLaunchKernelGGL(k_densify<TILE_F16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad,
+                               colsum, gram);
+        } else {
+            hipLaunchKernelGGL(k_densify<TILE_BF16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad,
+                               colsum, gram);
+        }
         break;
     }
 }

@@ -12,6 +12,7 @@ namespace crest {
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) double f64x4;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 __device__ inline bf16x8 as_bf16x8(uint4 v) {
@@ -20,6 +21,16 @@ __device__ inline bf16x8 as_bf16x8(uint4 v) {
     return r;
 }
 __device__ inline bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// One K-step of 16 on v_mfma_f32_32x32x16_{bf16,f16}: TY 0 = bf16, 1 = fp16.
+template <int TY> __device__ inline f32x16 mfma_x16(u32x4 a, u32x4 b, f32x16 c) {
+    if constexpr (TY == 0)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
+}
 
 // blk -> (bi, bj) over the upper triangle of nb x nb blocks, row by row.
 __device__ inline void tri_block(int blk, int nb, int &bi, int &bj) {
@@ -51,13 +62,15 @@ __device__ inline void store_partial32(double *dst, const double (&v)[16], int m
     }
 }
 
-// ---------------------------------------------------------------- split-BF16
-// S = H + L (bf16 hi/lo); S^T S = H^T H + H^T L + L^T H + L^T L on
-// v_mfma_f32_32x32x16_bf16.  The L^T L product is kept: on the diagonal it is a
-// positive bias of up to 2^-18 relative that would otherwise shift every variance.
-// FP32 accumulators are added into FP64 every FLUSH K-steps (16 rows each).
-template <int FLUSH>
-__global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
+// ---------------------------------------------------------------- split 16-bit (BF16 or FP16)
+// S = H + L (16-bit hi and the 16-bit rounding of the residual); S^T S = H^T H +
+// H^T L + L^T H + L^T L.  The first three run on MFMA.  L^T L is ~2^-2b of |s_a s_b|
+// per row (b = 8 for bf16, 11 for fp16): off the diagonal its signs vary and it sums
+// to nothing measurable, but on the diagonal it is a positive bias, so k_densify adds
+// the exact diagonal Σ l² into the Gram instead.  FP32 accumulators are added into
+// FP64 every FLUSH K-steps (16 rows each).
+template <int TY, int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_x16(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
                                                      int64_t n_groups, int d_pad, int n_split, double *partial) {
     const int nb = d_pad / kGramBlock;
     int blk, split, bi, bj;
@@ -102,14 +115,9 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(const uint4 *__restrict__ t
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.ah[i]), as_bf16x8(o.bh[j]), acc[i][j],
-                                                                    0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.ah[i]), as_bf16x8(o.bl[j]), acc[i][j],
-                                                                    0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.al[i]), as_bf16x8(o.bh[j]), acc[i][j],
-                                                                    0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(o.al[i]), as_bf16x8(o.bl[j]), acc[i][j],
-                                                                    0, 0, 0);
+                acc[i][j] = mfma_x16<TY>(o.ah[i], o.bh[j], acc[i][j]);
+                acc[i][j] = mfma_x16<TY>(o.ah[i], o.bl[j], acc[i][j]);
+                acc[i][j] = mfma_x16<TY>(o.al[i], o.bh[j], acc[i][j]);
             }
         if (++cnt == FLUSH) {
             cnt = 0;
@@ -301,8 +309,12 @@ void launch_gram(int fmt, const void *tile_a, const void *tile_b, int64_t rows_p
         hipLaunchKernelGGL(k_gram_f32<32>, grid, block, 0, s, (const float4 *)tile_a, n_groups, d_pad, n_split, partial);
         break;
     default:
-        hipLaunchKernelGGL(k_gram_bf16x3<16>, grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b, n_groups,
-                           d_pad, n_split, partial);
+        if (fmt == TILE_F16X2)
+            hipLaunchKernelGGL((k_gram_x16<1, 16>), grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
+                               n_groups, d_pad, n_split, partial);
+        else
+            hipLaunchKernelGGL((k_gram_x16<0, 16>), grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
+                               n_groups, d_pad, n_split, partial);
         break;
     }
 }

@@ -36,9 +36,14 @@ extern "C" {
  * neighbours and distances always come from the exact rerank.                    */
 #define CREST_PREC_FP64 0   /* Gram on FP64 MFMA; kNN screen on FP32 MFMA          */
 #define CREST_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation flushed to FP64          */
-#define CREST_PREC_BF16X3 2 /* split-BF16 hi+lo, three MFMA products, FP32 accum.   */
-#define CREST_PREC_FP16 3   /* kNN screen only: FP16 operands (power-of-two
-                               prescaled), one MFMA product, FP32 accumulation       */
+#define CREST_PREC_BF16X3 2 /* split-BF16 hi+lo, three MFMA products (the Gram adds
+                               the exact diagonal of the lo.lo term), FP32 accum.   */
+#define CREST_PREC_FP16 3   /* Gram: split-FP16 hi+lo, three MFMA products plus the
+                               exact diagonal of the lo.lo term, FP32 accumulation;
+                               needs scaled values clipped at 0 < max_value <= 30000
+                               (else CREST_E_VALUE).  kNN screen: FP16 operands
+                               (power-of-two prescaled), one MFMA product, FP32
+                               accumulation                                         */
 
 /* One chunk of raw counts plus the recorded filters and lazy transform: the
  * 9-tuple `(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr,
@@ -176,7 +181,7 @@ typedef struct crest_pipeline_args {
     double max_value;   /* clip (<= 0: off)                                          */
     size_t n_comps;     /* requested PCs                                             */
     size_t k;           /* neighbours per cell (knn_graph k)                         */
-    int32_t prec_gram;  /* CREST_PREC_* for the Gram                                 */
+    int32_t prec_gram;  /* CREST_PREC_* for the Gram (FP16: split-FP16)              */
     int32_t prec_knn;   /* CREST_PREC_FP64/FP32 (FP32 screen), BF16X3 or FP16         */
     size_t chunk_rows;  /* cells per densified tile (0 = default)                    */
     /* outputs in device memory */

@@ -128,7 +128,7 @@ def _gram_inputs(bf, oracle, d_top=300):
     return gmap, np.ascontiguousarray(mean[mask]), np.ascontiguousarray(sd[mask]), d
 
 
-@pytest.mark.parametrize("prec,rtol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-5)])
+@pytest.mark.parametrize("prec,rtol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-5), ("fp16", 4e-6)])
 def test_gram_accumulate(counts, oracle, prec, rtol):
     bf = _frame(counts, 5000)
     gmap, mean, sd, d = _gram_inputs(bf, oracle)
@@ -146,6 +146,21 @@ def test_gram_accumulate(counts, oracle, prec, rtol):
     assert np.abs(g_got[iu] - g_ref[iu]).max() <= rtol * scale
     assert np.all(np.tril(g_got, -1) == 0)  # lower triangle untouched
     np.testing.assert_allclose(c_got, c_ref, rtol=1e-12, atol=1e-9)
+
+
+def test_fp16_gram_needs_clipped_values(counts, oracle):
+    """Split-FP16 tiles hold |s| <= 65504, which clipping at max_value guarantees."""
+    bf = _frame(counts, 5000)
+    gmap, mean, sd, d = _gram_inputs(bf, oracle)
+    ca.set_precision("fp16")
+    try:
+        ctx = next(bf.iter_ctx())
+        with pytest.raises(ValueError):
+            N.gram_accumulate(*ctx, gmap, np.zeros((d, d)), np.zeros(d), False, None, None, 0.0)
+        with pytest.raises(ValueError):
+            N.gram_accumulate(*ctx, gmap, np.zeros((d, d)), np.zeros(d), True, mean, sd, 0.0)
+    finally:
+        ca.set_precision("fp64")
 
 
 def test_pca_from_gram(counts, oracle):
@@ -229,7 +244,9 @@ def test_knn_precision_knob():
         with pytest.raises(ValueError):
             ca.set_knn_precision(9)
         with pytest.raises(ValueError):
-            ca.set_precision("fp16")  # FP16 is a screen-only precision
+            ca.set_precision(7)
+        ca.set_precision("fp16")  # split-FP16 Gram + FP16 screen
+        assert ca.get_precision() == "fp16"
     finally:
         ca.set_knn_precision(None)
         ca.set_precision("fp64")

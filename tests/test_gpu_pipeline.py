@@ -46,7 +46,8 @@ def reference(data, oracle):
 @pytest.mark.parametrize("prec_gram,prec_knn,var_rtol,tot_rtol", [("fp64", "fp32", 1e-9, 1e-9),
                                                                   ("fp32", "fp32", 1e-5, 1e-7),
                                                                   ("bf16x3", "bf16x3", 1e-4, 1e-6),
-                                                                  ("bf16x3", "fp16", 1e-4, 1e-6)])
+                                                                  ("bf16x3", "fp16", 1e-4, 1e-6),
+                                                                  ("fp16", "fp16", 1e-5, 1e-7)])
 def test_pipeline(data, reference, oracle, prec_gram, prec_knn, var_rtol, tot_rtol):
     from crest_amd.pipeline import DeviceCSR, Pipeline
     indptr, indices, vals, g = data
