@@ -6,7 +6,8 @@ libcrest_hip.so, reached through a C ABI (include/crest_hip.h).
 """
 from . import pp, tl
 from ._lib import get_knn_precision, get_precision, set_knn_precision, set_precision
-from .core import DEFAULT_CHUNK_NNZ, BioFrame, CSRStore, Frame, TripletStore
+from .core import DEFAULT_CHUNK_NNZ, BioFrame, CSRStore, Frame, ParquetStore, TripletStore, read_parquet
 
-__all__ = ["pp", "tl", "BioFrame", "CSRStore", "TripletStore", "Frame", "DEFAULT_CHUNK_NNZ",
+__all__ = ["pp", "tl", "BioFrame", "CSRStore", "TripletStore", "ParquetStore", "read_parquet", "Frame",
+           "DEFAULT_CHUNK_NNZ",
            "set_precision", "get_precision", "set_knn_precision", "get_knn_precision"]

@@ -117,6 +117,43 @@ class TripletStore:
             lo = hi
 
 
+class ParquetStore:
+    """Raw counts on disk (out-of-core), crest/core.py:108-134: a directory of
+    ``part-*.parquet`` files of (cell_id, gene_id, count) triplets for ranges of whole
+    cells.  One chunk = one file (``chunk_nnz`` is ignored), read with pyarrow."""
+
+    def __init__(self, path, n_cells: int, n_genes: int):
+        from pathlib import Path
+        self.path = Path(path)
+        self.parts = sorted(self.path.glob("part-*.parquet"))
+        if not self.parts:
+            raise FileNotFoundError(f"no part-*.parquet files in {self.path}")
+        self.n_cells, self.n_genes = int(n_cells), int(n_genes)
+
+    @property
+    def nnz(self) -> int:
+        import pyarrow.parquet as pq
+        return int(sum(pq.ParquetFile(p).metadata.num_rows for p in self.parts))
+
+    def chunks(self, chunk_nnz: int = DEFAULT_CHUNK_NNZ):
+        import pyarrow.parquet as pq
+        for p in self.parts:
+            t = pq.read_table(p, columns=["cell_id", "gene_id", "count"])
+            cols = [np.ascontiguousarray(t.column(c).to_numpy(), dt)
+                    for c, dt in (("gene_id", np.uint32), ("count", np.float32), ("cell_id", np.uint32))]
+            del t
+            yield cols[0], cols[1], cols[2], None, 0
+
+
+def _frame_to_arrow(f: "Frame"):
+    import pyarrow as pa
+    return pa.table({k: pa.array(np.asarray(f[k])) for k in f.columns})
+
+
+def _frame_from_arrow(t) -> "Frame":
+    return Frame({c: np.asarray(t.column(c).to_numpy(zero_copy_only=False)) for c in t.column_names})
+
+
 @dataclass
 class BioFrame:
     """A single-cell dataset: raw counts (never modified) plus annotations."""
@@ -229,6 +266,52 @@ class BioFrame:
     def filter_cells(self, mask) -> "BioFrame":
         return self._subset(obs_mask=np.asarray(mask, dtype=bool))
 
+    # ---- out-of-core (crest/core.py:368-397, 418-425)
+    def write_parquet(self, path, nnz_per_part: int = 1 << 23) -> None:
+        """Write the raw counts of the kept cells/genes (current ids) as a directory of
+        whole-cell ``part-*.parquet`` files of about nnz_per_part entries, plus
+        obs.parquet / var.parquet."""
+        import pyarrow as pa
+        import pyarrow.parquet as pq
+        from pathlib import Path
+        from . import _native
+        path = Path(path)
+        path.mkdir(parents=True, exist_ok=True)
+        for old in path.glob("part-*.parquet"):
+            old.unlink()
+        buf, n_buf, part = [], 0, 0
+
+        def flush():
+            nonlocal buf, n_buf, part
+            if buf:
+                r = np.concatenate([b[0] for b in buf])
+                g = np.concatenate([b[1] for b in buf])
+                v = np.concatenate([b[2] for b in buf])
+                pq.write_table(pa.table({"cell_id": r, "gene_id": g, "count": v}),
+                               path / f"part-{part:05d}.parquet", compression="zstd")
+                part += 1
+            buf, n_buf = [], 0
+
+        for ctx in self.iter_ctx(transform=False):
+            r, g, v = _native.materialize(*ctx)
+            lo = 0
+            while lo < len(r):
+                hi = min(lo + nnz_per_part - n_buf, len(r))
+                if hi < len(r):  # end on a cell boundary
+                    hi = int(np.searchsorted(r, r[hi - 1], side="right"))
+                buf.append((r[lo:hi], g[lo:hi], v[lo:hi]))
+                n_buf += hi - lo
+                lo = hi
+                if n_buf >= nnz_per_part:
+                    flush()
+        flush()
+        obs = self.obs.copy()
+        obs["cell_id"] = np.arange(self.n_obs, dtype=np.uint32)
+        var = self.var.copy()
+        var["gene_id"] = np.arange(self.n_vars, dtype=np.uint32)
+        pq.write_table(_frame_to_arrow(obs), path / "obs.parquet")
+        pq.write_table(_frame_to_arrow(var), path / "var.parquet")
+
     def filter_genes(self, mask) -> "BioFrame":
         return self._subset(var_mask=np.asarray(mask, dtype=bool))
 
@@ -240,3 +323,15 @@ class BioFrame:
             r, g, x = _native.materialize(*ctx)
             D[r, g] = x
         return D
+
+
+def read_parquet(path, chunk_nnz: int = DEFAULT_CHUNK_NNZ) -> BioFrame:
+    """Open a dataset written by BioFrame.write_parquet (crest/core.py:418-425)."""
+    import pyarrow.parquet as pq
+    from pathlib import Path
+    path = Path(path)
+    obs = _frame_from_arrow(pq.read_table(path / "obs.parquet"))
+    var = _frame_from_arrow(pq.read_table(path / "var.parquet"))
+    bf = BioFrame._with_store(ParquetStore(path, obs.height, var.height), obs, var)
+    bf.chunk_nnz = chunk_nnz
+    return bf

@@ -390,3 +390,22 @@ def test_neighbors_api(counts):
     assert nb["indices"].shape == (bf.n_obs, 9) and len(nb["rows"]) > 0
     C = ca.pp.connectivities_matrix(bf)
     assert (abs(C - C.T) > 0).nnz == 0
+
+
+def test_parquet_roundtrip(counts, tmp_path, oracle):
+    """write_parquet of a filtered frame, read back: the same raw counts (QC bit-exact)
+    and the same transformed statistics, one chunk per part file."""
+    bf = _frame(counts, 4000)
+    bf.write_parquet(tmp_path / "ds", nnz_per_part=7000)
+    back = ca.read_parquet(tmp_path / "ds")
+    assert len(back.store.parts) > 1 and back.n_obs == bf.n_obs and back.n_vars == bf.n_vars
+    D0 = bf.to_dense(transform=False)
+    D1 = back.to_dense(transform=False)
+    np.testing.assert_array_equal(D0, D1)
+    ca.pp.normalize_total(back, 1e4)
+    ca.pp.log1p(back)
+    np.testing.assert_allclose(ca.pp.gene_stats(back), ca.pp.gene_stats(bf), rtol=1e-12, atol=1e-12)
+    o0, v0 = ca.pp.calculate_qc_metrics(bf, inplace=False)
+    o1, v1 = ca.pp.calculate_qc_metrics(back, inplace=False)
+    np.testing.assert_array_equal(o0["total_counts"], o1["total_counts"])
+    np.testing.assert_array_equal(v0["n_cells_by_counts"], v1["n_cells_by_counts"])

@@ -115,3 +115,40 @@ def test_streamed_errors():
     v = np.zeros(4)
     assert lib().crest_pipeline_run_streamed(C.byref(a), None, None, None, 0, 0, v.ctypes.data, v.ctypes.data,
                                               C.byref(r)) != 0
+
+
+def _oracle_pca(oracle, indptr, indices, vals, g, n_comps):
+    n = len(indptr) - 1
+    cmap, gmap = np.arange(n, dtype=np.int64), np.arange(g, dtype=np.int32)
+    ctx = (indices, vals, cmap, gmap, 1e4, True, None, np.asarray(indptr, np.int64), 0)
+    st = np.zeros((g, 5))
+    oracle.gene_stats(*ctx, st)
+    mean = st[:, 0] / n
+    sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+    sd[sd == 0] = 1
+    gram, colsum = np.zeros((g, g)), np.zeros(g)
+    oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
+    return oracle.pca_from_gram(gram, colsum, n, n_comps)
+
+
+def test_many_subsets(oracle):
+    """The batching configuration as a parity case: independent subsets of log-uniform
+    sizes, each with its own generator and genes, through the same pipeline object
+    types one after another (sizes below one workgroup's 512 queries included)."""
+    from crest_amd.pipeline import DeviceCSR, Pipeline
+    rng = np.random.default_rng(11)
+    sizes = np.exp(rng.uniform(np.log(200), np.log(6000), 6)).astype(int)
+    for i, n in enumerate(sizes):
+        g = int(rng.integers(250, 700))
+        indptr, indices, vals, _ = synth.make_counts(int(n), g, 10, density=0.07, seed=100 + i)
+        p = Pipeline(DeviceCSR.upload(indptr, indices, vals, g), np.arange(g, dtype=np.int32), n_comps=30, k=15,
+                     prec_gram="fp16", prec_knn="fp16", chunk_rows=2048)
+        out = p.run()
+        var_o, load_o, tot_o = _oracle_pca(oracle, indptr, indices, vals, g, 30)
+        np.testing.assert_allclose(out.variance, var_o, rtol=1e-5, err_msg=f"subset {i} n={n} g={g}")
+        X = p.x_pca()
+        idx, dist = p.knn()
+        q = np.arange(min(int(n), 300))
+        ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
+        bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
+        assert bad == ties, (i, n, bad, ties)
