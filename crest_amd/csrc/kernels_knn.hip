@@ -896,11 +896,107 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
         }
 }
 
+// The same selection with LPC lanes per candidate (64/LPC candidate slots, one wave
+// per query).  Lane `sub` of a candidate's group reads features sub*W .. sub*W+W-1
+// (W = 8/LPC) of every 8-block, so a group loads whole 32-byte runs of the row and
+// holds exactly the reference's FP32 partial sums sub*W .. sub*W+W-1; the FP32
+// distance is then summed in the reference's order after a gather of the 8 partial
+// sums.  The FP64 distance of the same features is summed by a butterfly over the
+// group, which leaves the same value in every lane of it.
+template <int LPC>
+__global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict__ xqs, int64_t nq,
+                                                        const float *__restrict__ x, int64_t n, int p,
+                                                        const uint32_t *__restrict__ perm_q,
+                                                        const uint32_t *__restrict__ perm, int self,
+                                                        const uint32_t *__restrict__ cand_i, int n_cand, int k,
+                                                        uint32_t *idx_out, float *dist_out) {
+    constexpr int W = 8 / LPC;
+    constexpr int S = 64 / LPC;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / LPC, sub = lane % LPC;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;  // sorted query position
+    if (i >= nq) return;  // uniform across the wave
+    const int64_t q = perm_q[i];
+    const float *xq = xqs + q * (int64_t)p;
+    const bool has = slot < n_cand;
+    const uint32_t cpos = has ? cand_i[i * (int64_t)n_cand + slot] : 0xFFFFFFFFu;
+    const bool valid = has && (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+    const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
+    const float *xc = x + (int64_t)(valid ? c : 0u) * p;
+    const int m = p - p % 8;
+    float acc[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc[w] = 0.f;
+    double part = 0.0;
+    if (valid) {
+        for (int b = sub * W; b < m; b += 8)
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const float a = xq[b + w], e = xc[b + w];
+                const float t = a - e;
+                const float sq = t * t;
+                acc[w] = acc[w] + sq;
+                const double dd = (double)a - (double)e;
+                part += dd * dd;
+            }
+        for (int f = m + sub; f < p; f += LPC) {
+            const double dd = (double)xq[f] - (double)xc[f];
+            part += dd * dd;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o, 64);
+    // acc[0] + acc[1] + ... + acc[7] in order, then the tail (knn.rs:48-66)
+    const int base = lane - sub;
+    float s32 = 0.f;
+#pragma unroll
+    for (int j = 0; j < LPC; ++j)
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const float v = __shfl(acc[w], base + j, 64);
+            s32 = (j == 0 && w == 0) ? v : s32 + v;
+        }
+    if (valid)
+        for (int f = m; f < p; ++f) {
+            const float t = xq[f] - xc[f];
+            const float sq = t * t;
+            s32 = s32 + sq;
+        }
+    const double d = valid ? part : (double)INFINITY;
+    const int ns = n_cand < S ? n_cand : S;
+    int rank = 0;
+    for (int s = 0; s < ns; ++s) {
+        const double ds = __shfl(d, s * LPC, 64);
+        const uint32_t cs = __shfl(c, s * LPC, 64);
+        const int vs = __shfl((int)valid, s * LPC, 64);
+        if (vs && (ds < d || (ds == d && cs < c))) ++rank;
+    }
+    const bool sel = valid && rank < k;
+    const float dfl = sel ? sqrtf(s32) : INFINITY;
+    int rank2 = 0;
+    for (int s = 0; s < ns; ++s) {
+        const float fs = __shfl(dfl, s * LPC, 64);
+        const uint32_t cs = __shfl(c, s * LPC, 64);
+        const int ss = __shfl((int)sel, s * LPC, 64);
+        if (ss && (fs < dfl || (fs == dfl && cs < c))) ++rank2;
+    }
+    if (sel && sub == 0) {
+        idx_out[q * k + rank2] = c;
+        dist_out[q * k + rank2] = dfl;
+    }
+}
+
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o, int self,
                        const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
                        hipStream_t s) {
-    const int g = (int)ceil_div(nq, 4);
-    if (n_cand <= 64)
+    const int g = (int)ceil_div(nq, 4);  // one wave per query, four per workgroup
+    if (n_cand <= 16)
+        hipLaunchKernelGGL(k_knn_rerank_grp<4>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
+                           cand_i, n_cand, k, idx_out, dist_out);
+    else if (n_cand <= 32)
+        hipLaunchKernelGGL(k_knn_rerank_grp<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
+                           cand_i, n_cand, k, idx_out, dist_out);
+    else if (n_cand <= 64)
         hipLaunchKernelGGL(k_knn_rerank<1>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
                            cand_i, n_cand, k, idx_out, dist_out);
     else
