@@ -318,13 +318,61 @@ static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *d
 // k nearest rows of dev_xc [nc, p] for each row of dev_xq [nq, p] into
 // dev_idx/dev_dist [nq, k]; dev_xq == nullptr gives the kNN graph of dev_xc (the
 // queries are the candidates, each row's own entry excluded).
+// k > 64 (knn_device): FP32 dot products of a block of queries (<= 2 GiB) with all candidates
+// (rocBLAS), per-row radix selection of k2 = k + max(32, k/8) screened candidates
+// (rounded up to 64, at most the candidate count), exact rank and output order.
+static void knn_device_large(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
+                             uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
+    const bool graph = dev_xq == nullptr;
+    if (graph) {
+        dev_xq = dev_xc;
+        nq = nc;
+    }
+    const int64_t n_valid = nc - (graph ? 1 : 0);
+    int64_t k2 = ((int64_t)k + std::max(32, k / 8) + 63) / 64 * 64;
+    k2 = std::min(k2, n_valid);
+    if (k2 > kKnnLargeMaxCand || k > n_valid)
+        value_error("k is too large for the exact GPU kNN (at most 3584 neighbours)");
+    if (nc > INT32_MAX || p > INT32_MAX) value_error("too many points for the large-k GPU kNN");
+    const int cap = (int)std::min<int64_t>(4 * k2, nc);  // per-row copy of the keys below the group bound
+    const int64_t qb = std::max<int64_t>(
+        1, std::min<int64_t>({nq, ((int64_t)1 << 29) / (nc + 3), ((int64_t)1 << 26) / cap}));  // dots <= 2 GiB
+    const int64_t ld = (nc + 3) / 4 * 4;  // dot rows padded for four-candidate loads
+    float *cn = x.get<float>("knnl_cn", (size_t)ld);
+    float *qn = graph ? cn : x.get<float>("knnl_qn", (size_t)nq);
+    float *dots = x.get<float>("knnl_dots", (size_t)qb * ld);
+    uint32_t *cand = x.get<uint32_t>("knnl_cand", (size_t)qb * k2);
+    uint32_t *comp_key = x.get<uint32_t>("knnl_comp_key", (size_t)qb * cap);
+    uint32_t *comp_idx = x.get<uint32_t>("knnl_comp_idx", (size_t)qb * cap);
+    if (ev3) HIPCHK(hipEventRecord(ev3[0], x.st));
+    launch_knn_sqnorm(dev_xc, nc, p, cn, x.st);
+    if (!graph) launch_knn_sqnorm(dev_xq, nq, p, qn, x.st);
+    if (ev3) HIPCHK(hipEventRecord(ev3[1], x.st));
+    const float one = 1.0f, zero = 0.0f;
+    for (int64_t q0 = 0; q0 < nq; q0 += qb) {
+        const int64_t rows = std::min(qb, nq - q0);
+        // dots[r][c] = x_c . q_(q0+r): column-major (nc x rows) = Xc^T (nc x p) * Q (p x rows)
+        BLASCHK(rocblas_sgemm(x.blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nc,
+                              (rocblas_int)rows, p, &one, dev_xc, p, dev_xq + q0 * p, p, &zero, dots,
+                              (rocblas_int)ld));
+        launch_knn_select_rows(dots, ld, nc, qn, cn, q0, rows, graph ? 1 : 0, (int)k2, cap, comp_key, comp_idx, cand,
+                               x.st);
+        launch_knn_rerank_sort(dev_xq, dev_xc, p, cand, (int)k2, k, q0, rows, dev_idx, dev_dist, x.st);
+    }
+    if (ev3) {  // one timed span: screen and rank interleave per query block
+        HIPCHK(hipEventRecord(ev3[2], x.st));
+        HIPCHK(hipEventRecord(ev3[3], x.st));
+    }
+    x.check_launch();
+}
+
 static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
                        int prec, uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
+    if (k > 64) return knn_device_large(x, dev_xc, nc, dev_xq, nq, p, k, dev_idx, dev_dist, ev3);
     const bool graph = dev_xq == nullptr;
     if (graph) nq = nc;
     const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
     const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
-    if (k > 64) value_error("k > 64 is not supported by the exact GPU kNN");
     const int kpad = knn_kpad(kp, p);
     if (kpad < 0)
         value_error("data has too many columns for the GPU kNN (at most 134 for fp32/fp64, 137 for bf16x3, "

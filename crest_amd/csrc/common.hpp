@@ -122,6 +122,17 @@ bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o,
                        int self, const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
                        hipStream_t s);
+// Large-k kNN: f32 row norms; per query row of an FP32 dot-product block (row
+// stride ld, a multiple of 4; cn readable up to ld entries), the indices of the k2
+// smallest screened keys; exact rank of those k2 and the k
+// nearest in the reference's output order (k2 <= kKnnLargeMaxCand).
+constexpr int kKnnLargeMaxCand = 4096;
+void launch_knn_sqnorm(const float *x, int64_t n, int p, float *out, hipStream_t s);
+void launch_knn_select_rows(const float *dots, int64_t ld, int64_t nc, const float *qn, const float *cn,
+                            int64_t q0, int64_t rows, int self, int k2, int cap, uint32_t *comp_key,
+                            uint32_t *comp_idx, uint32_t *cand, hipStream_t s);
+void launch_knn_rerank_sort(const float *xq, const float *xc, int p, const uint32_t *cand, int k2, int k,
+                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s);
 size_t conn_temp_bytes(int64_t m);
 void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, int k, int n_neighbors,
                            uint64_t *keys, uint64_t *keys2, float *w, float *w2, float *wu, uint32_t *flag,

@@ -1004,4 +1004,378 @@ void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc,
                            cand_i, n_cand, k, idx_out, dist_out);
 }
 
+// ---------------------------------------------------------------- large k
+// k beyond the register lists (Scrublet asks for k ~ 1.5 sqrt(n), doublets.py:
+// 131-137).  The dot products of a block of queries with every candidate come from
+// one FP32 GEMM; a workgroup per query then radix-selects the k2 smallest screened
+// keys of its row, max(|q|^2 + |c|^2 - 2 q.c, 0) as knn.rs:126 forms it (the query
+// itself excluded in a graph), and a second workgroup per query ranks those k2
+// exactly (FP64 squared distance, ties by index), keeps k and orders them by the
+// reference's f32 distance and index (knn.rs:356-367).
+
+// knn.rs:346: sum of squares of a row in f32, in column order.
+__global__ void k_knn_sqnorm(const float *__restrict__ x, int64_t n, int p, float *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *r = x + i * p;
+    float s = 0.f;
+    for (int f = 0; f < p; ++f) {
+        const float sq = r[f] * r[f];
+        s = s + sq;
+    }
+    out[i] = s;
+}
+
+// Non-negative f32 distances order like their bit patterns.
+__device__ inline uint32_t screen_key(float dot, float qn, float cn) {
+    const float s = qn + cn;
+    const float t = 2.0f * dot;
+    const float d = s - t;
+    return __float_as_uint(d > 0.f ? d : 0.f);
+}
+
+constexpr int kSelBins = 2048;
+constexpr int kSelUnroll = 4;
+constexpr int kSelGroups = 4096;  // >= the largest k2
+
+struct SelShared {
+    uint32_t hist[kSelBins];
+    uint32_t bin, before;
+    uint32_t cnt[2][4];
+    uint32_t pcnt[8][4];
+};
+
+// The k-th smallest (1-based) of `count` keys key_at(i): three radix rounds over
+// key bits 31-21, 20-10 and 9-0.  Returns it and sets need = k - #(keys below it).
+// Called by all 256 threads of the workgroup.
+template <class KeyAt>
+__device__ uint32_t radix_kth(int64_t count, KeyAt key_at, uint32_t k, uint32_t &need_out, SelShared &sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t prefix = 0, need = k;
+    for (int round = 0; round < 3; ++round) {
+        const int shift = round == 0 ? 21 : round == 1 ? 10 : 0;
+        const uint32_t hmask = round == 0 ? 0u : round == 1 ? 0xFFE00000u : 0xFFFFFC00u;  // bits already fixed
+        const uint32_t dmask = round == 2 ? 0x3FFu : 0x7FFu;
+        for (int i = tid; i < kSelBins; i += 256) sh.hist[i] = 0;
+        __syncthreads();
+        for (int64_t i0 = 0; i0 < count; i0 += 256 * kSelUnroll) {
+            uint32_t kk[kSelUnroll];
+#pragma unroll
+            for (int u = 0; u < kSelUnroll; ++u) {
+                const int64_t i = i0 + u * 256 + tid;
+                kk[u] = i < count ? key_at(i) : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int u = 0; u < kSelUnroll; ++u) {
+                const bool act = i0 + u * 256 + tid < count && (kk[u] & hmask) == prefix;
+                const uint32_t bin = (kk[u] >> shift) & dmask;
+                uint64_t m = __ballot(act);
+                while (m) {  // one LDS add per distinct bin of the wave
+                    const int leader = __builtin_ctzll(m);
+                    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
+                    const uint64_t same = __ballot(act && bin == b);
+                    if (lane == leader) atomicAdd(&sh.hist[b], (uint32_t)__popcll(same));
+                    m &= ~same;
+                }
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {  // the bin holding the need-th key: 32 bins per lane, then a wave scan
+            constexpr int per = kSelBins / 64;
+            uint32_t sum = 0;
+            for (int b = 0; b < per; ++b) sum += sh.hist[lane * per + b];
+            uint32_t inc = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += v;
+            }
+            const uint64_t hit = __ballot(inc >= need);
+            if (lane == __builtin_ctzll(hit)) {
+                uint32_t acc = inc - sum;
+                int b = lane * per;
+                while (acc + sh.hist[b] < need) acc += sh.hist[b++];
+                sh.bin = (uint32_t)b;
+                sh.before = acc;
+            }
+        }
+        __syncthreads();
+        prefix |= sh.bin << shift;
+        need -= sh.before;
+        __syncthreads();
+    }
+    need_out = need;
+    return prefix;
+}
+
+// Among `count` entries in candidate order (key key_at(i), candidate idx_at(i)),
+// write to out[0..k2) every candidate whose key is below T, the k2-th smallest key,
+// then the first ones whose key equals T.  Called by all 256 threads.
+template <class KeyAt, class IdxAt>
+__device__ void select_ordered(int64_t count, KeyAt key_at, IdxAt idx_at, int k2, uint32_t *out, SelShared &sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t need = 0;
+    const uint32_t T = radix_kth(count, key_at, (uint32_t)k2, need, sh);
+    const uint32_t n_lt = (uint32_t)k2 - need;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t base_lt = 0, base_eq = 0;
+    for (int64_t i0 = 0; i0 < count; i0 += 256) {
+        const int64_t i = i0 + tid;
+        const uint32_t kk = i < count ? key_at(i) : 0xFFFFFFFFu;
+        const bool lt = i < count && kk < T, eq = i < count && kk == T;
+        const uint64_t blt = __ballot(lt), beq = __ballot(eq);
+        if (lane == 0) {
+            sh.cnt[0][wave] = (uint32_t)__popcll(blt);
+            sh.cnt[1][wave] = (uint32_t)__popcll(beq);
+        }
+        __syncthreads();
+        uint32_t off_lt = base_lt, off_eq = base_eq;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            if (w < wave) {
+                off_lt += sh.cnt[0][w];
+                off_eq += sh.cnt[1][w];
+            }
+            base_lt += sh.cnt[0][w];
+            base_eq += sh.cnt[1][w];
+        }
+        __syncthreads();
+        if (lt) out[off_lt + __popcll(blt & below)] = idx_at(i);
+        if (eq) {
+            const uint32_t pos = off_eq + (uint32_t)__popcll(beq & below);
+            if (pos < need) out[n_lt + pos] = idx_at(i);
+        }
+    }
+}
+
+// One workgroup (4 waves) per query row of `dots` (row r = query q0 + r at dots +
+// r ld, ld a multiple of 4, cn readable up to ld): the k2 candidates of smallest
+// screened key, in candidate order, into cand[r].  Pass 1 takes the minimum key of
+// each of G = 4096 >= k2 groups (a fixed map of c mod G) and U, the k2-th smallest
+// of those minima: k2 distinct candidates have keys <= U, so the k2-th smallest key
+// of the row is <= U, while few others are (about k2 (1 + k2/G)).  Pass 2 copies the
+// candidates with key <= U in candidate order into comp[r] (capacity cap); the
+// selection runs on that copy, or on the whole row when it overflowed.  Rows of at
+// most 2G candidates are selected directly.  Both passes read four candidates per
+// lane per load.
+__global__ __launch_bounds__(256) void k_knn_select_rows(const float *__restrict__ dots, int64_t ld, int64_t nc,
+                                                         const float *__restrict__ qn_all,
+                                                         const float *__restrict__ cn, int64_t q0, int self, int k2,
+                                                         int cap, uint32_t *__restrict__ comp_key,
+                                                         uint32_t *__restrict__ comp_idx, uint32_t *__restrict__ cand) {
+    __shared__ SelShared sh;
+    __shared__ uint32_t gmin[kSelGroups];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r = blockIdx.x;
+    const int64_t q = q0 + r;
+    const float *row = dots + r * ld;
+    const float qn = qn_all[q];
+    auto key = [&](int64_t c) -> uint32_t {
+        if (self && c == q) return 0x7F800000u;  // the query itself: +inf
+        return screen_key(row[c], qn, cn[c]);
+    };
+    // keys of candidates c .. c+3 (c a multiple of 4 below ld); past nc: all ones
+    auto key4 = [&](int64_t c, uint32_t *kk) {
+        const float4 d = *reinterpret_cast<const float4 *>(row + c);
+        const float4 n4 = *reinterpret_cast<const float4 *>(cn + c);
+        const float dv[4] = {d.x, d.y, d.z, d.w}, nv[4] = {n4.x, n4.y, n4.z, n4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t cj = c + j;
+            kk[j] = cj >= nc ? 0xFFFFFFFFu : (self && cj == q) ? 0x7F800000u : screen_key(dv[j], qn, nv[j]);
+        }
+    };
+    uint32_t *out = cand + r * (int64_t)k2;
+    if (nc <= 2 * kSelGroups) {
+        select_ordered(
+            nc, [&](int64_t i) { return key(i); }, [&](int64_t i) { return (uint32_t)i; }, k2, out, sh);
+        return;
+    }
+    // pass 1: group minima (group of c: (c/4) mod 1024 + 1024 (c mod 4)), then their k2-th smallest
+    constexpr int V = 4;  // float4 loads per lane per step
+    for (int i = tid; i < kSelGroups; i += 256) gmin[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < nc; c0 += 1024 * V) {
+        uint32_t kk[V][4];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int64_t c = c0 + (int64_t)(v * 256 + tid) * 4;
+            if (c < nc) key4(c, kk[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int64_t c = c0 + (int64_t)(v * 256 + tid) * 4;
+            if (c < nc) {
+                const uint32_t g = (uint32_t)((c >> 2) & 1023);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (kk[v][j] != 0xFFFFFFFFu) atomicMin(&gmin[g + 1024 * j], kk[v][j]);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t need_g = 0;
+    const uint32_t U = radix_kth((int64_t)kSelGroups, [&](int64_t i) { return gmin[i]; }, (uint32_t)k2, need_g, sh);
+    // pass 2: ordered copy of the keys <= U; order within a step: v, wave, lane, j
+    uint32_t *ck = comp_key + r * (int64_t)cap;
+    uint32_t *cix = comp_idx + r * (int64_t)cap;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t m = 0;
+    for (int64_t c0 = 0; c0 < nc && m <= (uint32_t)cap; c0 += 1024 * V) {
+        uint32_t kk[V][4];
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int64_t c = c0 + (int64_t)(v * 256 + tid) * 4;
+            if (c < nc)
+                key4(c, kk[v]);
+            else
+                kk[v][0] = kk[v][1] = kk[v][2] = kk[v][3] = 0xFFFFFFFFu;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) any |= kk[v][j] <= U;  // U < all ones
+        }
+        if (!__syncthreads_or(any)) continue;
+        uint32_t pre[V];  // hits of the lanes below, over all four slots
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            uint32_t tot = 0;
+            pre[v] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t b = __ballot(kk[v][j] <= U);
+                pre[v] += (uint32_t)__popcll(b & below);
+                tot += (uint32_t)__popcll(b);
+            }
+            if (lane == 0) sh.pcnt[v][wave] = tot;
+        }
+        __syncthreads();
+        uint32_t off[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            off[v] = m;
+            for (int w = 0; w < 4; ++w) {
+                if (w < wave) off[v] += sh.pcnt[v][w];
+                m += sh.pcnt[v][w];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            uint32_t pos = off[v] + pre[v];
+            const int64_t c = c0 + (int64_t)(v * 256 + tid) * 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (kk[v][j] <= U) {
+                    if (pos < (uint32_t)cap) {
+                        ck[pos] = kk[v][j];
+                        cix[pos] = (uint32_t)(c + j);
+                    }
+                    ++pos;
+                }
+        }
+    }
+    __syncthreads();  // the copy is complete and visible to the workgroup
+    if (m <= (uint32_t)cap)
+        select_ordered(
+            (int64_t)m, [&](int64_t i) { return ck[i]; }, [&](int64_t i) { return cix[i]; }, k2, out, sh);
+    else
+        select_ordered(
+            nc, [&](int64_t i) { return key(i); }, [&](int64_t i) { return (uint32_t)i; }, k2, out, sh);
+}
+
+// Bitonic sort of n2 (a power of two) (distance, index) pairs in LDS, ascending.
+__device__ void lds_bitonic(double *sd, uint32_t *si, int n2) {
+    for (int size = 2; size <= n2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+                const int lo = 2 * t - (t & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const double da = sd[lo], db = sd[hi];
+                const uint32_t ia = si[lo], ib = si[hi];
+                const bool gt = da > db || (da == db && ia > ib);
+                if (gt == up) {
+                    sd[lo] = db;
+                    sd[hi] = da;
+                    si[lo] = ib;
+                    si[hi] = ia;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One workgroup per query: FP64 squared distances of its k2 candidates, sorted by
+// (distance, index); the first k get the reference's f32 distance and are sorted by
+// (that distance, index) for output.  LDS: n2 doubles + n2 indices, n2 = 2^m >= k2.
+__global__ __launch_bounds__(256) void k_knn_rerank_sort(const float *__restrict__ xq_all,
+                                                         const float *__restrict__ xc, int p,
+                                                         const uint32_t *__restrict__ cand, int k2, int n2, int k,
+                                                         int64_t q0, uint32_t *idx_out, float *dist_out) {
+    extern __shared__ double lds_d[];
+    double *sd = lds_d;
+    uint32_t *si = reinterpret_cast<uint32_t *>(lds_d + n2);
+    const int64_t q = q0 + blockIdx.x;
+    const float *xq = xq_all + q * (int64_t)p;
+    const uint32_t *ci = cand + (int64_t)blockIdx.x * k2;
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        double d = INFINITY;
+        uint32_t c = 0xFFFFFFFFu;
+        if (j < k2) {
+            c = ci[j];
+            const float *xr = xc + (int64_t)c * p;
+            double acc = 0.0;
+            for (int f = 0; f < p; ++f) {
+                const double dd = (double)xq[f] - (double)xr[f];
+                acc += dd * dd;
+            }
+            d = acc;
+        }
+        sd[j] = d;
+        si[j] = c;
+    }
+    __syncthreads();
+    lds_bitonic(sd, si, n2);
+    int m = 1;
+    while (m < k) m <<= 1;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        float f = INFINITY;
+        uint32_t c = 0xFFFFFFFFu;
+        if (j < k) {
+            c = si[j];
+            f = sqrtf(ref_sq_dist(xq, xc + (int64_t)c * p, p));
+        }
+        sd[j] = (double)f;
+        si[j] = c;
+    }
+    __syncthreads();
+    lds_bitonic(sd, si, m);
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        idx_out[q * k + j] = si[j];
+        dist_out[q * k + j] = (float)sd[j];
+    }
+}
+
+void launch_knn_sqnorm(const float *x, int64_t n, int p, float *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_knn_sqnorm, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, n, p, out);
+}
+
+void launch_knn_select_rows(const float *dots, int64_t ld, int64_t nc, const float *qn, const float *cn,
+                            int64_t q0, int64_t rows, int self, int k2, int cap, uint32_t *comp_key,
+                            uint32_t *comp_idx, uint32_t *cand, hipStream_t s) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_knn_select_rows, dim3((unsigned)rows), dim3(256), 0, s, dots, ld, nc, qn, cn, q0, self, k2,
+                       cap, comp_key, comp_idx, cand);
+}
+
+void launch_knn_rerank_sort(const float *xq, const float *xc, int p, const uint32_t *cand, int k2, int k,
+                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s) {
+    if (rows <= 0) return;
+    int n2 = 1;
+    while (n2 < k2) n2 <<= 1;
+    const size_t lds = (size_t)n2 * (sizeof(double) + sizeof(uint32_t));
+    hipLaunchKernelGGL(k_knn_rerank_sort, dim3((unsigned)rows), dim3(256), lds, s, xq, xc, p, cand, k2, n2, k, q0,
+                       idx_out, dist_out);
+}
+
 }  // namespace crest

@@ -235,6 +235,80 @@ def test_knn_graph(oracle, prec, n, d, k, scale, offset):
     np.testing.assert_array_equal(dist, pd)
 
 
+@pytest.mark.parametrize("n,d,k", [(3000, 30, 100), (2500, 10, 300), (700, 5, 650), (1200, 50, 1190),
+                                   (20000, 30, 212)])
+def test_knn_graph_large_k(oracle, n, d, k):
+    """k beyond the register lists (Scrublet's k_adj ~ 1.5 sqrt(n), doublets.py:131-137):
+    same bar as the small-k path -- sets equal to the FP64 exact result up to ties,
+    ascending, self excluded, distances the reference's f32 formula."""
+    x = _clustered(n, d, 3 * n + d)
+    idx, dist = N.knn_graph(x, k)
+    assert idx.shape == (n, k) and dist.shape == (n, k)
+    assert not (idx == np.arange(n)[:, None]).any()
+    assert (np.diff(dist, axis=1) >= 0).all()
+    assert all(len(set(r.tolist())) == k for r in idx[:: max(1, n // 200)])
+    rows = np.arange(n) if n <= 3000 else np.arange(0, n, 17)
+    ref_i, ref_d2 = oracle.knn_exact_f64(x, k, queries=rows)
+    bad, ties = oracle.knn_mismatch_ties(idx[rows], x, rows, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    pd = oracle.pair_dist(x, np.repeat(rows, k), idx[rows].ravel()).reshape(len(rows), k)
+    np.testing.assert_array_equal(dist[rows], pd)
+
+
+def test_knn_large_k_duplicates(oracle):
+    """Many exact duplicates put ties at the selection threshold: the lowest indices win."""
+    base = _clustered(300, 6, 11)
+    x = np.repeat(base, 5, axis=0)  # every point five times
+    k = 120
+    idx, dist = N.knn_graph(x, k)
+    n = len(x)
+    X64 = x.astype(np.float64)
+    D = ((X64[:, None, :] - X64[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(D, np.inf)
+    ref_i = np.lexsort((np.broadcast_to(np.arange(n), D.shape), D), axis=1)[:, :k]  # (distance, index)
+    np.testing.assert_array_equal(np.sort(idx, 1), np.sort(ref_i, 1))
+    pd = oracle.pair_dist(x, np.repeat(np.arange(len(x)), k), idx.ravel()).reshape(len(x), k)
+    np.testing.assert_array_equal(dist, pd)
+
+
+def test_knn_large_k_overflow_ties():
+    """20,000 rows that are 5 distinct points: thousands of candidates share the k-th
+    key, so the bounded copy overflows and the whole row is selected; ties go to the
+    lowest indices."""
+    base = np.random.default_rng(3).normal(size=(5, 8)).astype(np.float32) * 4
+    n, k = 20000, 100
+    x = base[np.arange(n) % 5]
+    idx, dist = N.knn_graph(x, k)
+    for i in (0, 1, 7, 12345, n - 1):
+        want = [j for j in range(i % 5, n, 5) if j != i][:k]
+        assert idx[i].tolist() == want
+    assert (dist == 0).all()
+
+
+def test_knn_query_large_k(oracle):
+    R = _clustered(4000, 20, 5)
+    Q = _clustered(500, 20, 6)
+    k = 200
+    idx, dist = N.knn_query(R, Q, k)
+    assert idx.shape == (500, k)
+    assert (np.diff(dist, axis=1) >= 0).all()
+    ref_i, ref_d2 = oracle.knn_query_exact_f64(R, Q, k)
+    bad, ties = oracle.knn_query_mismatch_ties(idx, R, Q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    X = np.vstack([Q, R])
+    pd = oracle.pair_dist(X, np.repeat(np.arange(500), k), 500 + idx.ravel()).reshape(500, k)
+    np.testing.assert_array_equal(dist, pd)
+    idx, dist = N.knn_query(R[:150], Q[:7], 400)  # k above n_ref: every reference row
+    assert idx.shape == (7, 150)
+    ref_i, _ = oracle.knn_query_exact_f64(R[:150], Q[:7], 150)
+    np.testing.assert_array_equal(idx, ref_i)
+
+
+def test_knn_large_k_limit():
+    with pytest.raises(ValueError):
+        N.knn_graph(np.random.default_rng(0).normal(size=(5000, 4)).astype(np.float32), 4000)
+
+
 def test_knn_precision_knob():
     ca.set_precision("bf16x3")
     try:
