@@ -51,6 +51,25 @@ class PipelineResult(C.Structure):
     ]
 
 
+class BatchArgs(C.Structure):
+    """struct crest_batch_args."""
+    _fields_ = [
+        ("pipe", PipelineArgs), ("n_subsets", C.c_size_t), ("subset_off", C.c_void_p),
+        ("tile_rows", C.c_size_t), ("gram_rows", C.c_size_t), ("eig_batch", C.c_size_t), ("use_graph", C.c_int32),
+    ]
+
+
+class BatchResult(C.Structure):
+    """struct crest_batch_result."""
+    _fields_ = [
+        ("k_pcs", C.c_size_t), ("k_nn", C.c_size_t),
+        ("ms_stats", C.c_double), ("ms_gram", C.c_double), ("ms_eig", C.c_double), ("ms_project", C.c_double),
+        ("ms_knn_prep", C.c_double), ("ms_knn_screen", C.c_double), ("ms_knn_rerank", C.c_double),
+        ("ms_total", C.c_double), ("tile_rows", C.c_size_t), ("gram_rows", C.c_size_t),
+        ("graph_replayed", C.c_int32), ("gram_tuned", C.c_int32),
+    ]
+
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -92,6 +111,8 @@ _SIGS = {
     "crest_pipeline_run_streamed": (C.c_int, [C.POINTER(PipelineArgs), C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_int32, C.c_size_t, C.c_void_p, C.c_void_p,
                                               C.POINTER(PipelineResult)]),
+    "crest_pipeline_run_batched": (C.c_int, [C.POINTER(BatchArgs), C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.POINTER(BatchResult)]),
     "crest_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "crest_host_free": (C.c_int, [C.c_void_p]),
 }

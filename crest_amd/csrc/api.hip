@@ -68,6 +68,11 @@ struct Ctx {
     hipEvent_t ev[12] = {};
     hipStream_t copy = nullptr;        // host -> device uploads of streamed chunks
     hipEvent_t copied[2] = {}, consumed[2] = {};
+    // batch of subsets: the two captured graphs and the arguments they were captured for
+    std::string graph_key;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // autotuner: shape key -> chosen Gram row range
+    std::map<std::string, int64_t> tuned;
 
     void init(int d) {
         dev = d;
@@ -118,8 +123,17 @@ struct Ctx {
     }
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     void check_launch() { HIPCHK(hipGetLastError()); }
+    void drop_graphs() {
+        for (auto &g : graph)
+            if (g) {
+                hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        graph_key.clear();
+    }
     void release() {
         HIPCHK(hipStreamSynchronize(copy));
+        drop_graphs();
         for (auto &kv : bufs)
             if (kv.second.first) hipFree(kv.second.first);
         bufs.clear();
@@ -397,10 +411,18 @@ static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev
 }
 
 // mean/std of the HVGs from per-gene stats (pp.py:123-126, tl.py:31-34).
+// blockIdx.y = subset y of a batch: stats + y n_genes 5, mean/sd + y d, n from cell_off.
 __global__ void k_mean_std(const double *stats, const int32_t *gene_of_hvg, int d, double n, double *mean,
-                           double *sd) {
+                           double *sd, int64_t n_genes, const int64_t *cell_off) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= d) return;
+    if (cell_off) {
+        const int y = blockIdx.y;
+        stats += (int64_t)y * n_genes * 5;
+        mean += (int64_t)y * d;
+        sd += (int64_t)y * d;
+        n = (double)(cell_off[y + 1] - cell_off[y]);
+    }
     const int g = gene_of_hvg[h];
     const double m = stats[(int64_t)g * 5] / n;
     double var = 0.0;
@@ -508,7 +530,8 @@ static void run_pipeline(Ctx &x, const crest_pipeline_args *a, Each &&each, doub
     if (a->scale) {
         launch_fill_f64(stats, (int64_t)a->n_genes * 5, 0.0, x.st);
         each(tmpl, [&](const DevChunk &c) { launch_gene_stats(c, stats, (int)a->n_genes, x.st); });
-        hipLaunchKernelGGL(k_mean_std, dim3((d + 255) / 256), dim3(256), 0, x.st, stats, goh, d, (double)n, mean, sd);
+        hipLaunchKernelGGL(k_mean_std, dim3((d + 255) / 256), dim3(256), 0, x.st, stats, goh, d, (double)n, mean, sd,
+                           (int64_t)0, (const int64_t *)nullptr);
     }
     HIPCHK(hipEventRecord(ev[1], x.st));
     // PASS 1: Gram + colsum (tl.py:68-71)
@@ -567,6 +590,649 @@ static void par_copy(void *dst, const void *src, size_t bytes) {
         th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, len); });
     }
     for (auto &t : th) t.join();
+}
+
+// ---------------------------------------------------------------- batch of subsets
+// Host-side plan of a batch: where each subset's rows sit in the densified tiles, the
+// row ranges each workgroup takes, the Gram work items and the reduce items.
+struct BatchTile {
+    int64_t rows_pad = 0;
+    std::vector<SegDesc> segs;
+    std::vector<RowRange> dens;
+    std::vector<GramItem> items;
+    std::vector<RedItem> red;
+    int64_t n_slots = 0;
+};
+
+constexpr int64_t kDensRows = 64;     // rows densified per workgroup
+constexpr int64_t kStatsRows = 2048;  // rows per gene-statistics workgroup
+constexpr int64_t kProjRows = 128;    // rows per projection workgroup
+
+// Subsets laid out one after another in tiles of at most tile_rows rows (each segment
+// padded to 16 rows); a subset that does not fit the rest of a tile continues in the
+// next one.
+static std::vector<BatchTile> plan_tiles(const std::vector<int64_t> &off, int64_t tile_rows) {
+    std::vector<BatchTile> tiles(1);
+    const int n_sub = (int)off.size() - 1;
+    for (int y = 0; y < n_sub; ++y) {
+        int64_t c = off[y];
+        while (c < off[y + 1]) {
+            if (tile_rows - tiles.back().rows_pad < 16) tiles.emplace_back();
+            BatchTile &t = tiles.back();
+            const int64_t take = std::min(off[y + 1] - c, tile_rows - t.rows_pad);
+            const int64_t rows16 = round_up(take, 16);
+            t.segs.push_back(SegDesc{c, take, t.rows_pad / kRowGroup, y, 0});
+            for (int64_t r = 0; r < rows16; r += kDensRows)
+                t.dens.push_back(RowRange{r, std::min(r + kDensRows, rows16), (int32_t)t.segs.size() - 1, 0});
+            t.rows_pad += rows16;
+            c += take;
+        }
+    }
+    if (tiles.back().segs.empty()) tiles.pop_back();
+    return tiles;
+}
+
+// Gram items of one tile: a segment's rows are cut into ceil(rows / gram_rows) ranges
+// of equal length (whole 16-row steps).  A segment of one range adds its blocks
+// straight into its subset's Gram (one item per block and launch, so no two items
+// touch the same block); the ranges of a longer segment write partial slots that a
+// reduce item sums in range order.  Ranges are dealt round-robin, longest first, to
+// 8 queues that are then interleaved: with a persistent grid of a multiple of 8
+// workgroups (dispatched round-robin over the 8 XCDs) every block of one row range
+// runs on one XCD and reads those rows through that XCD's L2.
+static void plan_items(BatchTile &t, int nblk, int64_t gram_rows) {
+    struct Range {
+        int32_t subset, n_split, j;
+        int64_t g0, g1, slot0;
+    };
+    std::vector<Range> ranges;
+    t.items.clear();
+    t.red.clear();
+    t.n_slots = 0;
+    for (const SegDesc &sd : t.segs) {
+        const int64_t rows16 = round_up(sd.n, 16);
+        const int64_t groups = rows16 / kRowGroup;
+        const int64_t want = std::max<int64_t>(1, ceil_div(rows16, gram_rows));
+        const int64_t per = round_up(ceil_div(groups, want), 2);
+        const int n_split = (int)ceil_div(groups, per);
+        int64_t slot0 = -1;
+        if (n_split > 1) {
+            slot0 = t.n_slots;
+            t.n_slots += (int64_t)nblk * n_split;
+            for (int b = 0; b < nblk; ++b) t.red.push_back(RedItem{b, sd.subset, slot0 + (int64_t)b * n_split, n_split, 0});
+        }
+        for (int j = 0; j < n_split; ++j)
+            ranges.push_back(Range{sd.subset, n_split, j, sd.grp0 + j * per, sd.grp0 + std::min(groups, (j + 1) * per),
+                                   slot0});
+    }
+    std::stable_sort(ranges.begin(), ranges.end(),
+                     [](const Range &a, const Range &b) { return a.g1 - a.g0 > b.g1 - b.g0; });
+    std::vector<GramItem> q[8];
+    for (size_t r = 0; r < ranges.size(); ++r) {
+        const Range &R = ranges[r];
+        for (int b = 0; b < nblk; ++b)
+            q[r % 8].push_back(GramItem{b, R.subset, R.g0, R.g1, R.slot0 < 0 ? -1 : R.slot0 + (int64_t)b * R.n_split + R.j});
+    }
+    size_t qmax = 0;
+    for (auto &v : q) qmax = std::max(qmax, v.size());
+    t.items.assign(8 * qmax, GramItem{-1, 0, 0, 0, 0});
+    for (size_t j = 0; j < qmax; ++j)
+        for (int z = 0; z < 8; ++z)
+            if (j < q[z].size()) t.items[8 * j + z] = q[z][j];
+}
+
+// Host tables packed into one device buffer (256-byte aligned pieces).
+struct TablePack {
+    std::vector<uint8_t> bytes;
+    template <class T> size_t add(const std::vector<T> &v) {
+        const size_t o = round_up((int64_t)bytes.size(), 256);
+        bytes.resize(o + std::max<size_t>(v.size() * sizeof(T), 1));
+        if (!v.empty()) std::memcpy(bytes.data() + o, v.data(), v.size() * sizeof(T));
+        return o;
+    }
+};
+
+// The autotuner's persistent cache: "key value" lines in the file named by
+// CREST_TUNE_CACHE (read once per process, appended on every new choice).
+static std::mutex g_tune_mu;
+static std::map<std::string, int64_t> g_tune;
+static bool g_tune_loaded = false;
+static bool tune_lookup(const std::string &key, int64_t &v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_loaded) {
+        g_tune_loaded = true;
+        if (const char *f = std::getenv("CREST_TUNE_CACHE")) {
+            if (FILE *fp = std::fopen(f, "r")) {
+                char k[512];
+                long long val;
+                while (std::fscanf(fp, "%511s %lld", k, &val) == 2) g_tune[k] = val;
+                std::fclose(fp);
+            }
+        }
+    }
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    v = it->second;
+    return true;
+}
+static void tune_store(const std::string &key, int64_t v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = v;
+    if (const char *f = std::getenv("CREST_TUNE_CACHE"))
+        if (FILE *fp = std::fopen(f, "a")) {
+            std::fprintf(fp, "%s %lld\n", key.c_str(), (long long)v);
+            std::fclose(fp);
+        }
+}
+
+static void check_batch_args(const crest_batch_args *b, double *variance, double *loadings, double *total,
+                             crest_batch_result *res) {
+    if (!b || !variance || !loadings || !total || !res || !b->subset_off) value_error("NULL argument");
+    const crest_pipeline_args *a = &b->pipe;
+    if (!a->dev_indptr || !a->dev_genes || !a->dev_values) value_error("NULL argument");
+    if (!a->hvg_of_gene || !a->dev_x_pca || !a->dev_knn_idx || !a->dev_knn_dist) value_error("NULL argument");
+    if (b->n_subsets < 1 || b->n_subsets > (1u << 20)) value_error("n_subsets must be in 1 .. 2^20");
+    if (a->d == 0) fail(CREST_E_RUNTIME, "need at least 1 gene");
+    if (a->n_cells >= (size_t)UINT32_MAX) value_error("too many cells for one batch");
+    if (a->prec_gram != CREST_PREC_FP64 && a->prec_gram != CREST_PREC_FP32 && a->prec_gram != CREST_PREC_BF16X3 &&
+        a->prec_gram != CREST_PREC_FP16)
+        value_error("unknown Gram precision");
+    check_gram_prec(a->prec_gram, a->scale, a->max_value);
+    if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
+        a->prec_knn != CREST_PREC_FP16)
+        value_error("unknown kNN screen precision");
+    if (a->k < 1 || a->k > 64) value_error("the batched kNN needs 1 <= k <= 64");
+    const int64_t kp = (int64_t)std::min<size_t>(a->n_comps, a->d);
+    if (kp < 1 || kp > 256) value_error("n_comps must be in 1 .. 256");
+    const int64_t need = std::max<int64_t>(kp, (int64_t)a->k) + 1;
+    const int64_t *o = b->subset_off;
+    if (o[0] != 0 || o[b->n_subsets] != (int64_t)a->n_cells) value_error("subset_off must run from 0 to n_cells");
+    for (size_t i = 0; i < b->n_subsets; ++i)
+        if (o[i + 1] - o[i] < need)
+            value_error("subset " + std::to_string(i) + " has " + std::to_string(o[i + 1] - o[i]) +
+                        " cells; a batch needs at least max(n_comps, k) + 1 = " + std::to_string(need) +
+                        " per subset");
+}
+
+// Everything one call needs on the device, planned on the host.
+struct BatchPlan {
+    std::string pre;  // buffer-name prefix: graph-mode calls own their buffers ("bg_"), eager ones "bt_"
+    int n_sub = 0, d = 0, d_pad = 0, kp = 0, kn = 0, fmt = 0, kprec = 0, L = 0, kpad = 0, nblk = 0, grid = 0;
+    int64_t n = 0, max_rows = 0, n_pad = 0, max_pad = 0, tile_rows = 0, gram_rows = 0, eig_batch = 0;
+    std::vector<int64_t> off, row0, nrow;
+    std::vector<BatchTile> tiles;
+    // device tables
+    const int64_t *d_off = nullptr, *d_row0 = nullptr, *d_nrow = nullptr;
+    const SegDesc *d_sub_segs = nullptr;
+    const RowRange *d_stats_ranges = nullptr, *d_proj_ranges = nullptr;
+    int64_t n_stats_ranges = 0, n_proj_ranges = 0;
+    const int32_t *d_blk_seg = nullptr, *d_order = nullptr;
+    std::vector<const SegDesc *> d_tile_segs;
+    std::vector<const RowRange *> d_tile_dens;
+    std::vector<const GramItem *> d_tile_items;
+    std::vector<const RedItem *> d_tile_red;
+    int64_t max_slots = 0, max_tile_rows = 0;
+};
+
+// Builds every host table and uploads them into one buffer (synchronously).
+static void batch_tables(Ctx &x, BatchPlan &P) {
+    TablePack tp;
+    const size_t o_off = tp.add(P.off), o_row0 = tp.add(P.row0), o_nrow = tp.add(P.nrow);
+    std::vector<SegDesc> sub_segs;
+    std::vector<RowRange> stats_r, proj_r;
+    for (int y = 0; y < P.n_sub; ++y) {
+        const int64_t ny = P.off[y + 1] - P.off[y];
+        sub_segs.push_back(SegDesc{P.off[y], ny, 0, y, 0});
+        for (int64_t r = 0; r < ny; r += kStatsRows) stats_r.push_back(RowRange{r, std::min(ny, r + kStatsRows), y, 0});
+        for (int64_t r = 0; r < ny; r += kProjRows) proj_r.push_back(RowRange{r, std::min(ny, r + kProjRows), y, 0});
+    }
+    P.n_stats_ranges = (int64_t)stats_r.size();
+    P.n_proj_ranges = (int64_t)proj_r.size();
+    const size_t o_segs = tp.add(sub_segs), o_sr = tp.add(stats_r), o_pr = tp.add(proj_r);
+    // kNN: subset of each 256-row block of operand rows; 512-row query blocks ordered by
+    // subset size (largest first, so the longest-running workgroups start first)
+    std::vector<int32_t> blk_seg(P.n_pad / kSegBlock), order;
+    std::vector<int> by_size(P.n_sub);
+    for (int y = 0; y < P.n_sub; ++y) {
+        by_size[y] = y;
+        for (int64_t r = P.row0[y]; r < P.row0[y + 1]; r += kSegBlock) blk_seg[r / kSegBlock] = y;
+    }
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return P.nrow[a] > P.nrow[b]; });
+    for (int y : by_size)
+        for (int64_t r = P.row0[y]; r < P.row0[y + 1]; r += 512) order.push_back((int32_t)(r / 512));
+    const size_t o_bs = tp.add(blk_seg), o_ord = tp.add(order);
+    std::vector<size_t> o_ts, o_td, o_ti, o_tr;
+    for (const BatchTile &t : P.tiles) {
+        o_ts.push_back(tp.add(t.segs));
+        o_td.push_back(tp.add(t.dens));
+        o_ti.push_back(tp.add(t.items));
+        o_tr.push_back(tp.add(t.red));
+    }
+    uint8_t *dev = x.get<uint8_t>(P.pre + "tables", tp.bytes.size());
+    HIPCHK(hipMemcpyAsync(dev, tp.bytes.data(), tp.bytes.size(), hipMemcpyHostToDevice, x.st));
+    x.sync();
+    P.d_off = (const int64_t *)(dev + o_off);
+    P.d_row0 = (const int64_t *)(dev + o_row0);
+    P.d_nrow = (const int64_t *)(dev + o_nrow);
+    P.d_sub_segs = (const SegDesc *)(dev + o_segs);
+    P.d_stats_ranges = (const RowRange *)(dev + o_sr);
+    P.d_proj_ranges = (const RowRange *)(dev + o_pr);
+    P.d_blk_seg = (const int32_t *)(dev + o_bs);
+    P.d_order = (const int32_t *)(dev + o_ord);
+    P.d_tile_segs.clear();
+    P.d_tile_dens.clear();
+    P.d_tile_items.clear();
+    P.d_tile_red.clear();
+    for (size_t i = 0; i < P.tiles.size(); ++i) {
+        P.d_tile_segs.push_back((const SegDesc *)(dev + o_ts[i]));
+        P.d_tile_dens.push_back((const RowRange *)(dev + o_td[i]));
+        P.d_tile_items.push_back((const GramItem *)(dev + o_ti[i]));
+        P.d_tile_red.push_back((const RedItem *)(dev + o_tr[i]));
+    }
+}
+
+// Device buffers of a batch (grow-only, named for the batch path alone).
+struct BatchBufs {
+    int64_t *cmap;
+    int32_t *gmap;
+    const int32_t *sub, *goh;
+    double *stats, *mean, *sd, *gram, *colsum, *cov, *w, *e, *var, *load, *tot;
+    rocblas_int *info;
+    float *shift;
+    void *ta, *tb;
+    double *partial;
+    void *xa, *xb, *kwork;
+    float *cd;
+    uint32_t *ci;
+};
+
+static BatchBufs batch_buffers(Ctx &x, const crest_batch_args *b, const BatchPlan &P,
+                               const std::vector<int32_t> &gene_of_hvg) {
+    const crest_pipeline_args *a = &b->pipe;
+    BatchBufs B{};
+    const size_t S = (size_t)P.n_sub, d = (size_t)P.d, dp = (size_t)P.d_pad;
+    B.cmap = x.get<int64_t>(P.pre + "cmap", P.n);
+    B.gmap = x.get<int32_t>(P.pre + "gmap", a->n_genes);
+    hipLaunchKernelGGL(k_iota_i64, dim3(1024), dim3(256), 0, x.st, B.cmap, P.n);
+    hipLaunchKernelGGL(k_iota_i32, dim3(256), dim3(256), 0, x.st, B.gmap, (int64_t)a->n_genes);
+    B.sub = x.upload(P.pre + "sub", a->hvg_of_gene, a->n_genes);
+    B.goh = x.upload(P.pre + "goh", gene_of_hvg.data(), d);
+    B.stats = x.get<double>(P.pre + "stats", S * a->n_genes * 5);
+    B.mean = x.get<double>(P.pre + "mean", S * d);
+    B.sd = x.get<double>(P.pre + "sd", S * d);
+    B.gram = x.get<double>(P.pre + "gram", S * dp * dp);
+    B.colsum = x.get<double>(P.pre + "colsum", S * dp);
+    const size_t eb = (size_t)P.eig_batch;
+    B.cov = x.get<double>(P.pre + "cov", eb * d * d);
+    B.w = x.get<double>(P.pre + "eig_w", eb * d);
+    B.e = x.get<double>(P.pre + "eig_e", eb * d);
+    B.info = x.get<rocblas_int>(P.pre + "eig_info", S);
+    B.var = x.get<double>(P.pre + "var", S * P.kp);
+    B.load = x.get<double>(P.pre + "load", S * d * P.kp);
+    B.tot = x.get<double>(P.pre + "total", S);
+    B.shift = x.get<float>(P.pre + "shift", S * P.kp);
+    const size_t tile_elems = (size_t)P.max_tile_rows * dp;
+    B.ta = x.get<uint8_t>(P.pre + "tile_a", tile_elems * tile_elem_bytes(P.fmt));
+    B.tb = (P.fmt == TILE_BF16X2 || P.fmt == TILE_F16X2) ? x.get<uint8_t>(P.pre + "tile_b", tile_elems * 2) : nullptr;
+    B.partial = x.get<double>(P.pre + "partial", (size_t)std::max<int64_t>(P.max_slots, 1) * kGramBlock * kGramBlock);
+    const size_t kb = P.kprec == CREST_PREC_FP32 ? 4 : 2;
+    B.xa = x.get<uint8_t>(P.pre + "knn_xa", (size_t)P.n_pad * P.kpad * kb);
+    B.xb = x.get<uint8_t>(P.pre + "knn_xb", (size_t)P.n_pad * P.kpad * kb);
+    B.kwork = x.get<uint8_t>(P.pre + "knn_work", knn_batch_work_bytes(P.n, P.n_pad, P.n_sub, P.kp));
+    B.cd = x.get<float>(P.pre + "knn_cd", (size_t)P.n_pad * 2 * P.L);
+    B.ci = x.get<uint32_t>(P.pre + "knn_ci", (size_t)P.n_pad * 2 * P.L);
+    return B;
+}
+
+static DevChunk batch_chunk(const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    const crest_pipeline_args *a = &b->pipe;
+    DevChunk c{};
+    c.span_off = a->dev_indptr;
+    c.span_cell = nullptr;
+    c.first_cell = 0;
+    c.n_spans = P.n;
+    c.genes = a->dev_genes;
+    c.values = a->dev_values;
+    c.cell_map = B.cmap;
+    c.n_cell_map = P.n;
+    c.gene_var = B.gmap;
+    c.n_gene_map = (int64_t)a->n_genes;
+    c.target_sum = a->target_sum;
+    c.log1p = a->log1p ? 1 : 0;
+    c.base = 0;
+    return c;
+}
+
+static DevScale batch_scale(const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    DevScale sc{};
+    sc.sub = B.sub;
+    sc.scale = b->pipe.scale ? 1 : 0;
+    sc.max_value = b->pipe.max_value;
+    sc.d = P.d;
+    sc.mean = B.mean;
+    sc.sd = B.sd;
+    return sc;
+}
+
+// Stage launches (no host synchronisation inside, so they can be captured).
+static void batch_stats(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    if (!b->pipe.scale) return;
+    const int64_t ng = (int64_t)b->pipe.n_genes;
+    launch_fill_f64(B.stats, (int64_t)P.n_sub * ng * 5, 0.0, x.st);
+    launch_gene_stats_ranges(batch_chunk(b, P, B), P.d_sub_segs, P.d_stats_ranges, P.n_stats_ranges, B.stats, (int)ng,
+                             x.st);
+    hipLaunchKernelGGL(k_mean_std, dim3((P.d + 255) / 256, P.n_sub), dim3(256), 0, x.st, B.stats, B.goh, P.d, 0.0,
+                       B.mean, B.sd, ng, P.d_off);
+}
+static void batch_gram(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    launch_fill_f64(B.gram, (int64_t)P.n_sub * P.d_pad * P.d_pad, 0.0, x.st);
+    launch_fill_f64(B.colsum, (int64_t)P.n_sub * P.d_pad, 0.0, x.st);
+    const DevChunk c = batch_chunk(b, P, B);
+    const DevScale sc = batch_scale(b, P, B);
+    for (size_t i = 0; i < P.tiles.size(); ++i) {
+        const BatchTile &t = P.tiles[i];
+        launch_densify_ranges(c, sc, P.fmt, B.ta, B.tb, P.d_pad, P.d_tile_segs[i], P.d_tile_dens[i],
+                              (int64_t)t.dens.size(), B.colsum, B.gram, x.st);
+        launch_gram_items(P.fmt, B.ta, B.tb, P.d_pad, P.d_tile_items[i], (int64_t)t.items.size(), P.grid, B.partial,
+                          B.gram, x.st);
+        launch_gram_reduce_items(B.partial, P.d_tile_red[i], (int64_t)t.red.size(), P.d_pad, B.gram, x.st);
+    }
+}
+static void batch_project(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    launch_shift_fill_batch(B.colsum, P.d_pad, B.load, P.d, P.kp, P.d_off, P.n_sub, P.max_rows, b->pipe.dev_x_pca,
+                            B.shift, x.st);
+    launch_project_ranges(batch_chunk(b, P, B), batch_scale(b, P, B), B.load, P.kp, P.d_sub_segs, P.d_proj_ranges,
+                          P.n_proj_ranges, b->pipe.dev_x_pca, x.st);
+}
+static KnnOrder batch_knn_prep(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    return launch_knn_prep_batch(P.kprec, b->pipe.dev_x_pca, P.n, P.d_off, P.n_sub, P.max_rows, P.d_row0, P.n_pad,
+                                 P.max_pad, P.kp, P.kpad, B.kwork, B.xa, B.xb, x.st);
+}
+static void batch_knn_screen(Ctx &x, const BatchPlan &P, const BatchBufs &B) {
+    const KnnSeg sg{P.d_row0, P.d_nrow, P.d_blk_seg, P.d_order};
+    if (!launch_knn_screen_batch(P.kprec, B.xa, B.xb, P.n_pad, P.kpad, P.L, sg, B.cd, B.ci, x.st))
+        value_error("no kNN kernel for this width");
+}
+static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B,
+                             const KnnOrder &o) {
+    launch_knn_rerank(b->pipe.dev_x_pca, P.n_pad, b->pipe.dev_x_pca, P.n_pad, P.kp, o, 1, B.ci, 2 * P.L, P.kn,
+                      b->pipe.dev_knn_idx, b->pipe.dev_knn_dist, x.st, P.d_blk_seg, P.d_off);
+}
+// Covariance, batched FP64 eigensolve and selection, eig_batch subsets per call
+// (rocSOLVER: eager, never captured).
+static void batch_eig(Ctx &x, const BatchPlan &P, const BatchBufs &B) {
+    for (int g0 = 0; g0 < P.n_sub; g0 += (int)P.eig_batch) {
+        const int ng = (int)std::min<int64_t>(P.eig_batch, P.n_sub - g0);
+        launch_cov_batch(B.gram + (int64_t)g0 * P.d_pad * P.d_pad, B.colsum + (int64_t)g0 * P.d_pad, P.d, P.d_pad,
+                         P.d_off + g0, ng, B.cov, B.tot + g0, x.st);
+        x.check_launch();
+        if (ng == 1)
+            BLASCHK(rocsolver_dsyevd(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov, P.d, B.w, B.e,
+                                     B.info + g0));
+        else
+            BLASCHK(rocsolver_dsyevd_strided_batched(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov,
+                                                     P.d, (rocblas_stride)P.d * P.d, B.w, P.d, B.e, P.d, B.info + g0,
+                                                     ng));
+        launch_eig_select_batch(B.cov, B.w, P.d, P.kp, ng, B.var + (int64_t)g0 * P.kp,
+                                B.load + (int64_t)g0 * P.d * P.kp, x.st);
+        x.check_launch();
+    }
+}
+
+// gram_rows from the autotuner: the first tile is densified once, then the Gram
+// launch is timed for each candidate range length (after one untimed run) and the
+// fastest is kept, cached per shape in-process and in CREST_TUNE_CACHE.
+static std::string tune_key(const BatchPlan &P) {
+    int64_t lg = 0;
+    while (((int64_t)1 << lg) < P.max_rows) ++lg;
+    return "gram_rows:fmt" + std::to_string(P.fmt) + ":dpad" + std::to_string(P.d_pad) + ":tile" +
+           std::to_string(P.tile_rows) + ":maxrows2^" + std::to_string(lg) + ":cu" + std::to_string(P.grid);
+}
+static int64_t tune_gram_rows(Ctx &x, const crest_batch_args *b, BatchPlan &P, const BatchBufs &B0, bool &ran) {
+    ran = false;
+    const std::string key = tune_key(P);
+    int64_t v = 0;
+    static const int64_t cand[] = {2048, 4096, 8192, 16384, 1 << 20};
+    std::vector<int64_t> cands;
+    for (int64_t c : cand)
+        if (c < round_up(P.max_rows, 16) || cands.empty()) cands.push_back(c);
+    if (cands.size() == 1) cands[0] = cand[sizeof(cand) / sizeof(cand[0]) - 1];
+    if (cands.size() > 1) {
+        ran = true;
+        BatchTile t = P.tiles[0];
+        // densify the first tile (its column sums and Gram diagonal are cleared again
+        // before the real pass)
+        batch_stats(x, b, P, B0);
+        launch_fill_f64(B0.gram, (int64_t)P.n_sub * P.d_pad * P.d_pad, 0.0, x.st);
+        launch_fill_f64(B0.colsum, (int64_t)P.n_sub * P.d_pad, 0.0, x.st);
+        launch_densify_ranges(batch_chunk(b, P, B0), batch_scale(b, P, B0), P.fmt, B0.ta, B0.tb, P.d_pad,
+                              P.d_tile_segs[0], P.d_tile_dens[0], (int64_t)t.dens.size(), B0.colsum, B0.gram, x.st);
+        float best = 0.f;
+        for (int64_t c : cands) {
+            plan_items(t, P.nblk, c);
+            TablePack tp;
+            const size_t oi = tp.add(t.items), orr = tp.add(t.red);
+            uint8_t *dev = x.get<uint8_t>("bt_tune_tables", tp.bytes.size());
+            HIPCHK(hipMemcpyAsync(dev, tp.bytes.data(), tp.bytes.size(), hipMemcpyHostToDevice, x.st));
+            double *part = x.get<double>("bt_tune_partial", (size_t)std::max<int64_t>(t.n_slots, 1) * kGramBlock *
+                                                               kGramBlock);
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                HIPCHK(hipEventRecord(x.ev[10], x.st));
+                launch_gram_items(P.fmt, B0.ta, B0.tb, P.d_pad, (const GramItem *)(dev + oi), (int64_t)t.items.size(),
+                                  P.grid, part, B0.gram, x.st);
+                launch_gram_reduce_items(part, (const RedItem *)(dev + orr), (int64_t)t.red.size(), P.d_pad, B0.gram,
+                                         x.st);
+                HIPCHK(hipEventRecord(x.ev[11], x.st));
+                x.check_launch();
+                HIPCHK(hipEventSynchronize(x.ev[11]));
+                HIPCHK(hipEventElapsedTime(&ms, x.ev[10], x.ev[11]));
+            }
+            if (v == 0 || ms < best) {
+                best = ms;
+                v = c;
+            }
+        }
+    } else {
+        v = cands[0];
+    }
+    tune_store(key, v);
+    return v;
+}
+
+static std::string batch_key(const crest_batch_args *b, const BatchPlan &P) {
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    put(&b->pipe, sizeof(crest_pipeline_args));
+    put(b->pipe.hvg_of_gene, b->pipe.n_genes * sizeof(int32_t));
+    put(P.off.data(), P.off.size() * sizeof(int64_t));
+    const int64_t extra[4] = {P.tile_rows, P.gram_rows, P.eig_batch, (int64_t)P.grid};
+    put(extra, sizeof(extra));
+    return k;
+}
+
+static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, double *loadings, double *total,
+                      crest_batch_result *res) {
+    const crest_pipeline_args *a = &b->pipe;
+    BatchPlan P;
+    P.n_sub = (int)b->n_subsets;
+    P.n = (int64_t)a->n_cells;
+    P.d = (int)a->d;
+    P.d_pad = (int)round_up(P.d, kGramBlock);
+    P.kp = (int)std::min<size_t>(a->n_comps, a->d);
+    P.kn = (int)a->k;
+    P.fmt = tile_fmt(a->prec_gram);
+    P.kprec = a->prec_knn == CREST_PREC_FP64 ? CREST_PREC_FP32 : a->prec_knn;
+    P.L = P.kn <= 16 ? 16 : P.kn <= 32 ? 32 : 64;
+    P.kpad = knn_kpad(P.kprec, P.kp);
+    if (P.kpad < 0) value_error("too many components for the GPU kNN screen");
+    P.nblk = (P.d_pad / kGramBlock) * (P.d_pad / kGramBlock + 1) / 2;
+    P.off.assign(b->subset_off, b->subset_off + P.n_sub + 1);
+    P.row0.assign(P.n_sub + 1, 0);
+    P.nrow.assign(P.n_sub, 0);
+    for (int y = 0; y < P.n_sub; ++y) {
+        const int64_t ny = P.off[y + 1] - P.off[y];
+        P.nrow[y] = ny;
+        P.row0[y + 1] = P.row0[y] + knn_rows_padded(ny);
+        P.max_rows = std::max(P.max_rows, ny);
+        P.max_pad = std::max(P.max_pad, knn_rows_padded(ny));
+    }
+    P.n_pad = P.row0[P.n_sub];
+    P.tile_rows = b->tile_rows ? round_up((int64_t)b->tile_rows, 16) : 524288;
+    P.eig_batch = b->eig_batch ? (int64_t)b->eig_batch : 64;
+    int n_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, x.dev));
+    P.grid = (int)round_up((int64_t)n_cu * gram_items_occupancy(P.fmt), 8);
+    P.tiles = plan_tiles(P.off, P.tile_rows);
+    for (const BatchTile &t : P.tiles) P.max_tile_rows = std::max(P.max_tile_rows, t.rows_pad);
+
+    std::vector<int32_t> gene_of_hvg(P.d, -1);
+    for (size_t g = 0; g < a->n_genes; ++g) {
+        const int32_t h = a->hvg_of_gene[g];
+        if (h >= P.d) value_error("hvg_of_gene values must be < d");
+        if (h >= 0) gene_of_hvg[h] = (int32_t)g;
+    }
+    for (int h = 0; h < P.d; ++h)
+        if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+
+    // Gram row ranges: given, cached by the autotuner for this shape, or tuned now (the
+    // tuner needs the first tile's tables and buffers sized for the smallest candidate)
+    bool tuned_now = false;
+    P.pre = b->use_graph ? "bg_" : "bt_";
+    P.gram_rows = b->gram_rows ? round_up((int64_t)b->gram_rows, 16) : 0;
+    auto plan_all = [&](int64_t rows) {
+        P.max_slots = 0;
+        for (BatchTile &t : P.tiles) {
+            plan_items(t, P.nblk, rows);
+            P.max_slots = std::max(P.max_slots, t.n_slots);
+        }
+    };
+    if (!P.gram_rows && !tune_lookup(tune_key(P), P.gram_rows)) {
+        // a new shape: time the candidates (a graph captured for other arguments goes,
+        // since the buffers it reads may be resized here)
+        if (b->use_graph) x.drop_graphs();
+        plan_all(2048);
+        densify_set_attrs();
+        batch_tables(x, P);
+        BatchBufs B0 = batch_buffers(x, b, P, gene_of_hvg);
+        P.gram_rows = tune_gram_rows(x, b, P, B0, tuned_now);
+    }
+    plan_all(P.gram_rows);
+    densify_set_attrs();
+    const std::string key = batch_key(b, P);
+    // graphs are replayed only by graph-mode calls with identical arguments; eager calls
+    // use their own buffers and leave them alone
+    const bool replay = b->use_graph && x.graph_key == key && x.graph[0] && x.graph[1];
+    if (b->use_graph && !replay) x.drop_graphs();
+    batch_tables(x, P);
+    const BatchBufs B = batch_buffers(x, b, P, gene_of_hvg);
+    x.sync();
+
+    hipEvent_t *ev = x.ev;
+    float ms[8] = {};
+    if (b->use_graph) {
+        if (!replay) {
+            KnnOrder o{};
+            for (int gi = 0; gi < 2; ++gi) {
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(x.st, hipStreamCaptureModeRelaxed));
+                try {
+                    if (gi == 0) {
+                        batch_stats(x, b, P, B);
+                        batch_gram(x, b, P, B);
+                    } else {
+                        batch_project(x, b, P, B);
+                        o = batch_knn_prep(x, b, P, B);
+                        batch_knn_screen(x, P, B);
+                        batch_knn_rerank(x, b, P, B, o);
+                    }
+                } catch (...) {
+                    hipStreamEndCapture(x.st, &g);
+                    if (g) hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(x.st, &g));
+                const hipError_t e = hipGraphInstantiate(&x.graph[gi], g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    x.drop_graphs();
+                    fail(CREST_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                }
+            }
+            x.graph_key = key;
+        }
+        HIPCHK(hipEventRecord(ev[0], x.st));
+        HIPCHK(hipGraphLaunch(x.graph[0], x.st));
+        HIPCHK(hipEventRecord(ev[1], x.st));
+        batch_eig(x, P, B);
+        HIPCHK(hipEventRecord(ev[2], x.st));
+        HIPCHK(hipGraphLaunch(x.graph[1], x.st));
+        HIPCHK(hipEventRecord(ev[3], x.st));
+        x.sync();
+        for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        res->ms_stats = 0.0;
+        res->ms_gram = ms[0];
+        res->ms_eig = ms[1];
+        res->ms_project = ms[2];
+        res->ms_knn_prep = res->ms_knn_screen = res->ms_knn_rerank = 0.0;
+        float tot = 0.f;
+        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3]));
+        res->ms_total = tot;
+    } else {
+        HIPCHK(hipEventRecord(ev[0], x.st));
+        batch_stats(x, b, P, B);
+        HIPCHK(hipEventRecord(ev[1], x.st));
+        batch_gram(x, b, P, B);
+        HIPCHK(hipEventRecord(ev[2], x.st));
+        batch_eig(x, P, B);
+        HIPCHK(hipEventRecord(ev[3], x.st));
+        batch_project(x, b, P, B);
+        HIPCHK(hipEventRecord(ev[4], x.st));
+        const KnnOrder o = batch_knn_prep(x, b, P, B);
+        HIPCHK(hipEventRecord(ev[5], x.st));
+        batch_knn_screen(x, P, B);
+        HIPCHK(hipEventRecord(ev[6], x.st));
+        batch_knn_rerank(x, b, P, B, o);
+        HIPCHK(hipEventRecord(ev[7], x.st));
+        x.check_launch();
+        x.sync();
+        for (int i = 0; i < 7; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        res->ms_stats = ms[0];
+        res->ms_gram = ms[1];
+        res->ms_eig = ms[2];
+        res->ms_project = ms[3];
+        res->ms_knn_prep = ms[4];
+        res->ms_knn_screen = ms[5];
+        res->ms_knn_rerank = ms[6];
+        float tot = 0.f;
+        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[7]));
+        res->ms_total = tot;
+    }
+    res->k_pcs = (size_t)P.kp;
+    res->k_nn = (size_t)P.kn;
+    res->tile_rows = (size_t)P.tile_rows;
+    res->gram_rows = (size_t)P.gram_rows;
+    res->graph_replayed = replay ? 1 : 0;
+    res->gram_tuned = tuned_now ? 1 : 0;
+    // results
+    std::vector<rocblas_int> info(P.n_sub);
+    x.download(info.data(), B.info, P.n_sub);
+    std::vector<double> hv((size_t)P.n_sub * P.kp), hl((size_t)P.n_sub * P.d * P.kp);
+    x.download(hv.data(), B.var, hv.size());
+    x.download(hl.data(), B.load, hl.size());
+    x.download(total, B.tot, P.n_sub);
+    x.sync();
+    for (int y = 0; y < P.n_sub; ++y)
+        if (info[y] != 0)
+            fail(CREST_E_RUNTIME, "eigensolver did not converge for subset " + std::to_string(y) + " (info " +
+                                      std::to_string(info[y]) + ")");
+    const size_t nc = b->pipe.n_comps;
+    for (int y = 0; y < P.n_sub; ++y)
+        for (int j = 0; j < P.kp; ++j) {
+            variance[(size_t)y * nc + j] = hv[(size_t)y * P.kp + j];
+            for (int i = 0; i < P.d; ++i)
+                loadings[((size_t)y * P.d + i) * nc + j] = hl[((size_t)y * P.d + i) * P.kp + j];
+        }
 }
 
 extern "C" {
@@ -1069,6 +1735,16 @@ int crest_host_alloc(size_t bytes, void **ptr) {
 int crest_host_free(void *ptr) {
     return guarded([&] {
         if (ptr) HIPCHK(hipHostFree(ptr));
+    });
+}
+
+int crest_pipeline_run_batched(const crest_batch_args *b, double *variance, double *loadings, double *total,
+                               crest_batch_result *res) {
+    return guarded([&] {
+        check_batch_args(b, variance, loadings, total, res);
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        run_batch(x, b, variance, loadings, total, res);
     });
 }
 

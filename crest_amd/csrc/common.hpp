@@ -83,7 +83,63 @@ struct DevScale {
     int d;
 };
 
+// Batches of independent cell subsets (many subsets in one launch).  A segment is a
+// contiguous run of cells [cell0, cell0 + n) of one subset; in a densified tile its
+// rows start at row group grp0 (rows padded to 16 with zeros).  A row range is a
+// piece [r0, r1) of a segment's rows handled by one workgroup.
+struct SegDesc {
+    int64_t cell0, n, grp0;
+    int32_t subset, pad_;
+};
+struct RowRange {
+    int64_t r0, r1;
+    int32_t seg, pad_;
+};
+// One Gram block over row groups [g0, g1) of a tile into partial slot `slot`
+// (blk < 0: no work).  A reduce item sums slots [slot0, slot0 + n_slots) of block
+// blk into the Gram of `subset`.
+struct GramItem {
+    int32_t blk, subset;
+    int64_t g0, g1, slot;  // slot < 0: add the block into the Gram of `subset` directly
+};
+struct RedItem {
+    int32_t blk, subset;
+    int64_t slot0;
+    int32_t n_slots, pad_;
+};
+
+// Rows [a, b) of a chunk as a chunk of their own (span offsets stay absolute).
+__host__ __device__ inline DevChunk seg_rows(DevChunk c, int64_t a, int64_t b) {
+    c.span_off += a;
+    if (c.span_cell)
+        c.span_cell += a;
+    else
+        c.first_cell += (uint32_t)a;
+    c.n_spans = b - a;
+    return c;
+}
+
 // ---- kernel launchers (each defined next to its kernel) ----------------------
+void launch_gene_stats_ranges(const DevChunk &c, const SegDesc *segs, const RowRange *ranges, int64_t n_ranges,
+                              double *out, int n_vars, hipStream_t s);
+void densify_set_attrs();
+void launch_densify_ranges(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b, int d_pad,
+                           const SegDesc *segs, const RowRange *ranges, int64_t n_ranges, double *colsum,
+                           double *gram, hipStream_t s);
+void launch_project_ranges(const DevChunk &c, const DevScale &sc, const double *V, int k, const SegDesc *segs,
+                           const RowRange *ranges, int64_t n_ranges, float *out, hipStream_t s);
+void launch_gram_items(int fmt, const void *tile_a, const void *tile_b, int d_pad, const GramItem *items,
+                       int64_t n_items, int grid, double *partial, double *gram, hipStream_t s);
+int gram_items_occupancy(int fmt);
+void launch_gram_reduce_items(const double *partial, const RedItem *items, int64_t n_items, int d_pad,
+                              double *gram, hipStream_t s);
+void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
+                      int n_sub, double *cov, double *total, hipStream_t s);
+void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
+                             double *loadings, hipStream_t s);
+void launch_shift_fill_batch(const double *colsum, int ld_c, const double *V, int d, int k,
+                             const int64_t *dev_cell_off, int n_sub, int64_t max_rows, float *x, float *shift_f,
+                             hipStream_t s);
 void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s);
 void launch_qc(const DevChunk &c, const uint8_t *flag, double *row_total, double *row_ngenes, double *row_flag,
                uint32_t *gene_ncells, double *gene_total, hipStream_t s);
@@ -112,6 +168,18 @@ void launch_shift_fill(const double *colsum, const double *V, int d, int k, doub
 struct KnnOrder {
     const uint32_t *perm_c, *perm_q, *keys_c, *keys_q;
 };
+// A batch of independent subsets in one screen: subset y's operand rows are
+// row0[y] .. row0[y] + nrow[y] - 1 (then padding up to a multiple of 512); blk_seg
+// gives the subset of every kSegBlock-row block of operand rows, and order lists the
+// 512-row query blocks in the order their workgroups are dispatched.  All null for a
+// single set of rows.
+constexpr int kSegBlock = 256;
+struct KnnSeg {
+    const int64_t *row0;
+    const int64_t *nrow;
+    const int32_t *blk_seg;
+    const int32_t *order;
+};
 int64_t knn_rows_padded(int64_t n);
 size_t knn_work_bytes(int64_t nc, int64_t nq, int p);
 KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq, int64_t nq, int p, int kpad,
@@ -121,7 +189,15 @@ bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int
                        int self, const KnnOrder &o, float *cand_d, uint32_t *cand_i, hipStream_t s);
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o,
                        int self, const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
-                       hipStream_t s);
+                       hipStream_t s, const int32_t *blk_seg = nullptr, const int64_t *cell_off = nullptr);
+// Batch forms (subset y = rows cell_off[y] .. cell_off[y+1]-1 of x, operand rows
+// row0[y] .. of n_pad in all; max_rows / max_pad are the largest n_y and padded n_y).
+size_t knn_batch_work_bytes(int64_t n, int64_t n_pad, int n_sub, int p);
+KnnOrder launch_knn_prep_batch(int prec, const float *x, int64_t n, const int64_t *cell_off, int n_sub,
+                               int64_t max_rows, const int64_t *row0, int64_t n_pad, int64_t max_pad, int p,
+                               int kpad, void *work, void *xa, void *xb, hipStream_t s);
+bool launch_knn_screen_batch(int prec, const void *xa, const void *xb, int64_t n_pad, int kpad, int L,
+                             const KnnSeg &sg, float *cand_d, uint32_t *cand_i, hipStream_t s);
 // Large-k kNN: f32 row norms; per query row of an FP32 dot-product block (row
 // stride ld, a multiple of 4; cn readable up to ld entries), the indices of the k2
 // smallest screened keys; exact rank of those k2 and the k

@@ -45,13 +45,26 @@ __device__ inline double cell_total(const DevChunk &c, int64_t lo, int64_t hi, i
 // ---------------------------------------------------------------- gene_stats
 // kernels.rs:141-160.  Each workgroup owns a contiguous block of cells and
 // accumulates a window of vars in LDS (FP64), then adds it to `out` once.
-__global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int n_vars) {
+// With `ranges`, workgroup b takes row range ranges[b] of segment segs[ranges[b].seg]
+// and accumulates into that segment's subset's block of `out` ([subset][n_vars][5]).
+__global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int n_vars, const SegDesc *segs,
+                                                    const RowRange *ranges) {
     __shared__ double acc[5 * kStatsWindow];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int64_t base = c.base;
-    const int64_t per = (c.n_spans + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blockIdx.x * per;
-    const int64_t r1 = r0 + per < c.n_spans ? r0 + per : c.n_spans;
+    int64_t r0, r1;
+    if (ranges) {
+        const RowRange w = ranges[blockIdx.x];
+        const SegDesc sd = segs[w.seg];
+        c = seg_rows(c, sd.cell0 + w.r0, sd.cell0 + w.r1);
+        out += (int64_t)sd.subset * n_vars * 5;
+        r0 = 0;
+        r1 = c.n_spans;
+    } else {
+        const int64_t per = (c.n_spans + gridDim.x - 1) / gridDim.x;
+        r0 = (int64_t)blockIdx.x * per;
+        r1 = r0 + per < c.n_spans ? r0 + per : c.n_spans;
+    }
     for (int w0 = 0; w0 < n_vars; w0 += kStatsWindow) {
         for (int i = threadIdx.x; i < 5 * kStatsWindow; i += blockDim.x) acc[i] = 0.0;
         __syncthreads();
@@ -87,7 +100,13 @@ __global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int
 void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s) {
     if (c.n_spans <= 0 || n_vars <= 0) return;
     int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 16), 1024);
-    hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(256), 0, s, c, out, n_vars);
+    hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(256), 0, s, c, out, n_vars, (const SegDesc *)nullptr,
+                       (const RowRange *)nullptr);
+}
+void launch_gene_stats_ranges(const DevChunk &c, const SegDesc *segs, const RowRange *ranges, int64_t n_ranges,
+                              double *out, int n_vars, hipStream_t s) {
+    if (n_ranges <= 0 || n_vars <= 0) return;
+    hipLaunchKernelGGL(k_gene_stats, dim3((unsigned)n_ranges), dim3(256), 0, s, c, out, n_vars, segs, ranges);
 }
 
 // ---------------------------------------------------------------- qc
@@ -156,9 +175,14 @@ template <> struct TileElem<TILE_F32> { typedef float T; static constexpr int DW
 template <> struct TileElem<TILE_BF16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
 template <> struct TileElem<TILE_F16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
 
+// With `ranges`, workgroup b densifies row groups [r0/8, r1/8) of segment
+// segs[ranges[b].seg] into tile groups grp0 + g, with that subset's scale parameters
+// (mean/sd + subset d), column sums (colsum + subset d_pad) and Gram diagonal
+// (gram + subset d_pad^2).
 template <int FMT>
 __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *tile_a, void *tile_b,
-                                                 int64_t n_groups, int d_pad, double *colsum, double *gram) {
+                                                 int64_t n_groups, int d_pad, double *colsum, double *gram,
+                                                 const SegDesc *segs, const RowRange *ranges) {
     typedef typename TileElem<FMT>::T T;
     constexpr int DW = TileElem<FMT>::DW;
     constexpr bool X2 = FMT == TILE_BF16X2 || FMT == TILE_F16X2;  // hi/lo pairs
@@ -171,13 +195,27 @@ __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int nwave = blockDim.x / kWave;
     const int64_t base = c.base;
+    int64_t g_begin = blockIdx.x, g_end = n_groups, g_step = gridDim.x, grp0 = 0;
+    if (ranges) {
+        const RowRange w = ranges[blockIdx.x];
+        const SegDesc sd = segs[w.seg];
+        c = seg_rows(c, sd.cell0, sd.cell0 + sd.n);
+        g_begin = w.r0 / kRowGroup;
+        g_end = w.r1 / kRowGroup;
+        g_step = 1;
+        grp0 = sd.grp0;
+        colsum += (int64_t)sd.subset * d_pad;
+        gram += (int64_t)sd.subset * d_pad * d_pad;
+        sc.mean += (int64_t)sd.subset * sc.d;
+        sc.sd += (int64_t)sd.subset * sc.d;
+    }
     for (int h0 = 0; h0 < d_pad; h0 += DW) {
         const int dw = d_pad - h0 < DW ? d_pad - h0 : DW;
         for (int i = threadIdx.x; i < DW; i += blockDim.x) {
             cs[i] = 0.0;
             if constexpr (LL) cl[i] = 0.0;
         }
-        for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+        for (int64_t g = g_begin; g < g_end; g += g_step) {
             // zero the stage (vectorised)
             uint4 *st4 = reinterpret_cast<uint4 *>(stage);
             const int n4 = (int)((int64_t)NST * DW * 8 * sizeof(T) / 16);
@@ -222,7 +260,7 @@ __global__ __launch_bounds__(256) void k_densify(DevChunk c, DevScale sc, void *
             __syncthreads();
             // stage -> tile (contiguous run of dw*8 elements per stage)
             for (int t = 0; t < NST; ++t) {
-                T *dst = reinterpret_cast<T *>(t == 0 ? tile_a : tile_b) + ((int64_t)g * d_pad + h0) * kRowGroup;
+                T *dst = reinterpret_cast<T *>(t == 0 ? tile_a : tile_b) + ((grp0 + g) * d_pad + h0) * kRowGroup;
                 const uint4 *src = reinterpret_cast<const uint4 *>(stage + (int64_t)t * DW * kRowGroup);
                 uint4 *dst4 = reinterpret_cast<uint4 *>(dst);
                 const int m4 = (int)((int64_t)dw * kRowGroup * sizeof(T) / 16);
@@ -251,14 +289,14 @@ void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a
         g = (int)std::min<int64_t>(n_groups, 512);
         hipFuncSetAttribute((const void *)k_densify<TILE_F64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_densify<TILE_F64>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum,
-                           gram);
+                           gram, (const SegDesc *)nullptr, (const RowRange *)nullptr);
         break;
     case TILE_F32:
         lds = TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float));
         g = (int)std::min<int64_t>(n_groups, 1024);
         hipFuncSetAttribute((const void *)k_densify<TILE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_densify<TILE_F32>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum,
-                           gram);
+                           gram, (const SegDesc *)nullptr, (const RowRange *)nullptr);
         break;
     default:
         lds = TileElem<TILE_BF16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t));
@@ -268,11 +306,52 @@ void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a
             hipFuncSetAttribute((const void *)k_densify<TILE_F16X2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
             hipLaunchKernelGGL(k_densify<TILE_F16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad,
-                               colsum, gram);
+                               colsum, gram, (const SegDesc *)nullptr, (const RowRange *)nullptr);
         } else {
             hipLaunchKernelGGL(k_densify<TILE_BF16X2>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad,
-                               colsum, gram);
+                               colsum, gram, (const SegDesc *)nullptr, (const RowRange *)nullptr);
         }
+        break;
+    }
+}
+
+// The dynamic LDS limits of every k_densify form (set once before any graph capture).
+void densify_set_attrs() {
+    hipFuncSetAttribute((const void *)k_densify<TILE_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)(TileElem<TILE_F64>::DW * (sizeof(double) + 8 * sizeof(double))));
+    hipFuncSetAttribute((const void *)k_densify<TILE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)(TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float))));
+    const int l16 = (int)(TileElem<TILE_BF16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t)));
+    hipFuncSetAttribute((const void *)k_densify<TILE_BF16X2>, hipFuncAttributeMaxDynamicSharedMemorySize, l16);
+    hipFuncSetAttribute((const void *)k_densify<TILE_F16X2>, hipFuncAttributeMaxDynamicSharedMemorySize, l16);
+}
+
+void launch_densify_ranges(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b, int d_pad,
+                           const SegDesc *segs, const RowRange *ranges, int64_t n_ranges, double *colsum,
+                           double *gram, hipStream_t s) {
+    if (n_ranges <= 0) return;
+    const dim3 g((unsigned)n_ranges), b(256);
+    size_t lds;
+    switch (fmt) {
+    case TILE_F64:
+        lds = TileElem<TILE_F64>::DW * (sizeof(double) + 8 * sizeof(double));
+        hipLaunchKernelGGL(k_densify<TILE_F64>, g, b, lds, s, c, sc, tile_a, tile_b, (int64_t)0, d_pad, colsum, gram,
+                           segs, ranges);
+        break;
+    case TILE_F32:
+        lds = TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float));
+        hipLaunchKernelGGL(k_densify<TILE_F32>, g, b, lds, s, c, sc, tile_a, tile_b, (int64_t)0, d_pad, colsum, gram,
+                           segs, ranges);
+        break;
+    case TILE_F16X2:
+        lds = TileElem<TILE_F16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t));
+        hipLaunchKernelGGL(k_densify<TILE_F16X2>, g, b, lds, s, c, sc, tile_a, tile_b, (int64_t)0, d_pad, colsum, gram,
+                           segs, ranges);
+        break;
+    default:
+        lds = TileElem<TILE_BF16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t));
+        hipLaunchKernelGGL(k_densify<TILE_BF16X2>, g, b, lds, s, c, sc, tile_a, tile_b, (int64_t)0, d_pad, colsum,
+                           gram, segs, ranges);
         break;
     }
 }
@@ -283,12 +362,24 @@ void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a
 // reference does), then out[row, j] += (float)acc[j] in FP32.  Lane j owns column
 // j (and j+64, ...); the (h, s) of 64 entries at a time are computed one per lane
 // and broadcast with shuffles.
+// With `ranges`, workgroup b projects row range ranges[b] of its segment with that
+// subset's loadings (V + subset d k) and scale parameters.
 template <int NC>
 __global__ __launch_bounds__(256) void k_project(DevChunk c, DevScale sc, const double *V, int k, float *out,
-                                                  int64_t row0) {
+                                                  int64_t row0, const SegDesc *segs, const RowRange *ranges) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    int64_t n_waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    if (ranges) {
+        const RowRange w = ranges[blockIdx.x];
+        const SegDesc sd = segs[w.seg];
+        c = seg_rows(c, sd.cell0 + w.r0, sd.cell0 + w.r1);
+        V += (int64_t)sd.subset * sc.d * k;
+        sc.mean += (int64_t)sd.subset * sc.d;
+        sc.sd += (int64_t)sd.subset * sc.d;
+        wave_id = threadIdx.x / kWave;
+        n_waves = blockDim.x / kWave;
+    }
     const int64_t base = c.base;
     for (int64_t span = wave_id; span < c.n_spans; span += n_waves) {
         const int64_t row = chunk_row(c, span);
@@ -356,12 +447,25 @@ void launch_project(const DevChunk &c, const DevScale &sc, const double *V, int 
                     hipStream_t s) {
     if (c.n_spans <= 0 || k <= 0) return;
     int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 4), 8192);
+    const SegDesc *ns = nullptr;
+    const RowRange *nr = nullptr;
     if (k <= 64)
-        hipLaunchKernelGGL(k_project<1>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+        hipLaunchKernelGGL(k_project<1>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0, ns, nr);
     else if (k <= 128)
-        hipLaunchKernelGGL(k_project<2>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+        hipLaunchKernelGGL(k_project<2>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0, ns, nr);
     else
-        hipLaunchKernelGGL(k_project<4>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0);
+        hipLaunchKernelGGL(k_project<4>, dim3(g), dim3(256), 0, s, c, sc, V, k, out, row0, ns, nr);
+}
+void launch_project_ranges(const DevChunk &c, const DevScale &sc, const double *V, int k, const SegDesc *segs,
+                           const RowRange *ranges, int64_t n_ranges, float *out, hipStream_t s) {
+    if (n_ranges <= 0 || k <= 0) return;
+    const dim3 g((unsigned)n_ranges), b(256);
+    if (k <= 64)
+        hipLaunchKernelGGL(k_project<1>, g, b, 0, s, c, sc, V, k, out, (int64_t)0, segs, ranges);
+    else if (k <= 128)
+        hipLaunchKernelGGL(k_project<2>, g, b, 0, s, c, sc, V, k, out, (int64_t)0, segs, ranges);
+    else
+        hipLaunchKernelGGL(k_project<4>, g, b, 0, s, c, sc, V, k, out, (int64_t)0, segs, ranges);
 }
 
 // ---------------------------------------------------------------- materialize

@@ -52,14 +52,31 @@ __device__ inline void gram_work(int n_split, int nb, int &blk, int &split, int 
     tri_block(blk, nb, bi, bj);
 }
 
-// Output of a 32x32 MFMA tile: register r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l%32].
-__device__ inline void store_partial32(double *dst, const double (&v)[16], int m0, int n0, int lane) {
+// Output of a 32x32 MFMA tile: register r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l%32],
+// stored (or, with add, added) at dst[m * ld + n].
+__device__ inline void store_partial32(double *dst, const double (&v)[16], int m0, int n0, int lane, int ld = kGramBlock,
+                                       bool add = false) {
     const int col = lane & 31, half = lane >> 5;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = m0 + 8 * (r / 4) + 4 * half + (r % 4);
-        dst[m * kGramBlock + n0 + col] = v[r];
+        double *o = dst + (int64_t)m * ld + n0 + col;
+        *o = add ? *o + v[r] : v[r];
     }
+}
+
+// Destination of a work item: its partial slot, or (slot < 0: the item is the only
+// one for its block in this launch) the block of its subset's Gram, added in place.
+__device__ inline double *item_dst(const GramItem &w, int bi, int bj, int d_pad, double *partial, double *gram,
+                                   int &ld, bool &add) {
+    if (w.slot >= 0) {
+        ld = kGramBlock;
+        add = false;
+        return partial + w.slot * kGramBlock * kGramBlock;
+    }
+    ld = d_pad;
+    add = true;
+    return gram + (int64_t)w.subset * d_pad * d_pad + (int64_t)bi * kGramBlock * d_pad + bj * kGramBlock;
 }
 
 // ---------------------------------------------------------------- split 16-bit (BF16 or FP16)
@@ -69,19 +86,14 @@ __device__ inline void store_partial32(double *dst, const double (&v)[16], int m
 // to nothing measurable, but on the diagonal it is a positive bias, so k_densify adds
 // the exact diagonal Σ l² into the Gram instead.  FP32 accumulators are added into
 // FP64 every FLUSH K-steps (16 rows each).
+// Block (bi, bj) over the K-steps [p0, p1) of 16 rows into dst (128 x 128 FP64).
 template <int TY, int FLUSH>
-__global__ __launch_bounds__(256) void k_gram_x16(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
-                                                     int64_t n_groups, int d_pad, int n_split, double *partial) {
-    const int nb = d_pad / kGramBlock;
-    int blk, split, bi, bj;
-    gram_work(n_split, nb, blk, split, bi, bj);
+__device__ inline void gram_x16_block(const uint4 *__restrict__ th, const uint4 *__restrict__ tl, int d_pad, int bi,
+                                      int bj, int64_t p0, int64_t p1, double *dst, int ld = kGramBlock,
+                                      bool add = false) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
     const int col = lane & 31, half = lane >> 5;
     const int ga = bi * kGramBlock + wm * 64 + col, gb = bj * kGramBlock + wn * 64 + col;
-    const int64_t n_pairs = n_groups / 2;
-    const int64_t per = (n_pairs + n_split - 1) / n_split;
-    const int64_t p0 = (int64_t)split * per;
-    const int64_t p1 = p0 + per < n_pairs ? p0 + per : n_pairs;
     f32x16 acc[2][2];
     double dacc[2][2][16];
 #pragma unroll
@@ -149,32 +161,58 @@ __global__ __launch_bounds__(256) void k_gram_x16(const uint4 *__restrict__ th, 
             if (p + 2 >= p1) break;
         }
     }
-    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
-            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane);
+            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane, ld, add);
         }
+}
+
+template <int TY, int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_x16(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
+                                                     int64_t n_groups, int d_pad, int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int64_t n_pairs = n_groups / 2;
+    const int64_t per = (n_pairs + n_split - 1) / n_split;
+    const int64_t p0 = (int64_t)split * per;
+    const int64_t p1 = p0 + per < n_pairs ? p0 + per : n_pairs;
+    gram_x16_block<TY, FLUSH>(th, tl, d_pad, bi, bj, p0, p1,
+                              partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock);
+}
+
+// Persistent form over a work list (many independent cell subsets in one tile):
+// workgroup w takes items w, w + gridDim.x, ...; an item is one Gram block over one
+// row range of one subset's rows, written to its own partial slot.
+template <int TY, int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_x16_items(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
+                                                           int d_pad, const GramItem *items, int64_t n_items,
+                                                           double *partial, double *gram) {
+    const int nb = d_pad / kGramBlock;
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const GramItem w = items[it];
+        if (w.blk < 0) continue;
+        int bi, bj, ld;
+        bool add;
+        tri_block(w.blk, nb, bi, bj);
+        double *dst = item_dst(w, bi, bj, d_pad, partial, gram, ld, add);
+        gram_x16_block<TY, FLUSH>(th, tl, d_pad, bi, bj, w.g0 / 2, w.g1 / 2, dst, ld, add);
+    }
 }
 
 // ---------------------------------------------------------------- FP32
 // v_mfma_f32_32x32x2_f32; lane half h takes rows 4h..4h+3 of each row group, so the
 // four MFMAs of a group cover its eight rows.
 template <int FLUSH>
-__global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, int64_t n_groups, int d_pad,
-                                                  int n_split, double *partial) {
-    const int nb = d_pad / kGramBlock;
-    int blk, split, bi, bj;
-    gram_work(n_split, nb, blk, split, bi, bj);
+__device__ inline void gram_f32_block(const float4 *__restrict__ t, int d_pad, int bi, int bj, int64_t g0, int64_t g1,
+                                      double *dst, int ld = kGramBlock, bool add = false) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
     const int col = lane & 31, half = lane >> 5;
     const int ga = bi * kGramBlock + wm * 64 + col, gb = bj * kGramBlock + wn * 64 + col;
-    const int64_t per = (n_groups + n_split - 1) / n_split;
-    const int64_t g0 = (int64_t)split * per;
-    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
     f32x16 acc[2][2];
     double dacc[2][2][16];
 #pragma unroll
@@ -235,32 +273,52 @@ __global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, 
             if (g + 2 >= g1) break;
         }
     }
-    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
-            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane);
+            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane, ld, add);
         }
+}
+
+template <int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_f32(const float4 *__restrict__ t, int64_t n_groups, int d_pad,
+                                                  int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int64_t per = (n_groups + n_split - 1) / n_split;
+    const int64_t g0 = (int64_t)split * per;
+    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
+    gram_f32_block<FLUSH>(t, d_pad, bi, bj, g0, g1, partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock);
+}
+
+template <int FLUSH>
+__global__ __launch_bounds__(256) void k_gram_f32_items(const float4 *__restrict__ t, int d_pad, const GramItem *items,
+                                                        int64_t n_items, double *partial, double *gram) {
+    const int nb = d_pad / kGramBlock;
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const GramItem w = items[it];
+        if (w.blk < 0) continue;
+        int bi, bj, ld;
+        bool add;
+        tri_block(w.blk, nb, bi, bj);
+        double *dst = item_dst(w, bi, bj, d_pad, partial, gram, ld, add);
+        gram_f32_block<FLUSH>(t, d_pad, bi, bj, w.g0, w.g1, dst, ld, add);
+    }
 }
 
 // ---------------------------------------------------------------- FP64
 // v_mfma_f64_16x16x4_f64; lane quarter q = l/16 takes rows 2q, 2q+1 of each row
 // group.  Output (FP64 layout, rows interleaved across the lane quarters): register
 // r of lane l is C[4*r + l/16][l%16].
-__global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t, int64_t n_groups, int d_pad,
-                                                  int n_split, double *partial) {
-    const int nb = d_pad / kGramBlock;
-    int blk, split, bi, bj;
-    gram_work(n_split, nb, blk, split, bi, bj);
+__device__ inline void gram_f64_block(const double2 *__restrict__ t, int d_pad, int bi, int bj, int64_t g0, int64_t g1,
+                                      double *dst, int ld = kGramBlock, bool add = false) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
     const int c16 = lane & 15, q = lane >> 4;
     const int ga = bi * kGramBlock + wm * 64 + c16, gb = bj * kGramBlock + wn * 64 + c16;
-    const int64_t per = (n_groups + n_split - 1) / n_split;
-    const int64_t g0 = (int64_t)split * per;
-    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
     f64x4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -282,7 +340,6 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t,
                 acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
             }
     }
-    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -291,8 +348,34 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t,
             for (int r = 0; r < 4; ++r) {
                 const int m = wm * 64 + i * 16 + 4 * r + q;
                 const int n = wn * 64 + j * 16 + c16;
-                dst[m * kGramBlock + n] = acc[i][j][r];
+                double *o = dst + (int64_t)m * ld + n;
+                *o = add ? *o + acc[i][j][r] : acc[i][j][r];
             }
+}
+
+__global__ __launch_bounds__(256) void k_gram_f64(const double2 *__restrict__ t, int64_t n_groups, int d_pad,
+                                                  int n_split, double *partial) {
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int64_t per = (n_groups + n_split - 1) / n_split;
+    const int64_t g0 = (int64_t)split * per;
+    const int64_t g1 = g0 + per < n_groups ? g0 + per : n_groups;
+    gram_f64_block(t, d_pad, bi, bj, g0, g1, partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock);
+}
+
+__global__ __launch_bounds__(256) void k_gram_f64_items(const double2 *__restrict__ t, int d_pad, const GramItem *items,
+                                                        int64_t n_items, double *partial, double *gram) {
+    const int nb = d_pad / kGramBlock;
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const GramItem w = items[it];
+        if (w.blk < 0) continue;
+        int bi, bj, ld;
+        bool add;
+        tri_block(w.blk, nb, bi, bj);
+        double *dst = item_dst(w, bi, bj, d_pad, partial, gram, ld, add);
+        gram_f64_block(t, d_pad, bi, bj, w.g0, w.g1, dst, ld, add);
+    }
 }
 
 void launch_gram(int fmt, const void *tile_a, const void *tile_b, int64_t rows_pad, int d_pad, int n_split,
@@ -336,6 +419,68 @@ __global__ void k_gram_reduce(const double *partial, int n_split, int nb, int d_
     }
 }
 
+// Work-list form: item r sums slots slot0 .. slot0 + n_slots - 1 (in order) of one
+// block and adds them into that block of its subset's Gram (gram + subset d_pad^2).
+// No two items of one launch name the same (subset, block), so the adds do not race.
+__global__ void k_gram_reduce_items(const double *partial, const RedItem *items, int64_t n_items, int nb, int d_pad,
+                                    double *gram) {
+    const int64_t total = n_items * kGramBlock * kGramBlock;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const RedItem r = items[e / (kGramBlock * kGramBlock)];
+        const int rem = (int)(e % (kGramBlock * kGramBlock));
+        int bi, bj;
+        tri_block(r.blk, nb, bi, bj);
+        const double *src = partial + r.slot0 * kGramBlock * kGramBlock + rem;
+        double s = 0.0;
+        for (int sp = 0; sp < r.n_slots; ++sp) s += src[(int64_t)sp * kGramBlock * kGramBlock];
+        const int m = rem / kGramBlock, n = rem % kGramBlock;
+        gram[(int64_t)r.subset * d_pad * d_pad + (int64_t)(bi * kGramBlock + m) * d_pad + bj * kGramBlock + n] += s;
+    }
+}
+
+void launch_gram_items(int fmt, const void *tile_a, const void *tile_b, int d_pad, const GramItem *items,
+                       int64_t n_items, int grid, double *partial, double *gram, hipStream_t s) {
+    if (n_items <= 0) return;
+    const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, n_items))), block(256);
+    switch (fmt) {
+    case TILE_F64:
+        hipLaunchKernelGGL(k_gram_f64_items, g, block, 0, s, (const double2 *)tile_a, d_pad, items, n_items, partial,
+                           gram);
+        break;
+    case TILE_F32:
+        hipLaunchKernelGGL(k_gram_f32_items<32>, g, block, 0, s, (const float4 *)tile_a, d_pad, items, n_items,
+                           partial, gram);
+        break;
+    default:
+        if (fmt == TILE_F16X2)
+            hipLaunchKernelGGL((k_gram_x16_items<1, 16>), g, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
+                               d_pad, items, n_items, partial, gram);
+        else
+            hipLaunchKernelGGL((k_gram_x16_items<0, 16>), g, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
+                               d_pad, items, n_items, partial, gram);
+        break;
+    }
+}
+
+int gram_items_occupancy(int fmt) {
+    int n = 0;
+    const void *f = fmt == TILE_F64 ? (const void *)k_gram_f64_items
+                    : fmt == TILE_F32 ? (const void *)k_gram_f32_items<32>
+                    : fmt == TILE_F16X2 ? (const void *)k_gram_x16_items<1, 16>
+                                        : (const void *)k_gram_x16_items<0, 16>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+
+void launch_gram_reduce_items(const double *partial, const RedItem *items, int64_t n_items, int d_pad, double *gram,
+                              hipStream_t s) {
+    if (n_items <= 0) return;
+    const int64_t total = n_items * kGramBlock * kGramBlock;
+    const int g = (int)std::min<int64_t>(ceil_div(total, 256), 16384);
+    hipLaunchKernelGGL(k_gram_reduce_items, dim3(g), dim3(256), 0, s, partial, items, n_items, d_pad / kGramBlock,
+                       d_pad, gram);
+}
+
 void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *gram, hipStream_t s) {
     const int nb = d_pad / kGramBlock;
     const int64_t total = (int64_t)nb * (nb + 1) / 2 * kGramBlock * kGramBlock;
@@ -346,7 +491,15 @@ void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *g
 // ---------------------------------------------------------------- covariance
 // cov[i, j] = (G[a, b] - c_a c_b / n) / (n - 1) with a = min, b = max (kernels.rs:318-322),
 // in the reference's operation order; written full and column-major (symmetric).
-__global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov) {
+// blockIdx.y = subset of a batch (gram + y ld_g^2, colsum + y ld_g, cov + y d^2), with
+// n = cell_off[y+1] - cell_off[y] when cell_off is given.
+__global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov,
+                      const int64_t *cell_off) {
+    const int y = blockIdx.y;
+    gram += (int64_t)y * ld_g * ld_g;
+    colsum += (int64_t)y * ld_g;
+    cov += (int64_t)y * d * d;
+    if (cell_off) n = (double)(cell_off[y + 1] - cell_off[y]);
     const int64_t total = (int64_t)d * d;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(e / d), i = (int)(e % d);
@@ -361,6 +514,8 @@ __global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g,
 // gathered into LDS by the whole block, then summed sequentially by one thread.
 __global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double *total) {
     __shared__ double diag[4096];
+    cov += (int64_t)blockIdx.x * d * d;  // one block per subset of a batch
+    total += blockIdx.x;
     double s = 0.0;
     for (int i0 = 0; i0 < d; i0 += 4096) {
         const int m = d - i0 < 4096 ? d - i0 : 4096;
@@ -376,8 +531,15 @@ __global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double 
 void launch_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov, double *total,
                 hipStream_t s) {
     const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 8192);
-    hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov);
+    hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov, (const int64_t *)nullptr);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, cov, d, total);
+}
+
+void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
+                      int n_sub, double *cov, double *total, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 2048);
+    hipLaunchKernelGGL(k_cov, dim3(g, n_sub), dim3(256), 0, s, gram, colsum, d, ld_g, 0.0, cov, dev_cell_off);
+    hipLaunchKernelGGL(k_trace, dim3(n_sub), dim3(256), 0, s, cov, d, total);
 }
 
 // ---------------------------------------------------------------- eigen selection
@@ -388,6 +550,11 @@ __global__ __launch_bounds__(256) void k_eig_select(const double *evecs, const d
                                                     double *variance, double *loadings) {
     __shared__ double sbest[256];
     __shared__ int sidx[256];
+    const int y = blockIdx.y;  // subset of a batch
+    evecs += (int64_t)y * d * d;
+    evals += (int64_t)y * d;
+    variance += (int64_t)y * k;
+    loadings += (int64_t)y * d * k;
     const int c = blockIdx.x;
     const int src = d - 1 - c;
     const double *u = evecs + (int64_t)src * d;
@@ -430,14 +597,24 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
     if (k <= 0) return;
     hipLaunchKernelGGL(k_eig_select, dim3(k), dim3(256), 0, s, evecs, evals, d, k, variance, loadings);
 }
+void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
+                             double *loadings, hipStream_t s) {
+    if (k <= 0 || n_sub <= 0) return;
+    hipLaunchKernelGGL(k_eig_select, dim3(k, n_sub), dim3(256), 0, s, evecs, evals, d, k, variance, loadings);
+}
 
 // ---------------------------------------------------------------- shift prefill
 // shift = (colsum / n) @ V (tl.py:76), one block per component (a matrix-vector
 // product, so the summation order is the block reduction's), then
 // X[:] = -(f32)shift (tl.py:77-78).
 __global__ __launch_bounds__(256) void k_shift(const double *colsum, const double *V, int d, int k, double n,
-                                               float *shift_f) {
+                                               float *shift_f, int ld_c, const int64_t *cell_off) {
     __shared__ double part[256];
+    const int y = blockIdx.y;  // subset of a batch
+    colsum += (int64_t)y * ld_c;
+    V += (int64_t)y * d * k;
+    shift_f += (int64_t)y * k;
+    if (cell_off) n = (double)(cell_off[y + 1] - cell_off[y]);
     const int c = blockIdx.x;
     double s = 0.0;
     for (int a = threadIdx.x; a < d; a += blockDim.x) {
@@ -453,7 +630,13 @@ __global__ __launch_bounds__(256) void k_shift(const double *colsum, const doubl
     }
     if (threadIdx.x == 0) shift_f[c] = -(float)part[0];
 }
-__global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x) {
+__global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x, const int64_t *cell_off) {
+    if (cell_off) {  // blockIdx.y = subset: its rows cell_off[y] .. cell_off[y+1]-1
+        const int y = blockIdx.y;
+        shift_f += (int64_t)y * k;
+        x += cell_off[y] * k;
+        n_rows = cell_off[y + 1] - cell_off[y];
+    }
     const int64_t total = n_rows * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
         x[e] = shift_f[e % k];
@@ -461,9 +644,17 @@ __global__ void k_fill_rows(const float *shift_f, int k, int64_t n_rows, float *
 
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows, float *x,
                        float *shift_f, hipStream_t s) {
-    hipLaunchKernelGGL(k_shift, dim3(k), dim3(256), 0, s, colsum, V, d, k, n, shift_f);
+    hipLaunchKernelGGL(k_shift, dim3(k), dim3(256), 0, s, colsum, V, d, k, n, shift_f, 0, (const int64_t *)nullptr);
     const int g = (int)std::min<int64_t>(ceil_div(n_rows * k, 256), 8192);
-    hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x);
+    hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x, (const int64_t *)nullptr);
+}
+
+void launch_shift_fill_batch(const double *colsum, int ld_c, const double *V, int d, int k,
+                             const int64_t *dev_cell_off, int n_sub, int64_t max_rows, float *x, float *shift_f,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_shift, dim3(k, n_sub), dim3(256), 0, s, colsum, V, d, k, 0.0, shift_f, ld_c, dev_cell_off);
+    const int g = (int)std::min<int64_t>(ceil_div(max_rows * k, 256), 1024);
+    hipLaunchKernelGGL(k_fill_rows, dim3(g, n_sub), dim3(256), 0, s, shift_f, k, (int64_t)0, x, dev_cell_off);
 }
 
 }  // namespace crest

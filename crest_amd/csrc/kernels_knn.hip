@@ -138,9 +138,20 @@ size_t knn_work_bytes(int64_t nc, int64_t nq, int p) {
 // Block b sums rows [b*rows_per, (b+1)*rows_per) of x (mode 0) or of (x - mean)^2
 // (mode 1): thread t takes column t % p of every (256/p)-th row, and the row groups
 // are combined in order through LDS.
+// With cell_off (a batch of subsets), blockIdx.y = subset y: rows cell_off[y] ..
+// cell_off[y+1]-1 split into gridDim.x parts, its means at mean + y p, its parts at
+// part + y gridDim.x p.
 __global__ __launch_bounds__(256) void k_knn_colsum(const float *x, const float *mean, int mode, int64_t n, int p,
-                                                    int64_t rows_per, double *part) {
+                                                    int64_t rows_per, double *part, const int64_t *cell_off) {
     __shared__ double red[256];
+    if (cell_off) {
+        const int y = blockIdx.y;
+        x += cell_off[y] * p;
+        n = cell_off[y + 1] - cell_off[y];
+        rows_per = (n + gridDim.x - 1) / gridDim.x;
+        part += (int64_t)y * gridDim.x * p;
+        mean += (int64_t)y * p;
+    }
     const int g = 256 / p, t = threadIdx.x, f = t % p, j = t / p;
     const int64_t r0 = blockIdx.x * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
     double acc = 0.0;
@@ -162,9 +173,19 @@ __global__ __launch_bounds__(256) void k_knn_colsum(const float *x, const float 
 
 // mode 0: mean[f] = Σ/n and the running maximum norm is reset; mode 1: sd[f] and the
 // key columns (largest variance first, ties to the lower column).
+// With cell_off, block y finalises subset y (its own mean, sd, key columns, maximum norm).
 __global__ void k_knn_colfinal(const double *part, int nparts, int64_t n, int p, int mode, float *mean, float *sd,
-                               int32_t *cols, uint32_t *maxbits) {
+                               int32_t *cols, uint32_t *maxbits, const int64_t *cell_off) {
     __shared__ float var[256];
+    if (cell_off) {
+        const int y = blockIdx.x;
+        part += (int64_t)y * nparts * p;
+        n = cell_off[y + 1] - cell_off[y];
+        mean += (int64_t)y * p;
+        sd += (int64_t)y * p;
+        cols += (int64_t)y * kKeyCols;
+        maxbits += y;
+    }
     for (int f = threadIdx.x; f < p; f += blockDim.x) {
         double s = 0.0;
         for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * p + f];
@@ -193,11 +214,7 @@ __global__ void k_knn_colfinal(const double *part, int nparts, int64_t n, int p,
     }
 }
 
-__global__ __launch_bounds__(256) void k_knn_keys(const float *x, int64_t n, int p, const float *mean,
-                                                  const float *sd, const int32_t *cols, uint32_t *keys,
-                                                  uint32_t *vals) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline uint32_t morton_key(const float *row, const float *mean, const float *sd, const int32_t *cols) {
     int m = 0;
     uint32_t q[kKeyCols];
     for (int c = 0; c < kKeyCols; ++c)
@@ -206,7 +223,7 @@ __global__ __launch_bounds__(256) void k_knn_keys(const float *x, int64_t n, int
     for (int c = 0; c < m; ++c) {
         const int f = cols[c];
         const float s = sd[f];
-        const float z = s > 0.0f ? (x[i * p + f] - mean[f]) / s : 0.0f;
+        const float z = s > 0.0f ? (row[f] - mean[f]) / s : 0.0f;
         float v = floorf(z * (float)(1 << (bits - 3)) + (float)(1 << (bits - 1)));
         v = v < 0.0f ? 0.0f : v;  // NaN rows end at 0 as well
         const float top = (float)((1 << bits) - 1);
@@ -215,7 +232,29 @@ __global__ __launch_bounds__(256) void k_knn_keys(const float *x, int64_t n, int
     uint32_t key = 0;
     for (int b = bits - 1; b >= 0; --b)
         for (int c = 0; c < m; ++c) key = (key << 1) | ((q[c] >> b) & 1u);
-    keys[i] = key;
+    return key;
+}
+
+__global__ __launch_bounds__(256) void k_knn_keys(const float *x, int64_t n, int p, const float *mean,
+                                                  const float *sd, const int32_t *cols, uint32_t *keys,
+                                                  uint32_t *vals) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_key(x + i * p, mean, sd, cols);
+    vals[i] = (uint32_t)i;
+}
+
+// Batch of subsets (blockIdx.y = subset y): key (y << 32) | Morton key of the row
+// under subset y's statistics, so one sort orders the subsets and, within each, the
+// rows spatially.
+__global__ __launch_bounds__(256) void k_knn_keys_seg(const float *x, const int64_t *cell_off, int p,
+                                                      const float *mean, const float *sd, const int32_t *cols,
+                                                      uint64_t *keys, uint32_t *vals) {
+    const int y = blockIdx.y;
+    const int64_t i = cell_off[y] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= cell_off[y + 1]) return;
+    keys[i] = ((uint64_t)y << 32) | morton_key(x + i * p, mean + (int64_t)y * p, sd + (int64_t)y * p,
+                                               cols + (int64_t)y * kKeyCols);
     vals[i] = (uint32_t)i;
 }
 
@@ -230,7 +269,14 @@ __device__ inline float centred_norm(const float *r, const float *mean, int p, f
 }
 
 __global__ __launch_bounds__(256) void k_knn_maxnorm(const float *x, int64_t n, int p, const float *mean,
-                                                     uint32_t *maxbits) {
+                                                     uint32_t *maxbits, const int64_t *cell_off) {
+    if (cell_off) {  // blockIdx.y = subset
+        const int y = blockIdx.y;
+        x += cell_off[y] * p;
+        n = cell_off[y + 1] - cell_off[y];
+        mean += (int64_t)y * p;
+        maxbits += y;
+    }
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     float nn = i < n ? centred_norm(x + i * p, mean, p, 1.0f) : 0.0f;
     for (int o = 32; o > 0; o >>= 1) nn = fmaxf(nn, __shfl_xor(nn, o, 64));
@@ -248,6 +294,37 @@ __device__ inline float fp16_prescale(const uint32_t *maxbits) {
     e = e >= 0 ? e / 2 : -((1 - e) / 2);  // floor(e / 2)
     e = e < -60 ? -60 : (e > 60 ? 60 : e);
     return ldexpf(1.0f, e);
+}
+
+// Operand row resolution.  Plain: row i (sorted position) holds data row perm[i],
+// rows i >= n are padding.  Batch (sg.cell_off set, blockIdx.y = subset y): subset y
+// owns operand rows row0[y] .. row0[y+1]-1, of which the first n_y hold its rows in
+// sorted order (perm[cell_off[y] + local]); perm_pad records the data row of every
+// operand row (0xFFFFFFFF for padding).
+struct PrepSeg {
+    const int64_t *cell_off;
+    const int64_t *row0;
+    uint32_t *perm_pad;
+};
+__device__ inline bool prep_row(const PrepSeg &sg, const uint32_t *perm, int64_t n, int64_t n_pad, int p,
+                                int64_t &i, bool &real, int64_t &src, const float *&mean, const uint32_t *&maxbits) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (sg.cell_off) {
+        const int y = blockIdx.y;
+        if (t >= sg.row0[y + 1] - sg.row0[y]) return false;
+        i = sg.row0[y] + t;
+        real = t < sg.cell_off[y + 1] - sg.cell_off[y];
+        src = real ? (int64_t)perm[sg.cell_off[y] + t] : 0;
+        mean += (int64_t)y * p;
+        if (maxbits) maxbits += y;
+        sg.perm_pad[i] = real ? (uint32_t)src : 0xFFFFFFFFu;
+        return true;
+    }
+    if (t >= n_pad) return false;
+    i = t;
+    real = t < n;
+    src = real ? (int64_t)perm[t] : 0;
+    return true;
 }
 
 // Element k of the split-BF16 operands of a centred row r' (values r - mean) with
@@ -274,11 +351,12 @@ __device__ inline void bf16_elem(const float *r, const float *mean, int p, float
 
 // Operand row i (sorted position) holds data row perm[i]; rows i >= n are padding.
 __global__ void k_knn_prep_bf16(const float *x, const uint32_t *perm, const float *mean, int64_t n, int64_t n_pad,
-                                int p, int kc, uint4 *xa, uint4 *xb) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n_pad) return;
-    const bool real = i < n;
-    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
+                                int p, int kc, uint4 *xa, uint4 *xb, PrepSeg sg) {
+    int64_t i, src;
+    bool real;
+    const uint32_t *nomax = nullptr;
+    if (!prep_row(sg, perm, n, n_pad, p, i, real, src, mean, nomax)) return;
+    const float *r = x + src * p;
     const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j) {
         uint16_t ea[8], eb[8];
@@ -303,12 +381,12 @@ __global__ void k_knn_prep_bf16(const float *x, const uint32_t *perm, const floa
 
 // FP16 operands of a centred, prescaled row; fp16 conversions round to nearest even.
 __global__ void k_knn_prep_f16(const float *x, const uint32_t *perm, const float *mean, const uint32_t *maxbits,
-                               int64_t n, int64_t n_pad, int p, int kc, uint4 *xa, uint4 *xb) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n_pad) return;
-    const bool real = i < n;
+                               int64_t n, int64_t n_pad, int p, int kc, uint4 *xa, uint4 *xb, PrepSeg sg) {
+    int64_t i, src;
+    bool real;
+    if (!prep_row(sg, perm, n, n_pad, p, i, real, src, mean, maxbits)) return;
     const float s = fp16_prescale(maxbits);
-    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
+    const float *r = x + src * p;
     const float nn = real ? centred_norm(r, mean, p, s) : INFINITY;
     const _Float16 nh = (_Float16)nn;
     const _Float16 nl = real ? (_Float16)(nn - (float)nh) : (_Float16)0.0f;
@@ -356,11 +434,12 @@ __device__ inline void f32_elem(const float *r, const float *mean, int p, float 
 }
 
 __global__ void k_knn_prep_f32(const float *x, const uint32_t *perm, const float *mean, int64_t n, int64_t n_pad,
-                               int p, int kc, float2 *xa, float2 *xb) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n_pad) return;
-    const bool real = i < n;
-    const float *r = x + (real ? (int64_t)perm[i] : 0) * p;
+                               int p, int kc, float2 *xa, float2 *xb, PrepSeg sg) {
+    int64_t i, src;
+    bool real;
+    const uint32_t *nomax = nullptr;
+    if (!prep_row(sg, perm, n, n_pad, p, i, real, src, mean, nomax)) return;
+    const float *r = x + src * p;
     const float nn = real ? centred_norm(r, mean, p, 1.0f) : INFINITY;
     for (int j = 0; j < kc; ++j)
         for (int h = 0; h < 2; ++h) {
@@ -397,13 +476,13 @@ static void write_operands(int prec, const float *x, const uint32_t *perm, const
     const int g = (int)ceil_div(n_pad, 256);
     if (prec == kPrecFP16)
         hipLaunchKernelGGL(k_knn_prep_f16, dim3(g), dim3(256), 0, s, x, perm, w.mean, w.maxbits, n, n_pad, p,
-                           kpad / 8, (uint4 *)xa, (uint4 *)xb);
+                           kpad / 8, (uint4 *)xa, (uint4 *)xb, PrepSeg{});
     else if (prec == kPrecBF16X3)
         hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, perm, w.mean, n, n_pad, p, kpad / 8,
-                           (uint4 *)xa, (uint4 *)xb);
+                           (uint4 *)xa, (uint4 *)xb, PrepSeg{});
     else
         hipLaunchKernelGGL(k_knn_prep_f32, dim3(g), dim3(256), 0, s, x, perm, w.mean, n, n_pad, p, kpad / 4,
-                           (float2 *)xa, (float2 *)xb);
+                           (float2 *)xa, (float2 *)xb, PrepSeg{});
 }
 
 KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq, int64_t nq, int p, int kpad,
@@ -412,20 +491,21 @@ KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq,
     const KnnWork w = knn_work(work, nc, graph ? 0 : nq, p, nullptr);
     const int64_t rows_per = ceil_div(nc, kColParts);
     const int nparts = (int)ceil_div(nc, rows_per);
-    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 0, nc, p, rows_per, w.part);
+    const int64_t *no = nullptr;
+    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 0, nc, p, rows_per, w.part, no);
     hipLaunchKernelGGL(k_knn_colfinal, dim3(1), dim3(256), 0, s, w.part, nparts, nc, p, 0, w.mean, w.sd, w.cols,
-                       w.maxbits);
-    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 1, nc, p, rows_per, w.part);
+                       w.maxbits, no);
+    hipLaunchKernelGGL(k_knn_colsum, dim3(nparts), dim3(256), 0, s, xc, w.mean, 1, nc, p, rows_per, w.part, no);
     hipLaunchKernelGGL(k_knn_colfinal, dim3(1), dim3(256), 0, s, w.part, nparts, nc, p, 1, w.mean, w.sd, w.cols,
-                       w.maxbits);
+                       w.maxbits, no);
     sort_rows(w, xc, nc, p, w.keys_c, w.perm_c, s);
     if (!graph) sort_rows(w, xq, nq, p, w.keys_q, w.perm_q, s);
     if (prec == kPrecFP16) {
         hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, s, xc, nc, p, w.mean,
-                           w.maxbits);
+                           w.maxbits, no);
         if (!graph)
             hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, xq, nq, p, w.mean,
-                               w.maxbits);
+                               w.maxbits, no);
     }
     if (graph) {
         write_operands(prec, xc, w.perm_c, w, nc, p, kpad, xa, xb, s);
@@ -438,6 +518,100 @@ KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq,
     o.perm_q = w.perm_q;
     o.keys_c = graph ? nullptr : w.keys_c;
     o.keys_q = graph ? nullptr : w.keys_q;
+    return o;
+}
+
+// ---------------------------------------------------------------- batch prep
+// The same three steps for a batch of independent subsets of the rows of x (subset y
+// = rows cell_off[y] .. cell_off[y+1]-1): statistics, key columns and FP16 prescale
+// per subset; one stable sort of 64-bit keys (subset, Morton key) orders the subsets
+// and each subset spatially; subset y's operands occupy rows row0[y] .. row0[y+1]-1
+// (its rows padded to kPadRows), and perm_pad maps every operand row back to its data
+// row (0xFFFFFFFF for padding).
+constexpr int kBatchColParts = 64;
+
+struct KnnBatchWork {
+    double *part;
+    float *mean, *sd;
+    int32_t *cols;
+    uint32_t *maxbits;
+    uint64_t *keys_in, *keys_out;
+    uint32_t *vals_in, *perm, *perm_pad;
+    void *temp;
+    size_t temp_bytes;
+};
+
+static KnnBatchWork knn_batch_work(void *w, int64_t n, int64_t n_pad, int n_sub, int p, size_t *total) {
+    uint8_t *b = (uint8_t *)w;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *r = b ? b + o : nullptr;
+        o = round_up((int64_t)(o + bytes), 256);
+        return (void *)r;
+    };
+    KnnBatchWork k;
+    k.part = (double *)take((size_t)n_sub * kBatchColParts * p * sizeof(double));
+    k.mean = (float *)take((size_t)n_sub * p * sizeof(float));
+    k.sd = (float *)take((size_t)n_sub * p * sizeof(float));
+    k.cols = (int32_t *)take((size_t)n_sub * kKeyCols * sizeof(int32_t));
+    k.maxbits = (uint32_t *)take((size_t)n_sub * sizeof(uint32_t));
+    k.keys_in = (uint64_t *)take((size_t)n * 8);
+    k.keys_out = (uint64_t *)take((size_t)n * 8);
+    k.vals_in = (uint32_t *)take((size_t)n * 4);
+    k.perm = (uint32_t *)take((size_t)n * 4);
+    k.perm_pad = (uint32_t *)take((size_t)n_pad * 4);
+    size_t tb = 0;
+    (void)rocprim::radix_sort_pairs((void *)nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                    (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0, 64);
+    k.temp_bytes = tb;
+    k.temp = take(tb);
+    if (total) *total = o;
+    return k;
+}
+
+size_t knn_batch_work_bytes(int64_t n, int64_t n_pad, int n_sub, int p) {
+    size_t t = 0;
+    (void)knn_batch_work(nullptr, n, n_pad, n_sub, p, &t);
+    return t;
+}
+
+KnnOrder launch_knn_prep_batch(int prec, const float *x, int64_t n, const int64_t *cell_off, int n_sub,
+                               int64_t max_rows, const int64_t *row0, int64_t n_pad, int64_t max_pad, int p, int kpad,
+                               void *work, void *xa, void *xb, hipStream_t s) {
+    const KnnBatchWork w = knn_batch_work(work, n, n_pad, n_sub, p, nullptr);
+    const dim3 gp(kBatchColParts, n_sub), grow((unsigned)ceil_div(max_rows, 256), n_sub);
+    hipLaunchKernelGGL(k_knn_colsum, gp, dim3(256), 0, s, x, w.mean, 0, (int64_t)0, p, (int64_t)0, w.part, cell_off);
+    hipLaunchKernelGGL(k_knn_colfinal, dim3(n_sub), dim3(256), 0, s, w.part, kBatchColParts, (int64_t)0, p, 0, w.mean,
+                       w.sd, w.cols, w.maxbits, cell_off);
+    hipLaunchKernelGGL(k_knn_colsum, gp, dim3(256), 0, s, x, w.mean, 1, (int64_t)0, p, (int64_t)0, w.part, cell_off);
+    hipLaunchKernelGGL(k_knn_colfinal, dim3(n_sub), dim3(256), 0, s, w.part, kBatchColParts, (int64_t)0, p, 1, w.mean,
+                       w.sd, w.cols, w.maxbits, cell_off);
+    hipLaunchKernelGGL(k_knn_keys_seg, grow, dim3(256), 0, s, x, cell_off, p, w.mean, w.sd, w.cols, w.keys_in,
+                       w.vals_in);
+    int end_bit = 33;
+    while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < n_sub) ++end_bit;
+    size_t tb = w.temp_bytes;
+    if (rocprim::radix_sort_pairs(w.temp, tb, w.keys_in, w.keys_out, w.vals_in, w.perm, (size_t)n, 0, end_bit, s) !=
+        hipSuccess)
+        throw std::runtime_error("rocprim radix_sort_pairs failed");
+    if (prec == kPrecFP16)
+        hipLaunchKernelGGL(k_knn_maxnorm, grow, dim3(256), 0, s, x, (int64_t)0, p, w.mean, w.maxbits, cell_off);
+    const PrepSeg sg{cell_off, row0, w.perm_pad};
+    const dim3 gpad((unsigned)ceil_div(max_pad, 256), n_sub);
+    if (prec == kPrecFP16)
+        hipLaunchKernelGGL(k_knn_prep_f16, gpad, dim3(256), 0, s, x, w.perm, w.mean, w.maxbits, (int64_t)0, (int64_t)0,
+                           p, kpad / 8, (uint4 *)xa, (uint4 *)xb, sg);
+    else if (prec == kPrecBF16X3)
+        hipLaunchKernelGGL(k_knn_prep_bf16, gpad, dim3(256), 0, s, x, w.perm, w.mean, (int64_t)0, (int64_t)0, p,
+                           kpad / 8, (uint4 *)xa, (uint4 *)xb, sg);
+    else
+        hipLaunchKernelGGL(k_knn_prep_f32, gpad, dim3(256), 0, s, x, w.perm, w.mean, (int64_t)0, (int64_t)0, p,
+                           kpad / 4, (float2 *)xa, (float2 *)xb, sg);
+    KnnOrder o;
+    o.perm_c = w.perm_pad;
+    o.perm_q = w.perm_pad;
+    o.keys_c = nullptr;
+    o.keys_q = nullptr;
     return o;
 }
 
@@ -558,7 +732,7 @@ template <int KC> constexpr int screen_tb() { return KC <= 8 ? 4 : KC <= 16 ? 2 
 template <int TY, int KC, int L, int NW, int TB>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_waves<KC, L>()))) void k_knn_screen16(
     const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t nc, int64_t nq, const uint32_t *keys_c,
-    const uint32_t *keys_q, int self, float *cand_d, uint32_t *cand_i) {
+    const uint32_t *keys_q, int self, float *cand_d, uint32_t *cand_i, KnnSeg sg) {
     constexpr int KS = KC / 2;     // MFMA K-steps of 16
     constexpr int BU = KC * kTC;   // 16-byte units per 64-candidate block
     constexpr int TU = TB * BU;    // units per stage
@@ -572,7 +746,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     // contiguous range of query blocks, so the workgroups sharing an XCD's L2 have
     // neighbouring home stages and walk overlapping candidate streams.
     const int64_t nb = gridDim.x, xcd = blockIdx.x % 8, per = nb / 8, rem = nb % 8;
-    const int64_t wg = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / 8;
+    int64_t wg = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / 8;
+    if (sg.order) {  // batch: query blocks in the order of the table (largest subsets first)
+        constexpr int SB = kPadRows / NT;
+        wg = (int64_t)sg.order[blockIdx.x / SB] * SB + blockIdx.x % SB;
+    }
     const int64_t q0 = wg * NT + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
     const u32x4 *qbase = xb + (q0 / 64) * KC * 64;
@@ -608,7 +786,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
         topk_update<L>(d0, i0, c0, cb, half);
         topk_update<L>(d1, i1, c1, cb, half);
     };
-    const int64_t n_stages = (nc + kPadRows - 1) / kPadRows * (kPadRows / (kTC * TB));
+    int64_t n_stages = (nc + kPadRows - 1) / kPadRows * (kPadRows / (kTC * TB));
     // Stages are visited outward from the one holding the middle of this
     // workgroup's queries (h, h+1, h-1, h+2, ...), so in the spatial order the
     // nearest candidates come first and the lists tighten early.  For a separate
@@ -631,9 +809,18 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
         }
         mid = lo;
     }
+    int64_t st0 = 0;  // first stage of the candidate range
+    if (sg.order) {   // batch: the candidates are the rows of this block's subset
+        const int sub = sg.blk_seg[wg * NT / kSegBlock];
+        const int64_t r0 = sg.row0[sub], nv = sg.nrow[sub];
+        st0 = r0 / (kTC * TB);
+        n_stages = (nv + kTC * TB - 1) / (kTC * TB);
+        mid = wg * NT + NT / 2;
+        mid = mid < r0 ? r0 : (mid > r0 + nv - 1 ? r0 + nv - 1 : mid);
+    }
     const int64_t home = mid / (kTC * TB);
-    const int64_t h = home < n_stages ? home : n_stages - 1;
-    const int64_t below = h, above = n_stages - 1 - h, m = below < above ? below : above;
+    const int64_t h = home < st0 + n_stages ? home : st0 + n_stages - 1;
+    const int64_t below = h - st0, above = st0 + n_stages - 1 - h, m = below < above ? below : above;
     auto stage_at = [&](int64_t k) -> int64_t {
         if (k <= 2 * m) return (k & 1) ? h + (k + 1) / 2 : h - k / 2;
         return above > below ? h + (k - m) : h - (k - m);
@@ -685,13 +872,16 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
 // two MFMAs, so the chunk's four K values are covered by the two half-waves.
 template <int KC, int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 && KC <= 16) ? 2 : 1))) void k_knn_screen_f32(const float2 *__restrict__ xa, const float2 *__restrict__ xb,
-                                                        int64_t nc_pad, int self, float *cand_d, uint32_t *cand_i) {
+                                                        int64_t nc_pad, int self, float *cand_d, uint32_t *cand_i,
+                                                        KnnSeg sg) {
     constexpr int TU = KC * 2 * kTC;  // float2 units per tile
     static_assert(TU % 256 == 0, "tile must split evenly over 256 threads");
     constexpr int PER = TU / 256;
     __shared__ float2 lds[2][TU];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
-    const int64_t q0 = (int64_t)blockIdx.x * 256 + wave * kQW;
+    int64_t wg = blockIdx.x;
+    if (sg.order) wg = (int64_t)sg.order[blockIdx.x / (kPadRows / 256)] * (kPadRows / 256) + blockIdx.x % (kPadRows / 256);
+    const int64_t q0 = wg * 256 + wave * kQW;
     const uint32_t qa = (uint32_t)(q0 + col), qb = (uint32_t)(q0 + 32 + col);
     const float2 *qbase = xb + (q0 / 64) * KC * 2 * 64;
     float2 b0[KC], b1[KC];
@@ -707,7 +897,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
         d0[j] = d1[j] = INFINITY;
         i0[j] = i1[j] = 0xFFFFFFFFu;
     }
-    const int64_t n_tiles = nc_pad / kTC;
+    int64_t n_tiles = nc_pad / kTC, t0 = 0;
+    if (sg.order) {  // batch: the candidates are the rows of this block's subset
+        const int sub = sg.blk_seg[wg * 256 / kSegBlock];
+        t0 = sg.row0[sub] / kTC;
+        n_tiles = (sg.nrow[sub] + kTC - 1) / kTC;
+    }
+    xa += t0 * TU;
     // same loop-carried prefetch as the 16-bit screen
     for (int i = threadIdx.x; i < TU; i += 256) lds[0][i] = xa[i];
     Regs<float2, PER> pre;
@@ -726,7 +922,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
                 a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b1[j].x, a1, 0, 0, 0);
                 a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b1[j].y, a1, 0, 0, 0);
             }
-            const uint32_t cb = (uint32_t)(t * kTC + mt * 32);
+            const uint32_t cb = (uint32_t)((t0 + t) * kTC + mt * 32);
             if (self && (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32)) {
                 self_mask<L>(a0, cb, half, qa);
                 self_mask<L>(a1, cb, half, qb);
@@ -746,20 +942,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((L <= 16 &&
 template <int L>
 static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int self,
                             const uint32_t *keys_c, const uint32_t *keys_q, float *cd, uint32_t *ci,
-                            hipStream_t s) {
+                            hipStream_t s, const KnnSeg &sg) {
     const int64_t nq_pad = knn_rows_padded(nq), nc_pad = knn_rows_padded(nc);
 #define CREST_X16(TY, KCV)                                                                                   \
     case KCV: {                                                                                             \
         constexpr int NW = screen_nw<TY, KCV, L>();                                                         \
         hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW, screen_tb<KCV>()>), dim3((unsigned)(nq_pad / (NW * 64))), \
                            dim3(NW * 64), 0, s, (const u32x4 *)xa, (const u32x4 *)xb, nc, nq, keys_c, keys_q, \
-                           self, cd, ci);                                                                   \
+                           self, cd, ci, sg);                                                               \
         return true;                                                                                        \
     }
 #define CREST_F(KCV)                                                                                          \
     case KCV:                                                                                                 \
         hipLaunchKernelGGL((k_knn_screen_f32<KCV, L>), dim3((unsigned)(nq_pad / 256)), dim3(256), 0, s,      \
-                           (const float2 *)xa, (const float2 *)xb, nc_pad, self, cd, ci);                    \
+                           (const float2 *)xa, (const float2 *)xb, nc_pad, self, cd, ci, sg);                \
         return true;
     if (prec == kPrecBF16X3) {
         switch (kpad / 8) {
@@ -784,9 +980,20 @@ static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t nc
 // kernel is instantiated for (prec, kpad, L).
 bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int L, int self,
                        const KnnOrder &o, float *cand_d, uint32_t *cand_i, hipStream_t s) {
-    if (L == 16) return launch_screen_L<16>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
-    if (L == 32) return launch_screen_L<32>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
-    if (L == 64) return launch_screen_L<64>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s);
+    const KnnSeg sg{};
+    if (L == 16) return launch_screen_L<16>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s, sg);
+    if (L == 32) return launch_screen_L<32>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s, sg);
+    if (L == 64) return launch_screen_L<64>(prec, xa, xb, nc, nq, kpad, self, o.keys_c, o.keys_q, cand_d, cand_i, s, sg);
+    return false;
+}
+
+// Batch: n_pad operand rows holding every subset's rows (subset y at row0[y], n_y
+// real rows, padded to kPadRows); each query block screens only its own subset.
+bool launch_knn_screen_batch(int prec, const void *xa, const void *xb, int64_t n_pad, int kpad, int L,
+                             const KnnSeg &sg, float *cand_d, uint32_t *cand_i, hipStream_t s) {
+    if (L == 16) return launch_screen_L<16>(prec, xa, xb, n_pad, n_pad, kpad, 1, nullptr, nullptr, cand_d, cand_i, s, sg);
+    if (L == 32) return launch_screen_L<32>(prec, xa, xb, n_pad, n_pad, kpad, 1, nullptr, nullptr, cand_d, cand_i, s, sg);
+    if (L == 64) return launch_screen_L<64>(prec, xa, xb, n_pad, n_pad, kpad, 1, nullptr, nullptr, cand_d, cand_i, s, sg);
     return false;
 }
 
@@ -823,11 +1030,15 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
                                                     const uint32_t *__restrict__ perm_q,
                                                     const uint32_t *__restrict__ perm, int self,
                                                     const uint32_t *__restrict__ cand_i, int n_cand, int k,
-                                                    uint32_t *idx_out, float *dist_out) {
+                                                    uint32_t *idx_out, float *dist_out, const int32_t *blk_seg,
+                                                    const int64_t *cell_off) {
     const int lane = threadIdx.x & 63;
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;  // sorted query position
     if (i >= nq) return;
     const int64_t q = perm_q[i];
+    if (q == 0xFFFFFFFFu) return;  // padding row of a batch
+    // batch: indices are reported within the query's subset
+    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[blk_seg[i / kSegBlock]] : 0u;
     const float *xq = xqs + q * (int64_t)p;
     const uint32_t *ci = cand_i + i * (int64_t)n_cand;
     double dd[NCL];
@@ -841,8 +1052,9 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
     }
     for (int j = 0; j < n_cand; ++j) {
         const uint32_t cpos = ci[j];
-        const bool valid = (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+        bool valid = (int64_t)cpos < n && !(self && (int64_t)cpos == i);
         const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
+        valid = valid && c != 0xFFFFFFFFu;
         double part = 0.0;
         if (valid) {
             const float *xc = x + (int64_t)c * p;
@@ -891,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
 #pragma unroll
     for (int s = 0; s < NCL; ++s)
         if (sel[s]) {
-            idx_out[q * k + rank2[s]] = cc[s];
+            idx_out[q * k + rank2[s]] = cc[s] - ibase;
             dist_out[q * k + rank2[s]] = df[s];
         }
 }
@@ -909,7 +1121,8 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
                                                         const uint32_t *__restrict__ perm_q,
                                                         const uint32_t *__restrict__ perm, int self,
                                                         const uint32_t *__restrict__ cand_i, int n_cand, int k,
-                                                        uint32_t *idx_out, float *dist_out) {
+                                                        uint32_t *idx_out, float *dist_out, const int32_t *blk_seg,
+                                                        const int64_t *cell_off) {
     constexpr int W = 8 / LPC;
     constexpr int S = 64 / LPC;
     const int lane = threadIdx.x & 63;
@@ -917,11 +1130,14 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;  // sorted query position
     if (i >= nq) return;  // uniform across the wave
     const int64_t q = perm_q[i];
+    if (q == 0xFFFFFFFFu) return;  // padding row of a batch
+    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[blk_seg[i / kSegBlock]] : 0u;
     const float *xq = xqs + q * (int64_t)p;
     const bool has = slot < n_cand;
     const uint32_t cpos = has ? cand_i[i * (int64_t)n_cand + slot] : 0xFFFFFFFFu;
-    const bool valid = has && (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+    bool valid = has && (int64_t)cpos < n && !(self && (int64_t)cpos == i);
     const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
+    valid = valid && c != 0xFFFFFFFFu;
     const float *xc = x + (int64_t)(valid ? c : 0u) * p;
     const int m = p - p % 8;
     float acc[W];
@@ -981,27 +1197,27 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
         if (ss && (fs < dfl || (fs == dfl && cs < c))) ++rank2;
     }
     if (sel && sub == 0) {
-        idx_out[q * k + rank2] = c;
+        idx_out[q * k + rank2] = c - ibase;
         dist_out[q * k + rank2] = dfl;
     }
 }
 
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o, int self,
                        const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
-                       hipStream_t s) {
+                       hipStream_t s, const int32_t *blk_seg, const int64_t *cell_off) {
     const int g = (int)ceil_div(nq, 4);  // one wave per query, four per workgroup
     if (n_cand <= 16)
         hipLaunchKernelGGL(k_knn_rerank_grp<4>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
     else if (n_cand <= 32)
         hipLaunchKernelGGL(k_knn_rerank_grp<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
     else if (n_cand <= 64)
         hipLaunchKernelGGL(k_knn_rerank<1>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
     else
         hipLaunchKernelGGL(k_knn_rerank<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
 }
 
 // ---------------------------------------------------------------- large k

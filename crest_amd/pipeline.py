@@ -3,7 +3,9 @@ Gram -> FP64 eigensolve -> projection -> exact kNN on the library's stream and
 reports per-stage HIP-event times.  The CSR is either resident in device memory
 (DeviceCSR, crest_pipeline_run) or held in host memory and streamed through two
 device slots chunk by chunk (HostCSR, crest_pipeline_run_streamed; pinned host
-arrays are uploaded without a staging copy).  Device and pinned buffers come from
+arrays are uploaded without a staging copy).  BatchPipeline runs many independent
+cell subsets of one device CSR together (crest_pipeline_run_batched): each subset
+gets exactly the result of its own Pipeline run.  Device and pinned buffers come from
 the library itself (crest_dev_alloc / crest_host_alloc), so no second HIP runtime is
 loaded into the process.
 """
@@ -14,7 +16,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import PREC_NAMES, PipelineArgs, PipelineResult, check, lib
+from ._lib import PREC_NAMES, BatchArgs, BatchResult, PipelineArgs, PipelineResult, check, lib
 
 
 class DeviceBuffer:
@@ -175,6 +177,69 @@ class Pipeline:
                                                        "knn_screen", "knn_rerank", "total")}
         kp = res.k_pcs
         return PipelineOutput(self.variance[:kp].copy(), self.loadings[:, :kp].copy(), res.total_variance, times)
+
+    def x_pca(self) -> np.ndarray:
+        return self.x.to_numpy(np.float32, (self.n, self.kp))
+
+    def knn(self):
+        return self.idx.to_numpy(np.uint32, (self.n, self.kn)), self.dist.to_numpy(np.float32, (self.n, self.kn))
+
+
+@dataclass
+class BatchOutput:
+    variance: np.ndarray   # [n_subsets, k_pcs]
+    loadings: np.ndarray   # [n_subsets, d, k_pcs]
+    total_variance: np.ndarray  # [n_subsets]
+    times_ms: dict
+    tile_rows: int
+    gram_rows: int
+    graph_replayed: bool
+    gram_tuned: bool
+
+
+class BatchPipeline:
+    """Independent cell subsets (cells subset_off[i] .. subset_off[i+1]-1 of a
+    DeviceCSR) through gene statistics -> Gram -> eigensolve -> projection -> kNN,
+    one launch per stage for all subsets.  use_graph captures the launch sequences in
+    HIP graphs on the first run and replays them on later runs."""
+
+    def __init__(self, csr: DeviceCSR, subset_off, hvg_of_gene: np.ndarray, n_comps: int = 30, k: int = 15,
+                 target_sum: float = 1e4, log1p: bool = True, scale: bool = True, max_value: float = 10.0,
+                 prec_gram: str = "fp16", prec_knn: str = "fp16", tile_rows: int = 0, gram_rows: int = 0,
+                 eig_batch: int = 0, use_graph: bool = False):
+        self.csr = csr
+        self.off = np.ascontiguousarray(subset_off, np.int64)
+        self.n_sub = len(self.off) - 1
+        self.hvg = np.ascontiguousarray(hvg_of_gene, np.int32)
+        self.d = int(self.hvg.max()) + 1
+        n = csr.n_cells
+        self.n = n
+        self.n_comps = int(n_comps)
+        self.kp = min(self.n_comps, self.d)
+        self.kn = int(k)
+        self.x = DeviceBuffer(4 * n * self.kp)
+        self.idx = DeviceBuffer(4 * n * self.kn)
+        self.dist = DeviceBuffer(4 * n * self.kn)
+        pipe = PipelineArgs(
+            csr.indptr.ptr, csr.indices.ptr, csr.data.ptr, n, csr.n_genes,
+            self.hvg.ctypes.data, self.d, float(target_sum), int(log1p), int(scale), float(max_value or 0.0),
+            self.n_comps, self.kn, PREC_NAMES[prec_gram], PREC_NAMES[prec_knn], 0,
+            self.x.ptr, self.idx.ptr, self.dist.ptr)
+        self.args = BatchArgs(pipe, self.n_sub, self.off.ctypes.data, int(tile_rows), int(gram_rows),
+                              int(eig_batch), int(use_graph))
+        self.variance = np.zeros((self.n_sub, self.n_comps), np.float64)
+        self.loadings = np.zeros((self.n_sub, self.d, self.n_comps), np.float64)
+        self.total = np.zeros(self.n_sub, np.float64)
+
+    def run(self) -> BatchOutput:
+        res = BatchResult()
+        check(lib().crest_pipeline_run_batched(C.byref(self.args), self.variance.ctypes.data,
+                                               self.loadings.ctypes.data, self.total.ctypes.data, C.byref(res)))
+        times = {k: getattr(res, "ms_" + k) for k in ("stats", "gram", "eig", "project", "knn_prep",
+                                                       "knn_screen", "knn_rerank", "total")}
+        kp = res.k_pcs
+        return BatchOutput(self.variance[:, :kp].copy(), self.loadings[:, :, :kp].copy(), self.total.copy(), times,
+                           int(res.tile_rows), int(res.gram_rows), bool(res.graph_replayed), bool(res.gram_tuned))
 
     def x_pca(self) -> np.ndarray:
         return self.x.to_numpy(np.float32, (self.n, self.kp))

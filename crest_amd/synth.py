@@ -94,3 +94,23 @@ def make_counts(n_cells: int, n_genes: int, n_clusters: int, density: float | No
                  indptr.ctypes.data, indices.ctypes.data, data.ctypes.data, threads)
     info = {"density": nnz / float(n_cells * n_genes), "nnz": nnz, "lib_scale": factor, "clusters": cluster}
     return indptr, indices, data, info
+
+
+def make_subsets(sizes, n_genes: int, n_clusters: int, density: float | None = None, seed: int = 0,
+                 threads: int | None = None):
+    """Independent subsets, each from its own generator (seed + 1000 + i), concatenated
+    into one CSR.  Returns (indptr int64, indices uint32, data float32, subset_off
+    int64[len(sizes)+1])."""
+    ips, ixs, dvs = [], [], []
+    off = [0]
+    base = 0
+    for i, n in enumerate(sizes):
+        ip, ix, dv, _ = make_counts(int(n), n_genes, n_clusters, density=density, seed=seed + 1000 + i,
+                                    threads=threads)
+        ips.append(ip[:-1] + base if i + 1 < len(sizes) else ip + base)
+        base += int(ip[-1])
+        ixs.append(ix)
+        dvs.append(dv)
+        off.append(off[-1] + int(n))
+    return (np.concatenate(ips).astype(np.int64), np.concatenate(ixs).astype(np.uint32),
+            np.concatenate(dvs).astype(np.float32), np.asarray(off, np.int64))

@@ -220,6 +220,46 @@ int crest_pipeline_run_streamed(const crest_pipeline_args *args, const int64_t *
                                 size_t stream_rows, double *variance, double *loadings,
                                 crest_pipeline_result *result);
 
+/* A batch of independent cell subsets (no reference counterpart: the reference runs
+ * tl.pca + pp.neighbors once per subset, crest/tl.py:37-91 and crest/pp.py:296-315).
+ * Subset i is cells subset_off[i] .. subset_off[i+1]-1 of the device CSR in `pipe`;
+ * each gets its own gene statistics and mean/std (over its own cells), its own
+ * Gram, covariance, eigensolve and projection, and its own exact kNN graph among its
+ * own cells — exactly what crest_pipeline_run computes for that subset alone.
+ * Every subset needs at least max(n_comps, k) + 1 cells (smaller ones go through
+ * crest_pipeline_run); all share hvg_of_gene, the transform and the precisions.
+ * Outputs: pipe.dev_x_pca [n_cells, k_pcs] and pipe.dev_knn_idx / dev_knn_dist
+ * [n_cells, k] by cell (kNN indices counted from the subset's first cell);
+ * variance [n_subsets, n_comps], loadings [n_subsets, d, n_comps] and total
+ * [n_subsets] in host memory.
+ * The stages run as one launch each over all subsets (the Gram as a persistent
+ * kernel over a work list of (subset, row range, block) items), except the FP64
+ * eigensolve, which runs batched over eig_batch subsets per call.               */
+typedef struct crest_batch_args {
+    crest_pipeline_args pipe;
+    size_t n_subsets;
+    const int64_t *subset_off; /* host [n_subsets+1], non-decreasing, 0 .. n_cells      */
+    size_t tile_rows;   /* rows of one densified batch tile (0 = 524,288)                 */
+    size_t gram_rows;   /* largest row range of one Gram work item (0 = autotuned)        */
+    size_t eig_batch;   /* subsets per batched eigensolve call (0 = 64)                   */
+    int32_t use_graph;  /* capture the launches before and after the eigensolve into two
+                           HIP graphs, replayed by later calls with identical arguments  */
+} crest_batch_args;
+
+typedef struct crest_batch_result {
+    size_t k_pcs;      /* min(n_comps, d)                                             */
+    size_t k_nn;       /* k                                                           */
+    double ms_stats, ms_gram, ms_eig, ms_project, ms_knn_prep, ms_knn_screen, ms_knn_rerank;
+    double ms_total;   /* with use_graph: ms_stats+ms_gram is the first graph, ms_eig the
+                          eigensolve, ms_project the second graph (stage split not timed) */
+    size_t tile_rows, gram_rows; /* shapes used (gram_rows from the autotuner if 0)   */
+    int32_t graph_replayed;      /* 1 when captured graphs were replayed              */
+    int32_t gram_tuned;          /* 1 when this call ran the gram_rows autotuner      */
+} crest_batch_result;
+
+int crest_pipeline_run_batched(const crest_batch_args *args, double *variance, double *loadings, double *total,
+                               crest_batch_result *result);
+
 /* Pinned (page-locked) host memory for crest_pipeline_run_streamed's inputs. */
 int crest_host_alloc(size_t bytes, void **ptr);
 int crest_host_free(void *ptr);

@@ -165,3 +165,44 @@ def test_pca_argument_errors():
         ca.tl.pca(bf)
     with pytest.raises(KeyError):
         ca.pp.neighbors(bf)
+
+
+def _batch_args(n_cells, off, n_comps=30, k=15):
+    """crest_batch_args with placeholder device pointers: validation fails before any
+    pointer is used or a device is touched."""
+    import ctypes as C
+    hvg = np.arange(100, dtype=np.int32)
+    fake = 256
+    pipe = _lib.PipelineArgs(fake, fake, fake, n_cells, 100, hvg.ctypes.data, 100, 1e4, 1, 1, 10.0, n_comps, k,
+                             _lib.PREC_FP16, _lib.PREC_FP16, 0, fake, fake, fake)
+    off = np.asarray(off, np.int64)
+    args = _lib.BatchArgs(pipe, len(off) - 1, off.ctypes.data, 0, 0, 0, 0)
+    out = np.zeros(1 << 16)
+    return args, (off, hvg, out), C
+
+
+@pytest.mark.parametrize("n_cells,off,n_comps,k,match", [
+    (100, [0, 20, 100], 30, 15, "subset 0 has 20 cells"),
+    (100, [0, 60, 99], 30, 15, "subset_off must run from 0 to n_cells"),
+    (100, [0, 60, 100], 30, 80, r"1 <= k <= 64"),
+    (100, [0, 50, 100], 50, 15, "subset 0 has 50 cells; a batch needs at least max\\(n_comps, k\\) \\+ 1 = 51"),
+])
+def test_batch_validation_errors(n_cells, off, n_comps, k, match):
+    args, keep, C = _batch_args(n_cells, off, n_comps, k)
+    out = keep[2]
+    res = _lib.BatchResult()
+    rc = _lib.lib().crest_pipeline_run_batched(C.byref(args), out.ctypes.data, out.ctypes.data, out.ctypes.data,
+                                               C.byref(res))
+    with pytest.raises(ValueError, match=match):
+        _lib.check(rc)
+
+
+def test_batch_plan_of_subsets_is_contiguous():
+    """The synthetic batch generator: subsets back to back, offsets spanning all cells."""
+    from crest_amd import synth
+    ip, ix, dv, off = synth.make_subsets([40, 7, 90], 50, 3, density=0.1, seed=4)
+    assert list(off) == [0, 40, 47, 137]
+    assert len(ip) == 138 and ip[0] == 0 and ip[-1] == len(ix) == len(dv)
+    a, b, c, _ = synth.make_counts(7, 50, 3, density=0.1, seed=4 + 1000 + 1)
+    np.testing.assert_array_equal(ix[ip[40]:ip[47]], b)
+    np.testing.assert_array_equal(np.diff(ip[40:48]), np.diff(a))

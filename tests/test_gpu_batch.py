@@ -1,0 +1,150 @@
+"""The batch of independent subsets (crest_pipeline_run_batched) against one
+crest_pipeline_run per subset and against the oracle: variance, loadings, scores and
+kNN of every subset, with subsets cut across tiles, split into several Gram row
+ranges, eigensolved in small batches, and replayed from captured HIP graphs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from crest_amd import synth  # noqa: E402
+
+G = 400
+# a subset of exactly max(n_comps, k) + 1 cells, ones below and above one 512-row
+# query block, and ones spanning several blocks and Gram row ranges
+SIZES = [31, 200, 513, 700, 1300, 2600, 4100]
+
+
+@pytest.fixture(scope="module")
+def batch():
+    indptr, indices, data, off = synth.make_subsets(SIZES, G, 10, density=0.07, seed=21)
+    return indptr, indices, data, off
+
+
+def _run(batch, **kw):
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR
+    indptr, indices, data, off = batch
+    csr = DeviceCSR.upload(indptr, indices, data, G)
+    p = BatchPipeline(csr, off, np.arange(G, dtype=np.int32), n_comps=30, k=15, **kw)
+    out = p.run()
+    idx, dist = p.knn()
+    return p, out, p.x_pca(), idx, dist
+
+
+def _single(batch, i, prec_gram, prec_knn):
+    from crest_amd.pipeline import DeviceCSR, Pipeline
+    indptr, indices, data, off = batch
+    a, b = int(off[i]), int(off[i + 1])
+    ip = indptr[a:b + 1] - indptr[a]
+    ix = indices[indptr[a]:indptr[b]]
+    dv = data[indptr[a]:indptr[b]]
+    p = Pipeline(DeviceCSR.upload(ip, ix, dv, G), np.arange(G, dtype=np.int32), n_comps=30, k=15,
+                 prec_gram=prec_gram, prec_knn=prec_knn, chunk_rows=2048)
+    out = p.run()
+    idx, dist = p.knn()
+    return out, p.x_pca(), idx, dist, (ip, ix, dv)
+
+
+def _angle(oracle, A, B, r):
+    return oracle.principal_angle(A[:, :r], B[:, :r])
+
+
+def _check_knn(oracle, X, idx, n):
+    assert idx.max() < n
+    assert not (idx == np.arange(n)[:, None]).any()
+    q = np.arange(n)
+    ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
+    bad, ties = oracle.knn_mismatch_ties(idx, X, q, ref_i, ref_d2)
+    assert bad == ties, (n, bad, ties)
+
+
+# Split precisions accumulate 16 K-steps in FP32 before adding into FP64; the batch's
+# row ranges move those flush points, so its sums differ from the single run's at the
+# FP32 accumulation level (the same bound the single pipeline meets against the oracle).
+@pytest.mark.parametrize("prec_gram,prec_knn,var_rtol,tot_rtol", [("fp64", "fp32", 1e-9, 1e-9),
+                                                                  ("fp16", "fp16", 1e-6, 1e-7),
+                                                                  ("bf16x3", "bf16x3", 1e-5, 1e-6)])
+def test_batch_equals_single_runs(batch, oracle, prec_gram, prec_knn, var_rtol, tot_rtol):
+    p, out, X, idx, dist = _run(batch, prec_gram=prec_gram, prec_knn=prec_knn)
+    off = batch[3]
+    for i in range(len(SIZES)):
+        a, b = int(off[i]), int(off[i + 1])
+        o1, X1, i1, d1, csr1 = _single(batch, i, prec_gram, prec_knn)
+        np.testing.assert_allclose(out.variance[i], o1.variance, rtol=var_rtol, err_msg=f"subset {i}")
+        assert abs(out.total_variance[i] - o1.total_variance) <= tot_rtol * o1.total_variance
+        # the 10 leading components: the subsets have 10 clusters, so the signal
+        # subspace is well separated from the noise
+        assert _angle(oracle, out.loadings[i], o1.loadings, 9) < 1e-6, i
+        # kNN of the batch is exact on the batch's own scores
+        _check_knn(oracle, X[a:b], idx[a:b].astype(np.int64), b - a)
+        # and the subset's own pipeline agrees on the oracle's variance
+        if prec_gram == "fp64":
+            ip, ix, dv = csr1
+            n = b - a
+            cmap, gmap = np.arange(n, dtype=np.int64), np.arange(G, dtype=np.int32)
+            ctx = (ix, dv, cmap, gmap, 1e4, True, None, ip, 0)
+            st = np.zeros((G, 5))
+            oracle.gene_stats(*ctx, st)
+            mean = st[:, 0] / n
+            sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+            sd[sd == 0] = 1
+            gram, colsum = np.zeros((G, G)), np.zeros(G)
+            oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
+            var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, 30)
+            np.testing.assert_allclose(out.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
+            assert abs(out.total_variance[i] - tot_o) <= 1e-11 * tot_o
+            # scores: (Z - mean Z) V with the oracle's loadings, through the oracle's
+            # projection (FP64 accumulation, f32 output)
+            Xo = np.empty((n, 30), np.float32)
+            Xo[:] = -((colsum / n) @ load_o).astype(np.float32)
+            oracle.project(*ctx, gmap, load_o, Xo, True, mean, sd, 10.0)
+            sign = np.sign((X[a:b, :9] * Xo[:, :9]).sum(0))
+            np.testing.assert_allclose(X[a:b, :9] * sign, Xo[:, :9], atol=1e-4 * np.abs(Xo[:, :9]).max())
+
+
+def test_batch_tiles_ranges_and_eig_groups(batch, oracle):
+    """Subsets cut across 1,024-row tiles, Gram row ranges of 256 rows (every subset
+    above 256 rows goes through partial slots and the ordered reduce), eigensolves 3
+    subsets per call: the same results as the default plan."""
+    _, o_def, X_def, i_def, _ = _run(batch, prec_gram="fp64", prec_knn="fp32")
+    _, o, X, idx, _ = _run(batch, prec_gram="fp64", prec_knn="fp32", tile_rows=1024, gram_rows=256, eig_batch=3)
+    assert o.tile_rows == 1024 and o.gram_rows == 256
+    np.testing.assert_allclose(o.variance, o_def.variance, rtol=1e-11)
+    np.testing.assert_allclose(o.total_variance, o_def.total_variance, rtol=1e-12)
+    for i in range(len(SIZES)):
+        assert _angle(oracle, o.loadings[i], o_def.loadings[i], 9) < 1e-9
+    np.testing.assert_allclose(X, X_def, rtol=1e-5, atol=1e-5)
+    assert (np.sort(idx, 1) == np.sort(i_def, 1)).all(1).mean() > 0.999
+
+
+def test_batch_autotuner_reports_its_choice(batch):
+    _, o1, _, _, _ = _run(batch, prec_gram="fp16", prec_knn="fp16", tile_rows=4096)
+    assert o1.gram_rows in (2048, 4096, 8192, 16384, 1 << 20)
+    _, o2, _, _, _ = _run(batch, prec_gram="fp16", prec_knn="fp16", tile_rows=4096)
+    assert o2.gram_rows == o1.gram_rows and not o2.gram_tuned  # cached per shape
+
+
+def test_batch_graph_replays(batch, oracle):
+    """Captured graphs replayed several times, with eager batch runs and device
+    synchronisations in between: every replay matches the eager run."""
+    from crest_amd._lib import check, lib
+    _, o_e, X_e, i_e, d_e = _run(batch, prec_gram="fp16", prec_knn="fp16")
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR
+    indptr, indices, data, off = batch
+    csr = DeviceCSR.upload(indptr, indices, data, G)
+    pg = BatchPipeline(csr, off, np.arange(G, dtype=np.int32), n_comps=30, k=15, prec_gram="fp16",
+                       prec_knn="fp16", use_graph=True)
+    pe = BatchPipeline(csr, off, np.arange(G, dtype=np.int32), n_comps=30, k=15, prec_gram="fp16",
+                       prec_knn="fp16")
+    for rep in range(4):
+        o = pg.run()
+        assert o.graph_replayed == (rep > 0)
+        X, (idx, dist) = pg.x_pca(), pg.knn()
+        np.testing.assert_allclose(o.variance, o_e.variance, rtol=1e-9)
+        np.testing.assert_allclose(X, X_e, rtol=1e-5, atol=1e-5)
+        assert (np.sort(idx, 1) == np.sort(i_e, 1)).all(1).mean() > 0.999
+        np.testing.assert_allclose(dist, d_e, rtol=1e-5, atol=1e-6)
+        pe.run()  # eager work on the same stream between replays
+        check(lib().crest_synchronize())
+    # the eager object's own results are unchanged by the replays
+    np.testing.assert_allclose(pe.x_pca(), X_e, rtol=1e-5, atol=1e-5)
