@@ -202,6 +202,83 @@ def streamed_leg(args) -> dict:
             "stages_ms": {k: round(v, 3) for k, v in stages.items()}}
 
 
+def batched_leg(args) -> dict:
+    """The batching configuration: n_subsets independent subsets of log-uniform sizes,
+    each from its own NB generator, through one crest_pipeline_run_batched call
+    (replayed from its captured HIP graphs), against one crest_pipeline_run per subset
+    one after another.  Both at the headline precisions; inputs resident in HBM."""
+    from crest_amd import synth
+    from crest_amd._lib import check, lib
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR, Pipeline
+    rng = np.random.default_rng(args.batch_seed)
+    sizes = np.exp(rng.uniform(np.log(args.batch_min), np.log(args.batch_max), args.batch_subsets)).astype(np.int64)
+    t0 = time.perf_counter()
+    ip, ix, dv, off = synth.make_subsets(sizes, args.genes, args.batch_clusters, density=args.batch_density,
+                                         seed=args.batch_seed)
+    n, nnz = int(off[-1]), int(ip[-1])
+    log(f"batch: generated {args.batch_subsets} subsets, {n} cells, nnz {nnz} in {time.perf_counter() - t0:.1f}s")
+    hvg = np.arange(args.genes, dtype=np.int32)
+    csr = DeviceCSR.upload(ip, ix, dv, args.genes)
+    kw = dict(n_comps=args.batch_pcs, k=args.k, prec_gram=args.gram_prec, prec_knn=args.knn_prec)
+    # one eager run for the per-stage times
+    pe = BatchPipeline(csr, off, hvg, **kw)
+    oe = pe.run()
+    oe = pe.run()
+    log(f"batch eager: {oe.times_ms['total']:.1f} ms, stages "
+        f"{json.dumps({k: round(v, 2) for k, v in oe.times_ms.items()})}, gram_rows {oe.gram_rows}")
+    del pe
+    pg = BatchPipeline(csr, off, hvg, use_graph=True, **kw)
+    pg.run()  # captures the graphs (and tunes, if this shape is new)
+    check(lib().crest_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        og = pg.run()
+    check(lib().crest_synchronize())
+    ms_b = (time.perf_counter() - t0) * 1e3 / args.batch_steps
+    assert og.graph_replayed
+    log(f"batch (graph replay): {ms_b:.1f} ms/step ({og.times_ms['gram']:.1f} gram graph, "
+        f"{og.times_ms['eig']:.1f} eig, {og.times_ms['project']:.1f} projection+kNN graph)")
+    Xb = pg.x_pca()
+    ib, _ = pg.knn()
+    # sequential: one pipeline per subset (its own CSR upload outside the timed region)
+    singles = []
+    for i in range(args.batch_subsets):
+        a, b = int(off[i]), int(off[i + 1])
+        sip = ip[a:b + 1] - ip[a]
+        c1 = DeviceCSR.upload(sip, ix[ip[a]:ip[b]], dv[ip[a]:ip[b]], args.genes)
+        singles.append(Pipeline(c1, hvg, chunk_rows=65536, **kw))
+    big = int(np.argmax(sizes))
+    singles[big].run()  # warm: the largest subset sizes every library buffer
+    check(lib().crest_synchronize())
+    t0 = time.perf_counter()
+    outs = [s_.run() for s_ in singles]
+    check(lib().crest_synchronize())
+    ms_s = (time.perf_counter() - t0) * 1e3
+    log(f"sequential ({args.batch_subsets} crest_pipeline_run calls): {ms_s:.1f} ms")
+    # agreement of the batch with the per-subset runs
+    var_rel = max(float(np.max(np.abs(og.variance[i] - o.variance) / o.variance)) for i, o in enumerate(outs))
+    agree = []
+    for i in range(0, args.batch_subsets, max(1, args.batch_subsets // 16)):
+        a, b = int(off[i]), int(off[i + 1])
+        i1, _ = singles[i].knn()
+        agree.append((np.sort(ib[a:b], 1) == np.sort(i1, 1)).all(1).mean())
+    seq_stage = {k: round(float(sum(o.times_ms[k] for o in outs)), 2) for k in outs[0].times_ms}
+    return {"workload": (f"{args.batch_subsets} subsets of log-uniform [{args.batch_min}, {args.batch_max}] cells "
+                         f"({n} cells) x {args.genes} HVGs, each its own NB generator ({args.batch_clusters} clusters, "
+                         f"density {nnz / n / args.genes:.4f}), PCA {args.batch_pcs}, exact kNN k={args.k}, "
+                         f"seed {args.batch_seed}"),
+            "n_subsets": args.batch_subsets, "n_cells": n, "nnz": nnz,
+            "ms_batched": ms_b, "ms_sequential": ms_s, "speedup_vs_sequential": ms_s / ms_b,
+            "value_batched": n / (ms_b * 1e-3), "value_sequential": n / (ms_s * 1e-3), "unit": "cells/s",
+            "graph_ms": {"stats_gram": round(og.times_ms["gram"], 2), "eig": round(og.times_ms["eig"], 2),
+                         "project_knn": round(og.times_ms["project"], 2)},
+            "stages_ms_eager": {k: round(v, 2) for k, v in oe.times_ms.items()},
+            "stages_ms_sequential_sum": seq_stage,
+            "tile_rows": og.tile_rows, "gram_rows": og.gram_rows,
+            "max_variance_rel_diff_vs_sequential": var_rel,
+            "knn_set_agreement_vs_sequential": float(np.mean(agree))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +310,15 @@ def main() -> int:
                     help="also time the out-of-core form: N x genes NB counts in pinned host memory, streamed in "
                          "chunks of --stream-rows cells (reported under 'streamed', not the headline)")
     ap.add_argument("--stream-rows", type=int, default=32768)
+    ap.add_argument("--batch-subsets", type=int, default=256,
+                    help="also time the batching configuration with this many subsets (0: skip)")
+    ap.add_argument("--batch-min", type=int, default=200)
+    ap.add_argument("--batch-max", type=int, default=50000)
+    ap.add_argument("--batch-clusters", type=int, default=10)
+    ap.add_argument("--batch-density", type=float, default=0.07)
+    ap.add_argument("--batch-pcs", type=int, default=30)
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -347,6 +433,8 @@ def main() -> int:
         line["speedup_vs_fp32"] = ms32 / ms
     if args.streamed_cells:
         line["streamed"] = streamed_leg(args)
+    if args.batch_subsets:
+        line["batch"] = batched_leg(args)
     if not args.no_cpu:
         t0 = time.perf_counter()
         base, acc = cpu_baseline_leg(indptr, indices, data, args.genes, gpu, args)
