@@ -136,7 +136,7 @@ void launch_gram_reduce_items(const double *partial, const RedItem *items, int64
 void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
                       int n_sub, double *cov, double *total, hipStream_t s);
 void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
-                             double *loadings, hipStream_t s);
+                             double *loadings, hipStream_t s, int ncols = 0);
 void launch_shift_fill_batch(const double *colsum, int ld_c, const double *V, int d, int k,
                              const int64_t *dev_cell_off, int n_sub, int64_t max_rows, float *x, float *shift_f,
                              hipStream_t s);
@@ -156,7 +156,13 @@ void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *g
 void launch_cov(const double *gram, const double *colsum, int d, int d_pad, double n, double *cov,
                 double *total, hipStream_t s);
 void launch_eig_select(const double *evecs, const double *evals, int d, int k, double *variance,
-                       double *loadings, hipStream_t s);
+                       double *loadings, hipStream_t s, int ncols = 0);
+// Top-p eigenpairs after a lower Householder tridiagonalization (kernels_eig.hip).
+bool tri_topp_supported(int n);
+void tri_set_attrs();
+void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, int64_t sTau, const double *D,
+                     int64_t sD, const double *E, int64_t sE, int n, int p, int n_mat, double *W, double *Y,
+                     double *X, double *fac, uint8_t *swp, int *info, hipStream_t s);
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
                        float *x, float *shift_scratch, hipStream_t s);
 // kNN screen operands: rows are padded to knn_rows_padded(n) and stored in a

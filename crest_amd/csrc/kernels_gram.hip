@@ -543,20 +543,21 @@ void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g,
 }
 
 // ---------------------------------------------------------------- eigen selection
-// Component c takes eigenpair d-1-c of the ascending solver output; variance is
+// Component c takes eigenpair ncols-1-c of the ascending solver output (ncols = d for
+// a full eigensolve, the number of computed eigenpairs otherwise); variance is
 // floored at 0; the sign makes the first largest-|loading| entry positive
 // (kernels.rs:328-346).  evecs is column-major (column j = eigenvector j).
 __global__ __launch_bounds__(256) void k_eig_select(const double *evecs, const double *evals, int d, int k,
-                                                    double *variance, double *loadings) {
+                                                    double *variance, double *loadings, int ncols) {
     __shared__ double sbest[256];
     __shared__ int sidx[256];
     const int y = blockIdx.y;  // subset of a batch
-    evecs += (int64_t)y * d * d;
-    evals += (int64_t)y * d;
+    evecs += (int64_t)y * d * ncols;
+    evals += (int64_t)y * ncols;
     variance += (int64_t)y * k;
     loadings += (int64_t)y * d * k;
     const int c = blockIdx.x;
-    const int src = d - 1 - c;
+    const int src = ncols - 1 - c;
     const double *u = evecs + (int64_t)src * d;
     double best = 0.0;
     int bidx = -1;
@@ -593,14 +594,16 @@ __global__ __launch_bounds__(256) void k_eig_select(const double *evecs, const d
 }
 
 void launch_eig_select(const double *evecs, const double *evals, int d, int k, double *variance, double *loadings,
-                       hipStream_t s) {
+                       hipStream_t s, int ncols) {
     if (k <= 0) return;
-    hipLaunchKernelGGL(k_eig_select, dim3(k), dim3(256), 0, s, evecs, evals, d, k, variance, loadings);
+    hipLaunchKernelGGL(k_eig_select, dim3(k), dim3(256), 0, s, evecs, evals, d, k, variance, loadings,
+                       ncols > 0 ? ncols : d);
 }
 void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
-                             double *loadings, hipStream_t s) {
+                             double *loadings, hipStream_t s, int ncols) {
     if (k <= 0 || n_sub <= 0) return;
-    hipLaunchKernelGGL(k_eig_select, dim3(k, n_sub), dim3(256), 0, s, evecs, evals, d, k, variance, loadings);
+    hipLaunchKernelGGL(k_eig_select, dim3(k, n_sub), dim3(256), 0, s, evecs, evals, d, k, variance, loadings,
+                       ncols > 0 ? ncols : d);
 }
 
 // ---------------------------------------------------------------- shift prefill

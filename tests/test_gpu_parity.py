@@ -483,3 +483,61 @@ def test_parquet_roundtrip(counts, tmp_path, oracle):
     o1, v1 = ca.pp.calculate_qc_metrics(back, inplace=False)
     np.testing.assert_array_equal(o0["total_counts"], o1["total_counts"])
     np.testing.assert_array_equal(v0["n_cells_by_counts"], v1["n_cells_by_counts"])
+
+
+def _gram_with_spectrum(lam, n, seed):
+    """Gram and colsum (zero) whose covariance (G - c c^T / n) / (n - 1) has eigenvalues lam."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.normal(size=(len(lam), len(lam))))
+    C = (Q * lam) @ Q.T
+    C = (C + C.T) / 2
+    return C * (n - 1), np.zeros(len(lam)), C
+
+
+def _subspace_angle(A, B):
+    s = np.linalg.svd(A.T @ B, compute_uv=False)
+    return float(np.arccos(np.clip(s.min(), -1, 1))) if s.min() < 1 - 1e-14 else float(
+        np.linalg.norm(B - A @ (A.T @ B), 2))
+
+
+@pytest.mark.parametrize("case", ["cluster", "rank_deficient", "dense_distinct"])
+def test_pca_from_gram_top_p_eigensolver(case, monkeypatch):
+    """The top-p eigensolver (tridiagonalization, bisection, inverse iteration with
+    in-cluster orthogonalisation, back-transform) on spectra with an exactly repeated
+    cluster, with zero eigenvalues inside the requested p, and fully distinct: the
+    eigenvalues of numpy's eigh, orthonormal loadings, the determined subspaces, and
+    the same results as rocSOLVER's full dsyevd."""
+    d, n, k = 300, 5000, 40
+    rng = np.random.default_rng(3)
+    if case == "cluster":
+        lam = np.concatenate([np.linspace(10, 1, 10), np.full(30, 0.5), np.sort(rng.uniform(0, 0.3, d - 40))[::-1]])
+    elif case == "rank_deficient":
+        lam = np.concatenate([np.sort(rng.uniform(1, 5, 25))[::-1], np.zeros(d - 25)])
+    else:
+        lam = np.sort(rng.uniform(0.01, 20, d))[::-1]
+    g, c, C = _gram_with_spectrum(lam, n, 7)
+    monkeypatch.setenv("CREST_EIGENSOLVER", "topp")
+    var, load, tot = N.pca_from_gram(g, c, n, k)
+    ev, V = np.linalg.eigh(C)
+    ev, V = ev[::-1], V[:, ::-1]
+    np.testing.assert_allclose(var, np.maximum(ev[:k], 0), atol=1e-12 * ev[0])
+    assert abs(tot - np.trace(C)) <= 1e-12 * abs(np.trace(C))
+    assert np.abs(load.T @ load - np.eye(k)).max() < 1e-12
+    # residual of every computed pair
+    assert np.abs(C @ load - load * var).max() < 1e-11 * ev[0]
+    if case == "cluster":
+        assert _subspace_angle(V[:, :10], load[:, :10]) < 1e-10
+        assert _subspace_angle(V[:, :40], load[:, :40]) < 1e-10
+    elif case == "dense_distinct":
+        assert _subspace_angle(V[:, :k], load) < 1e-9
+    else:
+        assert _subspace_angle(V[:, :25], load[:, :25]) < 1e-10
+    # sign rule: the first largest-|loading| entry of every component is positive
+    idx = np.argmax(np.abs(load), axis=0)
+    assert (load[idx, np.arange(k)] > 0).all()
+    monkeypatch.setenv("CREST_EIGENSOLVER", "dsyevd")
+    var2, load2, tot2 = N.pca_from_gram(g, c, n, k)
+    np.testing.assert_allclose(var, var2, atol=1e-12 * ev[0])
+    assert tot == tot2
+    if case == "dense_distinct":
+        np.testing.assert_allclose(load, load2, atol=1e-9)
