@@ -240,6 +240,18 @@ def batched_leg(args) -> dict:
         f"{og.times_ms['eig']:.1f} eig, {og.times_ms['project']:.1f} projection+kNN graph)")
     Xb = pg.x_pca()
     ib, _ = pg.knn()
+    base = {"workload": (f"{args.batch_subsets} subsets of log-uniform [{args.batch_min}, {args.batch_max}] cells "
+                         f"({n} cells) x {args.genes} HVGs, each its own NB generator ({args.batch_clusters} clusters, "
+                         f"density {nnz / n / args.genes:.4f}), PCA {args.batch_pcs}, exact kNN k={args.k}, "
+                         f"seed {args.batch_seed}"),
+            "n_subsets": args.batch_subsets, "n_cells": n, "nnz": nnz, "ms_batched": ms_b,
+            "value_batched": n / (ms_b * 1e-3), "unit": "cells/s",
+            "graph_ms": {"stats_gram": round(og.times_ms["gram"], 2), "eig": round(og.times_ms["eig"], 2),
+                         "project_knn": round(og.times_ms["project"], 2)},
+            "stages_ms_eager": {k: round(v, 2) for k, v in oe.times_ms.items()},
+            "tile_rows": og.tile_rows, "gram_rows": og.gram_rows}
+    if args.batch_no_seq:
+        return base
     # sequential: one pipeline per subset (its own CSR upload outside the timed region)
     singles = []
     for i in range(args.batch_subsets):
@@ -263,20 +275,10 @@ def batched_leg(args) -> dict:
         i1, _ = singles[i].knn()
         agree.append((np.sort(ib[a:b], 1) == np.sort(i1, 1)).all(1).mean())
     seq_stage = {k: round(float(sum(o.times_ms[k] for o in outs)), 2) for k in outs[0].times_ms}
-    return {"workload": (f"{args.batch_subsets} subsets of log-uniform [{args.batch_min}, {args.batch_max}] cells "
-                         f"({n} cells) x {args.genes} HVGs, each its own NB generator ({args.batch_clusters} clusters, "
-                         f"density {nnz / n / args.genes:.4f}), PCA {args.batch_pcs}, exact kNN k={args.k}, "
-                         f"seed {args.batch_seed}"),
-            "n_subsets": args.batch_subsets, "n_cells": n, "nnz": nnz,
-            "ms_batched": ms_b, "ms_sequential": ms_s, "speedup_vs_sequential": ms_s / ms_b,
-            "value_batched": n / (ms_b * 1e-3), "value_sequential": n / (ms_s * 1e-3), "unit": "cells/s",
-            "graph_ms": {"stats_gram": round(og.times_ms["gram"], 2), "eig": round(og.times_ms["eig"], 2),
-                         "project_knn": round(og.times_ms["project"], 2)},
-            "stages_ms_eager": {k: round(v, 2) for k, v in oe.times_ms.items()},
-            "stages_ms_sequential_sum": seq_stage,
-            "tile_rows": og.tile_rows, "gram_rows": og.gram_rows,
-            "max_variance_rel_diff_vs_sequential": var_rel,
-            "knn_set_agreement_vs_sequential": float(np.mean(agree))}
+    base.update({"ms_sequential": ms_s, "speedup_vs_sequential": ms_s / ms_b, "value_sequential": n / (ms_s * 1e-3),
+                 "stages_ms_sequential_sum": seq_stage, "max_variance_rel_diff_vs_sequential": var_rel,
+                 "knn_set_agreement_vs_sequential": float(np.mean(agree))})
+    return base
 
 
 def main() -> int:
@@ -319,6 +321,8 @@ def main() -> int:
     ap.add_argument("--batch-pcs", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-only", action="store_true", help="run only the batching configuration (profiling)")
+    ap.add_argument("--batch-no-seq", action="store_true", help="skip the sequential comparison of the batch leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -331,6 +335,9 @@ def main() -> int:
 
     if args.knn_only:
         return knn_only(args)
+    if args.batch_only:
+        print(json.dumps({"batch_only": True, "batch": batched_leg(args)}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     indptr, indices, data, info = synth.make_counts(args.cells, args.genes, args.clusters, density=args.density,
