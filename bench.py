@@ -281,6 +281,140 @@ def batched_leg(args) -> dict:
     return base
 
 
+def c2_leg(args) -> dict:
+    """Configuration 2 through the reference's own operator path: 100,000 cells x
+    33,538 genes (20 clusters), normalize_total 1e4, log1p, seurat HVG 2,000, scale
+    max_value 10, tl.pca 50, knn_graph(X, 15), chunks of 16,384 cells from host
+    memory (each chunk uploaded by its call, as the reference's Python layer passes
+    numpy views), plain FP32 GPU path (FP32 MFMA Gram, FP32 screen).  The second of two
+    runs is timed."""
+    import crest_amd as ca
+    from crest_amd import _native as N
+    from crest_amd import synth
+    n, G = args.c2_cells, args.c2_genes
+    t0 = time.perf_counter()
+    ip, ix, dv, info = synth.make_counts(n, G, 20, density=0.07, seed=1)
+    log(f"c2: generated {n} x {G}, nnz {info['nnz']} (density {info['density']:.4f}) in "
+        f"{time.perf_counter() - t0:.1f}s")
+    chunk_nnz = int(ip[min(n, 16384)])
+    prev, prev_knn = ca.get_precision(), ca.get_knn_precision()
+    ca.set_precision("fp32")
+    ca.set_knn_precision(None)
+
+    def run():
+        bf = ca.BioFrame.from_csr(ip, ix, dv, G)
+        bf.chunk_nnz = chunk_nnz
+        t = {}
+        ca.pp.normalize_total(bf, 1e4)
+        ca.pp.log1p(bf)
+        a = time.perf_counter()
+        ca.pp.highly_variable_genes(bf, n_top_genes=2000)
+        ca.pp.scale(bf, max_value=10)
+        b = time.perf_counter()
+        ca.tl.pca(bf, n_comps=50)
+        c = time.perf_counter()
+        idx, dist = N.knn_graph(bf.obsm["X_pca"], 15)
+        e = time.perf_counter()
+        t.update({"hvg": (b - a) * 1e3, "pca": (c - b) * 1e3, "knn": (e - c) * 1e3, "total": (e - a) * 1e3})
+        return bf, idx, t
+
+    try:
+        run()
+        bf, idx, t = run()
+    finally:
+        ca.set_precision(prev)
+        ca.set_knn_precision(None if prev_knn == prev else prev_knn)
+    log(f"c2 (fp32, operator path): {t['total']:.1f} ms, stages {json.dumps({k: round(v, 1) for k, v in t.items()})}")
+    out = {"workload": (f"{n} cells x {G} genes, NB counts density {info['density']:.4f}, 20 clusters, seed 1; "
+                        "normalize_total 1e4, log1p, seurat HVG 2000, scale max_value 10, PCA 50, "
+                        "knn_graph(X, 15); chunks of 16,384 cells from host memory; FP32 Gram and screen"),
+           "n_cells": n, "n_genes": G, "nnz": info["nnz"], "ms": t["total"], "value": n / (t["total"] * 1e-3),
+           "unit": "cells/s", "stages_ms": {k: round(v, 2) for k, v in t.items()}}
+    if not args.no_cpu:
+        out.update(c2_cpu_baseline(ip, ix, dv, G, chunk_nnz, bf, idx, args))
+    return out
+
+
+def c2_cpu_baseline(ip, ix, dv, G, chunk_nnz, bf, idx, args) -> dict:
+    """The oracle (C restatement of kernels.rs/knn.rs) on configuration 2: gene
+    statistics, HVG selection, FP64 Gram over all cells, numpy eigh, projection of
+    20,000 cells and exact kNN of 2,048 queries (both scaled to all cells); and the
+    GPU results checked against it (HVG index set, variances, subspace, kNN sets)."""
+    from crest_amd.pp import hvg_from_stats
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    n = len(ip) - 1
+    cmap, gall = np.arange(n, dtype=np.int64), np.arange(G, dtype=np.int32)
+
+    def chunks():
+        lo = 0
+        while lo < n:
+            hi = int(np.searchsorted(ip, ip[lo] + chunk_nnz, side="right")) - 1
+            hi = max(min(hi, n), lo + 1)
+            a, b = int(ip[lo]), int(ip[hi])
+            yield lo, hi, (ix[a:b], dv[a:b], cmap, gall, 1e4, True, None, ip[lo:hi + 1].copy(), lo)
+            lo = hi
+
+    t = {}
+    t0 = time.perf_counter()
+    st = np.zeros((G, 5))
+    for _, _, ctx in chunks():
+        O.gene_stats(*ctx, st)
+    cols = hvg_from_stats(st, n, 2000)
+    t["hvg"] = time.perf_counter() - t0
+    hv = cols["highly_variable"]
+    d = int(hv.sum())
+    gmap = np.full(G, -1, np.int32)
+    gmap[hv] = np.arange(d, dtype=np.int32)
+    mean, sd = cols["mean_log"][hv].copy(), cols["std_log"][hv].copy()
+    t0 = time.perf_counter()
+    gram, colsum = np.zeros((d, d)), np.zeros(d)
+    for _, _, ctx in chunks():
+        O.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=threads)
+    t["gram"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    var_o, load_o, tot_o = O.pca_from_gram(gram, colsum, n, 50)
+    t["eig"] = time.perf_counter() - t0
+    ns = min(n, 20_000)
+    Xs = np.zeros((ns, 50), np.float32)
+    t0 = time.perf_counter()
+    for lo, hi, ctx in chunks():
+        if lo >= ns:
+            break
+        a, b = int(ip[lo]), int(ip[min(hi, ns)])
+        c2 = (ix[a:b], dv[a:b], cmap[:ns], gall, 1e4, True, None, ip[lo:min(hi, ns) + 1].copy(), lo)
+        O.project(*c2, gmap, load_o, Xs, True, mean, sd, 10.0)
+    t["project"] = (time.perf_counter() - t0) * n / ns
+    Xg = bf.obsm["X_pca"]
+    q = min(n, 2048)
+    t0 = time.perf_counter()
+    O.knn(Xg, 15, 0, q, threads=threads)
+    t["knn"] = (time.perf_counter() - t0) * n / q
+    total = sum(t.values())
+    # accuracy of the GPU operator path
+    hv_gpu = np.asarray(bf.var["highly_variable"], bool)
+    ev = np.linalg.eigvalsh(O.cov_from_gram(gram, colsum, n)[0])[::-1]
+    rel = (ev[:-1] - ev[1:]) / ev[:-1]
+    r = int(np.argmax(rel[1:48]) + 2)
+    load_g = bf.uns["pca"]["projection"]["loadings"]
+    qs = np.random.default_rng(0).choice(n, 1000, replace=False)
+    ref_i, ref_d2 = O.knn_exact_f64(Xg, 15, qs)
+    bad, ties = O.knn_mismatch_ties(idx[qs], Xg, qs, ref_i, ref_d2)
+    acc = {"hvg_set_identical": bool((hv_gpu == hv).all()), "hvg_count": d,
+           "variance_ratio_err": float(np.max(np.abs(bf.uns["pca"]["variance_ratio"] - var_o / tot_o))),
+           "pca_signal_r": r, "pca_angle_signal": float(O.principal_angle(load_g[:, :r], load_o[:, :r])),
+           "knn_set_agreement": float(1.0 - bad / len(qs)), "knn_mismatch_ties": [int(bad), int(ties)],
+           "queries": len(qs)}
+    log(f"c2 cpu baseline {total:.1f}s ({json.dumps({k: round(v, 2) for k, v in t.items()})}), accuracy {acc}")
+    return {"cpu_baseline": {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port",
+                             "sample": ("C restatement of kernels.rs/knn.rs: gene statistics, HVG and FP64 Gram over "
+                                        f"all {n} cells, numpy eigh, projection of {ns} cells and exact kNN of {q} "
+                                        f"queries x {n} points, both scaled linearly to {n}"),
+                             "stage_s": {k: round(v, 3) for k, v in t.items()}},
+            "accuracy": acc}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +456,11 @@ def main() -> int:
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-only", action="store_true", help="run only the batching configuration (profiling)")
+    ap.add_argument("--c2-cells", type=int, default=100_000,
+                    help="also run configuration 2 (this many cells x --c2-genes, HVG selection, FP32 operator "
+                         "path) with its CPU reference (0: skip)")
+    ap.add_argument("--c2-genes", type=int, default=33_538)
+    ap.add_argument("--c2-only", action="store_true", help="run only configuration 2")
     ap.add_argument("--batch-no-seq", action="store_true", help="skip the sequential comparison of the batch leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB MiB (HBM counter calibration)")
@@ -337,6 +476,9 @@ def main() -> int:
         return knn_only(args)
     if args.batch_only:
         print(json.dumps({"batch_only": True, "batch": batched_leg(args)}), flush=True)
+        return 0
+    if args.c2_only:
+        print(json.dumps({"c2_only": True, "c2": c2_leg(args)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -442,6 +584,8 @@ def main() -> int:
         line["streamed"] = streamed_leg(args)
     if args.batch_subsets:
         line["batch"] = batched_leg(args)
+    if args.c2_cells:
+        line["c2"] = c2_leg(args)
     if not args.no_cpu:
         t0 = time.perf_counter()
         base, acc = cpu_baseline_leg(indptr, indices, data, args.genes, gpu, args)
