@@ -442,7 +442,7 @@ def main() -> int:
     ap.add_argument("--alt", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
                     help="also time one step with this precision for both Gram and screen (reported, not the "
                          "headline; bf16x3 = the all-split-BF16 configuration)")
-    ap.add_argument("--streamed-cells", type=int, default=0, metavar="N",
+    ap.add_argument("--streamed-cells", type=int, default=2_000_000, metavar="N",
                     help="also time the out-of-core form: N x genes NB counts in pinned host memory, streamed in "
                          "chunks of --stream-rows cells (reported under 'streamed', not the headline)")
     ap.add_argument("--stream-rows", type=int, default=32768)

@@ -313,21 +313,18 @@ static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, i
 
 // Eigen-decomposition of the covariance from (gram [ld_g], colsum, n): variance and
 // loadings (device), total (device scalar).
-// The eigensolver.  "topp": rocSOLVER's Householder tridiagonalization (dsytrd) and
-// the top-p kernels of kernels_eig.hip (bisection, inverse iteration, back-transform
-// of the p vectors); "dsyevd": rocSOLVER's full divide-and-conquer eigensolve.  The
-// top-p kernels are latency-bound chains per matrix, so they pay off when several
-// matrices run side by side: by default a batch of n_mat >= 4 uses them and a single
-// matrix uses dsyevd (measured at d = 2,000: 56 vs 51 ms for one matrix, 6.8 vs 10.2
-// ms per matrix for 256).  CREST_EIGENSOLVER=topp or =dsyevd forces one.  dsyevd
-// also runs for orders beyond the top-p kernels' LDS and for any matrix whose
+// The eigensolver.  "topp" (default): rocSOLVER's Householder tridiagonalization
+// (dsytrd) and the top-p kernels of kernels_eig.hip (bisection, inverse iteration,
+// back-transform of the p vectors); "dsyevd": rocSOLVER's full divide-and-conquer
+// eigensolve (measured at d = 2,000, p = 50: 49 vs 51 ms for one matrix, 6.7 vs 10.2
+// ms per matrix for 256 side by side).  CREST_EIGENSOLVER=dsyevd selects dsyevd; it
+// also runs for orders beyond the top-p kernels' limits and for any matrix whose
 // inverse iteration did not converge.
 static bool use_topp(int d, int k, int n_mat) {
+    (void)n_mat;
     if (!(k >= 1 && k <= d && tri_topp_supported(d))) return false;
     const char *e = std::getenv("CREST_EIGENSOLVER");
-    if (e && std::string(e) == "dsyevd") return false;
-    if (e && std::string(e) == "topp") return true;
-    return n_mat >= 4;
+    return !(e && std::string(e) == "dsyevd");
 }
 
 static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *dev_colsum, int d, double n, int k,

@@ -11,11 +11,11 @@
 //     LU of the tridiagonal, one workgroup per cluster of close eigenvalues (gap below
 //     1e-3 ||T||): the cluster's vectors iterate together, one lane each with its own
 //     shift (coincident eigenvalues pushed apart by 10 ulp ||T||), and after every
-//     round they are orthogonalised in order (modified Gram-Schmidt) and normalised;
+//     round they are orthogonalised in order (Gram-Schmidt, twice) and normalised;
 //     rounds stop once every residual 1/||x|| has been at rounding level twice.
 //  3. k_tri_back: X = Q Y with Q = H(0) H(1) ... H(n-2), the reflectors of a lower
 //     tridiagonalization (H(i) = I - tau_i v v^T, v = [0..0, 1, A(i+2:n, i)]), applied
-//     to a block of columns held in LDS, last reflector first.
+//     to a block of columns held in registers, last reflector first.
 #include <cmath>
 
 #include "common.hpp"
@@ -36,6 +36,18 @@ __device__ inline double wave_min_f64(double v) {
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+// N wave sums at once, level by level, so the lane exchanges of one level are all in
+// flight together instead of N dependent chains.
+template <int N> __device__ inline void wave_sum_n(double (&v)[N]) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double t[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) t[c] = __shfl_xor(v[c], o, 64);
+#pragma unroll
+        for (int c = 0; c < N; ++c) v[c] += t[c];
+    }
 }
 
 // Number of eigenvalues of T below x (diagonal d, squared off-diagonal e2): the
@@ -151,7 +163,7 @@ constexpr int kTriChunk = 16;
 // Workgroup c computes the vectors of cluster c (if it exists) of matrix blockIdx.y:
 // lane t iterates vector c0 + t of the cluster (64 at a time), all with their own
 // shifts at once, and after every round the cluster's vectors are orthogonalised in
-// order (modified Gram-Schmidt) and normalised.  Per matrix, the LU factors (fac:
+// order (Gram-Schmidt, twice) and normalised.  Per matrix, the LU factors (fac:
 // 3 arrays [n][p]: reciprocal pivots, U's first superdiagonal, multipliers; swp
 // [n][p] bytes: row swaps) and the vectors Y [n][p] are stored row-interleaved, so
 // the lanes' accesses of one step are contiguous.  info[mat] is set when a vector's
@@ -309,14 +321,35 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
             }
             __threadfence_block();
             __syncthreads();
-            // modified Gram-Schmidt over the cluster in order, rows spread over the lanes
+            // Gram-Schmidt over the cluster in order, twice: column t loses its components
+            // along columns c0..t-1, taken 16 at a time so that each lane reads 16
+            // consecutive values of its rows; rows spread over the lanes
             for (int t = g0; t < g1; ++t) {
-                for (int s2 = c0; s2 < t; ++s2) {
-                    double dot = 0.0;
-                    for (int r = lane; r < n; r += 64) dot += Y[(int64_t)r * p + s2] * Y[(int64_t)r * p + t];
-                    dot = wave_sum(dot);
-                    for (int r = lane; r < n; r += 64) Y[(int64_t)r * p + t] -= dot * Y[(int64_t)r * p + s2];
-                }
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int s0 = c0; s0 < t; s0 += 16) {
+                        const int ns = t - s0 < 16 ? t - s0 : 16;
+                        double acc[16];
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+#pragma unroll 4
+                        for (int r = lane; r < n; r += 64) {
+                            const double *row = Y + (int64_t)r * p;
+                            const double xt = row[t];
+#pragma unroll
+                            for (int q = 0; q < 16; ++q)
+                                if (q < ns) acc[q] += row[s0 + q] * xt;
+                        }
+                        wave_sum_n<16>(acc);
+#pragma unroll 4
+                        for (int r = lane; r < n; r += 64) {
+                            double *row = Y + (int64_t)r * p;
+                            double v = row[t];
+#pragma unroll
+                            for (int q = 0; q < 16; ++q)
+                                if (q < ns) v -= acc[q] * row[s0 + q];
+                            row[t] = v;
+                        }
+                    }
                 double nx = 0.0;
                 for (int r = lane; r < n; r += 64) {
                     const double v = Y[(int64_t)r * p + t];
@@ -347,19 +380,18 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
     if (failed && lane == 0) atomicOr(info + mat, 1);
 }
 
-// grid (ceil(p/CB), n_mat), 256 threads; dynamic LDS CB n + 2*4*CB doubles.
+// grid (ceil(p/CB), n_mat), 256 threads.
 // X[:, c] = Q Y[:, c] for the CB columns of this block (Y [n][p] row-major, X column-major).
-// Thread t owns rows t + 256 u (u < RT, n <= 256 RT); per reflector the CB dot products
-// are reduced over the four waves through a double-buffered LDS slot (one barrier per
-// reflector), and the thread's entries of the next reflector are loaded into
-// registers while the current one is applied.
+// Thread t owns rows t + 256 u (u < RT, n <= 256 RT) of the CB columns, in registers;
+// per reflector the CB dot products are reduced over the four waves through a
+// double-buffered LDS slot (one barrier per reflector, which waits for the LDS writes
+// only, so loads in flight stay in flight), and the thread's entries of the
+// reflectors two steps ahead are loaded while the current one is applied.
 template <int CB, int RT>
 __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, int lda, const double *tau,
                                                   int64_t sTau, int n, int p, const double *Y, int64_t sY, double *X,
                                                   int64_t sX) {
-    extern __shared__ double sm[];
-    double *y = sm;                     // [CB][n]
-    double *red = sm + (int64_t)CB * n; // [2][4][CB]
+    __shared__ double red[2][4][CB];
     const int mat = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     A += mat * sA;
     tau += mat * sTau;
@@ -367,8 +399,14 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
     X += mat * sX;
     const int cb0 = blockIdx.x * CB;
     const int ncol = p - cb0 < CB ? p - cb0 : CB;
+    double y[CB][RT];
+#pragma unroll
     for (int c = 0; c < CB; ++c)
-        for (int r = tid; r < n; r += 256) y[c * n + r] = c < ncol ? Y[(int64_t)r * p + cb0 + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            const int r = tid + 256 * u;
+            y[c][u] = (r < n && c < ncol) ? Y[(int64_t)r * p + cb0 + c] : 0.0;
+        }
     // v entries of reflector i for the thread's rows: 1 at row i+1, A(r, i) below, 0 above
     auto load_v = [&](int i, double (&v)[RT], double &t) {
         const double *col = A + (int64_t)i * lda;
@@ -379,66 +417,61 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
         }
         t = tau[i];
     };
-    double vc[RT], vn[RT], tc = 0.0, tn = 0.0;
+    double vc[RT], vn[RT], vnn[RT], tc = 0.0, tn = 0.0, tnn = 0.0;
     if (n >= 2) load_v(n - 2, vc, tc);
-    // each thread touches only its own rows of y, so no barrier is needed before the loop
+    if (n >= 3) load_v(n - 3, vn, tn);
     for (int i = n - 2; i >= 0; --i) {
-        if (i > 0) load_v(i - 1, vn, tn);
+        if (i > 1) load_v(i - 2, vnn, tnn);
+        double *slot = &red[i & 1][0][0];
         double part[CB];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) part[c] = 0.0;
+        for (int c = 0; c < CB; ++c) {
+            double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-        for (int u = 0; u < RT; ++u) {
-            const int r = tid + 256 * u;
-            if (r < n && r > i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) part[c] += vc[u] * y[c * n + r];
+            for (int u = 0; u < RT; u += 2) {
+                a0 += vc[u] * y[c][u];
+                if (u + 1 < RT) a1 += vc[u + 1] * y[c][u + 1];
+            }
+            part[c] = a0 + a1;
         }
-        double *slot = red + (i & 1) * 4 * CB;
+        wave_sum_n<CB>(part);
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) slot[wave * CB + c] = part[c];
+        // s_waitcnt lgkmcnt(0) (vmcnt and expcnt left at their maxima), then the barrier
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_s_barrier();
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-            const double s = wave_sum(part[c]);
-            if (lane == 0) slot[wave * CB + c] = s;
-        }
-        __syncthreads();
-        double w[CB];
+            const double w = tc * (((slot[c] + slot[CB + c]) + slot[2 * CB + c]) + slot[3 * CB + c]);
 #pragma unroll
-        for (int c = 0; c < CB; ++c) w[c] = tc * (((slot[c] + slot[CB + c]) + slot[2 * CB + c]) + slot[3 * CB + c]);
+            for (int u = 0; u < RT; ++u) y[c][u] -= w * vc[u];
+        }
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            vc[u] = vn[u];
+            vn[u] = vnn[u];
+        }
+        tc = tn;
+        tn = tnn;
+    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
 #pragma unroll
         for (int u = 0; u < RT; ++u) {
             const int r = tid + 256 * u;
-            if (r < n && r > i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) y[c * n + r] -= w[c] * vc[u];
+            if (r < n && c < ncol) X[(int64_t)(cb0 + c) * n + r] = y[c][u];
         }
-#pragma unroll
-        for (int u = 0; u < RT; ++u) vc[u] = vn[u];
-        tc = tn;
-    }
-    __syncthreads();
-    for (int c = 0; c < ncol; ++c)
-        for (int r = tid; r < n; r += 256) X[(int64_t)(cb0 + c) * n + r] = y[c * n + r];
 }
 
 // Scratch of launch_tri_topp per matrix: 3 n p doubles (fac) and n p bytes (swp).
 // LDS limits of the three kernels for order n (0 when the order is too large).
 size_t tri_vectors_lds(int n) { return (size_t)2 * n * sizeof(double); }
-int tri_back_cb(int n) {
-    const size_t budget = 150 * 1024;
-    for (int cb : {8, 4, 2, 1})
-        if ((size_t)cb * n * sizeof(double) + 64 * sizeof(double) <= budget) return cb;
-    return 0;
-}
-bool tri_topp_supported(int n) { return n <= 4096 && tri_vectors_lds(n) <= 150 * 1024 && tri_back_cb(n) > 0; }
+bool tri_topp_supported(int n) { return n <= 4096 && tri_vectors_lds(n) <= 150 * 1024; }
 
 void tri_set_attrs() {
     hipFuncSetAttribute((const void *)k_tri_bisect, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
     hipFuncSetAttribute((const void *)k_tri_vectors, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    for (const void *f : {(const void *)k_tri_back<8, 8>, (const void *)k_tri_back<4, 8>, (const void *)k_tri_back<2, 8>,
-                          (const void *)k_tri_back<1, 8>, (const void *)k_tri_back<8, 16>,
-                          (const void *)k_tri_back<4, 16>, (const void *)k_tri_back<2, 16>,
-                          (const void *)k_tri_back<1, 16>})
-        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
 }
 
 // Top-p eigenpairs of n_mat tridiagonalised matrices: A (reflectors, lower, lda,
@@ -453,27 +486,13 @@ void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, in
                        sE, n, p, W, sW);
     hipLaunchKernelGGL(k_tri_vectors, dim3(p, n_mat), dim3(64), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
                        (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
-    const int cb = tri_back_cb(n);
-    const size_t lds = ((size_t)cb * n + 8 * cb) * sizeof(double);
-    const dim3 g((p + cb - 1) / cb, n_mat);
-#define CREST_BACK(CBV, RTV)                                                                                  \
-    hipLaunchKernelGGL((k_tri_back<CBV, RTV>), g, dim3(256), lds, s, A, sA, lda, tau, sTau, n, p, Y, sY, X, sY)
-    if (n <= 2048) {
-        switch (cb) {
-        case 8: CREST_BACK(8, 8); break;
-        case 4: CREST_BACK(4, 8); break;
-        case 2: CREST_BACK(2, 8); break;
-        default: CREST_BACK(1, 8); break;
-        }
-    } else {
-        switch (cb) {
-        case 8: CREST_BACK(8, 16); break;
-        case 4: CREST_BACK(4, 16); break;
-        case 2: CREST_BACK(2, 16); break;
-        default: CREST_BACK(1, 16); break;
-        }
-    }
-#undef CREST_BACK
+    // columns per block: the thread's rows of CB columns stay in registers (CB * RT <= 64)
+    if (n <= 2048)
+        hipLaunchKernelGGL((k_tri_back<8, 8>), dim3((p + 7) / 8, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
+                           p, Y, sY, X, sY);
+    else
+        hipLaunchKernelGGL((k_tri_back<4, 16>), dim3((p + 3) / 4, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
+                           p, Y, sY, X, sY);
 }
 
 }  // namespace crest
