@@ -171,9 +171,11 @@ __device__ inline float bf16_to_float(uint16_t b) { return __uint_as_float((uint
 
 template <int FMT> struct TileElem;
 template <> struct TileElem<TILE_F64> { typedef double T; static constexpr int DW = 1024; };
-template <> struct TileElem<TILE_F32> { typedef float T; static constexpr int DW = 2048; };
-template <> struct TileElem<TILE_BF16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
-template <> struct TileElem<TILE_F16X2> { typedef uint16_t T; static constexpr int DW = 2048; };
+// Column window per pass: small enough that three workgroups fit a CU's LDS, so
+// enough tile stores are in flight (the tile is written at close to HBM speed).
+template <> struct TileElem<TILE_F32> { typedef float T; static constexpr int DW = 1024; };
+template <> struct TileElem<TILE_BF16X2> { typedef uint16_t T; static constexpr int DW = 1024; };
+template <> struct TileElem<TILE_F16X2> { typedef uint16_t T; static constexpr int DW = 1024; };
 
 // With `ranges`, workgroup b densifies row groups [r0/8, r1/8) of segment
 // segs[ranges[b].seg] into tile groups grp0 + g, with that subset's scale parameters
@@ -293,14 +295,14 @@ void launch_densify(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a
         break;
     case TILE_F32:
         lds = TileElem<TILE_F32>::DW * (sizeof(double) + 8 * sizeof(float));
-        g = (int)std::min<int64_t>(n_groups, 1024);
+        g = (int)std::min<int64_t>(n_groups, 2048);
         hipFuncSetAttribute((const void *)k_densify<TILE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_densify<TILE_F32>, dim3(g), dim3(256), lds, s, c, sc, tile_a, tile_b, n_groups, d_pad, colsum,
                            gram, (const SegDesc *)nullptr, (const RowRange *)nullptr);
         break;
     default:
         lds = TileElem<TILE_BF16X2>::DW * (2 * sizeof(double) + 16 * sizeof(uint16_t));
-        g = (int)std::min<int64_t>(n_groups, 1024);
+        g = (int)std::min<int64_t>(n_groups, 2048);
         hipFuncSetAttribute((const void *)k_densify<TILE_BF16X2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (fmt == TILE_F16X2) {
             hipFuncSetAttribute((const void *)k_densify<TILE_F16X2>, hipFuncAttributeMaxDynamicSharedMemorySize,
