@@ -220,14 +220,15 @@ def batched_leg(args) -> dict:
     hvg = np.arange(args.genes, dtype=np.int32)
     csr = DeviceCSR.upload(ip, ix, dv, args.genes)
     kw = dict(n_comps=args.batch_pcs, k=args.k, prec_gram=args.gram_prec, prec_knn=args.knn_prec)
+    bkw = dict(eig_batch=args.batch_eig)
     # one eager run for the per-stage times
-    pe = BatchPipeline(csr, off, hvg, **kw)
+    pe = BatchPipeline(csr, off, hvg, **kw, **bkw)
     oe = pe.run()
     oe = pe.run()
     log(f"batch eager: {oe.times_ms['total']:.1f} ms, stages "
         f"{json.dumps({k: round(v, 2) for k, v in oe.times_ms.items()})}, gram_rows {oe.gram_rows}")
     del pe
-    pg = BatchPipeline(csr, off, hvg, use_graph=True, **kw)
+    pg = BatchPipeline(csr, off, hvg, use_graph=True, **kw, **bkw)
     pg.run()  # captures the graphs (and tunes, if this shape is new)
     check(lib().crest_synchronize())
     t0 = time.perf_counter()
@@ -455,6 +456,7 @@ def main() -> int:
     ap.add_argument("--batch-pcs", type=int, default=30)
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-eig", type=int, default=0, help="subsets per batched eigensolve call (0: library default)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batching configuration (profiling)")
     ap.add_argument("--c2-cells", type=int, default=100_000,
                     help="also run configuration 2 (this many cells x --c2-genes, HVG selection, FP32 operator "

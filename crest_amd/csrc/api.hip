@@ -1007,52 +1007,130 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 }
 // Covariance, batched FP64 eigensolve and selection, eig_batch subsets per call
 // (rocSOLVER: eager, never captured).
-static void batch_eig(Ctx &x, const BatchPlan &P, const BatchBufs &B) {
+//
+// Subsets with fewer cells than genes have a covariance of rank below n; with the
+// top-p solver, a subset whose padded cell count m (256, 512, 1,024 or 1,536) is below
+// d takes the n x n dual instead: Zc = its centred scaled rows in FP64 (m x d, rows
+// beyond n zero), K = Zc Zc^T (rocBLAS dgemm), the top p eigenpairs (lambda, u) of K,
+// then C's eigenpairs (lambda / (n - 1), Zc^T u normalised) — the same nonzero
+// eigenpairs as C = Zc^T Zc / (n - 1) at O(m^3) instead of O(d^3).  The trace comes
+// from the Gram diagonal with the covariance's own arithmetic.  CREST_BATCH_DUAL=0
+// keeps every subset on the d x d covariance.
+static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
     const bool topp = use_topp(P.d, P.kp, (int)std::min<int64_t>(P.eig_batch, P.n_sub));
     if (topp) tri_set_attrs();
-    for (int g0 = 0; g0 < P.n_sub; g0 += (int)P.eig_batch) {
-        const int ng = (int)std::min<int64_t>(P.eig_batch, P.n_sub - g0);
-        launch_cov_batch(B.gram + (int64_t)g0 * P.d_pad * P.d_pad, B.colsum + (int64_t)g0 * P.d_pad, P.d, P.d_pad,
-                         P.d_off + g0, ng, B.cov, B.tot + g0, x.st);
+    const char *dual_env = std::getenv("CREST_BATCH_DUAL");
+    // the dense centred rows are staged in LDS, four rows of d doubles per workgroup
+    const bool dual_on = topp && P.d <= 4096 && !(dual_env && std::string(dual_env) == "0");
+    static const int kBuckets[] = {256, 512, 1024, 1536};
+    std::vector<int32_t> primal;
+    std::vector<int32_t> dual[4];
+    for (int y = 0; y < P.n_sub; ++y) {
+        const int64_t ny = P.off[y + 1] - P.off[y];
+        int bk = -1;
+        if (dual_on)
+            for (int i = 0; i < 4; ++i)
+                if (ny <= kBuckets[i] && kBuckets[i] < P.d && P.kp <= kBuckets[i] && tri_topp_supported(kBuckets[i])) {
+                    bk = i;
+                    break;
+                }
+        if (bk < 0)
+            primal.push_back(y);
+        else
+            dual[bk].push_back(y);
+    }
+    int32_t *ids_dev = x.get<int32_t>(P.pre + "eig_ids", (size_t)P.n_sub);
+    std::vector<int32_t> all_ids;  // primal ids, then each bucket's, in one upload
+    all_ids.insert(all_ids.end(), primal.begin(), primal.end());
+    for (auto &v : dual) all_ids.insert(all_ids.end(), v.begin(), v.end());
+    HIPCHK(hipMemcpyAsync(ids_dev, all_ids.data(), all_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, x.st));
+    const int64_t dd = (int64_t)P.d * P.d;
+    if (topp) launch_fill_u32(reinterpret_cast<uint32_t *>(B.info), P.n_sub, 0u, x.st);  // a fallback sets its own
+    // a subset whose inverse iteration did not converge: full eigensolve of its covariance
+    auto fallback = [&](int y) {
+        launch_cov_batch(B.gram + (int64_t)y * P.d_pad * P.d_pad, B.colsum + (int64_t)y * P.d_pad, P.d, P.d_pad,
+                         P.d_off + y, 1, B.cov, B.tot + y, x.st);
+        BLASCHK(rocsolver_dsyevd(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov, P.d, B.w, B.e,
+                                 B.info + y));
+        launch_eig_select_batch(B.cov, B.w, P.d, P.kp, 1, B.var + (int64_t)y * P.kp, B.load + (int64_t)y * P.d * P.kp,
+                                x.st);
+        x.check_launch();
+    };
+    // ---- the d x d covariances
+    for (int g0 = 0; g0 < (int)primal.size(); g0 += (int)P.eig_batch) {
+        const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)primal.size() - g0);
+        const int32_t *ids = ids_dev + g0;
+        launch_cov_batch(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ng, B.cov, B.tot, x.st, ids);
         x.check_launch();
         if (topp) {
-            const int64_t dd = (int64_t)P.d * P.d;
             BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, P.d, B.cov, P.d, dd, B.w, P.d, B.e,
                                                      P.d, B.tau, P.d, ng));
             launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
-            launch_fill_u32(reinterpret_cast<uint32_t *>(B.info + g0), ng, 0u, x.st);
             launch_tri_topp(B.cov, dd, P.d, B.tau, P.d, B.w, P.d, B.e, P.d, P.d, P.kp, ng, B.topw, B.y, B.x, B.fac,
                             B.swp, B.tinfo, x.st);
-            launch_eig_select_batch(B.x, B.topw, P.d, P.kp, ng, B.var + (int64_t)g0 * P.kp,
-                                    B.load + (int64_t)g0 * P.d * P.kp, x.st, P.kp);
+            launch_eig_select_batch(B.x, B.topw, P.d, P.kp, ng, B.var, B.load, x.st, P.kp, ids);
             x.check_launch();
             std::vector<int> hi(ng);
             x.download(hi.data(), B.tinfo, ng);
             x.sync();
-            for (int i = 0; i < ng; ++i) {
-                if (!hi[i]) continue;
-                // inverse iteration did not converge for this subset: full eigensolve
-                const int y = g0 + i;
-                launch_cov_batch(B.gram + (int64_t)y * P.d_pad * P.d_pad, B.colsum + (int64_t)y * P.d_pad, P.d,
-                                 P.d_pad, P.d_off + y, 1, B.cov, B.tot + y, x.st);
-                BLASCHK(rocsolver_dsyevd(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov, P.d, B.w,
-                                         B.e, B.info + y));
-                launch_eig_select_batch(B.cov, B.w, P.d, P.kp, 1, B.var + (int64_t)y * P.kp,
-                                        B.load + (int64_t)y * P.d * P.kp, x.st);
-                x.check_launch();
-            }
+            for (int i = 0; i < ng; ++i)
+                if (hi[i]) fallback(primal[g0 + i]);
             continue;
         }
+        // without the top-p solver no subset takes the dual, so primal[g0 + i] = g0 + i
         if (ng == 1)
             BLASCHK(rocsolver_dsyevd(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov, P.d, B.w, B.e,
                                      B.info + g0));
         else
             BLASCHK(rocsolver_dsyevd_strided_batched(x.blas, rocblas_evect_original, rocblas_fill_upper, P.d, B.cov,
-                                                     P.d, (rocblas_stride)P.d * P.d, B.w, P.d, B.e, P.d, B.info + g0,
-                                                     ng));
-        launch_eig_select_batch(B.cov, B.w, P.d, P.kp, ng, B.var + (int64_t)g0 * P.kp,
-                                B.load + (int64_t)g0 * P.d * P.kp, x.st);
+                                                     P.d, dd, B.w, P.d, B.e, P.d, B.info + g0, ng));
+        launch_eig_select_batch(B.cov, B.w, P.d, P.kp, ng, B.var, B.load, x.st, 0, ids);
         x.check_launch();
+    }
+    // ---- the n x n duals, bucket by bucket
+    int64_t base = (int64_t)primal.size();
+    const DevChunk c = batch_chunk(b, P, B);
+    const DevScale sc = batch_scale(b, P, B);
+    for (int bi = 0; bi < 4; ++bi) {
+        const int m = kBuckets[bi];
+        const std::vector<int32_t> &L = dual[bi];
+        for (int g0 = 0; g0 < (int)L.size(); g0 += (int)P.eig_batch) {
+            const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)L.size() - g0);
+            const int32_t *ids = ids_dev + base + g0;
+            std::vector<SegDesc> segs(ng);
+            for (int i = 0; i < ng; ++i) {
+                const int y = L[g0 + i];
+                segs[i] = SegDesc{P.off[y], P.off[y + 1] - P.off[y], 0, y, 0};
+            }
+            SegDesc *segs_dev = x.get<SegDesc>(P.pre + "dual_segs", (size_t)ng);
+            HIPCHK(hipMemcpyAsync(segs_dev, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, x.st));
+            double *zc = x.get<double>(P.pre + "dual_zc", (size_t)ng * m * P.d);
+            double *kv = B.cov;  // K [ng][m x m], then V [ng][d x p] (K is dead by then)
+            double *ev = x.get<double>(P.pre + "dual_ev", (size_t)ng * P.kp);
+            const int64_t mm = (int64_t)m * m, md = (int64_t)m * P.d;
+            launch_dense_centred(c, sc, segs_dev, ng, m, B.colsum, P.d_pad, zc, x.st);
+            const double one = 1.0, zero = 0.0;
+            BLASCHK(rocblas_dgemm_strided_batched(x.blas, rocblas_operation_transpose, rocblas_operation_none, m, m, P.d,
+                                                  &one, zc, P.d, md, zc, P.d, md, &zero, kv, m, mm, ng));
+            BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, m, kv, m, mm, B.w, m, B.e, m, B.tau,
+                                                     m, ng));
+            launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
+            launch_tri_topp(kv, mm, m, B.tau, m, B.w, m, B.e, m, m, P.kp, ng, B.topw, B.y, B.x, B.fac, B.swp, B.tinfo,
+                            x.st);
+            BLASCHK(rocblas_dgemm_strided_batched(x.blas, rocblas_operation_none, rocblas_operation_none, P.d, P.kp, m,
+                                                  &one, zc, P.d, md, B.x, m, (int64_t)m * P.kp, &zero, kv, P.d,
+                                                  (int64_t)P.d * P.kp, ng));
+            launch_dual_finish(kv, B.topw, P.d, P.kp, P.d_off, ids, ng, ev, x.st);
+            launch_eig_select_batch(kv, ev, P.d, P.kp, ng, B.var, B.load, x.st, P.kp, ids);
+            launch_trace_gram(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ids, ng, B.tot, x.st);
+            x.check_launch();
+            std::vector<int> hi(ng);
+            x.download(hi.data(), B.tinfo, ng);
+            x.sync();
+            for (int i = 0; i < ng; ++i)
+                if (hi[i]) fallback(L[g0 + i]);
+        }
+        base += (int64_t)L.size();
     }
 }
 
@@ -1156,7 +1234,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
     }
     P.n_pad = P.row0[P.n_sub];
     P.tile_rows = b->tile_rows ? round_up((int64_t)b->tile_rows, 16) : 524288;
-    P.eig_batch = b->eig_batch ? (int64_t)b->eig_batch : 64;
+    P.eig_batch = b->eig_batch ? (int64_t)b->eig_batch : 256;
     int n_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, x.dev));
     P.grid = (int)round_up((int64_t)n_cu * gram_items_occupancy(P.fmt), 8);
@@ -1241,7 +1319,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
         HIPCHK(hipEventRecord(ev[0], x.st));
         HIPCHK(hipGraphLaunch(x.graph[0], x.st));
         HIPCHK(hipEventRecord(ev[1], x.st));
-        batch_eig(x, P, B);
+        batch_eig(x, b, P, B);
         HIPCHK(hipEventRecord(ev[2], x.st));
         HIPCHK(hipGraphLaunch(x.graph[1], x.st));
         HIPCHK(hipEventRecord(ev[3], x.st));
@@ -1261,7 +1339,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
         HIPCHK(hipEventRecord(ev[1], x.st));
         batch_gram(x, b, P, B);
         HIPCHK(hipEventRecord(ev[2], x.st));
-        batch_eig(x, P, B);
+        batch_eig(x, b, P, B);
         HIPCHK(hipEventRecord(ev[3], x.st));
         batch_project(x, b, P, B);
         HIPCHK(hipEventRecord(ev[4], x.st));

@@ -134,9 +134,17 @@ int gram_items_occupancy(int fmt);
 void launch_gram_reduce_items(const double *partial, const RedItem *items, int64_t n_items, int d_pad,
                               double *gram, hipStream_t s);
 void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
-                      int n_sub, double *cov, double *total, hipStream_t s);
+                      int n_sub, double *cov, double *total, hipStream_t s, const int32_t *ids = nullptr);
 void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
-                             double *loadings, hipStream_t s, int ncols = 0);
+                             double *loadings, hipStream_t s, int ncols = 0, const int32_t *ids = nullptr);
+// The n x n dual of small subsets: dense centred scaled rows, the covariance trace from
+// the Gram diagonal, and the eigenpairs of C from those of Zc Zc^T.
+void launch_dense_centred(const DevChunk &c, const DevScale &sc, const SegDesc *segs, int n_seg, int m,
+                          const double *colsum, int ld_c, double *out, hipStream_t s);
+void launch_trace_gram(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
+                       const int32_t *ids, int n_ids, double *total, hipStream_t s);
+void launch_dual_finish(double *V, const double *W, int d, int p, const int64_t *dev_cell_off, const int32_t *ids,
+                        int n_ids, double *evals, hipStream_t s);
 void launch_shift_fill_batch(const double *colsum, int ld_c, const double *V, int d, int k,
                              const int64_t *dev_cell_off, int n_sub, int64_t max_rows, float *x, float *shift_f,
                              hipStream_t s);

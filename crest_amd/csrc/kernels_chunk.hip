@@ -358,6 +358,56 @@ void launch_densify_ranges(const DevChunk &c, const DevScale &sc, int fmt, void 
     }
 }
 
+// ---------------------------------------------------------------- dense centred rows
+// Row r < m of segment e = blockIdx.y (one subset's cells): the FP64 scaled values of
+// the cell minus the subset's column means colsum / n, every HVG column written (rows
+// n..m-1 zero).  One wave per row, the row staged in LDS (zeros become -mean).
+__global__ __launch_bounds__(256) void k_dense_centred(DevChunk c, DevScale sc, const SegDesc *segs, int m,
+                                                       const double *colsum, int ld_c, double *out) {
+    extern __shared__ double rowbuf[];  // [4][d]
+    const int e = blockIdx.y, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const SegDesc sd = segs[e];
+    const int d = sc.d;
+    const int64_t r = (int64_t)blockIdx.x * 4 + wave;
+    if (r >= m) return;
+    double *row = out + ((int64_t)e * m + r) * d;
+    if (r >= sd.n) {
+        for (int h = lane; h < d; h += kWave) row[h] = 0.0;
+        return;
+    }
+    double *buf = rowbuf + (int64_t)wave * d;
+    const double *cs = colsum + (int64_t)sd.subset * ld_c;
+    const double n = (double)sd.n;
+    for (int h = lane; h < d; h += kWave) buf[h] = -(cs[h] / n);
+    __builtin_amdgcn_wave_barrier();
+    const DevChunk cc = seg_rows(c, sd.cell0 + r, sd.cell0 + r + 1);
+    if (chunk_row(cc, 0) >= 0) {
+        const int64_t lo = cc.span_off[0] - cc.base, hi = cc.span_off[1] - cc.base;
+        const double ns = norm_factor(cell_total(cc, lo, hi, lane), cc.target_sum);
+        const double *mean = sc.mean + (int64_t)sd.subset * d, *sdv = sc.sd + (int64_t)sd.subset * d;
+        for (int64_t q = lo + lane; q < hi; q += kWave) {
+            const int32_t v = chunk_var(cc, cc.genes[q]);
+            if (v < 0) continue;
+            const int32_t h = sc.sub[v];
+            if (h < 0) continue;
+            const float x = lazy_transform(cc.values[q], ns, cc.target_sum, cc.log1p);
+            const double s = sc.scale ? scale_value(x, mean[h], sdv[h], sc.max_value, 1) : (double)x;
+            buf[h] = s - cs[h] / n;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int h = lane; h < d; h += kWave) row[h] = buf[h];
+}
+
+void launch_dense_centred(const DevChunk &c, const DevScale &sc, const SegDesc *segs, int n_seg, int m,
+                          const double *colsum, int ld_c, double *out, hipStream_t s) {
+    if (n_seg <= 0) return;
+    const size_t lds = (size_t)4 * sc.d * sizeof(double);
+    hipFuncSetAttribute((const void *)k_dense_centred, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_dense_centred, dim3((unsigned)ceil_div(m, 4), n_seg), dim3(256), lds, s, c, sc, segs, m,
+                       colsum, ld_c, out);
+}
+
 // ---------------------------------------------------------------- project
 // kernels.rs:350-371: per kept cell, acc[j] = Σ_e s_e * V[h_e, j] in FP64 over the
 // cell's HVG entries in chunk order (multiply and add rounded separately, as the

@@ -494,12 +494,12 @@ void launch_gram_reduce(const double *partial, int n_split, int d_pad, double *g
 // blockIdx.y = subset of a batch (gram + y ld_g^2, colsum + y ld_g, cov + y d^2), with
 // n = cell_off[y+1] - cell_off[y] when cell_off is given.
 __global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov,
-                      const int64_t *cell_off) {
-    const int y = blockIdx.y;
-    gram += (int64_t)y * ld_g * ld_g;
-    colsum += (int64_t)y * ld_g;
+                      const int64_t *cell_off, const int32_t *ids) {
+    const int y = blockIdx.y, sub = ids ? ids[y] : y;  // ids: the subsets of this group
+    gram += (int64_t)sub * ld_g * ld_g;
+    colsum += (int64_t)sub * ld_g;
     cov += (int64_t)y * d * d;
-    if (cell_off) n = (double)(cell_off[y + 1] - cell_off[y]);
+    if (cell_off) n = (double)(cell_off[sub + 1] - cell_off[sub]);
     const int64_t total = (int64_t)d * d;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(e / d), i = (int)(e % d);
@@ -512,10 +512,10 @@ __global__ void k_cov(const double *gram, const double *colsum, int d, int ld_g,
 }
 // total = Σ_i cov[i, i] in index order (kernels.rs:323): the strided diagonal is
 // gathered into LDS by the whole block, then summed sequentially by one thread.
-__global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double *total) {
+__global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double *total, const int32_t *ids) {
     __shared__ double diag[4096];
     cov += (int64_t)blockIdx.x * d * d;  // one block per subset of a batch
-    total += blockIdx.x;
+    total += ids ? ids[blockIdx.x] : blockIdx.x;
     double s = 0.0;
     for (int i0 = 0; i0 < d; i0 += 4096) {
         const int m = d - i0 < 4096 ? d - i0 : 4096;
@@ -531,15 +531,79 @@ __global__ __launch_bounds__(256) void k_trace(const double *cov, int d, double 
 void launch_cov(const double *gram, const double *colsum, int d, int ld_g, double n, double *cov, double *total,
                 hipStream_t s) {
     const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 8192);
-    hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov, (const int64_t *)nullptr);
-    hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, cov, d, total);
+    hipLaunchKernelGGL(k_cov, dim3(g), dim3(256), 0, s, gram, colsum, d, ld_g, n, cov, (const int64_t *)nullptr,
+                       (const int32_t *)nullptr);
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, cov, d, total, (const int32_t *)nullptr);
 }
 
 void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
-                      int n_sub, double *cov, double *total, hipStream_t s) {
+                      int n_sub, double *cov, double *total, hipStream_t s, const int32_t *ids) {
     const int g = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 2048);
-    hipLaunchKernelGGL(k_cov, dim3(g, n_sub), dim3(256), 0, s, gram, colsum, d, ld_g, 0.0, cov, dev_cell_off);
-    hipLaunchKernelGGL(k_trace, dim3(n_sub), dim3(256), 0, s, cov, d, total);
+    hipLaunchKernelGGL(k_cov, dim3(g, n_sub), dim3(256), 0, s, gram, colsum, d, ld_g, 0.0, cov, dev_cell_off, ids);
+    hipLaunchKernelGGL(k_trace, dim3(n_sub), dim3(256), 0, s, cov, d, total, ids);
+}
+
+// total = trace of the covariance, straight from the Gram diagonal and the column sums
+// with k_cov's operations and k_trace's order (for subsets whose covariance is never
+// formed): one block per subset ids[b].
+__global__ __launch_bounds__(256) void k_trace_gram(const double *gram, const double *colsum, int d, int ld_g,
+                                                    const int64_t *cell_off, const int32_t *ids, double *total) {
+    __shared__ double diag[4096];
+    const int sub = ids[blockIdx.x];
+    gram += (int64_t)sub * ld_g * ld_g;
+    colsum += (int64_t)sub * ld_g;
+    const double n = (double)(cell_off[sub + 1] - cell_off[sub]);
+    double s = 0.0;
+    for (int i0 = 0; i0 < d; i0 += 4096) {
+        const int m = d - i0 < 4096 ? d - i0 : 4096;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            const int a = i0 + i;
+            const double cc = colsum[a] * colsum[a];
+            const double t = cc / n;
+            const double u = gram[(int64_t)a * ld_g + a] - t;
+            diag[i] = u / (n - 1.0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; ++i) s += diag[i];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[sub] = s;
+}
+void launch_trace_gram(const double *gram, const double *colsum, int d, int ld_g, const int64_t *dev_cell_off,
+                       const int32_t *ids, int n_ids, double *total, hipStream_t s) {
+    if (n_ids <= 0) return;
+    hipLaunchKernelGGL(k_trace_gram, dim3(n_ids), dim3(256), 0, s, gram, colsum, d, ld_g, dev_cell_off, ids, total);
+}
+
+// Eigenpairs of C = Zc^T Zc / (n - 1) from those of K = Zc Zc^T: column j of V (d x p,
+// = Zc^T U) normalised, eigenvalue W[j] / (n - 1).  One block per (column, subset
+// ids[y]); zero columns stay zero.
+__global__ __launch_bounds__(256) void k_dual_finish(double *V, const double *W, int d, int p,
+                                                     const int64_t *cell_off, const int32_t *ids, double *evals) {
+    __shared__ double red[256];
+    const int j = blockIdx.x, y = blockIdx.y, sub = ids[y];
+    double *v = V + ((int64_t)y * p + j) * d;
+    double ss = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) ss += v[i] * v[i];
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double nrm = sqrt(red[0]);
+    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) v[i] *= inv;
+    if (threadIdx.x == 0) {
+        const double n = (double)(cell_off[sub + 1] - cell_off[sub]);
+        evals[(int64_t)y * p + j] = W[(int64_t)y * p + j] / (n - 1.0);
+    }
+}
+void launch_dual_finish(double *V, const double *W, int d, int p, const int64_t *dev_cell_off, const int32_t *ids,
+                        int n_ids, double *evals, hipStream_t s) {
+    if (n_ids <= 0 || p <= 0) return;
+    hipLaunchKernelGGL(k_dual_finish, dim3(p, n_ids), dim3(256), 0, s, V, W, d, p, dev_cell_off, ids, evals);
 }
 
 // ---------------------------------------------------------------- eigen selection
@@ -548,14 +612,14 @@ void launch_cov_batch(const double *gram, const double *colsum, int d, int ld_g,
 // floored at 0; the sign makes the first largest-|loading| entry positive
 // (kernels.rs:328-346).  evecs is column-major (column j = eigenvector j).
 __global__ __launch_bounds__(256) void k_eig_select(const double *evecs, const double *evals, int d, int k,
-                                                    double *variance, double *loadings, int ncols) {
+                                                    double *variance, double *loadings, int ncols, const int32_t *ids) {
     __shared__ double sbest[256];
     __shared__ int sidx[256];
-    const int y = blockIdx.y;  // subset of a batch
+    const int y = blockIdx.y, sub = ids ? ids[y] : y;  // subset of a batch
     evecs += (int64_t)y * d * ncols;
     evals += (int64_t)y * ncols;
-    variance += (int64_t)y * k;
-    loadings += (int64_t)y * d * k;
+    variance += (int64_t)sub * k;
+    loadings += (int64_t)sub * d * k;
     const int c = blockIdx.x;
     const int src = ncols - 1 - c;
     const double *u = evecs + (int64_t)src * d;
@@ -597,13 +661,13 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
                        hipStream_t s, int ncols) {
     if (k <= 0) return;
     hipLaunchKernelGGL(k_eig_select, dim3(k), dim3(256), 0, s, evecs, evals, d, k, variance, loadings,
-                       ncols > 0 ? ncols : d);
+                       ncols > 0 ? ncols : d, (const int32_t *)nullptr);
 }
 void launch_eig_select_batch(const double *evecs, const double *evals, int d, int k, int n_sub, double *variance,
-                             double *loadings, hipStream_t s, int ncols) {
+                             double *loadings, hipStream_t s, int ncols, const int32_t *ids) {
     if (k <= 0 || n_sub <= 0) return;
     hipLaunchKernelGGL(k_eig_select, dim3(k, n_sub), dim3(256), 0, s, evecs, evals, d, k, variance, loadings,
-                       ncols > 0 ? ncols : d);
+                       ncols > 0 ? ncols : d, ids);
 }
 
 // ---------------------------------------------------------------- shift prefill

@@ -241,7 +241,7 @@ typedef struct crest_batch_args {
     const int64_t *subset_off; /* host [n_subsets+1], non-decreasing, 0 .. n_cells      */
     size_t tile_rows;   /* rows of one densified batch tile (0 = 524,288)                 */
     size_t gram_rows;   /* largest row range of one Gram work item (0 = autotuned)        */
-    size_t eig_batch;   /* subsets per batched eigensolve call (0 = 64)                   */
+    size_t eig_batch;   /* subsets per batched eigensolve call (0 = 256)                  */
     int32_t use_graph;  /* capture the launches before and after the eigensolve into two
                            HIP graphs, replayed by later calls with identical arguments  */
 } crest_batch_args;

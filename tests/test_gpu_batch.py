@@ -69,28 +69,36 @@ def test_batch_equals_single_runs(batch, oracle, prec_gram, prec_knn, var_rtol, 
     off = batch[3]
     for i in range(len(SIZES)):
         a, b = int(off[i]), int(off[i + 1])
+        n = b - a
         o1, X1, i1, d1, csr1 = _single(batch, i, prec_gram, prec_knn)
         np.testing.assert_allclose(out.variance[i], o1.variance, rtol=var_rtol, err_msg=f"subset {i}")
         assert abs(out.total_variance[i] - o1.total_variance) <= tot_rtol * o1.total_variance
-        # the 10 leading components: the subsets have 10 clusters, so the signal
-        # subspace is well separated from the noise
-        assert _angle(oracle, out.loadings[i], o1.loadings, 9) < 1e-6, i
         # kNN of the batch is exact on the batch's own scores
-        _check_knn(oracle, X[a:b], idx[a:b].astype(np.int64), b - a)
+        _check_knn(oracle, X[a:b], idx[a:b].astype(np.int64), n)
+        # the oracle's PCA of the subset
+        ip, ix, dv = csr1
+        cmap, gmap = np.arange(n, dtype=np.int64), np.arange(G, dtype=np.int32)
+        ctx = (ix, dv, cmap, gmap, 1e4, True, None, ip, 0)
+        st = np.zeros((G, 5))
+        oracle.gene_stats(*ctx, st)
+        mean = st[:, 0] / n
+        sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+        sd[sd == 0] = 1
+        gram, colsum = np.zeros((G, G)), np.zeros(G)
+        oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
+        var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, 30)
+        # the 10 leading components: the subsets have 10 clusters, so the signal
+        # subspace is well separated from the noise.  Subsets of at most 256 cells (fewer
+        # than the 400 genes) take the n x n dual on the FP64 scaled rows whatever the
+        # Gram precision, so they match the oracle's eigenvectors, not the split-precision
+        # single run's.
+        if n <= 256:
+            np.testing.assert_allclose(out.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
+            assert _angle(oracle, out.loadings[i], load_o, 9) < 1e-9, i
+        else:
+            assert _angle(oracle, out.loadings[i], o1.loadings, 9) < 1e-6, i
         # and the subset's own pipeline agrees on the oracle's variance
         if prec_gram == "fp64":
-            ip, ix, dv = csr1
-            n = b - a
-            cmap, gmap = np.arange(n, dtype=np.int64), np.arange(G, dtype=np.int32)
-            ctx = (ix, dv, cmap, gmap, 1e4, True, None, ip, 0)
-            st = np.zeros((G, 5))
-            oracle.gene_stats(*ctx, st)
-            mean = st[:, 0] / n
-            sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
-            sd[sd == 0] = 1
-            gram, colsum = np.zeros((G, G)), np.zeros(G)
-            oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
-            var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, 30)
             np.testing.assert_allclose(out.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
             assert abs(out.total_variance[i] - tot_o) <= 1e-11 * tot_o
             # scores: (Z - mean Z) V with the oracle's loadings, through the oracle's
@@ -148,3 +156,58 @@ def test_batch_graph_replays(batch, oracle):
         check(lib().crest_synchronize())
     # the eager object's own results are unchanged by the replays
     np.testing.assert_allclose(pe.x_pca(), X_e, rtol=1e-5, atol=1e-5)
+
+
+# Subsets with fewer cells than genes: below 1,600 genes, the ones of at most 256, 512,
+# 1,024 and 1,536 cells take the n x n dual eigensolve (one bucket each), the 2,000-cell
+# one the d x d covariance.
+G2 = 1600
+SIZES2 = [51, 256, 300, 600, 1100, 1536, 2000]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_batch_dual_eigensolve_matches_covariance(oracle, monkeypatch, prec):
+    """The dual's variance, total and loadings against the covariance path of the same
+    batch (CREST_BATCH_DUAL=0) and, in FP64, against the oracle's eigensolve of each
+    subset's Gram."""
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR
+    indptr, indices, data, off = synth.make_subsets(SIZES2, G2, 10, density=0.05, seed=5)
+    csr = DeviceCSR.upload(indptr, indices, data, G2)
+    res = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CREST_BATCH_DUAL", mode)
+        p = BatchPipeline(csr, off, np.arange(G2, dtype=np.int32), n_comps=30, k=15, prec_gram=prec,
+                          prec_knn="fp32")
+        o = p.run()
+        res[mode] = (o, p.x_pca(), p.knn()[0])
+    o, X, idx = res["1"]
+    oc, Xc, ic = res["0"]
+    # the Gram (and so the total) is the same; the dual works on the FP64 scaled rows,
+    # the covariance path on the Gram of the chosen precision
+    np.testing.assert_allclose(o.total_variance, oc.total_variance, rtol=1e-14)
+    vr = 1e-10 if prec == "fp64" else 1e-6
+    np.testing.assert_allclose(o.variance, oc.variance, rtol=vr)
+    for i in range(len(SIZES2)):
+        assert _angle(oracle, o.loadings[i], oc.loadings[i], 9) < (1e-9 if prec == "fp64" else 1e-6), i
+        a, b = int(off[i]), int(off[i + 1])
+        _check_knn(oracle, X[a:b], idx[a:b].astype(np.int64), b - a)
+    if prec != "fp64":
+        return
+    gmap = np.arange(G2, dtype=np.int32)
+    for i in (0, 2, 4):  # one subset from each of three buckets
+        a, b = int(off[i]), int(off[i + 1])
+        n = b - a
+        ip = indptr[a:b + 1] - indptr[a]
+        ctx = (indices[indptr[a]:indptr[b]], data[indptr[a]:indptr[b]], np.arange(n, dtype=np.int64), gmap, 1e4,
+               True, None, ip, 0)
+        st = np.zeros((G2, 5))
+        oracle.gene_stats(*ctx, st)
+        mean = st[:, 0] / n
+        sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+        sd[sd == 0] = 1
+        gram, colsum = np.zeros((G2, G2)), np.zeros(G2)
+        oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
+        var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, 30)
+        np.testing.assert_allclose(o.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
+        assert abs(o.total_variance[i] - tot_o) <= 1e-11 * tot_o
+        assert _angle(oracle, o.loadings[i], load_o, 9) < 1e-9, i
