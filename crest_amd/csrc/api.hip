@@ -1016,6 +1016,26 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 // eigenpairs as C = Zc^T Zc / (n - 1) at O(m^3) instead of O(d^3).  The trace comes
 // from the Gram diagonal with the covariance's own arithmetic.  CREST_BATCH_DUAL=0
 // keeps every subset on the d x d covariance.
+// Tridiagonalization of ng matrices of order n (lower, LAPACK dsytrd layout): the
+// one-workgroup-per-matrix kernel for groups of at least 32 (it streams each matrix's
+// lower triangle once per column from HBM, so a full group keeps HBM busy), rocSOLVER's
+// batched dsytrd for smaller groups.  CREST_SYTRD=wg|rocsolver forces either.
+static void batch_sytrd(Ctx &x, const std::string &pre, double *A, int n, int ng, double *Dg, double *Eg,
+                        double *tau) {
+    const char *e = std::getenv("CREST_SYTRD");
+    const std::string mode = e ? e : "auto";
+    const bool wg = sytrd_wg_supported(n) && (mode == "wg" || (mode != "rocsolver" && ng >= 32));
+    const int64_t nn = (int64_t)n * n;
+    if (!wg) {
+        BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, n, A, n, nn, Dg, n, Eg, n, tau, n, ng));
+        return;
+    }
+    double *W = x.get<double>(pre + "sytrd_w", (size_t)ng * sytrd_wg_scratch(n));
+    sytrd_set_attrs();
+    launch_sytrd_lower(A, nn, n, Dg, n, Eg, n, tau, n, W, n, ng, x.st);
+    x.check_launch();
+}
+
 static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
     const bool topp = use_topp(P.d, P.kp, (int)std::min<int64_t>(P.eig_batch, P.n_sub));
     if (topp) tri_set_attrs();
@@ -1063,8 +1083,7 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
         launch_cov_batch(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ng, B.cov, B.tot, x.st, ids);
         x.check_launch();
         if (topp) {
-            BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, P.d, B.cov, P.d, dd, B.w, P.d, B.e,
-                                                     P.d, B.tau, P.d, ng));
+            batch_sytrd(x, P.pre, B.cov, P.d, ng, B.w, B.e, B.tau);
             launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
             launch_tri_topp(B.cov, dd, P.d, B.tau, P.d, B.w, P.d, B.e, P.d, P.d, P.kp, ng, B.topw, B.y, B.x, B.fac,
                             B.swp, B.tinfo, x.st);
@@ -1112,8 +1131,7 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
             const double one = 1.0, zero = 0.0;
             BLASCHK(rocblas_dgemm_strided_batched(x.blas, rocblas_operation_transpose, rocblas_operation_none, m, m, P.d,
                                                   &one, zc, P.d, md, zc, P.d, md, &zero, kv, m, mm, ng));
-            BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, m, kv, m, mm, B.w, m, B.e, m, B.tau,
-                                                     m, ng));
+            batch_sytrd(x, P.pre, kv, m, ng, B.w, B.e, B.tau);
             launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
             launch_tri_topp(kv, mm, m, B.tau, m, B.w, m, B.e, m, m, P.kp, ng, B.topw, B.y, B.x, B.fac, B.swp, B.tinfo,
                             x.st);

@@ -167,6 +167,14 @@ void launch_eig_select(const double *evecs, const double *evals, int d, int k, d
                        double *loadings, hipStream_t s, int ncols = 0);
 // Top-p eigenpairs after a lower Householder tridiagonalization (kernels_eig.hip).
 bool tri_topp_supported(int n);
+// kernels_sytrd.hip: lower Householder tridiagonalization (LAPACK dsytrd layout), one
+// workgroup per matrix; W scratch sytrd_wg_scratch(n) doubles per matrix.
+size_t sytrd_wg_lds(int n);
+bool sytrd_wg_supported(int n);
+size_t sytrd_wg_scratch(int n);
+void sytrd_set_attrs();
+void launch_sytrd_lower(double *A, int64_t sA, int lda, double *D, int64_t sD, double *E, int64_t sE, double *tau,
+                        int64_t sT, double *W, int n, int n_mat, hipStream_t s);
 void tri_set_attrs();
 void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, int64_t sTau, const double *D,
                      int64_t sD, const double *E, int64_t sE, int n, int p, int n_mat, double *W, double *Y,

@@ -211,3 +211,27 @@ def test_batch_dual_eigensolve_matches_covariance(oracle, monkeypatch, prec):
         np.testing.assert_allclose(o.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
         assert abs(o.total_variance[i] - tot_o) <= 1e-11 * tot_o
         assert _angle(oracle, o.loadings[i], load_o, 9) < 1e-9, i
+
+
+@pytest.mark.parametrize("genes,sizes,seed", [(45, [40, 100, 300], 8), (G, SIZES, 21), (G2, SIZES2, 5)])
+def test_batch_tridiagonalization_kernel_matches_rocsolver(oracle, monkeypatch, genes, sizes, seed):
+    """The one-workgroup-per-matrix tridiagonalization (CREST_SYTRD=wg) against
+    rocSOLVER's dsytrd (CREST_SYTRD=rocsolver) on the same batch: orders 45 (one tile),
+    400 (a partial last panel), 1,600 and the dual orders 256 ... 1,536."""
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR
+    indptr, indices, data, off = synth.make_subsets(sizes, genes, 10, density=0.07, seed=seed)
+    csr = DeviceCSR.upload(indptr, indices, data, genes)
+    res = {}
+    for mode in ("rocsolver", "wg"):
+        monkeypatch.setenv("CREST_SYTRD", mode)
+        p = BatchPipeline(csr, off, np.arange(genes, dtype=np.int32), n_comps=30, k=15, prec_gram="fp64",
+                          prec_knn="fp32")
+        res[mode] = p.run()
+    o, r = res["wg"], res["rocsolver"]
+    assert np.isfinite(o.variance).all() and np.isfinite(o.loadings).all()
+    np.testing.assert_allclose(o.total_variance, r.total_variance, rtol=1e-14)
+    np.testing.assert_allclose(o.variance, r.variance, rtol=1e-10, atol=1e-12 * r.variance.max())
+    for i in range(len(sizes)):
+        assert _angle(oracle, o.loadings[i], r.loadings[i], 9) < 1e-9, i
+        V = o.loadings[i]
+        np.testing.assert_allclose(V.T @ V, np.eye(V.shape[1]), atol=1e-10)
