@@ -74,6 +74,34 @@ __device__ inline double transpose_sum16(double *t, int lane) {
     return v + __shfl_xor(v, 32, 64);
 }
 
+// a[u] -= V(r_u, q) cw(q) + W(r_u, q) cv(q) for q = 0 .. jj-1 in order, for the thread's
+// rows r_u = tid + 512 u (clamped to n - 1; the caller discards rows it does not own).
+// The loads of four panel columns are issued together.
+template <class FW, class FV>
+__device__ inline void panel_correct(double (&a)[kSyRU], const double *A, int lda, const double *Wg, int n, int j0,
+                                     int jj, int tid, FW cw, FV cv) {
+    for (int q0 = 0; q0 < jj; q0 += 4) {
+        double vv[kSyRU][4], ww[kSyRU][4];
+#pragma unroll
+        for (int u = 0; u < kSyRU; ++u) {
+            const int r = tid + kSyThreads * u, rc = r < n ? r : n - 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int q = q0 + k < jj ? q0 + k : jj - 1;
+                vv[u][k] = A[rc + (int64_t)(j0 + q) * lda];
+                ww[u][k] = Wg[(int64_t)q * n + rc];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (q0 + k < jj) {
+                const double x = cw(q0 + k), y = cv(q0 + k);
+#pragma unroll
+                for (int u = 0; u < kSyRU; ++u) a[u] -= vv[u][k] * x + ww[u][k] * y;
+            }
+    }
+}
+
 }  // namespace
 
 size_t sytrd_wg_lds(int n) {
@@ -119,19 +147,24 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
             double ss = 0.0;
 #pragma unroll
             for (int u = 0; u < kSyRU; ++u) {
+                const int r = tid + kSyThreads * u, rc = r < n ? r : n - 1;
+                colv[u] = A[rc + (int64_t)i * lda];
+            }
+            panel_correct(colv, A, lda, Wg, n, j0, jj, tid, [&](int q) { return rowW[q]; },
+                          [&](int q) { return rowV[q]; });
+#pragma unroll
+            for (int u = 0; u < kSyRU; ++u) {
                 const int r = tid + kSyThreads * u;
-                colv[u] = 0.0;
                 if (r >= i && r < n) {
-                    double a = A[r + (int64_t)i * lda];
-                    for (int q = 0; q < jj; ++q)
-                        a -= A[r + (int64_t)(j0 + q) * lda] * rowW[q] + Wg[(int64_t)q * n + r] * rowV[q];
-                    colv[u] = a;
+                    const double a = colv[u];
                     if (r >= i + 2) ss += a * a;
                     if (r == i1) sc[0] = a;
                     if (r == i) {
                         A[r + (int64_t)i * lda] = a;
                         D[i] = a;
                     }
+                } else {
+                    colv[u] = 0.0;
                 }
             }
             ss = wsum(ss);
@@ -166,76 +199,112 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                 panE[jj] = beta;
             }
             lds_barrier();
-            // (c1) W^T v and V^T v over the panel's earlier columns, one wave per dot
-            // product: red64[q] = W(:, q)^T v, red64[32 + q] = V(:, q)^T v over rows i1..n-1
-            for (int k = wave; k < 2 * jj; k += kSyWaves) {
-                const bool isw = k < jj;
-                const int q = isw ? k : k - jj;
-                const double *src = isw ? Wg + (int64_t)q * n : A + (int64_t)(j0 + q) * lda;
-                double acc = 0.0;
-                for (int r = i1 + lane; r < n; r += 64) acc += src[r] * vl[r];
-                acc = wsum(acc);
-                if (lane == 0) red64[(isw ? 0 : kSyNB) + q] = acc;
-            }
-            // (c2) y = A22 v over the lower triangle of rows/columns i1..n-1, in tiles of
-            // 64 rows x 32 columns (block row R, column block C <= 2R + 1)
+            // (c1 + c2) one list of work items per wave:
+            //  * y = A22 v over the lower triangle of rows/columns i1..n-1 in tiles of 64
+            //    rows x 32 columns (block row R, column block C <= 2R + 1), partial sums
+            //    into the wave's own LDS vector;
+            //  * W^T v and V^T v over the panel's earlier columns, 64 rows x 32 columns
+            //    each (C = 2R + 2: W(r, 0:32); C = 2R + 3: V(r, 0:32)), partial sums into
+            //    the wave's own red64 row (entry k < 32: W(:, k)^T v, 32 + q: V(:, q)^T v).
             {
                 double *yp = ypart + wave * ylen;
                 for (int r = i1 + lane; r < n; r += 64) yp[r] = 0.0;
+                red64[wave * 64 + lane] = 0.0;
                 const int T = (n - i1 + 63) >> 6;
-                int t = 0;
-#pragma unroll 1
-                for (int R = 0; R < T; ++R)
-#pragma unroll 1
-                    for (int C = 0; C <= 2 * R + 1; ++C, ++t) {
-                        if ((t & (kSyWaves - 1)) != wave) continue;
-                        const int r = i1 + 64 * R + lane, cb = i1 + 32 * C;
-                        if (cb >= n) continue;
-                        // column offset from the block row's first row: >= 0 only on the
-                        // two blocks that hold the diagonal; entries above it are not read
-                        const int off = 32 * C - 64 * R;
-                        const bool rv = r < n;
-                        // every load in bounds (row and column clamped to n - 1; entries
-                        // above the diagonal are read and discarded), so they issue together
-                        const double *arow = A + (rv ? r : n - 1);
-                        double a[32];
+                const int extra = jj > 0 ? 2 : 0;
+                // item (R, C) of the wave: every 8th of the row-major list starting at `wave`
+                auto advance = [&](int &R, int &C, int step) __attribute__((always_inline)) {
+                    C += step;
+                    while (R < T && C >= 2 * R + 2 + extra) {
+                        C -= 2 * R + 2 + extra;
+                        ++R;
+                    }
+                };
+                auto load = [&](int R, int C, double (&x)[32]) __attribute__((always_inline)) {
+                    const int r = i1 + 64 * R + lane, rc = r < n ? r : n - 1;
+                    if (C <= 2 * R + 1) {
+                        const double *arow = A + rc;
+                        const int cb = i1 + 32 * C;
 #pragma unroll
                         for (int c = 0; c < 32; ++c) {
                             const int col = cb + c < n ? cb + c : n - 1;
-                            a[c] = arow[(int64_t)col * lda];
+                            x[c] = arow[(int64_t)col * lda];
                         }
+                    } else if (C == 2 * R + 2) {
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) a[c] = (rv && cb + c < n && off + c <= lane) ? a[c] : 0.0;
+                        for (int c = 0; c < 32; ++c) x[c] = Wg[(int64_t)(c < jj ? c : 0) * n + rc];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 32; ++c) x[c] = A[rc + (int64_t)(j0 + (c < jj ? c : 0)) * lda];
+                    }
+                };
+                auto process = [&](int R, int C, double (&x)[32]) __attribute__((always_inline)) {
+                    const int r = i1 + 64 * R + lane;
+                    const bool rv = r < n;
+                    const double vrow = rv ? vl[r] : 0.0;
+                    if (C <= 2 * R + 1) {
+                        const int cb = i1 + 32 * C;
+                        // column offset from the block row's first row: >= 0 only on the
+                        // two blocks that hold the diagonal; entries above it are dropped
+                        const int off = 32 * C - 64 * R;
+#pragma unroll
+                        for (int c = 0; c < 32; ++c) x[c] = (rv && cb + c < n && off + c <= lane) ? x[c] : 0.0;
                         double acc = 0.0;
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) acc += a[c] * vl[cb + c];
-                        const double vrow = rv ? vl[r] : 0.0;
+                        for (int c = 0; c < 32; ++c) acc += x[c] * vl[cb + c];
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) a[c] = off + c < lane ? a[c] * vrow : 0.0;
-                        const double ts0 = transpose_sum16(a, lane);
-                        const double ts1 = transpose_sum16(a + 16, lane);
+                        for (int c = 0; c < 32; ++c) x[c] = off + c < lane ? x[c] * vrow : 0.0;
+                        const double ts0 = transpose_sum16(x, lane);
+                        const double ts1 = transpose_sum16(x + 16, lane);
                         if (rv) yp[r] += acc;
                         if (lane < 16 && cb + lane < n) yp[cb + lane] += ts0;
                         if (lane < 16 && cb + 16 + lane < n) yp[cb + 16 + lane] += ts1;
+                    } else {
+                        const int k0 = C == 2 * R + 2 ? 0 : kSyNB;
+#pragma unroll
+                        for (int c = 0; c < 32; ++c) x[c] = (rv && c < jj) ? x[c] * vrow : 0.0;
+                        const double ts0 = transpose_sum16(x, lane);
+                        const double ts1 = transpose_sum16(x + 16, lane);
+                        if (lane < 16) {
+                            red64[wave * 64 + k0 + lane] += ts0;
+                            red64[wave * 64 + k0 + 16 + lane] += ts1;
+                        }
                     }
+                };
+                int R = 0, C = 0;
+                advance(R, C, wave);
+                while (R < T) {
+                    double x[32];
+                    load(R, C, x);
+                    process(R, C, x);
+                    advance(R, C, kSyWaves);
+                }
             }
             lds_barrier();
             // (c3) w = tau (y - V (W^T v) - W (V^T v)), then w -= tau/2 (w^T v) v
-            const double s64 = jj > 0 ? red64[lane] : 0.0;
+            double s64 = 0.0;
+            if (jj > 0)
+#pragma unroll
+                for (int w = 0; w < kSyWaves; ++w) s64 += red64[w * 64 + lane];
             double wl[kSyRU], part = 0.0;
 #pragma unroll
             for (int u = 0; u < kSyRU; ++u) {
-                const int r = tid + kSyThreads * u;
-                wl[u] = 0.0;
-                if (r >= i1 && r < n) {
-                    double y = 0.0;
+                const int r = tid + kSyThreads * u, rc = r < n ? r : n - 1;
+                double y = 0.0;
 #pragma unroll
-                    for (int w = 0; w < kSyWaves; ++w) y += ypart[w * ylen + r];
-                    for (int q = 0; q < jj; ++q)
-                        y -= A[r + (int64_t)(j0 + q) * lda] * readlane_f64(s64, q) +
-                             Wg[(int64_t)q * n + r] * readlane_f64(s64, kSyNB + q);
-                    wl[u] = tv * y;
+                for (int w = 0; w < kSyWaves; ++w) y += ypart[w * ylen + rc];
+                wl[u] = y;
+            }
+            panel_correct(wl, A, lda, Wg, n, j0, jj, tid, [&](int q) { return readlane_f64(s64, q); },
+                          [&](int q) { return readlane_f64(s64, kSyNB + q); });
+#pragma unroll
+            for (int u = 0; u < kSyRU; ++u) {
+                const int r = tid + kSyThreads * u;
+                if (r >= i1 && r < n) {
+                    wl[u] = tv * wl[u];
                     part += wl[u] * vr[u];
+                } else {
+                    wl[u] = 0.0;
                 }
             }
             part = wsum(part);
@@ -252,14 +321,16 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                     const double w = wl[u] + a2 * vr[u];
                     Wg[(int64_t)jj * n + r] = w;
                     if (r == i1 && jj + 1 < pw) {  // row i1's panel entries for the next column
-                        for (int q = 0; q < jj; ++q) {
-                            rowW[q] = Wg[(int64_t)q * n + r];
-                            rowV[q] = A[r + (int64_t)(j0 + q) * lda];
-                        }
                         rowW[jj] = w;
                         rowV[jj] = 1.0;
                     }
                 }
+            }
+            // entries q < jj of row i1 (written in earlier columns, so visible after the
+            // full barriers since), one per thread
+            if (tid < jj && jj + 1 < pw) {
+                rowW[tid] = Wg[(int64_t)tid * n + i1];
+                rowV[tid] = A[i1 + (int64_t)(j0 + tid) * lda];
             }
             // a full barrier: the next column's dot products read this column's W and V
             // entries written by other threads
@@ -289,6 +360,11 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                 }
                 const int cend = t0 + 64 * R + 64 < n ? t0 + 64 * R + 64 : n;
                 for (int c0 = t0; c0 < cend; c0 += 16) {
+                    // this sub-tile's entries of A, loaded before the staging so that both
+                    // are in flight together (clamped in bounds; written back masked)
+                    double av[16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) av[c] = A[rc + (int64_t)(c0 + c < n ? c0 + c : n - 1) * lda];
                     // lane (c, qg) stages column c0 + (lane & 15), entries q = (lane >> 4) + 4 k
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
@@ -308,14 +384,16 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 #pragma unroll 2
                     for (int c = 0; c < 16; ++c) {
-                        const int col = c0 + c;
-                        if (rv && col <= r && col < n) {
-                            const double *sv = stg + c * kSyStage;
-                            double acc = 0.0;
+                        const double *sv = stg + c * kSyStage;
+                        double acc = 0.0;
 #pragma unroll
-                            for (int q = 0; q < kSyNB; ++q) acc += vrw[q] * sv[kSyNB + q] + wrw[q] * sv[q];
-                            A[r + (int64_t)col * lda] -= acc;
-                        }
+                        for (int q = 0; q < kSyNB; ++q) acc += vrw[q] * sv[kSyNB + q] + wrw[q] * sv[q];
+                        av[c] -= acc;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const int col = c0 + c;
+                        if (rv && col <= r && col < n) A[r + (int64_t)col * lda] = av[c];
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
                     __builtin_amdgcn_wave_barrier();
@@ -332,7 +410,7 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
 }
 
 void sytrd_set_attrs() {
-    hipFuncSetAttribute((const void *)k_sytrd_lower, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_sytrd_lower, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
 void launch_sytrd_lower(double *A, int64_t sA, int lda, double *D, int64_t sD, double *E, int64_t sE, double *tau,
