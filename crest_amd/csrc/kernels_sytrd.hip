@@ -32,7 +32,8 @@ constexpr int kSyWaves = 8;     // 512 threads
 constexpr int kSyThreads = kSyWaves * 64;
 constexpr int kSyRU = 4;        // rows per thread: row r belongs to thread r % 512
 constexpr int kSyMaxN = kSyThreads * kSyRU;
-constexpr int kSyStage = 66;    // LDS stride of a staged column (16-byte aligned, 4 banks apart)
+constexpr int kSyStage = 66;    // sizes the per-wave LDS vectors (at least 16 x 66 doubles)
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -336,68 +337,73 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
             // entries written by other threads
             __syncthreads();
         }
-        // (e) trailing rows/columns t0..n-1: A -= V W^T + W V^T (lower triangle)
+        // (e) trailing rows/columns t0..n-1: A -= V W^T + W V^T (lower triangle), on FP64
+        // MFMA in 16 x 16 tiles.  Tile (rows r0.., columns c0..): D = P Q with K = 64,
+        // P[i][k] = W(c0 + i, k) for k < 32 and V(c0 + i, k - 32) above, Q[k][j] =
+        // V(r0 + j, k) for k < 32 and W(r0 + j, k - 32) above, so D[i][j] is the update of
+        // A(r0 + j, c0 + i).  v_mfma_f64_16x16x4f64 at step s takes P[l & 15][4s + (l >> 4)]
+        // and Q[4s + (l >> 4)][l & 15] from lane l and leaves D[4v + (l >> 4)][l & 15] in its
+        // register v: one lane's outputs are consecutive rows of four columns.  A wave
+        // takes 16-row strips round-robin; the strip's Q stays in registers and the next
+        // tile's P and A entries load while the current tile is computed.
         __syncthreads();
         const int t0 = j0 + pw;
         if (t0 < n) {
-            double *stg = ypart + wave * ylen;  // [16 columns][kSyStage]: V(c, 0:32), W(c, 0:32)
-            const int TB = (n - t0 + 63) >> 6;
-            for (int R = wave; R < TB; R += kSyWaves) {
-                const int r = t0 + 64 * R + lane;
-                const bool rv = r < n;
-                double vrw[kSyNB], wrw[kSyNB];
-                const int rc = rv ? r : n - 1;
+            const int S = (n - t0 + 15) >> 4, li = lane & 15, lk = lane >> 4;
+            for (int sR = wave; sR < S; sR += kSyWaves) {
+                const int row = t0 + 16 * sR + li, rowc = row < n ? row : n - 1;
+                double qb[16];
 #pragma unroll
-                for (int q = 0; q < kSyNB; ++q) {
-                    const int qc = q < pw ? q : 0;
-                    vrw[q] = A[rc + (int64_t)(j0 + qc) * lda];
-                    wrw[q] = Wg[(int64_t)qc * n + rc];
+                for (int s2 = 0; s2 < 8; ++s2) {
+                    const int q = 4 * s2 + lk, qc = q < pw ? q : 0;
+                    qb[s2] = A[rowc + (int64_t)(j0 + qc) * lda];
+                    qb[8 + s2] = Wg[(int64_t)qc * n + rowc];
                 }
 #pragma unroll
-                for (int q = 0; q < kSyNB; ++q) {
-                    vrw[q] = (rv && q < pw) ? vrw[q] : 0.0;
-                    wrw[q] = (rv && q < pw) ? wrw[q] : 0.0;
+                for (int s2 = 0; s2 < 8; ++s2) {
+                    const bool ok = row < n && 4 * s2 + lk < pw;
+                    qb[s2] = ok ? qb[s2] : 0.0;
+                    qb[8 + s2] = ok ? qb[8 + s2] : 0.0;
                 }
-                const int cend = t0 + 64 * R + 64 < n ? t0 + 64 * R + 64 : n;
-                for (int c0 = t0; c0 < cend; c0 += 16) {
-                    // this sub-tile's entries of A, loaded before the staging so that both
-                    // are in flight together (clamped in bounds; written back masked)
-                    double av[16];
+                auto load = [&](int cB, double (&pa)[16], double (&av)[4]) __attribute__((always_inline)) {
+                    const int c0 = t0 + 16 * cB, colp = c0 + li, colpc = colp < n ? colp : n - 1;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) av[c] = A[rc + (int64_t)(c0 + c < n ? c0 + c : n - 1) * lda];
-                    // lane (c, qg) stages column c0 + (lane & 15), entries q = (lane >> 4) + 4 k
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        const int c = lane & 15, q = (lane >> 4) + 4 * k, col = c0 + c;
-                        double val = 0.0;
-                        if (col < n) {
-                            if (q < kSyNB) {
-                                if (q < pw) val = A[col + (int64_t)(j0 + q) * lda];
-                            } else if (q - kSyNB < pw) {
-                                val = Wg[(int64_t)(q - kSyNB) * n + col];
-                            }
-                        }
-                        stg[c * kSyStage + q] = val;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-#pragma unroll 2
-                    for (int c = 0; c < 16; ++c) {
-                        const double *sv = stg + c * kSyStage;
-                        double acc = 0.0;
-#pragma unroll
-                        for (int q = 0; q < kSyNB; ++q) acc += vrw[q] * sv[kSyNB + q] + wrw[q] * sv[q];
-                        av[c] -= acc;
+                    for (int s2 = 0; s2 < 8; ++s2) {
+                        const int q = 4 * s2 + lk, qc = q < pw ? q : 0;
+                        pa[s2] = Wg[(int64_t)qc * n + colpc];
+                        pa[8 + s2] = A[colpc + (int64_t)(j0 + qc) * lda];
                     }
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
-                        const int col = c0 + c;
-                        if (rv && col <= r && col < n) A[r + (int64_t)col * lda] = av[c];
+                    for (int v = 0; v < 4; ++v) {
+                        const int col = c0 + 4 * v + lk;
+                        av[v] = A[rowc + (int64_t)(col < n ? col : n - 1) * lda];
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                };
+                auto tile = [&](int cB, double (&pa)[16], const double (&av)[4]) __attribute__((always_inline)) {
+                    const int c0 = t0 + 16 * cB, colp = c0 + li;
+#pragma unroll
+                    for (int s2 = 0; s2 < 8; ++s2) {
+                        const bool ok = colp < n && 4 * s2 + lk < pw;
+                        pa[s2] = ok ? pa[s2] : 0.0;
+                        pa[8 + s2] = ok ? pa[8 + s2] : 0.0;
+                    }
+                    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s2 = 0; s2 < 16; ++s2) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[s2], qb[s2], acc, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int col = c0 + 4 * v + lk;
+                        if (row < n && col <= row) A[row + (int64_t)col * lda] = av[v] - acc[v];
+                    }
+                };
+                double pa0[16], av0[4], pa1[16], av1[4];
+                load(0, pa0, av0);
+                for (int cB = 0; cB <= sR; cB += 2) {
+                    if (cB + 1 <= sR) load(cB + 1, pa1, av1);
+                    tile(cB, pa0, av0);
+                    if (cB + 1 > sR) break;
+                    if (cB + 2 <= sR) load(cB + 2, pa0, av0);
+                    tile(cB + 1, pa1, av1);
                 }
             }
         }
