@@ -1017,16 +1017,15 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 // from the Gram diagonal with the covariance's own arithmetic.  CREST_BATCH_DUAL=0
 // keeps every subset on the d x d covariance.
 // Tridiagonalization of ng matrices of order n (lower, LAPACK dsytrd layout): the
-// one-workgroup-per-matrix kernel for groups of at least 64 — one matrix of order 2,000
-// takes ~520 ms on its workgroup, all of a group side by side, against ~6.2 ms per
-// matrix for rocSOLVER's batched dsytrd, so the kernel wins from ~85 matrices on and is
-// even from ~64 — and rocSOLVER for smaller groups.  CREST_SYTRD=wg|rocsolver forces
-// either.
+// one-workgroup-per-matrix kernel for groups of at least 48 — one matrix of order 2,000
+// takes ~300 ms on its workgroup, a group side by side until HBM saturates (162 take
+// ~480 ms), against ~6.2 ms per matrix for rocSOLVER's batched dsytrd — and rocSOLVER
+// for smaller groups.  CREST_SYTRD=wg|rocsolver forces either.
 static void batch_sytrd(Ctx &x, const std::string &pre, double *A, int n, int ng, double *Dg, double *Eg,
                         double *tau) {
     const char *e = std::getenv("CREST_SYTRD");
     const std::string mode = e ? e : "auto";
-    const bool wg = sytrd_wg_supported(n) && (mode == "wg" || (mode != "rocsolver" && ng >= 64));
+    const bool wg = sytrd_wg_supported(n) && (mode == "wg" || (mode != "rocsolver" && ng >= 48));
     const int64_t nn = (int64_t)n * n;
     if (!wg) {
         BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, n, A, n, nn, Dg, n, Eg, n, tau, n, ng));

@@ -41,6 +41,20 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Loads through buffer descriptors: a uniform byte offset in a scalar register plus one
+// 32-bit vector offset per lane, so a tile's loads share one address register (a matrix
+// of order <= 2,048 spans at most 32 MB).  Offsets past the descriptor's range read 0.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+__device__ inline BufRsrc buf_rsrc(const double *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline double bload(BufRsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void bstore(BufRsrc r, uint32_t voff, uint32_t soff, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 0);
+}
+
 __device__ inline double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -79,18 +93,19 @@ __device__ inline double transpose_sum16(double *t, int lane) {
 // rows r_u = tid + 512 u (clamped to n - 1; the caller discards rows it does not own).
 // The loads of four panel columns are issued together.
 template <class FW, class FV>
-__device__ inline void panel_correct(double (&a)[kSyRU], const double *A, int lda, const double *Wg, int n, int j0,
-                                     int jj, int tid, FW cw, FV cv) {
+__device__ inline void panel_correct(double (&a)[kSyRU], BufRsrc ra, int lda, BufRsrc rw, int n, int j0, int jj,
+                                     int tid, FW cw, FV cv) {
     for (int q0 = 0; q0 < jj; q0 += 4) {
         double vv[kSyRU][4], ww[kSyRU][4];
 #pragma unroll
         for (int u = 0; u < kSyRU; ++u) {
-            const int r = tid + kSyThreads * u, rc = r < n ? r : n - 1;
+            const int r = tid + kSyThreads * u;
+            const uint32_t rc = (uint32_t)(r < n ? r : n - 1);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int q = q0 + k < jj ? q0 + k : jj - 1;
-                vv[u][k] = A[rc + (int64_t)(j0 + q) * lda];
-                ww[u][k] = Wg[(int64_t)q * n + rc];
+                vv[u][k] = bload(ra, rc * 8u, (uint32_t)((j0 + q) * lda) * 8u);
+                ww[u][k] = bload(rw, rc * 8u, (uint32_t)(q * n) * 8u);
             }
         }
 #pragma unroll
@@ -130,12 +145,16 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
     double *panE = rowV + kSyNB;                // [NB] E of the panel's columns
     double *sc = panE + kSyNB;                  // [8] broadcast scalars
 
-    const int mat = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index through readfirstlane, so that values derived from it (work item
+    // indices, column offsets) are known to be uniform and live in scalar registers
+    const int mat = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     A += mat * sA;
     D += mat * sD;
     E += mat * sE;
     tau += mat * sT;
     Wg += (int64_t)mat * kSyNB * n;
+    const BufRsrc ra = buf_rsrc(A, (uint32_t)n * (uint32_t)lda * 8u);
+    const BufRsrc rw = buf_rsrc(Wg, (uint32_t)kSyNB * (uint32_t)n * 8u);
     for (int r = tid; r < ldn + 64; r += kSyThreads) vl[r] = 0.0;
     lds_barrier();
 
@@ -148,10 +167,10 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
             double ss = 0.0;
 #pragma unroll
             for (int u = 0; u < kSyRU; ++u) {
-                const int r = tid + kSyThreads * u, rc = r < n ? r : n - 1;
-                colv[u] = A[rc + (int64_t)i * lda];
+                const int r = tid + kSyThreads * u;
+                colv[u] = bload(ra, (uint32_t)(r < n ? r : n - 1) * 8u, (uint32_t)(i * lda) * 8u);
             }
-            panel_correct(colv, A, lda, Wg, n, j0, jj, tid, [&](int q) { return rowW[q]; },
+            panel_correct(colv, ra, lda, rw, n, j0, jj, tid, [&](int q) { return rowW[q]; },
                           [&](int q) { return rowV[q]; });
 #pragma unroll
             for (int u = 0; u < kSyRU; ++u) {
@@ -200,86 +219,91 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                 panE[jj] = beta;
             }
             lds_barrier();
-            // (c1 + c2) one list of work items per wave:
-            //  * y = A22 v over the lower triangle of rows/columns i1..n-1 in tiles of 64
-            //    rows x 32 columns (block row R, column block C <= 2R + 1), partial sums
-            //    into the wave's own LDS vector;
-            //  * W^T v and V^T v over the panel's earlier columns, 64 rows x 32 columns
-            //    each (C = 2R + 2: W(r, 0:32); C = 2R + 3: V(r, 0:32)), partial sums into
-            //    the wave's own red64 row (entry k < 32: W(:, k)^T v, 32 + q: V(:, q)^T v).
+            // (c1 + c2) y = A22 v over the lower triangle of rows/columns i1..n-1, by
+            // column blocks of 32 (block C = columns i1 + 32C .., round-robin over the
+            // waves): the wave walks down the block in 64-row tiles (block rows R >= C/2),
+            // each lane adding its row's part to the wave's LDS vector and accumulating its
+            // share of the transposed part per column; one cross-lane sum per column block.
+            // Then W^T v and V^T v over the panel's earlier columns, in 64-row x 32-column
+            // items (block row R; W(r, 0:32) or V(r, 0:32)) continuing the round-robin,
+            // summed into the wave's own red64 row (k < 32: W(:, k)^T v, 32 + q: V(:, q)^T v).
             {
                 double *yp = ypart + wave * ylen;
                 for (int r = i1 + lane; r < n; r += 64) yp[r] = 0.0;
                 red64[wave * 64 + lane] = 0.0;
                 const int T = (n - i1 + 63) >> 6;
-                const int extra = jj > 0 ? 2 : 0;
-                // item (R, C) of the wave: every 8th of the row-major list starting at `wave`
-                auto advance = [&](int &R, int &C, int step) __attribute__((always_inline)) {
-                    C += step;
-                    while (R < T && C >= 2 * R + 2 + extra) {
-                        C -= 2 * R + 2 + extra;
-                        ++R;
-                    }
-                };
+                const int NC = (n - i1 + 31) >> 5;  // column blocks
                 auto load = [&](int R, int C, double (&x)[32]) __attribute__((always_inline)) {
-                    const int r = i1 + 64 * R + lane, rc = r < n ? r : n - 1;
-                    if (C <= 2 * R + 1) {
-                        const double *arow = A + rc;
-                        const int cb = i1 + 32 * C;
+                    const int r = i1 + 64 * R + lane;
+                    const uint32_t rc = (uint32_t)(r < n ? r : n - 1);
+                    const int cb = i1 + 32 * C;
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) {
-                            const int col = cb + c < n ? cb + c : n - 1;
-                            x[c] = arow[(int64_t)col * lda];
-                        }
-                    } else if (C == 2 * R + 2) {
-#pragma unroll
-                        for (int c = 0; c < 32; ++c) x[c] = Wg[(int64_t)(c < jj ? c : 0) * n + rc];
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < 32; ++c) x[c] = A[rc + (int64_t)(j0 + (c < jj ? c : 0)) * lda];
+                    for (int c = 0; c < 32; ++c) {
+                        const int col = cb + c < n ? cb + c : n - 1;
+                        x[c] = bload(ra, rc * 8u, (uint32_t)(col * lda) * 8u);
                     }
                 };
-                auto process = [&](int R, int C, double (&x)[32]) __attribute__((always_inline)) {
-                    const int r = i1 + 64 * R + lane;
+                double tacc[32];
+                auto tile = [&](int R, int C, double (&x)[32]) __attribute__((always_inline)) {
+                    const int r = i1 + 64 * R + lane, cb = i1 + 32 * C;
                     const bool rv = r < n;
                     const double vrow = rv ? vl[r] : 0.0;
-                    if (C <= 2 * R + 1) {
-                        const int cb = i1 + 32 * C;
-                        // column offset from the block row's first row: >= 0 only on the
-                        // two blocks that hold the diagonal; entries above it are dropped
-                        const int off = 32 * C - 64 * R;
+                    // column offset from the block row's first row: >= 0 only on the block
+                    // row that holds the diagonal; entries above it are dropped
+                    const int off = 32 * C - 64 * R;
+                    double acc = 0.0;
+                    // masks as 0/1 factors (every entry read is finite: the matrix is
+                    // fully written and never modified above the diagonal)
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) x[c] = (rv && cb + c < n && off + c <= lane) ? x[c] : 0.0;
-                        double acc = 0.0;
+                    for (int c = 0; c < 32; ++c) {
+                        const double xv = x[c] * ((rv && cb + c < n && off + c <= lane) ? 1.0 : 0.0);
+                        acc += xv * vl[cb + c];
+                        tacc[c] += xv * (off + c < lane ? vrow : 0.0);
+                    }
+                    if (rv) yp[r] += acc;
+                };
+#pragma unroll 1
+                for (int C = wave; C < NC; C += kSyWaves) {
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) acc += x[c] * vl[cb + c];
+                    for (int c = 0; c < 32; ++c) tacc[c] = 0.0;
+#pragma unroll 1
+                    for (int R = C >> 1; R < T; ++R) {
+                        double x[32];
+                        load(R, C, x);
+                        tile(R, C, x);
+                    }
+                    const int cb = i1 + 32 * C;
+                    const double ts0 = transpose_sum16(tacc, lane);
+                    const double ts1 = transpose_sum16(tacc + 16, lane);
+                    if (lane < 16 && cb + lane < n) yp[cb + lane] += ts0;
+                    if (lane < 16 && cb + 16 + lane < n) yp[cb + 16 + lane] += ts1;
+                }
+                // panel items (R, g), numbered after the column blocks
+                if (jj > 0)
+                    for (int it = (wave - NC % kSyWaves + kSyWaves) % kSyWaves; it < 2 * T; it += kSyWaves) {
+                        const int R = it >> 1, g = it & 1;
+                        const int r = i1 + 64 * R + lane;
+                        const uint32_t rc = (uint32_t)(r < n ? r : n - 1);
+                        const bool rv = r < n;
+                        double x[32];
+                        if (g == 0) {
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) x[c] = off + c < lane ? x[c] * vrow : 0.0;
-                        const double ts0 = transpose_sum16(x, lane);
-                        const double ts1 = transpose_sum16(x + 16, lane);
-                        if (rv) yp[r] += acc;
-                        if (lane < 16 && cb + lane < n) yp[cb + lane] += ts0;
-                        if (lane < 16 && cb + 16 + lane < n) yp[cb + 16 + lane] += ts1;
-                    } else {
-                        const int k0 = C == 2 * R + 2 ? 0 : kSyNB;
+                            for (int c = 0; c < 32; ++c) x[c] = bload(rw, rc * 8u, (uint32_t)((c < jj ? c : 0) * n) * 8u);
+                        } else {
 #pragma unroll
-                        for (int c = 0; c < 32; ++c) x[c] = (rv && c < jj) ? x[c] * vrow : 0.0;
+                            for (int c = 0; c < 32; ++c)
+                                x[c] = bload(ra, rc * 8u, (uint32_t)((j0 + (c < jj ? c : 0)) * lda) * 8u);
+                        }
+                        const double vrow = rv ? vl[r] : 0.0;
+#pragma unroll
+                        for (int c = 0; c < 32; ++c) x[c] *= (rv && c < jj) ? vrow : 0.0;
                         const double ts0 = transpose_sum16(x, lane);
                         const double ts1 = transpose_sum16(x + 16, lane);
                         if (lane < 16) {
-                            red64[wave * 64 + k0 + lane] += ts0;
-                            red64[wave * 64 + k0 + 16 + lane] += ts1;
+                            red64[wave * 64 + g * kSyNB + lane] += ts0;
+                            red64[wave * 64 + g * kSyNB + 16 + lane] += ts1;
                         }
                     }
-                };
-                int R = 0, C = 0;
-                advance(R, C, wave);
-                while (R < T) {
-                    double x[32];
-                    load(R, C, x);
-                    process(R, C, x);
-                    advance(R, C, kSyWaves);
-                }
             }
             lds_barrier();
             // (c3) w = tau (y - V (W^T v) - W (V^T v)), then w -= tau/2 (w^T v) v
@@ -296,7 +320,7 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                 for (int w = 0; w < kSyWaves; ++w) y += ypart[w * ylen + rc];
                 wl[u] = y;
             }
-            panel_correct(wl, A, lda, Wg, n, j0, jj, tid, [&](int q) { return readlane_f64(s64, q); },
+            panel_correct(wl, ra, lda, rw, n, j0, jj, tid, [&](int q) { return readlane_f64(s64, q); },
                           [&](int q) { return readlane_f64(s64, kSyNB + q); });
 #pragma unroll
             for (int u = 0; u < kSyRU; ++u) {
@@ -352,12 +376,13 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
             const int S = (n - t0 + 15) >> 4, li = lane & 15, lk = lane >> 4;
             for (int sR = wave; sR < S; sR += kSyWaves) {
                 const int row = t0 + 16 * sR + li, rowc = row < n ? row : n - 1;
+                const uint32_t span = (uint32_t)(j0 * lda) * 8u;  // byte offset of the panel
                 double qb[16];
 #pragma unroll
                 for (int s2 = 0; s2 < 8; ++s2) {
                     const int q = 4 * s2 + lk, qc = q < pw ? q : 0;
-                    qb[s2] = A[rowc + (int64_t)(j0 + qc) * lda];
-                    qb[8 + s2] = Wg[(int64_t)qc * n + rowc];
+                    qb[s2] = bload(ra, (uint32_t)(qc * lda + rowc) * 8u, span);
+                    qb[8 + s2] = bload(rw, (uint32_t)(qc * n + rowc) * 8u, 0u);
                 }
 #pragma unroll
                 for (int s2 = 0; s2 < 8; ++s2) {
@@ -370,13 +395,13 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
 #pragma unroll
                     for (int s2 = 0; s2 < 8; ++s2) {
                         const int q = 4 * s2 + lk, qc = q < pw ? q : 0;
-                        pa[s2] = Wg[(int64_t)qc * n + colpc];
-                        pa[8 + s2] = A[colpc + (int64_t)(j0 + qc) * lda];
+                        pa[s2] = bload(rw, (uint32_t)(qc * n + colpc) * 8u, 0u);
+                        pa[8 + s2] = bload(ra, (uint32_t)(qc * lda + colpc) * 8u, span);
                     }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int col = c0 + 4 * v + lk;
-                        av[v] = A[rowc + (int64_t)(col < n ? col : n - 1) * lda];
+                        av[v] = bload(ra, (uint32_t)((col < n ? col : n - 1) * lda + rowc) * 8u, 0u);
                     }
                 };
                 auto tile = [&](int cB, double (&pa)[16], const double (&av)[4]) __attribute__((always_inline)) {
@@ -393,7 +418,7 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int col = c0 + 4 * v + lk;
-                        if (row < n && col <= row) A[row + (int64_t)col * lda] = av[v] - acc[v];
+                        if (row < n && col <= row) bstore(ra, (uint32_t)(col * lda + row) * 8u, 0u, av[v] - acc[v]);
                     }
                 };
                 double pa0[16], av0[4], pa1[16], av1[4];
