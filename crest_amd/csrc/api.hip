@@ -68,6 +68,11 @@ struct Ctx {
     std::map<std::string, std::pair<void *, size_t>> pinned;  // host staging (hipHostMalloc)
     hipEvent_t ev[12] = {};
     hipStream_t copy = nullptr;        // host -> device uploads of streamed chunks
+    // a second compute stream (default priority) with its own rocBLAS handle, for work
+    // that overlaps the main stream's (the batch's n x n duals)
+    hipStream_t side = nullptr;
+    rocblas_handle blas_side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t copied[2] = {}, consumed[2] = {};
     // batch of subsets: the two captured graphs and the arguments they were captured for
     std::string graph_key;
@@ -82,6 +87,11 @@ struct Ctx {
         HIPCHK(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
         BLASCHK(rocblas_create_handle(&blas));
         BLASCHK(rocblas_set_stream(blas, st));
+        HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        BLASCHK(rocblas_create_handle(&blas_side));
+        BLASCHK(rocblas_set_stream(blas_side, side));
+        HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         for (int i = 0; i < 2; ++i) {
             HIPCHK(hipEventCreateWithFlags(&copied[i], hipEventDisableTiming));
@@ -134,6 +144,7 @@ struct Ctx {
     }
     void release() {
         HIPCHK(hipStreamSynchronize(copy));
+        HIPCHK(hipStreamSynchronize(side));
         drop_graphs();
         for (auto &kv : bufs)
             if (kv.second.first) hipFree(kv.second.first);
@@ -1022,18 +1033,18 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 // ~480 ms), against ~6.2 ms per matrix for rocSOLVER's batched dsytrd — and rocSOLVER
 // for smaller groups.  CREST_SYTRD=wg|rocsolver forces either.
 static void batch_sytrd(Ctx &x, const std::string &pre, double *A, int n, int ng, double *Dg, double *Eg,
-                        double *tau) {
+                        double *tau, hipStream_t st, rocblas_handle blas) {
     const char *e = std::getenv("CREST_SYTRD");
     const std::string mode = e ? e : "auto";
     const bool wg = sytrd_wg_supported(n) && (mode == "wg" || (mode != "rocsolver" && ng >= 48));
     const int64_t nn = (int64_t)n * n;
     if (!wg) {
-        BLASCHK(rocsolver_dsytrd_strided_batched(x.blas, rocblas_fill_lower, n, A, n, nn, Dg, n, Eg, n, tau, n, ng));
+        BLASCHK(rocsolver_dsytrd_strided_batched(blas, rocblas_fill_lower, n, A, n, nn, Dg, n, Eg, n, tau, n, ng));
         return;
     }
     double *W = x.get<double>(pre + "sytrd_w", (size_t)ng * sytrd_wg_scratch(n));
     sytrd_set_attrs();
-    launch_sytrd_lower(A, nn, n, Dg, n, Eg, n, tau, n, W, n, ng, x.st);
+    launch_sytrd_lower(A, nn, n, Dg, n, Eg, n, tau, n, W, n, ng, st);
     x.check_launch();
 }
 
@@ -1077,6 +1088,75 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
                                 x.st);
         x.check_launch();
     };
+    // ---- the n x n duals, bucket by bucket, on the side stream: they run while the
+    // covariances below are tridiagonalized (one workgroup per matrix leaves CUs idle).
+    // Every bucket and group has buffers of its own; they write only their own
+    // subsets' variance, loadings and totals.
+    struct DualGroup {
+        const std::vector<int32_t> *L;
+        int g0, ng;
+        int *tinfo;
+    };
+    std::vector<DualGroup> dgs;
+    std::vector<std::vector<SegDesc>> seg_host;  // kept until the join (async uploads)
+    bool any_dual = false;
+    for (auto &v : dual) any_dual = any_dual || !v.empty();
+    if (any_dual) {
+        HIPCHK(hipEventRecord(x.fork, x.st));
+        HIPCHK(hipStreamWaitEvent(x.side, x.fork, 0));
+        const DevChunk c = batch_chunk(b, P, B);
+        const DevScale sc = batch_scale(b, P, B);
+        int64_t base = (int64_t)primal.size();
+        for (int bi = 0; bi < 4; ++bi) {
+            const int m = kBuckets[bi];
+            const std::vector<int32_t> &L = dual[bi];
+            for (int g0 = 0; g0 < (int)L.size(); g0 += (int)P.eig_batch) {
+                const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)L.size() - g0);
+                const int32_t *ids = ids_dev + base + g0;
+                const std::string pf = P.pre + "dual" + std::to_string(bi) + "_" + std::to_string(g0) + "_";
+                seg_host.emplace_back(ng);
+                std::vector<SegDesc> &segs = seg_host.back();
+                for (int i = 0; i < ng; ++i) {
+                    const int y = L[g0 + i];
+                    segs[i] = SegDesc{P.off[y], P.off[y + 1] - P.off[y], 0, y, 0};
+                }
+                SegDesc *segs_dev = x.get<SegDesc>(pf + "segs", (size_t)ng);
+                HIPCHK(hipMemcpyAsync(segs_dev, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice,
+                                      x.side));
+                const int64_t mm = (int64_t)m * m, md = (int64_t)m * P.d, mp = (int64_t)m * P.kp;
+                double *zc = x.get<double>(pf + "zc", (size_t)(ng * md));
+                double *K = x.get<double>(pf + "k", (size_t)(ng * mm));
+                double *V = x.get<double>(pf + "v", (size_t)ng * P.d * P.kp);
+                double *ev = x.get<double>(pf + "ev", (size_t)ng * P.kp);
+                double *Dg = x.get<double>(pf + "d", (size_t)ng * m);
+                double *Eg = x.get<double>(pf + "e", (size_t)ng * m);
+                double *tau = x.get<double>(pf + "tau", (size_t)ng * m);
+                double *tw = x.get<double>(pf + "tw", (size_t)ng * P.kp);
+                double *ty = x.get<double>(pf + "ty", (size_t)(ng * mp));
+                double *tx = x.get<double>(pf + "tx", (size_t)(ng * mp));
+                double *fac = x.get<double>(pf + "fac", (size_t)(3 * ng * mp));
+                uint8_t *swp = x.get<uint8_t>(pf + "swp", (size_t)(ng * mp));
+                int *tinfo = x.get<int>(pf + "tinfo", (size_t)ng);
+                launch_dense_centred(c, sc, segs_dev, ng, m, B.colsum, P.d_pad, zc, x.side);
+                const double one = 1.0, zero = 0.0;
+                BLASCHK(rocblas_dgemm_strided_batched(x.blas_side, rocblas_operation_transpose, rocblas_operation_none,
+                                                      m, m, P.d, &one, zc, P.d, md, zc, P.d, md, &zero, K, m, mm, ng));
+                batch_sytrd(x, pf, K, m, ng, Dg, Eg, tau, x.side, x.blas_side);
+                launch_fill_u32(reinterpret_cast<uint32_t *>(tinfo), ng, 0u, x.side);
+                launch_tri_topp(K, mm, m, tau, m, Dg, m, Eg, m, m, P.kp, ng, tw, ty, tx, fac, swp, tinfo, x.side);
+                BLASCHK(rocblas_dgemm_strided_batched(x.blas_side, rocblas_operation_none, rocblas_operation_none, P.d,
+                                                      P.kp, m, &one, zc, P.d, md, tx, m, mp, &zero, V, P.d,
+                                                      (int64_t)P.d * P.kp, ng));
+                launch_dual_finish(V, tw, P.d, P.kp, P.d_off, ids, ng, ev, x.side);
+                launch_eig_select_batch(V, ev, P.d, P.kp, ng, B.var, B.load, x.side, P.kp, ids);
+                launch_trace_gram(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ids, ng, B.tot, x.side);
+                x.check_launch();
+                dgs.push_back(DualGroup{&L, g0, ng, tinfo});
+            }
+            base += (int64_t)L.size();
+        }
+        HIPCHK(hipEventRecord(x.join, x.side));
+    }
     // ---- the d x d covariances
     for (int g0 = 0; g0 < (int)primal.size(); g0 += (int)P.eig_batch) {
         const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)primal.size() - g0);
@@ -1084,7 +1164,7 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
         launch_cov_batch(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ng, B.cov, B.tot, x.st, ids);
         x.check_launch();
         if (topp) {
-            batch_sytrd(x, P.pre, B.cov, P.d, ng, B.w, B.e, B.tau);
+            batch_sytrd(x, P.pre, B.cov, P.d, ng, B.w, B.e, B.tau, x.st, x.blas);
             launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
             launch_tri_topp(B.cov, dd, P.d, B.tau, P.d, B.w, P.d, B.e, P.d, P.d, P.kp, ng, B.topw, B.y, B.x, B.fac,
                             B.swp, B.tinfo, x.st);
@@ -1107,49 +1187,17 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
         launch_eig_select_batch(B.cov, B.w, P.d, P.kp, ng, B.var, B.load, x.st, 0, ids);
         x.check_launch();
     }
-    // ---- the n x n duals, bucket by bucket
-    int64_t base = (int64_t)primal.size();
-    const DevChunk c = batch_chunk(b, P, B);
-    const DevScale sc = batch_scale(b, P, B);
-    for (int bi = 0; bi < 4; ++bi) {
-        const int m = kBuckets[bi];
-        const std::vector<int32_t> &L = dual[bi];
-        for (int g0 = 0; g0 < (int)L.size(); g0 += (int)P.eig_batch) {
-            const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)L.size() - g0);
-            const int32_t *ids = ids_dev + base + g0;
-            std::vector<SegDesc> segs(ng);
-            for (int i = 0; i < ng; ++i) {
-                const int y = L[g0 + i];
-                segs[i] = SegDesc{P.off[y], P.off[y + 1] - P.off[y], 0, y, 0};
-            }
-            SegDesc *segs_dev = x.get<SegDesc>(P.pre + "dual_segs", (size_t)ng);
-            HIPCHK(hipMemcpyAsync(segs_dev, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, x.st));
-            double *zc = x.get<double>(P.pre + "dual_zc", (size_t)ng * m * P.d);
-            double *kv = B.cov;  // K [ng][m x m], then V [ng][d x p] (K is dead by then)
-            double *ev = x.get<double>(P.pre + "dual_ev", (size_t)ng * P.kp);
-            const int64_t mm = (int64_t)m * m, md = (int64_t)m * P.d;
-            launch_dense_centred(c, sc, segs_dev, ng, m, B.colsum, P.d_pad, zc, x.st);
-            const double one = 1.0, zero = 0.0;
-            BLASCHK(rocblas_dgemm_strided_batched(x.blas, rocblas_operation_transpose, rocblas_operation_none, m, m, P.d,
-                                                  &one, zc, P.d, md, zc, P.d, md, &zero, kv, m, mm, ng));
-            batch_sytrd(x, P.pre, kv, m, ng, B.w, B.e, B.tau);
-            launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);
-            launch_tri_topp(kv, mm, m, B.tau, m, B.w, m, B.e, m, m, P.kp, ng, B.topw, B.y, B.x, B.fac, B.swp, B.tinfo,
-                            x.st);
-            BLASCHK(rocblas_dgemm_strided_batched(x.blas, rocblas_operation_none, rocblas_operation_none, P.d, P.kp, m,
-                                                  &one, zc, P.d, md, B.x, m, (int64_t)m * P.kp, &zero, kv, P.d,
-                                                  (int64_t)P.d * P.kp, ng));
-            launch_dual_finish(kv, B.topw, P.d, P.kp, P.d_off, ids, ng, ev, x.st);
-            launch_eig_select_batch(kv, ev, P.d, P.kp, ng, B.var, B.load, x.st, P.kp, ids);
-            launch_trace_gram(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ids, ng, B.tot, x.st);
-            x.check_launch();
-            std::vector<int> hi(ng);
-            x.download(hi.data(), B.tinfo, ng);
+    // ---- join: the duals' convergence flags, then their fallbacks on the main stream
+    if (!dgs.empty()) {
+        HIPCHK(hipStreamWaitEvent(x.st, x.join, 0));
+        for (const DualGroup &g : dgs) {
+            std::vector<int> hi(g.ng);
+            x.download(hi.data(), g.tinfo, g.ng);
             x.sync();
-            for (int i = 0; i < ng; ++i)
-                if (hi[i]) fallback(L[g0 + i]);
+            for (int i = 0; i < g.ng; ++i)
+                if (hi[i]) fallback((*g.L)[g.g0 + i]);
         }
-        base += (int64_t)L.size();
+        x.sync();
     }
 }
 
