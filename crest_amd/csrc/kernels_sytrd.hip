@@ -32,7 +32,6 @@ constexpr int kSyWaves = 8;     // 512 threads
 constexpr int kSyThreads = kSyWaves * 64;
 constexpr int kSyRU = 4;        // rows per thread: row r belongs to thread r % 512
 constexpr int kSyMaxN = kSyThreads * kSyRU;
-constexpr int kSyStage = 66;    // sizes the per-wave LDS vectors (at least 16 x 66 doubles)
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline void lds_barrier() {
@@ -122,8 +121,7 @@ __device__ inline void panel_correct(double (&a)[kSyRU], BufRsrc ra, int lda, Bu
 
 size_t sytrd_wg_lds(int n) {
     const int ldn = (n + 63) & ~63;
-    const int ypart = kSyWaves * (ldn > 16 * kSyStage ? ldn : 16 * kSyStage);
-    return (size_t)(ypart + ldn + 64 + kSyWaves * 64 + kSyWaves + 3 * kSyNB + 8) * sizeof(double);
+    return (size_t)(kSyWaves * ldn + ldn + 64 + kSyWaves * 64 + kSyWaves + 3 * kSyNB + 8) * sizeof(double);
 }
 bool sytrd_wg_supported(int n) { return n >= 1 && n <= kSyMaxN && sytrd_wg_lds(n) <= 160 * 1024; }
 
@@ -133,8 +131,8 @@ __global__ __launch_bounds__(kSyThreads) void k_sytrd_lower(double *A, int64_t s
                                                             double *Wg, int n) {
     extern __shared__ double lds[];
     const int ldn = (n + 63) & ~63;
-    const int ylen = ldn > 16 * kSyStage ? ldn : 16 * kSyStage;
-    double *ypart = lds;                        // [8][ylen]: symv partials, or syr2k staging
+    const int ylen = ldn;
+    double *ypart = lds;                        // [8][ylen]: the waves' partial products
     // [ldn + 64] current reflector, by row; zero beyond n (the last column tile of the
     // product reads up to 63 entries past the matrix)
     double *vl = ypart + kSyWaves * ylen;
