@@ -1048,7 +1048,19 @@ static void batch_sytrd(Ctx &x, const std::string &pre, double *A, int n, int ng
     x.check_launch();
 }
 
+static void batch_eig_impl(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B);
+// Any error leaves no work of this call queued on the side stream (its uploads read
+// host vectors that die with the call).
 static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
+    try {
+        batch_eig_impl(x, b, P, B);
+    } catch (...) {
+        (void)hipStreamSynchronize(x.side);
+        throw;
+    }
+}
+
+static void batch_eig_impl(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
     const bool topp = use_topp(P.d, P.kp, (int)std::min<int64_t>(P.eig_batch, P.n_sub));
     if (topp) tri_set_attrs();
     const char *dual_env = std::getenv("CREST_BATCH_DUAL");
