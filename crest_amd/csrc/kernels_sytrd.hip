@@ -13,12 +13,15 @@
 // rank-2NB update brings the trailing lower triangle up to date.
 //
 // A matrix of order 2,000 is 32 MB, so a batch streams from HBM: the symmetric
-// matrix-vector product reads each lower-triangle element once (64 x 64 tiles, one per
-// wave at a time: the row part accumulates per lane, the transposed part is summed
-// across lanes by a halving exchange), so the whole reduction reads ~n^3/6 doubles per
-// matrix, half of what a full-storage product reads.  The partial products of the eight
-// waves go to per-wave LDS vectors and are summed in wave order, so the result does
-// not depend on scheduling.
+// matrix-vector product reads each lower-triangle element once (64-row x 32-column
+// tiles; a wave walks down one 32-column block, the row part accumulating per lane,
+// the transposed part per lane and column, summed across lanes once per block by a
+// halving exchange), so the whole reduction reads ~n^3/6 doubles per matrix, half of
+// what a full-storage product reads.  The partial products of the eight waves go to
+// per-wave LDS vectors and are summed in wave order, so the result does not depend on
+// scheduling.  The trailing update after each panel runs on FP64 MFMA.  Loads go
+// through buffer descriptors with the column offset in a scalar register, so a tile's
+// 32 loads share one address register.
 #include <cmath>
 
 #include "common.hpp"
