@@ -417,11 +417,9 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
         }
         t = tau[i];
     };
-    double vc[RT], vn[RT], vnn[RT], tc = 0.0, tn = 0.0, tnn = 0.0;
-    if (n >= 2) load_v(n - 2, vc, tc);
-    if (n >= 3) load_v(n - 3, vn, tn);
-    for (int i = n - 2; i >= 0; --i) {
-        if (i > 1) load_v(i - 2, vnn, tnn);
+    // reflector i applied to the block: dot products, their sum over the four waves
+    // through a double-buffered LDS slot (one barrier that waits for LDS only), update
+    auto apply = [&](int i, const double (&v)[RT], double t) __attribute__((always_inline)) {
         double *slot = &red[i & 1][0][0];
         double part[CB];
 #pragma unroll
@@ -429,8 +427,8 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
             double a0 = 0.0, a1 = 0.0;
 #pragma unroll
             for (int u = 0; u < RT; u += 2) {
-                a0 += vc[u] * y[c][u];
-                if (u + 1 < RT) a1 += vc[u + 1] * y[c][u + 1];
+                a0 += v[u] * y[c][u];
+                if (u + 1 < RT) a1 += v[u + 1] * y[c][u + 1];
             }
             part[c] = a0 + a1;
         }
@@ -443,17 +441,29 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
         __builtin_amdgcn_s_barrier();
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-            const double w = tc * (((slot[c] + slot[CB + c]) + slot[2 * CB + c]) + slot[3 * CB + c]);
+            const double w = t * (((slot[c] + slot[CB + c]) + slot[2 * CB + c]) + slot[3 * CB + c]);
 #pragma unroll
-            for (int u = 0; u < RT; ++u) y[c][u] -= w * vc[u];
+            for (int u = 0; u < RT; ++u) y[c][u] -= w * v[u];
         }
-#pragma unroll
-        for (int u = 0; u < RT; ++u) {
-            vc[u] = vn[u];
-            vn[u] = vnn[u];
-        }
-        tc = tn;
-        tn = tnn;
+    };
+    // four register sets in a ring: the entries of the reflector three steps ahead are
+    // loaded while the current one is applied
+    double v0[RT], v1[RT], v2[RT], v3[RT], t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (n >= 2) load_v(n - 2, v0, t0);
+    if (n >= 3) load_v(n - 3, v1, t1);
+    if (n >= 4) load_v(n - 4, v2, t2);
+    for (int i = n - 2; i >= 0; i -= 4) {
+        if (i >= 3) load_v(i - 3, v3, t3);
+        apply(i, v0, t0);
+        if (i < 1) break;
+        if (i >= 4) load_v(i - 4, v0, t0);
+        apply(i - 1, v1, t1);
+        if (i < 2) break;
+        if (i >= 5) load_v(i - 5, v1, t1);
+        apply(i - 2, v2, t2);
+        if (i < 3) break;
+        if (i >= 6) load_v(i - 6, v2, t2);
+        apply(i - 3, v3, t3);
     }
 #pragma unroll
     for (int c = 0; c < CB; ++c)
