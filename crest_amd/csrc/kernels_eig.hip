@@ -474,6 +474,122 @@ __global__ __launch_bounds__(256) void k_tri_back(const double *A, int64_t sA, i
         }
 }
 
+// The same X = Q Y with the reflectors taken four at a time: one reduction per group
+// gives the four blocks of dot products v_a^T y and the six products v_a^T v_b, and
+// the group's four updates follow from the recurrence
+//   w_a = tau_a (v_a^T y - sum_{b<a} w_b v_a^T v_b),  y -= w_0 v_0 + ... + w_3 v_3
+// (v_0 = reflector i, applied first, down to v_3 = reflector i-3), so there is one
+// barrier per four reflectors.  The next group's entries load during the current
+// group's update.  grid (ceil(p/CB), n_mat), 256 threads, n <= 256 RT.
+template <int CB, int RT>
+__global__ __launch_bounds__(256) void k_tri_back4(const double *A, int64_t sA, int lda, const double *tau,
+                                                   int64_t sTau, int n, int p, const double *Y, int64_t sY, double *X,
+                                                   int64_t sX) {
+    constexpr int NV = 4 * CB + 6;  // dot products and reflector products per group
+    __shared__ double red[2][4][NV];
+    const int mat = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    A += mat * sA;
+    tau += mat * sTau;
+    Y += mat * sY;
+    X += mat * sX;
+    const int cb0 = blockIdx.x * CB;
+    const int ncol = p - cb0 < CB ? p - cb0 : CB;
+    double y[CB][RT];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            const int r = tid + 256 * u;
+            y[c][u] = (r < n && c < ncol) ? Y[(int64_t)r * p + cb0 + c] : 0.0;
+        }
+    // v entries of reflector i for the thread's rows (1 at row i+1, A(r, i) below, 0
+    // above); reflector i < 0 is the identity (v = 0, tau = 0)
+    auto load_v = [&](int i, double (&v)[RT], double &t) __attribute__((always_inline)) {
+        const int ic = i >= 0 ? i : 0;
+        const double *col = A + (int64_t)ic * lda;
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            const int r = tid + 256 * u;
+            v[u] = (i >= 0 && r < n && r > i + 1) ? col[r] : ((i >= 0 && r == i + 1) ? 1.0 : 0.0);
+        }
+        t = i >= 0 ? tau[ic] : 0.0;
+    };
+    double va[4][RT], vb[4][RT], ta[4], tb[4];
+    auto group = [&](int i, int g, double (&v)[4][RT], const double (&t)[4], double (&vn)[4][RT], double (&tn)[4])
+                     __attribute__((always_inline)) {
+        double part[NV];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                double s0 = 0.0;
+#pragma unroll
+                for (int u = 0; u < RT; ++u) s0 += v[a][u] * y[c][u];
+                part[a * CB + c] = s0;
+            }
+        int k = 4 * CB;
+#pragma unroll
+        for (int a = 1; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < a; ++b) {
+                double s0 = 0.0;
+#pragma unroll
+                for (int u = 0; u < RT; ++u) s0 += v[a][u] * v[b][u];
+                part[k++] = s0;
+            }
+        wave_sum_n<NV>(part);
+        double *slot = &red[g & 1][0][0];
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) slot[wave * NV + j] = part[j];
+        // the next group's entries load while this group is reduced and applied
+#pragma unroll
+        for (int a = 0; a < 4; ++a) load_v(i - 4 - a, vn[a], tn[a]);
+        // s_waitcnt lgkmcnt(0) (vmcnt and expcnt left at their maxima), then the barrier
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_s_barrier();
+        double gm[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            gm[j] = ((slot[4 * CB + j] + slot[NV + 4 * CB + j]) + slot[2 * NV + 4 * CB + j]) + slot[3 * NV + 4 * CB + j];
+        // gm: v1.v0, v2.v0, v2.v1, v3.v0, v3.v1, v3.v2
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            double sv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                sv[a] = ((slot[a * CB + c] + slot[NV + a * CB + c]) + slot[2 * NV + a * CB + c]) +
+                        slot[3 * NV + a * CB + c];
+            const double w0 = t[0] * sv[0];
+            const double w1 = t[1] * (sv[1] - w0 * gm[0]);
+            const double w2 = t[2] * ((sv[2] - w0 * gm[1]) - w1 * gm[2]);
+            const double w3 = t[3] * (((sv[3] - w0 * gm[3]) - w1 * gm[4]) - w2 * gm[5]);
+#pragma unroll
+            for (int u = 0; u < RT; ++u) {
+                y[c][u] -= w0 * v[0][u];
+                y[c][u] -= w1 * v[1][u];
+                y[c][u] -= w2 * v[2][u];
+                y[c][u] -= w3 * v[3][u];
+            }
+        }
+    };
+#pragma unroll
+    for (int a = 0; a < 4; ++a) load_v(n - 2 - a, va[a], ta[a]);
+    int g = 0;
+    for (int i = n - 2; i >= 0; i -= 8, g += 2) {
+        group(i, g, va, ta, vb, tb);
+        if (i - 4 < 0) break;
+        group(i - 4, g + 1, vb, tb, va, ta);
+    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            const int r = tid + 256 * u;
+            if (r < n && c < ncol) X[(int64_t)(cb0 + c) * n + r] = y[c][u];
+        }
+}
+
 // Scratch of launch_tri_topp per matrix: 3 n p doubles (fac) and n p bytes (swp).
 // LDS limits of the three kernels for order n (0 when the order is too large).
 size_t tri_vectors_lds(int n) { return (size_t)2 * n * sizeof(double); }
@@ -496,8 +612,13 @@ void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, in
                        sE, n, p, W, sW);
     hipLaunchKernelGGL(k_tri_vectors, dim3(p, n_mat), dim3(64), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
                        (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
-    // columns per block: the thread's rows of CB columns stay in registers (CB * RT <= 64)
-    if (n <= 2048)
+    // columns per block: the thread's rows of CB columns stay in registers.  Fewer
+    // blocks than CUs (one matrix): four reflectors per barrier; a batch fills the GPU
+    // with one reflector per barrier at twice the occupancy.
+    if (n <= 2048 && (int64_t)n_mat * ((p + 3) / 4) < 256)
+        hipLaunchKernelGGL((k_tri_back4<4, 8>), dim3((p + 3) / 4, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
+                           p, Y, sY, X, sY);
+    else if (n <= 2048)
         hipLaunchKernelGGL((k_tri_back<8, 8>), dim3((p + 7) / 8, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
                            p, Y, sY, X, sY);
     else
