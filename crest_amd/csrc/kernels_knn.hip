@@ -778,6 +778,22 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
             c1 = mfma16<TY>(a, b1[s], c1);
         }
     };
+    // the same, split: the A operands of fragment f from LDS, then its MFMAs, so that
+    // the next fragment's operands are read while this one's MFMAs run
+    auto lda = [&](const u32x4 *stage, int f, u32x4 (&a)[KS]) {
+        const u32x4 *base = stage + (f >> 1) * BU + (f & 1) * 32 + col;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) a[s] = base[(2 * s + half) * kTC];
+    };
+    auto mmaa = [&](const u32x4 (&a)[KS], f32x16 &c0, f32x16 &c1) {
+        c0 = f32x16{};
+        c1 = f32x16{};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            c0 = mfma16<TY>(a[s], b0[s], c0);
+            c1 = mfma16<TY>(a[s], b1[s], c1);
+        }
+    };
     auto update = [&](f32x16 &c0, f32x16 &c1, uint32_t cb) {
         if (self && (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32)) {
             self_mask<L>(c0, cb, half, qa);
@@ -842,11 +858,15 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
         const u32x4 *stage = lds[t & 1];
         const uint32_t c0 = (uint32_t)(stage_at(t) * kTC * TB);
         if constexpr (screen_pipe<KC, L>()) {
+            u32x4 ae[KS], ao[KS];  // operands of the even / odd fragment
+            lda(stage, 0, ae);
 #pragma unroll
             for (int f = 0; f < NF; f += 2) {
-                mma(stage, f, x0, x1);
+                lda(stage, f + 1, ao);
+                mmaa(ae, x0, x1);
                 if (f > 0 || t > 0) update(y0, y1, cy);
-                mma(stage, f + 1, y0, y1);
+                if (f + 2 < NF) lda(stage, f + 2, ae);
+                mmaa(ao, y0, y1);
                 cy = c0 + (uint32_t)((f + 1) * 32);
                 update(x0, x1, c0 + (uint32_t)(f * 32));
             }
