@@ -73,6 +73,9 @@ struct Ctx {
     hipStream_t side = nullptr;
     rocblas_handle blas_side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // the Gram's tile pipeline: tile buffer b densified on the side stream, then read by
+    // the Gram on the main stream
+    hipEvent_t tile_ready[2] = {}, tile_free[2] = {};
     hipEvent_t copied[2] = {}, consumed[2] = {};
     // batch of subsets: the two captured graphs and the arguments they were captured for
     std::string graph_key;
@@ -92,6 +95,10 @@ struct Ctx {
         BLASCHK(rocblas_set_stream(blas_side, side));
         HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        for (int i = 0; i < 2; ++i) {
+            HIPCHK(hipEventCreateWithFlags(&tile_ready[i], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&tile_free[i], hipEventDisableTiming));
+        }
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         for (int i = 0; i < 2; ++i) {
             HIPCHK(hipEventCreateWithFlags(&copied[i], hipEventDisableTiming));
@@ -303,23 +310,49 @@ static int choose_split(int nblk, int64_t rows_pad) {
 }
 
 // Gram + colsum of the rows of a device chunk, accumulated into dev_gram/dev_colsum.
+// With more than one tile, the tiles go through two buffers: tile t+1 is densified on
+// the side stream while the Gram of tile t runs on the main stream (densify is bound by
+// its tile writes, the Gram by L2 and MFMA, so the two share the CUs well).  Densify's
+// exact lo^2 diagonal then accumulates into a buffer of its own, added to the Gram's
+// diagonal at the end, since the split-K reduce adds into dev_gram without atomics.
 static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, int d_pad, int64_t tile_rows,
                       double *dev_gram, double *dev_colsum) {
     const int fmt = tile_fmt(prec);
     const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
-    for (int64_t r0 = 0; r0 < c.n_spans; r0 += tile_rows) {
-        const int64_t r1 = std::min<int64_t>(c.n_spans, r0 + tile_rows);
+    const int64_t n_tiles = ceil_div(c.n_spans, tile_rows);
+    const bool pipe = n_tiles > 1;
+    double *gdiag = dev_gram;
+    if (pipe) {
+        gdiag = x.get<double>("gram_diag_acc", (size_t)d_pad * d_pad);
+        launch_fill_f64(gdiag, (int64_t)d_pad * d_pad, 0.0, x.st);
+        HIPCHK(hipEventRecord(x.fork, x.st));
+        HIPCHK(hipStreamWaitEvent(x.side, x.fork, 0));
+    }
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        const int64_t r0 = t * tile_rows, r1 = std::min<int64_t>(c.n_spans, r0 + tile_rows);
         const int64_t rows_pad = round_up(r1 - r0, 16);
         const size_t tile_elems = (size_t)rows_pad * d_pad;
-        void *ta = x.get<uint8_t>("tile_a", tile_elems * tile_elem_bytes(fmt));
-        void *tb = (fmt == TILE_BF16X2 || fmt == TILE_F16X2) ? x.get<uint8_t>("tile_b", tile_elems * 2) : nullptr;
+        const int b = pipe ? (int)(t & 1) : 0;
+        const std::string sfx = b ? "1" : "";
+        void *ta = x.get<uint8_t>("tile_a" + sfx, tile_elems * tile_elem_bytes(fmt));
+        void *tb = (fmt == TILE_BF16X2 || fmt == TILE_F16X2) ? x.get<uint8_t>("tile_b" + sfx, tile_elems * 2)
+                                                             : nullptr;
         const int n_split = choose_split(nblk, rows_pad);
         double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
-        launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, dev_gram, x.st);
+        if (pipe) {
+            if (t >= 2) HIPCHK(hipStreamWaitEvent(x.side, x.tile_free[b], 0));  // the Gram of tile t-2 is done
+            launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, gdiag, x.side);
+            HIPCHK(hipEventRecord(x.tile_ready[b], x.side));
+            HIPCHK(hipStreamWaitEvent(x.st, x.tile_ready[b], 0));
+        } else {
+            launch_densify(sub_chunk(c, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, dev_colsum, dev_gram, x.st);
+        }
         launch_gram(fmt, ta, tb, rows_pad, d_pad, n_split, partial, x.st);
         launch_gram_reduce(partial, n_split, d_pad, dev_gram, x.st);
+        if (pipe) HIPCHK(hipEventRecord(x.tile_free[b], x.st));
         x.check_launch();
     }
+    if (pipe) launch_add_diag(gdiag, d_pad, dev_gram, x.st);  // after the last Gram, so after every densify
 }
 
 // Eigen-decomposition of the covariance from (gram [ld_g], colsum, n): variance and

@@ -128,6 +128,7 @@ void launch_densify_ranges(const DevChunk &c, const DevScale &sc, int fmt, void 
                            double *gram, hipStream_t s);
 void launch_project_ranges(const DevChunk &c, const DevScale &sc, const double *V, int k, const SegDesc *segs,
                            const RowRange *ranges, int64_t n_ranges, float *out, hipStream_t s);
+void launch_add_diag(const double *src, int d_pad, double *gram, hipStream_t s);
 void launch_gram_items(int fmt, const void *tile_a, const void *tile_b, int d_pad, const GramItem *items,
                        int64_t n_items, int grid, double *partial, double *gram, hipStream_t s);
 int gram_items_occupancy(int fmt);

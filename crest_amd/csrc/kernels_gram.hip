@@ -419,6 +419,16 @@ __global__ void k_gram_reduce(const double *partial, int n_split, int nb, int d_
     }
 }
 
+// gram[h, h] += src[h, h] for h < d_pad (both d_pad x d_pad): the densify's diagonal
+// terms accumulated apart from the split-K sums.
+__global__ void k_add_diag(const double *src, int d_pad, double *gram) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < d_pad) gram[(int64_t)h * d_pad + h] += src[(int64_t)h * d_pad + h];
+}
+void launch_add_diag(const double *src, int d_pad, double *gram, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_diag, dim3((unsigned)ceil_div(d_pad, 256)), dim3(256), 0, s, src, d_pad, gram);
+}
+
 // Work-list form: item r sums slots slot0 .. slot0 + n_slots - 1 (in order) of one
 // block and adds them into that block of its subset's Gram (gram + subset d_pad^2).
 // No two items of one launch name the same (subset, block), so the adds do not race.
