@@ -47,7 +47,7 @@ __device__ inline double cell_total(const DevChunk &c, int64_t lo, int64_t hi, i
 // accumulates a window of vars in LDS (FP64), then adds it to `out` once.
 // With `ranges`, workgroup b takes row range ranges[b] of segment segs[ranges[b].seg]
 // and accumulates into that segment's subset's block of `out` ([subset][n_vars][5]).
-__global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int n_vars, const SegDesc *segs,
+__global__ __launch_bounds__(1024) void k_gene_stats(DevChunk c, double *out, int n_vars, const SegDesc *segs,
                                                     const RowRange *ranges) {
     __shared__ double acc[5 * kStatsWindow];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -97,10 +97,13 @@ __global__ __launch_bounds__(256) void k_gene_stats(DevChunk c, double *out, int
     }
 }
 
+// One chunk: 1,024-thread workgroups (16 waves, one cell each at a time) so that the
+// two workgroups the 80 KB accumulator leaves room for per CU keep 32 waves' loads in
+// flight; at most two workgroups per CU are launched.
 void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s) {
     if (c.n_spans <= 0 || n_vars <= 0) return;
-    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 16), 1024);
-    hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(256), 0, s, c, out, n_vars, (const SegDesc *)nullptr,
+    int g = (int)std::min<int64_t>(ceil_div(c.n_spans, 64), 512);
+    hipLaunchKernelGGL(k_gene_stats, dim3(g), dim3(1024), 0, s, c, out, n_vars, (const SegDesc *)nullptr,
                        (const RowRange *)nullptr);
 }
 void launch_gene_stats_ranges(const DevChunk &c, const SegDesc *segs, const RowRange *ranges, int64_t n_ranges,
