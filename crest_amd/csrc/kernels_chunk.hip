@@ -109,7 +109,7 @@ void launch_gene_stats(const DevChunk &c, double *out, int n_vars, hipStream_t s
 void launch_gene_stats_ranges(const DevChunk &c, const SegDesc *segs, const RowRange *ranges, int64_t n_ranges,
                               double *out, int n_vars, hipStream_t s) {
     if (n_ranges <= 0 || n_vars <= 0) return;
-    hipLaunchKernelGGL(k_gene_stats, dim3((unsigned)n_ranges), dim3(256), 0, s, c, out, n_vars, segs, ranges);
+    hipLaunchKernelGGL(k_gene_stats, dim3((unsigned)n_ranges), dim3(1024), 0, s, c, out, n_vars, segs, ranges);
 }
 
 // ---------------------------------------------------------------- qc
