@@ -712,15 +712,17 @@ template <int TY> __device__ inline f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
 
 // Workgroup of NW waves, 64 queries per wave.  Two waves per SIMD fit when the
 // B fragments, both accumulator pairs and the lists leave room in 256 VGPRs
-// (checked against the compiler's register report: no spills).  Eight-wave
-// workgroups (half the tile traffic per query) are used where that budget holds.
+// (checked against the compiler's register report: no spills).  Four-wave
+// workgroups throughout: eight-wave ones halve the stage traffic per query but make
+// twice as many waves meet at every stage barrier, and with the A operands read a
+// fragment ahead four waves measured faster (1M x 50 FP16: 131.3 against 134.6 ms).
 // The software pipeline (two accumulator pairs, below) is used where it keeps two
 // waves per SIMD, or where one wave per SIMD is all that fits anyway.
 template <int KC, int L> constexpr bool screen_pipe() { return (L <= 16 && KC <= 16) || !(L <= 16 && KC <= 20); }
 template <int KC, int L> constexpr int screen_waves() {
     return screen_pipe<KC, L>() ? ((L <= 16 && KC <= 16) ? 2 : 1) : 2;
 }
-template <int TY, int KC, int L> constexpr int screen_nw() { return TY == 1 && L <= 16 && KC % 8 == 0 && KC <= 16 ? 8 : 4; }
+template <int TY, int KC, int L> constexpr int screen_nw() { return 4; }  // waves per workgroup
 // 64-candidate blocks per LDS stage: as many (up to four) as keep the
 // double-buffered stages within 64 KB of LDS.
 template <int KC> constexpr int screen_tb() { return KC <= 8 ? 4 : KC <= 16 ? 2 : 1; }
