@@ -642,6 +642,13 @@ template <class T, int N, int STRIDE> __device__ inline void regs_store(const Re
 // ld[j] = min(ld[j], max(d, ld[j-1])), and no value is carried from slot to slot.
 // Entries are +0.0 or positive (or +inf), so their bit patterns order as unsigned
 // integers and the min/max run on the integer unit without NaN canonicalisation.
+// The median of three unsigned integers in one instruction (v_med3_u32).
+__device__ inline uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 template <int L>
 __device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, uint32_t id) {
     const uint32_t db = __float_as_uint(d);
@@ -651,7 +658,8 @@ __device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, u
 #pragma unroll
     for (int j = L - 1; j > 0; --j) {
         li[j] = c[j - 1] ? li[j - 1] : (c[j] ? id : li[j]);
-        ld[j] = __uint_as_float(min(__float_as_uint(ld[j]), max(db, __float_as_uint(ld[j - 1]))));
+        // ld[j-1] <= ld[j]: min(ld[j], max(d, ld[j-1])) is the median of the three
+        ld[j] = __uint_as_float(med3_u32(db, __float_as_uint(ld[j - 1]), __float_as_uint(ld[j])));
     }
     li[0] = c[0] ? id : li[0];
     ld[0] = __uint_as_float(min(__float_as_uint(ld[0]), db));
