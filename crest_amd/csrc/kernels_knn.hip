@@ -672,11 +672,14 @@ __device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, u
 // threshold.
 template <int L>
 __device__ inline void topk_update(float (&ld)[L], uint32_t (&li)[L], const f32x16 &a, uint32_t cb, int half) {
-    float m = a[0];
+    // the minimum on the bit patterns as signed integers: non-negative floats order as
+    // their patterns, negative ones fall below any threshold (which is >= 0), so
+    // "some value < thr" is decided without canonicalising the MFMA outputs
+    int32_t m = __float_as_int(a[0]);
 #pragma unroll
-    for (int r = 1; r < 16; ++r) m = fminf(m, a[r]);
+    for (int r = 1; r < 16; ++r) m = min(m, __float_as_int(a[r]));
     const float thr = ld[L - 1];
-    if (__builtin_amdgcn_ballot_w64(m < thr) == 0) return;
+    if (__builtin_amdgcn_ballot_w64(m < __float_as_int(thr)) == 0) return;
     uint32_t u = 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) u |= (__builtin_amdgcn_ballot_w64(a[r] < thr) != 0 ? 1u : 0u) << r;
