@@ -123,6 +123,7 @@ __host__ __device__ inline DevChunk seg_rows(DevChunk c, int64_t a, int64_t b) {
 void launch_gene_stats_ranges(const DevChunk &c, const SegDesc *segs, const RowRange *ranges, int64_t n_ranges,
                               double *out, int n_vars, hipStream_t s);
 void densify_set_attrs();
+bool gram_dma_enabled();
 void launch_densify_ranges(const DevChunk &c, const DevScale &sc, int fmt, void *tile_a, void *tile_b, int d_pad,
                            const SegDesc *segs, const RowRange *ranges, int64_t n_ranges, double *colsum,
                            double *gram, hipStream_t s);

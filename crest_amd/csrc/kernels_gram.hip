@@ -171,6 +171,168 @@ __device__ inline void gram_x16_block(const uint4 *__restrict__ th, const uint4 
         }
 }
 
+// LDS-staged form of gram_x16_block with the same accumulation order and result.  Per
+// K-step the workgroup's distinct operands are hi and lo of its 128 A and 128 B
+// columns over both 8-row groups: 1,024 uint4 (16 KB), where the per-wave form loads
+// each of them twice (two waves share every A run, two every B run).  Here they go
+// from global memory straight into LDS (global_load_lds_dwordx4: each lane's 16 B
+// land at the wave's LDS base + 16·lane, no registers held), NB stages of KS K-steps
+// in a ring with NB-1 stages in flight, and the waves read their MFMA operands from
+// LDS.  LDS index of an operand within a stage: ks*1024 + q*256 + h*128 + c, q = 0 A
+// hi, 1 A lo, 2 B hi, 3 B lo; h = row group of the K-step; c = column of the run.
+// Waits are explicit: vmcnt counts this wave's loads in issue order, so waiting for
+// all but the (NB-2)·4·KS most recent ones completes stage st; the barrier after it
+// makes every wave's part of the stage visible.  A workgroup fence would wait for all
+// loads in flight (vmcnt(0)), so the barrier is a bare s_barrier between compiler-only
+// memory barriers.
+template <int KS, int NB> constexpr int gram_dma_lds_bytes() { return NB * KS * 1024 * 16; }
+// one K-step per stage, three stages (48 KB): measured against (K-steps, stages) =
+// (2, 3), (1, 4), (2, 4), (1, 2) and (1, 5) at 1,000,000 x 2,000, the fastest Gram stage
+// (23.6 ms against 24.0-27.9; a smaller ring also leaves LDS for the densify of the
+// next tile, which runs beside the Gram)
+constexpr int kGramKS = 1, kGramNB = 3;
+
+__device__ inline void lds_dma16(const u32x4 *src, u32x4 *dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+
+// s_waitcnt immediate for vmcnt(n), expcnt and lgkmcnt not waited on (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x70 | 0xF00; }
+
+__device__ inline void bare_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int TY, int FLUSH, int KS, int NB>
+__device__ inline void gram_x16_block_dma(const uint4 *__restrict__ th, const uint4 *__restrict__ tl, int d_pad,
+                                          int bi, int bj, int64_t p0, int64_t p1, double *dst, u32x4 *lds,
+                                          int ld = kGramBlock, bool add = false) {
+    static_assert(FLUSH % KS == 0, "the FP64 flush falls on stage boundaries");
+    static_assert(NB >= 2 && (NB - 2) * 4 * KS < 64, "vmcnt holds at most 63 loads");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+    const int half = lane >> 5, col = lane & 31;
+    f32x16 acc[2][2];
+    double dacc[2][2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (f32x16){};
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] = 0.0;
+        }
+    int cnt = 0;
+    auto mma = [&](const u32x4 *L) {
+        u32x4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            ah[i] = L[0 * 256 + half * 128 + wm * 64 + 32 * i + col];
+            al[i] = L[1 * 256 + half * 128 + wm * 64 + 32 * i + col];
+            bh[i] = L[2 * 256 + half * 128 + wn * 64 + 32 * i + col];
+            bl[i] = L[3 * 256 + half * 128 + wn * 64 + 32 * i + col];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = mfma_x16<TY>(ah[i], bh[j], acc[i][j]);
+                acc[i][j] = mfma_x16<TY>(ah[i], bl[j], acc[i][j]);
+                acc[i][j] = mfma_x16<TY>(al[i], bh[j], acc[i][j]);
+            }
+    };
+    // after every FLUSH K-steps counted from p0, as in gram_x16_block
+    auto flush = [&](int done) {
+        cnt += done;
+        if (cnt == FLUSH) {
+            cnt = 0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = (f32x16){};
+                }
+        }
+    };
+    if (p0 < p1) {
+        const u32x4 *th4 = reinterpret_cast<const u32x4 *>(th), *tl4 = reinterpret_cast<const u32x4 *>(tl);
+        const int64_t nst = (p1 - p0 + KS - 1) / KS;
+        const int h = tid >> 7;
+        const int wu = __builtin_amdgcn_readfirstlane(wave);
+        const int64_t ca = (int64_t)bi * kGramBlock + (tid & 127), cb = (int64_t)bj * kGramBlock + (tid & 127);
+        // stage st into ring slot st % NB; K-steps past p1 (the tail, and the stages
+        // issued past the last one so that every iteration waits on the same count)
+        // load the last K-step again: valid memory, never multiplied
+        auto issue = [&](int64_t st) {
+            u32x4 *B = lds + (int)(st % NB) * KS * 1024 + wu * 64;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                int64_t k = p0 + st * KS + ks;
+                k = k < p1 ? k : p1 - 1;
+                const int64_t u = (2 * k + h) * (int64_t)d_pad;
+                lds_dma16(th4 + u + ca, B + ks * 1024 + 0 * 256);
+                lds_dma16(tl4 + u + ca, B + ks * 1024 + 1 * 256);
+                lds_dma16(th4 + u + cb, B + ks * 1024 + 2 * 256);
+                lds_dma16(tl4 + u + cb, B + ks * 1024 + 3 * 256);
+            }
+        };
+#pragma unroll
+        for (int st = 0; st < NB - 1; ++st) issue(st);
+        for (int64_t st = 0; st < nst; ++st) {
+            // stages st .. st+NB-2 are in flight: complete stage st
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm((NB - 2) * 4 * KS));
+            // every wave's part of stage st is in LDS, and every wave has consumed
+            // stage st-1, whose slot the next issue refills
+            bare_barrier();
+            issue(st + NB - 1);
+            const u32x4 *L = lds + (int)(st % NB) * KS * 1024;
+            const int64_t left = p1 - p0 - st * KS;
+            if (left >= KS) {
+                // a full stage: straight-line code, so the LDS reads of K-step ks+1
+                // overlap the MFMAs of K-step ks
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) mma(L + ks * 1024);
+                flush(KS);
+            } else {
+                // the tail stage (left < KS K-steps; cnt stays below FLUSH)
+                for (int ks = 0; ks < left; ++ks) mma(L + ks * 1024);
+            }
+            asm volatile("" ::: "memory");
+        }
+        // no load may land in LDS after the workgroup ends
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+            store_partial32(dst, dacc[i][j], wm * 64 + i * 32, wn * 64 + j * 32, lane, ld, add);
+        }
+}
+
+template <int TY, int FLUSH, int KS, int NB>
+__global__ __launch_bounds__(256) void k_gram_x16_dma(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
+                                                         int64_t n_groups, int d_pad, int n_split, double *partial) {
+    extern __shared__ u32x4 gram_lds[];
+    const int nb = d_pad / kGramBlock;
+    int blk, split, bi, bj;
+    gram_work(n_split, nb, blk, split, bi, bj);
+    const int64_t n_pairs = n_groups / 2;
+    const int64_t per = (n_pairs + n_split - 1) / n_split;
+    const int64_t p0 = (int64_t)split * per;
+    const int64_t p1 = p0 + per < n_pairs ? p0 + per : n_pairs;
+    gram_x16_block_dma<TY, FLUSH, KS, NB>(th, tl, d_pad, bi, bj, p0, p1,
+                                          partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock,
+                                          gram_lds);
+}
+
 template <int TY, int FLUSH>
 __global__ __launch_bounds__(256) void k_gram_x16(const uint4 *__restrict__ th, const uint4 *__restrict__ tl,
                                                      int64_t n_groups, int d_pad, int n_split, double *partial) {
@@ -201,6 +363,28 @@ __global__ __launch_bounds__(256) void k_gram_x16_items(const uint4 *__restrict_
         tri_block(w.blk, nb, bi, bj);
         double *dst = item_dst(w, bi, bj, d_pad, partial, gram, ld, add);
         gram_x16_block<TY, FLUSH>(th, tl, d_pad, bi, bj, w.g0 / 2, w.g1 / 2, dst, ld, add);
+    }
+}
+
+// Work-list form of k_gram_x16_dma.  Between two items every wave has consumed the
+// previous item's last stage (its MFMAs read it before the barrier) before the next
+// item's first loads refill the ring.
+template <int TY, int FLUSH, int KS, int NB>
+__global__ __launch_bounds__(256) void k_gram_x16_items_dma(const uint4 *__restrict__ th,
+                                                               const uint4 *__restrict__ tl, int d_pad,
+                                                               const GramItem *items, int64_t n_items, double *partial,
+                                                               double *gram) {
+    extern __shared__ u32x4 gram_lds[];
+    const int nb = d_pad / kGramBlock;
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const GramItem w = items[it];
+        if (w.blk < 0) continue;
+        int bi, bj, ld;
+        bool add;
+        tri_block(w.blk, nb, bi, bj);
+        double *dst = item_dst(w, bi, bj, d_pad, partial, gram, ld, add);
+        __syncthreads();
+        gram_x16_block_dma<TY, FLUSH, KS, NB>(th, tl, d_pad, bi, bj, w.g0 / 2, w.g1 / 2, dst, gram_lds, ld, add);
     }
 }
 
@@ -391,15 +575,37 @@ void launch_gram(int fmt, const void *tile_a, const void *tile_b, int64_t rows_p
     case TILE_F32:
         hipLaunchKernelGGL(k_gram_f32<32>, grid, block, 0, s, (const float4 *)tile_a, n_groups, d_pad, n_split, partial);
         break;
-    default:
-        if (fmt == TILE_F16X2)
-            hipLaunchKernelGGL((k_gram_x16<1, 16>), grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
-                               n_groups, d_pad, n_split, partial);
-        else
-            hipLaunchKernelGGL((k_gram_x16<0, 16>), grid, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
-                               n_groups, d_pad, n_split, partial);
+    default: {
+        const uint4 *ta = (const uint4 *)tile_a, *tb = (const uint4 *)tile_b;
+        const bool dma = gram_dma_enabled();
+#define CREST_GRAM_DMA_LAUNCH(T, KS, NB)                                                                               \
+    do {                                                                                                               \
+        hipFuncSetAttribute((const void *)k_gram_x16_dma<T, 16, KS, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                            gram_dma_lds_bytes<KS, NB>());                                                             \
+        hipLaunchKernelGGL((k_gram_x16_dma<T, 16, KS, NB>), grid, block, (gram_dma_lds_bytes<KS, NB>()), s, ta, tb,   \
+                           n_groups, d_pad, n_split, partial);                                                         \
+    } while (0)
+        if (fmt == TILE_F16X2) {
+            if (dma) CREST_GRAM_DMA_LAUNCH(1, kGramKS, kGramNB);
+            else hipLaunchKernelGGL((k_gram_x16<1, 16>), grid, block, 0, s, ta, tb, n_groups, d_pad, n_split, partial);
+        } else {
+            if (dma) CREST_GRAM_DMA_LAUNCH(0, kGramKS, kGramNB);
+            else hipLaunchKernelGGL((k_gram_x16<0, 16>), grid, block, 0, s, ta, tb, n_groups, d_pad, n_split, partial);
+        }
+#undef CREST_GRAM_DMA_LAUNCH
         break;
     }
+    }
+}
+
+// Form of the split-16-bit Gram: LDS-staged through direct loads unless
+// CREST_GRAM_DMA=0, which selects the form that loads operands per wave.
+bool gram_dma_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("CREST_GRAM_DMA");
+        return !(e && e[0] == '0' && e[1] == 0);
+    }();
+    return on;
 }
 
 // Deterministic split-K sum into the FP64 Gram, splits in order.
@@ -461,24 +667,43 @@ void launch_gram_items(int fmt, const void *tile_a, const void *tile_b, int d_pa
         hipLaunchKernelGGL(k_gram_f32_items<32>, g, block, 0, s, (const float4 *)tile_a, d_pad, items, n_items,
                            partial, gram);
         break;
-    default:
-        if (fmt == TILE_F16X2)
-            hipLaunchKernelGGL((k_gram_x16_items<1, 16>), g, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
-                               d_pad, items, n_items, partial, gram);
-        else
-            hipLaunchKernelGGL((k_gram_x16_items<0, 16>), g, block, 0, s, (const uint4 *)tile_a, (const uint4 *)tile_b,
-                               d_pad, items, n_items, partial, gram);
+    default: {
+        const uint4 *ta = (const uint4 *)tile_a, *tb = (const uint4 *)tile_b;
+        const int lds = gram_dma_lds_bytes<kGramKS, kGramNB>();
+        if (gram_dma_enabled()) {
+            if (fmt == TILE_F16X2) {
+                hipFuncSetAttribute((const void *)k_gram_x16_items_dma<1, 16, kGramKS, kGramNB>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                hipLaunchKernelGGL((k_gram_x16_items_dma<1, 16, kGramKS, kGramNB>), g, block, lds, s, ta, tb, d_pad,
+                                   items, n_items, partial, gram);
+            } else {
+                hipFuncSetAttribute((const void *)k_gram_x16_items_dma<0, 16, kGramKS, kGramNB>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                hipLaunchKernelGGL((k_gram_x16_items_dma<0, 16, kGramKS, kGramNB>), g, block, lds, s, ta, tb, d_pad,
+                                   items, n_items, partial, gram);
+            }
+        } else if (fmt == TILE_F16X2) {
+            hipLaunchKernelGGL((k_gram_x16_items<1, 16>), g, block, 0, s, ta, tb, d_pad, items, n_items, partial, gram);
+        } else {
+            hipLaunchKernelGGL((k_gram_x16_items<0, 16>), g, block, 0, s, ta, tb, d_pad, items, n_items, partial, gram);
+        }
         break;
+    }
     }
 }
 
 int gram_items_occupancy(int fmt) {
     int n = 0;
-    const void *f = fmt == TILE_F64 ? (const void *)k_gram_f64_items
+    const bool x16 = fmt != TILE_F64 && fmt != TILE_F32, dma = x16 && gram_dma_enabled();
+    const void *f = fmt == TILE_F64   ? (const void *)k_gram_f64_items
                     : fmt == TILE_F32 ? (const void *)k_gram_f32_items<32>
-                    : fmt == TILE_F16X2 ? (const void *)k_gram_x16_items<1, 16>
-                                        : (const void *)k_gram_x16_items<0, 16>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess || n < 1) n = 1;
+                    : dma ? (fmt == TILE_F16X2 ? (const void *)k_gram_x16_items_dma<1, 16, kGramKS, kGramNB>
+                                               : (const void *)k_gram_x16_items_dma<0, 16, kGramKS, kGramNB>)
+                          : (fmt == TILE_F16X2 ? (const void *)k_gram_x16_items<1, 16>
+                                               : (const void *)k_gram_x16_items<0, 16>);
+    const size_t lds = dma ? (size_t)gram_dma_lds_bytes<kGramKS, kGramNB>() : 0;
+    if (dma) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, lds) != hipSuccess || n < 1) n = 1;
     return n;
 }
 
