@@ -159,22 +159,38 @@ __device__ inline double start_value(int i, int j) {
 // loaded into registers before any of them is used, so their latency overlaps.
 constexpr int kTriChunk = 16;
 
-// grid (p, n_mat), 64 threads (one wave); dynamic LDS tri_vectors_lds(n).
+// grid (p, n_mat), 256 threads; dynamic LDS tri_vectors_lds(n).
 // Workgroup c computes the vectors of cluster c (if it exists) of matrix blockIdx.y:
-// lane t iterates vector c0 + t of the cluster (64 at a time), all with their own
-// shifts at once, and after every round the cluster's vectors are orthogonalised in
-// order (Gram-Schmidt, twice) and normalised.  Per matrix, the LU factors (fac:
+// lane t of wave 0 iterates vector c0 + t of the cluster (64 at a time), all with
+// their own shifts at once, and after every round the cluster's vectors are
+// orthogonalised in order (Gram-Schmidt, twice) and normalised by all four waves,
+// rows spread over the threads.  Per matrix, the LU factors (fac:
 // 3 arrays [n][p]: reciprocal pivots, U's first superdiagonal, multipliers; swp
 // [n][p] bytes: row swaps) and the vectors Y [n][p] are stored row-interleaved, so
 // the lanes' accesses of one step are contiguous.  info[mat] is set when a vector's
 // solve overflows or never reaches rounding-level residual.
-__global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD, const double *E, int64_t sE, int n,
+__global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD, const double *E, int64_t sE, int n,
                                                     int p, const double *W, int64_t sW, double *Y, int64_t sY,
                                                     double *fac, int64_t sF, uint8_t *swpb, int64_t sB, int *info) {
     extern __shared__ double sm[];
     double *d = sm, *e = sm + n;
     __shared__ double inv_s[64];
-    const int mat = blockIdx.y, lane = threadIdx.x;
+    __shared__ double gs_red[2][4][16];
+    const int mat = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // sum over the four waves of 16 values already summed within each wave, in wave
+    // order; the two LDS slots alternate, so a slot is rewritten only after the barrier
+    // of the reduction in between, which every thread passes after reading it
+    int red_i = 0;
+    auto block_sum16 = [&](double (&v)[16]) {
+        double *slot = &gs_red[red_i & 1][0][0];
+        ++red_i;
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) slot[wave * 16 + q] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = ((slot[q] + slot[16 + q]) + slot[32 + q]) + slot[48 + q];
+    };
     D += mat * sD;
     E += mat * sE;
     W += mat * sW;
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
     const double tiny = kUlp * tnorm > 0.0 ? kUlp * tnorm : kSafeMin;
     // couplings at rounding level split T (a perturbation no larger than rounding), so
     // no pivot of the factorisation is one of them
-    for (int i = lane; i < n; i += 64) {
+    for (int i = tid; i < n; i += 256) {
         d[i] = D[i];
         e[i] = i + 1 < n && fabs(E[i]) > tiny ? E[i] : 0.0;
     }
@@ -200,7 +216,7 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
     for (int g0 = c0; g0 < c1; g0 += 64) {
         const int g1 = c1 - g0 < 64 ? c1 : g0 + 64;
         const int j = g0 + lane;
-        const bool act = j < g1;
+        const bool act = wave == 0 && j < g1;
         if (act) {
             // shift: eigenvalues closer than pertol to the previous (shifted) one of the
             // cluster are moved up by pertol, so no two solves share a shift
@@ -332,7 +348,7 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
 #pragma unroll
                         for (int q = 0; q < 16; ++q) acc[q] = 0.0;
 #pragma unroll 4
-                        for (int r = lane; r < n; r += 64) {
+                        for (int r = tid; r < n; r += 256) {
                             const double *row = Y + (int64_t)r * p;
                             const double xt = row[t];
 #pragma unroll
@@ -340,8 +356,9 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
                                 if (q < ns) acc[q] += row[s0 + q] * xt;
                         }
                         wave_sum_n<16>(acc);
+                        block_sum16(acc);
 #pragma unroll 4
-                        for (int r = lane; r < n; r += 64) {
+                        for (int r = tid; r < n; r += 256) {
                             double *row = Y + (int64_t)r * p;
                             double v = row[t];
 #pragma unroll
@@ -351,15 +368,19 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
                         }
                     }
                 double nx = 0.0;
-                for (int r = lane; r < n; r += 64) {
+                for (int r = tid; r < n; r += 256) {
                     const double v = Y[(int64_t)r * p + t];
                     nx += v * v;
                 }
-                nx = sqrt(wave_sum(nx));
+                {
+                    double one[16] = {wave_sum(nx)};
+                    block_sum16(one);
+                    nx = sqrt(one[0]);
+                }
                 const bool ok = nx > 0.0 && isfinite(nx);
                 const double inv = ok ? 1.0 / nx : 0.0;
-                for (int r = lane; r < n; r += 64) Y[(int64_t)r * p + t] *= inv;
-                if (lane == 0) inv_s[t - g0] = ok ? inv : INFINITY;
+                for (int r = tid; r < n; r += 256) Y[(int64_t)r * p + t] *= inv;
+                if (tid == 0) inv_s[t - g0] = ok ? inv : INFINITY;
                 __threadfence_block();
                 __syncthreads();
             }
@@ -377,7 +398,7 @@ __global__ __launch_bounds__(64) void k_tri_vectors(const double *D, int64_t sD,
         }
         if (conv == 0) failed = true;
     }
-    if (failed && lane == 0) atomicOr(info + mat, 1);
+    if (failed && tid == 0) atomicOr(info + mat, 1);
 }
 
 // grid (ceil(p/CB), n_mat), 256 threads.
@@ -610,7 +631,7 @@ void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, in
     const int64_t sW = p, sY = (int64_t)n * p;
     hipLaunchKernelGGL(k_tri_bisect, dim3((p + 3) / 4, n_mat), dim3(256), (size_t)2 * n * sizeof(double), s, D, sD, E,
                        sE, n, p, W, sW);
-    hipLaunchKernelGGL(k_tri_vectors, dim3(p, n_mat), dim3(64), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
+    hipLaunchKernelGGL(k_tri_vectors, dim3(p, n_mat), dim3(256), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
                        (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
     // columns per block: the thread's rows of CB columns stay in registers.  Fewer
     // blocks than CUs (one matrix): four reflectors per barrier; a batch fills the GPU
