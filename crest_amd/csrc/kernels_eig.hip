@@ -159,29 +159,31 @@ __device__ inline double start_value(int i, int j) {
 // loaded into registers before any of them is used, so their latency overlaps.
 constexpr int kTriChunk = 16;
 
-// grid (p, n_mat), 256 threads; dynamic LDS tri_vectors_lds(n).
+// grid (p, n_mat), NT threads (64 or 256); dynamic LDS tri_vectors_lds(n).
 // Workgroup c computes the vectors of cluster c (if it exists) of matrix blockIdx.y:
 // lane t of wave 0 iterates vector c0 + t of the cluster (64 at a time), all with
 // their own shifts at once, and after every round the cluster's vectors are
-// orthogonalised in order (Gram-Schmidt, twice) and normalised by all four waves,
+// orthogonalised in order (Gram-Schmidt, twice) and normalised by all the waves,
 // rows spread over the threads.  Per matrix, the LU factors (fac:
 // 3 arrays [n][p]: reciprocal pivots, U's first superdiagonal, multipliers; swp
 // [n][p] bytes: row swaps) and the vectors Y [n][p] are stored row-interleaved, so
 // the lanes' accesses of one step are contiguous.  info[mat] is set when a vector's
 // solve overflows or never reaches rounding-level residual.
-__global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD, const double *E, int64_t sE, int n,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tri_vectors(const double *D, int64_t sD, const double *E, int64_t sE, int n,
                                                     int p, const double *W, int64_t sW, double *Y, int64_t sY,
                                                     double *fac, int64_t sF, uint8_t *swpb, int64_t sB, int *info) {
     extern __shared__ double sm[];
     double *d = sm, *e = sm + n;
     __shared__ double inv_s[64];
-    __shared__ double gs_red[2][4][16];
+    __shared__ double gs_red[2][NT / 64][16];
     const int mat = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // sum over the four waves of 16 values already summed within each wave, in wave
-    // order; the two LDS slots alternate, so a slot is rewritten only after the barrier
-    // of the reduction in between, which every thread passes after reading it
+    // sum over the waves of 16 values already summed within each wave, in wave order;
+    // the two LDS slots alternate, so a slot is rewritten only after the barrier of the
+    // reduction in between, which every thread passes after reading it
     int red_i = 0;
     auto block_sum16 = [&](double (&v)[16]) {
+        if constexpr (NT == 64) return;  // one wave: its sums are the block's
         double *slot = &gs_red[red_i & 1][0][0];
         ++red_i;
         if (lane == 0)
@@ -189,7 +191,12 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
             for (int q = 0; q < 16; ++q) slot[wave * 16 + q] = v[q];
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = ((slot[q] + slot[16 + q]) + slot[32 + q]) + slot[48 + q];
+        for (int q = 0; q < 16; ++q) {
+            double a = slot[q];
+#pragma unroll
+            for (int w = 1; w < NT / 64; ++w) a += slot[16 * w + q];
+            v[q] = a;
+        }
     };
     D += mat * sD;
     E += mat * sE;
@@ -205,7 +212,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
     const double tiny = kUlp * tnorm > 0.0 ? kUlp * tnorm : kSafeMin;
     // couplings at rounding level split T (a perturbation no larger than rounding), so
     // no pivot of the factorisation is one of them
-    for (int i = tid; i < n; i += 256) {
+    for (int i = tid; i < n; i += NT) {
         d[i] = D[i];
         e[i] = i + 1 < n && fabs(E[i]) > tiny ? E[i] : 0.0;
     }
@@ -348,7 +355,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
 #pragma unroll
                         for (int q = 0; q < 16; ++q) acc[q] = 0.0;
 #pragma unroll 4
-                        for (int r = tid; r < n; r += 256) {
+                        for (int r = tid; r < n; r += NT) {
                             const double *row = Y + (int64_t)r * p;
                             const double xt = row[t];
 #pragma unroll
@@ -358,7 +365,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
                         wave_sum_n<16>(acc);
                         block_sum16(acc);
 #pragma unroll 4
-                        for (int r = tid; r < n; r += 256) {
+                        for (int r = tid; r < n; r += NT) {
                             double *row = Y + (int64_t)r * p;
                             double v = row[t];
 #pragma unroll
@@ -368,7 +375,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
                         }
                     }
                 double nx = 0.0;
-                for (int r = tid; r < n; r += 256) {
+                for (int r = tid; r < n; r += NT) {
                     const double v = Y[(int64_t)r * p + t];
                     nx += v * v;
                 }
@@ -379,7 +386,7 @@ __global__ __launch_bounds__(256) void k_tri_vectors(const double *D, int64_t sD
                 }
                 const bool ok = nx > 0.0 && isfinite(nx);
                 const double inv = ok ? 1.0 / nx : 0.0;
-                for (int r = tid; r < n; r += 256) Y[(int64_t)r * p + t] *= inv;
+                for (int r = tid; r < n; r += NT) Y[(int64_t)r * p + t] *= inv;
                 if (tid == 0) inv_s[t - g0] = ok ? inv : INFINITY;
                 __threadfence_block();
                 __syncthreads();
@@ -618,7 +625,8 @@ bool tri_topp_supported(int n) { return n <= 4096 && tri_vectors_lds(n) <= 150 *
 
 void tri_set_attrs() {
     hipFuncSetAttribute((const void *)k_tri_bisect, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    hipFuncSetAttribute((const void *)k_tri_vectors, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+    hipFuncSetAttribute((const void *)k_tri_vectors<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+    hipFuncSetAttribute((const void *)k_tri_vectors<256>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
 }
 
 // Top-p eigenpairs of n_mat tridiagonalised matrices: A (reflectors, lower, lda,
@@ -631,8 +639,15 @@ void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, in
     const int64_t sW = p, sY = (int64_t)n * p;
     hipLaunchKernelGGL(k_tri_bisect, dim3((p + 3) / 4, n_mat), dim3(256), (size_t)2 * n * sizeof(double), s, D, sD, E,
                        sE, n, p, W, sW);
-    hipLaunchKernelGGL(k_tri_vectors, dim3(p, n_mat), dim3(256), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
-                       (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
+    // one matrix: four waves share each cluster's Gram-Schmidt sweeps (47.0 -> 44.0 ms
+    // at n = 2,000, p = 50); a batch keeps one wave per workgroup, whose grid already
+    // fills the GPU (256 threads measured 5 ms slower over 256 subsets)
+    if ((int64_t)n_mat * p < 256)
+        hipLaunchKernelGGL(k_tri_vectors<256>, dim3(p, n_mat), dim3(256), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
+                           (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
+    else
+        hipLaunchKernelGGL(k_tri_vectors<64>, dim3(p, n_mat), dim3(64), tri_vectors_lds(n), s, D, sD, E, sE, n, p,
+                           (const double *)W, sW, Y, sY, fac, 3 * sY, swp, sY, info);
     // columns per block: the thread's rows of CB columns stay in registers.  Fewer
     // blocks than CUs (one matrix): four reflectors per barrier; a batch fills the GPU
     // with one reflector per barrier at twice the occupancy.
