@@ -651,10 +651,17 @@ void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, in
     // columns per block: the thread's rows of CB columns stay in registers.  Fewer
     // blocks than CUs (one matrix): four reflectors per barrier; a batch fills the GPU
     // with one reflector per barrier at twice the occupancy.
-    if (n <= 2048 && (int64_t)n_mat * ((p + 3) / 4) < 256)
-        hipLaunchKernelGGL((k_tri_back4<4, 8>), dim3((p + 3) / 4, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
-                           p, Y, sY, X, sY);
-    else if (n <= 2048)
+    if (n <= 2048 && (int64_t)n_mat * ((p + 3) / 4) < 256) {
+        // one column per block while the blocks fit on the CUs: the per-group reduction
+        // shrinks to 4 + 6 values (the n = 2,000, p = 50 eigensolve: 42.7 ms, against
+        // 44.2 with four columns per block and 43.1 with two)
+        if ((int64_t)n_mat * p <= 256)
+            hipLaunchKernelGGL((k_tri_back4<1, 8>), dim3(p, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n, p, Y,
+                               sY, X, sY);
+        else
+            hipLaunchKernelGGL((k_tri_back4<4, 8>), dim3((p + 3) / 4, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau,
+                               n, p, Y, sY, X, sY);
+    } else if (n <= 2048)
         hipLaunchKernelGGL((k_tri_back<8, 8>), dim3((p + 7) / 8, n_mat), dim3(256), 0, s, A, sA, lda, tau, sTau, n,
                            p, Y, sY, X, sY);
     else
