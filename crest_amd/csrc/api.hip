@@ -302,9 +302,14 @@ static void check_gram_prec(int prec, int scale, double max_value) {
 }
 static size_t tile_elem_bytes(int fmt) { return fmt == TILE_F64 ? 8 : fmt == TILE_F32 ? 4 : 2; }
 
-static int choose_split(int nblk, int64_t rows_pad) {
+static int choose_split(int nblk, int64_t rows_pad, int fmt) {
     const int64_t ksteps = rows_pad / 16;
-    int s = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, nblk), ksteps / 8));
+    // about 1,024 workgroups for the LDS-staged split-16-bit Gram (8 splits of the 136
+    // blocks at d = 2,000: 22.4 ms for the 1M-cell Gram stage, against 23.6 with 16
+    // splits and 24.8 with 4), about 2,048 for the other forms
+    const bool x16 = (fmt == TILE_F16X2 || fmt == TILE_BF16X2) && gram_dma_enabled();
+    const int target = x16 ? 1024 : 2048;
+    int s = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(target, nblk), ksteps / 8));
     if (s >= 8) s = s / 8 * 8;
     return std::max(s, 1);
 }
@@ -337,7 +342,7 @@ static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, i
         void *ta = x.get<uint8_t>("tile_a" + sfx, tile_elems * tile_elem_bytes(fmt));
         void *tb = (fmt == TILE_BF16X2 || fmt == TILE_F16X2) ? x.get<uint8_t>("tile_b" + sfx, tile_elems * 2)
                                                              : nullptr;
-        const int n_split = choose_split(nblk, rows_pad);
+        const int n_split = choose_split(nblk, rows_pad, fmt);
         double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
         if (pipe) {
             if (t >= 2) HIPCHK(hipStreamWaitEvent(x.side, x.tile_free[b], 0));  // the Gram of tile t-2 is done
