@@ -436,7 +436,7 @@ def main() -> int:
     ap.add_argument("--acc-queries", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
-    ap.add_argument("--gram-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
+    ap.add_argument("--gram-prec", default="fp16", choices=["fp64", "fp32", "bf16x3", "fp16"],
                     help="Gram of the headline leg (fp16: split-FP16 hi+lo, three MFMA products)")
     ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
                     help="kNN screen of the headline leg")
@@ -556,7 +556,7 @@ def main() -> int:
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (f"{ {'fp16': 'split-fp16', 'bf16x3': 'split-bf16', 'fp32': 'fp32'}[args.gram_prec] } Gram + "
+        "dtype": (f"{ {'fp16': 'split-fp16', 'bf16x3': 'split-bf16', 'fp32': 'fp32', 'fp64': 'fp64'}[args.gram_prec] } Gram + "
                   f"{args.knn_prec} kNN screen (MFMA, f32 accumulate); f64 centring, eigensolve, projection and kNN "
                   "rerank"),
         "data": "synthetic (seeded NB counts, crest_amd.synth)",

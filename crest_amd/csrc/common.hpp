@@ -15,13 +15,16 @@ constexpr int kStatsWindow = 2048;  // vars accumulated in LDS per gene_stats wi
 // Element formats of a densified tile.
 enum TileFmt : int { TILE_F64 = 0, TILE_F32 = 1, TILE_BF16X2 = 2, TILE_F16X2 = 3 };
 
+#define CREST_LOG1PF_FN __host__ __device__ inline
+#include "log1pf_glibc.h"
+
 // normalize_total + log1p of one entry (kernels.rs:77-82).  `ns` is the
-// per-cell factor target_sum/total (1 when total == 0).  log1p is the correctly
-// rounded float of the double log1p, identical on host and device.
+// per-cell factor target_sum/total (1 when total == 0).  log1p is f32 with the
+// rounding of the reference's f32::ln_1p (glibc log1pf), identical on host and device.
 __host__ __device__ inline float lazy_transform(float x, double ns, double target_sum, int log1p) {
     if (!(target_sum > 0.0 || log1p)) return x;
     float y = target_sum > 0.0 ? (float)((double)x * ns) : x;
-    return log1p ? (float)::log1p((double)y) : y;
+    return log1p ? crest_log1pf_glibc(y) : y;
 }
 
 __host__ __device__ inline double norm_factor(double total, double target_sum) {

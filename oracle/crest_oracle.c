@@ -9,9 +9,10 @@
  * OpenMP paths parallelise exactly where the reference does (thread-owned Gram row
  * blocks, query blocks of the kNN) without changing any per-element order.
  *
- * log1p of the lazy transform (kernels.rs:81 `y.ln_1p()`): mode 0 (default) is
- * the correctly rounded float of the double log1p; mode 1 is the C library's
- * log1pf, which on glibc 2.35 differs from mode 0 by one ulp on ~0.5% of inputs.
+ * log1p of the lazy transform (kernels.rs:81 `y.ln_1p()`, i.e. the C library's
+ * log1pf): mode 1 (default) calls log1pf, the reference's own rounding; mode 0 is
+ * the correctly rounded float of the double log1p, which on glibc 2.35 differs from
+ * mode 1 by one ulp on 9,149,209 of the floats >= 0.
  */
 #include <math.h>
 #include <omp.h>
@@ -35,7 +36,7 @@ typedef struct oc_chunk {
     uint32_t first_cell;
 } oc_chunk;
 
-static int g_log1p_mode = 0;
+static int g_log1p_mode = 1;
 void oc_set_log1p_mode(int mode) { g_log1p_mode = mode; }
 
 static float ln1p_f32(float y) { return g_log1p_mode ? log1pf(y) : (float)log1p((double)y); }
@@ -534,4 +535,123 @@ size_t oc_connectivities(const uint32_t *idx, const float *dist, size_t n, size_
     }
     free(e);
     return out;
+}
+
+/* ------------------------------------------------------------------ FP64 exact kNN check */
+/* Checker helper (no reference counterpart): per row r of a block of FP64 dot
+ * products G[r][c] (query r with candidate col0 + c), the squared distance
+ * d = (qn[r] + cn[c]) - 2 G[r][c]; keep the m smallest as a list sorted by
+ * (distance, candidate index), skipping the row's own index self[r] (none when self
+ * is NULL).  best_d / best_i [rows, m] carry the lists across blocks (start with
+ * +inf / -1).  Ties keep the smaller index, so the lists equal a full sort by
+ * (distance, index). */
+void oc_topk_update(const double *G, const double *qn, const double *cn, size_t rows, size_t cols, int64_t col0,
+                    const int64_t *self, size_t m, double *best_d, int64_t *best_i, int threads) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (size_t r = 0; r < rows; ++r) {
+        const double *row = G + r * cols;
+        double *bd = best_d + r * m;
+        int64_t *bi = best_i + r * m;
+        const int64_t me = self ? self[r] : -1;
+        const double q = qn[r];
+        double worst = bd[m - 1];
+        for (size_t c = 0; c < cols; ++c) {
+            const double d = (q + cn[c]) - 2.0 * row[c];
+            if (!(d <= worst)) continue;
+            const int64_t j = col0 + (int64_t)c;
+            if (j == me) continue;
+            if (d == worst && bi[m - 1] >= 0 && j > bi[m - 1]) continue;
+            size_t pos = m - 1;
+            while (pos > 0 && (bd[pos - 1] > d || (bd[pos - 1] == d && bi[pos - 1] > j))) {
+                bd[pos] = bd[pos - 1];
+                bi[pos] = bi[pos - 1];
+                --pos;
+            }
+            bd[pos] = d;
+            bi[pos] = j;
+            worst = bd[m - 1];
+        }
+    }
+}
+
+/* Exact FP64 squared distances sum_j (q_j - x_j)^2 of m (query, candidate) pairs of
+ * rows of x [n, p] (f32 rows widened to f64, summed in column order). */
+void oc_pair_dist_f64(const float *x, size_t p, const int64_t *a, const int64_t *b, size_t m, double *out,
+                      int threads) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (size_t i = 0; i < m; ++i) {
+        const float *u = x + (size_t)a[i] * p, *v = x + (size_t)b[i] * p;
+        double s = 0.0;
+        for (size_t j = 0; j < p; ++j) {
+            const double t = (double)u[j] - (double)v[j];
+            s += t * t;
+        }
+        out[i] = s;
+    }
+}
+
+/* Checker helper: the FP64 screen of oc_topk_update for query rows q[0..m) of x
+ * [n, p] against every row of x, without materialising dot products: blocks of 32
+ * queries stream all candidates once (queries transposed so the 32 dot products of
+ * one candidate are independent vector lanes).  d = (qn + cn) - 2 q.x in FP64 with
+ * fused multiply-adds (a screen: the kept candidates are recomputed exactly by the
+ * caller).  best_d / best_i [m, mk] as oc_topk_update; self excluded. */
+#define OC_QB 32
+void oc_knn_screen_f64(const float *x, size_t n, size_t p, const int64_t *q, size_t m, size_t mk, double *best_d,
+                       int64_t *best_i, int threads) {
+    double *cn = (double *)malloc(n * sizeof(double));
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (size_t c = 0; c < n; ++c) {
+        double s = 0.0;
+        for (size_t j = 0; j < p; ++j) s = fma((double)x[c * p + j], (double)x[c * p + j], s);
+        cn[c] = s;
+    }
+    const size_t nb = (m + OC_QB - 1) / OC_QB;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (size_t b = 0; b < nb; ++b) {
+        const size_t q0 = b * OC_QB, nq = (m - q0) < OC_QB ? (m - q0) : OC_QB;
+        double *qt = (double *)calloc(p * OC_QB, sizeof(double));
+        double qn[OC_QB];
+        int64_t self[OC_QB];
+        for (size_t i = 0; i < OC_QB; ++i) {
+            qn[i] = INFINITY;
+            self[i] = -1;
+        }
+        for (size_t i = 0; i < nq; ++i) {
+            const float *r = x + (size_t)q[q0 + i] * p;
+            qn[i] = cn[q[q0 + i]];
+            self[i] = q[q0 + i];
+            for (size_t j = 0; j < p; ++j) qt[j * OC_QB + i] = (double)r[j];
+        }
+        double worst[OC_QB];
+        for (size_t i = 0; i < nq; ++i) worst[i] = best_d[(q0 + i) * mk + mk - 1];
+        for (size_t c = 0; c < n; ++c) {
+            const float *xc = x + c * p;
+            double acc[OC_QB];
+            for (size_t i = 0; i < OC_QB; ++i) acc[i] = 0.0;
+            for (size_t j = 0; j < p; ++j) {
+                const double v = (double)xc[j];
+                const double *t = qt + j * OC_QB;
+                for (size_t i = 0; i < OC_QB; ++i) acc[i] = fma(t[i], v, acc[i]);
+            }
+            for (size_t i = 0; i < nq; ++i) {
+                const double d = (qn[i] + cn[c]) - 2.0 * acc[i];
+                if (!(d <= worst[i]) || (int64_t)c == self[i]) continue;
+                double *bd = best_d + (q0 + i) * mk;
+                int64_t *bi = best_i + (q0 + i) * mk;
+                if (d == worst[i] && bi[mk - 1] >= 0 && (int64_t)c > bi[mk - 1]) continue;
+                size_t pos = mk - 1;
+                while (pos > 0 && (bd[pos - 1] > d || (bd[pos - 1] == d && bi[pos - 1] > (int64_t)c))) {
+                    bd[pos] = bd[pos - 1];
+                    bi[pos] = bi[pos - 1];
+                    --pos;
+                }
+                bd[pos] = d;
+                bi[pos] = (int64_t)c;
+                worst[i] = bd[mk - 1];
+            }
+        }
+        free(qt);
+    }
+    free(cn);
 }

@@ -65,6 +65,15 @@ def lib():
         _LIB.oc_gene_stats.argtypes = [C.c_void_p, C.c_void_p]
         _LIB.oc_qc.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         _LIB.oc_set_log1p_mode.argtypes = [C.c_int]
+        _LIB.oc_topk_update.restype = None
+        _LIB.oc_topk_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int64,
+                                        C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]
+        _LIB.oc_knn_screen_f64.restype = None
+        _LIB.oc_knn_screen_f64.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                                           C.c_void_p, C.c_void_p, C.c_int]
+        _LIB.oc_pair_dist_f64.restype = None
+        _LIB.oc_pair_dist_f64.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                          C.c_int]
         _LIB.oc_connectivities.restype = C.c_size_t
         _LIB.oc_connectivities.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
                                            C.c_void_p, C.c_void_p]
@@ -84,7 +93,8 @@ def _chunk(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr, 
 
 
 def set_log1p_mode(mode: int) -> None:
-    """0: correctly rounded float of the double log1p; 1: the C library's log1pf."""
+    """1 (default): the C library's log1pf, as the reference's f32::ln_1p
+    (kernels.rs:81); 0: the correctly rounded float of the double log1p."""
     lib().oc_set_log1p_mode(int(mode))
 
 
@@ -207,30 +217,35 @@ def dense_scaled(D, mean, std, max_value):
     return Z
 
 
-def knn_exact_f64(x, k, queries=None):
-    """FP64 exact kNN of the given query rows (all rows as candidates, self excluded):
-    returns (indices [m, k], squared distances [m, k+1]) -- the extra column is the
-    (k+1)-th distance for tie accounting."""
-    x = np.asarray(x, np.float64)
-    q = np.arange(len(x)) if queries is None else np.asarray(queries)
-    nrm = (x * x).sum(1)
-    out_i = np.empty((len(q), k), np.int64)
-    out_d = np.empty((len(q), k + 1), np.float64)
-    for s in range(0, len(q), 256):
-        qq = q[s:s + 256]
-        xq = x[qq]
-        d2 = nrm[qq, None] + nrm[None, :] - 2.0 * (xq @ x.T)
-        d2[np.arange(len(qq)), qq] = np.inf
-        part = np.argpartition(d2, k + 1, axis=1)[:, :k + 1]
-        pd = np.take_along_axis(d2, part, 1)
-        # exact recompute of the candidates' distances
-        diff = xq[:, None, :] - x[part]
-        pd = (diff * diff).sum(-1)
-        o = np.lexsort((part, pd), axis=1)
-        part = np.take_along_axis(part, o, 1)
-        pd = np.take_along_axis(pd, o, 1)
-        out_i[s:s + 256] = part[:, :k]
-        out_d[s:s + 256] = pd
+def knn_exact_f64(x, k, queries=None, threads=None):
+    """FP64 exact kNN of the given query rows of x (f32 scores, widened to f64; all
+    rows as candidates, self excluded): returns (indices [m, k], squared distances [m, k+1]) -- the extra column is the
+    (k+1)-th distance for tie accounting.  Ordered by (distance, index).
+
+    Screen (oc_knn_screen_f64): FP64 d = |q|^2 + |x|^2 - 2 q.x for every pair, the
+    k + 9 smallest per query kept; then every kept candidate's distance recomputed exactly as
+    sum (q_j - x_j)^2 in FP64 and re-sorted.  The expanded form is within ~1e-14
+    relative of the exact one, so a true neighbour is lost only if eight other points
+    lie within that of the (k+1)-th distance."""
+    import os
+    x32 = np.ascontiguousarray(x, np.float32)
+    n = len(x32)
+    q = np.arange(n, dtype=np.int64) if queries is None else np.ascontiguousarray(queries, np.int64)
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = min(k + 9, n - 1)
+    best_d = np.full((len(q), m), np.inf)
+    best_i = np.full((len(q), m), -1, np.int64)
+    L = lib()
+    L.oc_knn_screen_f64(_p(x32), n, x32.shape[1], _p(q), len(q), m, _p(best_d), _p(best_i), threads)
+    # exact recompute of the kept candidates, then order by (distance, index)
+    a = np.repeat(q, m)
+    b = best_i.ravel()
+    ex = np.empty(len(a))
+    L.oc_pair_dist_f64(_p(x32), x32.shape[1], _p(a), _p(np.ascontiguousarray(b)), len(a), _p(ex), threads)
+    ex = ex.reshape(len(q), m)
+    o = np.lexsort((best_i, ex), axis=1)
+    out_i = np.take_along_axis(best_i, o, 1)[:, :k]
+    out_d = np.take_along_axis(ex, o, 1)[:, :k + 1]
     return out_i, out_d
 
 

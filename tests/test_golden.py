@@ -1,6 +1,7 @@
-"""The committed golden fixture (tests/golden/c1_small.npz, written by
-tests/golden/make_golden.py from the oracle) stays reproducible: the seeded input
-generator gives the same CSR, and the oracle gives the same outputs."""
+"""The committed golden fixture (tests/golden/c1.npz, written by
+tests/golden/make_golden.py from the oracle at the small plain-method configuration)
+stays reproducible: the seeded input generator gives the same CSR, and the oracle
+gives the same outputs."""
 import os
 import sys
 
@@ -13,7 +14,7 @@ import make_golden as MG  # noqa: E402
 
 
 def _fixture():
-    return np.load(os.path.join(HERE, "golden", "c1_small.npz"), allow_pickle=False)
+    return np.load(os.path.join(HERE, "golden", "c1.npz"), allow_pickle=False)
 
 
 def test_fixture_inputs_reproduce():

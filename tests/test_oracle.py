@@ -55,29 +55,44 @@ def test_materialize_hand_values(oracle):
     assert r2.tolist() == r.tolist() and g2.tolist() == g.tolist() and x2.tolist() == x.tolist()
 
 
+def _libm_log1pf(y: np.ndarray) -> np.ndarray:
+    """The C library's log1pf on each float32 of y, called one by one through ctypes
+    (the reference's f32::ln_1p, kernels.rs:81), independent of the oracle's C code."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.log1pf.restype = ctypes.c_float
+    libm.log1pf.argtypes = [ctypes.c_float]
+    return np.array([libm.log1pf(float(v)) for v in np.asarray(y, np.float32)], np.float32)
+
+
 def test_lazy_normalisation_closed_form(oracle, X):
-    """tests/test_crest.py:77-85, and bit-exact against the float32 rounding chain
-    (f32)(x * (1e4/total)) -> (f32)log1p(double)."""
+    """tests/test_crest.py:77-85, and bit-exact against the reference's float32
+    rounding chain y = (f32)(x * (1e4/total)), v = log1pf(y)."""
     r, g, x = oracle.materialize(*ctx_of(X))
     D = X.toarray().astype(np.float64)
     ref = np.log1p(D / D.sum(1, keepdims=True) * 1e4)
     np.testing.assert_allclose(x, ref[r, g], rtol=1e-5, atol=1e-5)
     tot = D.sum(1)
     y = (D[r, g] * (1e4 / tot[r])).astype(np.float32)
-    exact = np.log1p(y.astype(np.float64)).astype(np.float32)
-    np.testing.assert_array_equal(x, exact)
+    np.testing.assert_array_equal(x, _libm_log1pf(y))
 
 
 def test_log1p_modes_differ_by_at_most_one_ulp(oracle, X):
+    """Mode 0 (correctly rounded float of the double log1p) against the default mode 1
+    (log1pf): at most one ulp apart, and mode 0 equals the correctly rounded chain."""
     ctx = ctx_of(X)
-    _, _, a = oracle.materialize(*ctx)
-    oracle.set_log1p_mode(1)
+    _, _, b = oracle.materialize(*ctx)
+    oracle.set_log1p_mode(0)
     try:
-        _, _, b = oracle.materialize(*ctx)
+        r, g, a = oracle.materialize(*ctx)
     finally:
-        oracle.set_log1p_mode(0)
+        oracle.set_log1p_mode(1)
     ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
     assert ulp.max() <= 1
+    D = X.toarray().astype(np.float64)
+    y = (D[r, g] * (1e4 / D.sum(1)[r])).astype(np.float32)
+    np.testing.assert_array_equal(a, np.log1p(y.astype(np.float64)).astype(np.float32))
 
 
 def test_filters_and_maps(oracle, X):
@@ -97,10 +112,13 @@ def test_filters_and_maps(oracle, X):
 
 
 def lognorm(X):
-    """The reference's rounding chain: y = (f32)(x * (1e4 / total)), v = (f32)log1p(y)."""
+    """The reference's rounding chain: y = (f32)(x * (1e4 / total)), v = log1pf(y) (dense)."""
     D = X.toarray().astype(np.float64)
     y = (D * (1e4 / D.sum(1, keepdims=True))).astype(np.float32)
-    return np.log1p(y.astype(np.float64)).astype(np.float32).astype(np.float64)
+    out = np.zeros(y.shape, np.float64)
+    nz = y != 0
+    out[nz] = _libm_log1pf(y[nz]).astype(np.float64)
+    return out
 
 
 def test_gene_stats_dense(oracle, X):
@@ -233,8 +251,9 @@ def test_knn_brute_matches_naive(oracle):
 
 
 def test_knn_exact_f64_helper(oracle):
-    y = np.random.default_rng(3).normal(size=(500, 6))
+    y = np.random.default_rng(3).normal(size=(500, 6)).astype(np.float32)
     i, d2 = oracle.knn_exact_f64(y, 4, np.arange(0, 500, 7))
+    y = y.astype(np.float64)
     D = ((y[::7, None, :] - y[None]) ** 2).sum(-1)
     D[np.arange(len(D)), np.arange(0, 500, 7)] = np.inf
     np.testing.assert_array_equal(i, np.argsort(D, 1)[:, :4])
