@@ -1,9 +1,12 @@
 """Benchmark: cells/s of scale + PCA + kNN at 1,000,000 cells x 2,000 HVGs on one
-MI355X, split-BF16 against the plain FP32 GPU path, at fixed kNN agreement.
+MI355X against the plain FP32 GPU path, at fixed kNN agreement.
 
 A step is one pass of the hot path over the device-resident CSR: gene statistics ->
-mean/std -> Gram (SYRK on MFMA) -> FP64 eigensolve -> projection -> exact kNN (k=15),
-through crest_pipeline_run.  Inputs are resident in HBM before the timed region.
+mean/std -> Gram -> FP64 eigensolve -> projection -> exact kNN (k=15), through
+crest_pipeline_run.  Inputs are resident in HBM before the timed region.  The headline
+runs the FP64 Gram (every one of the 50 principal components within 1e-6 rad of the
+FP64 oracle) with the FP16 distance screen and FP64 rerank; the split-BF16 and
+split-FP16 configurations and the plain FP32 path are timed beside it.
 
 Prints one JSON line.  After the timed region, the cpu_baseline leg times the
 oracle (the C restatement of the reference's CPU algorithm) on a bounded sample and
@@ -22,11 +25,25 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# the CPU baseline's OpenMP threads stay on their cores (set before libgomp starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "cells/s scale+PCA+kNN @1M×2k HVG: split-BF16 vs FP32 at fixed kNN agreement"
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 / FP16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X dense FP32 MFMA, FLOP/s
-PEAK_FP32 = 157.3e12    # MI355X FP32 MFMA, FLOP/s
+PREC_LABEL = {"fp64": "fp64", "fp32": "fp32", "bf16x3": "split-bf16", "fp16": "split-fp16"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(msg: str) -> None:
@@ -100,7 +117,9 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     O.knn(Xg, args.k, 0, q, threads=threads)
     t["knn"] = (time.perf_counter() - t0) * n / q
     total = sum(t.values())
-    base = {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port",
+    base = {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "pinning": (f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                        f"OMP_PLACES={os.environ.get('OMP_PLACES')}, {threads} OpenMP threads"),
             "sample": (f"C restatement of kernels.rs/knn.rs: gene_stats and FP64 Gram over all {n} cells, "
                        f"numpy eigh, projection on {ns} cells and exact kNN of {q} queries x {n} points, "
                        f"both scaled linearly to {n}"),
@@ -115,15 +134,19 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     r_sig = args.lead if args.lead > 0 else int(np.argmax(rel[1:args.pcs - 1]) + 2)
     acc["pca_signal_r"] = r_sig
     acc["pca_signal_rel_gap"] = float(rel[r_sig - 1])
-    rng = np.random.default_rng(0)
-    qs = rng.choice(n, args.acc_queries, replace=False)
-    for tag in [t for t in ("head", "alt", "fp32") if f"x_{t}" in gpu]:
+    for tag in gpu["tags"]:
+        nq = args.acc_queries if tag == "head" else args.alt_acc_queries
+        qs = np.random.default_rng(0).choice(n, nq, replace=False)
         Xp = gpu[f"x_{tag}"]
-        ref_i, ref_d2 = O.knn_exact_f64(Xp, args.k, qs)
+        t0 = time.perf_counter()
+        ref_i, ref_d2 = O.knn_exact_f64(Xp, args.k, qs, threads=threads)
         got = gpu[f"idx_{tag}"][qs]
         bad, ties = O.knn_mismatch_ties(got, Xp, qs, ref_i, ref_d2)
+        log(f"accuracy {tag}: {nq} queries against the FP64 exact search in {time.perf_counter() - t0:.1f}s, "
+            f"{bad} differing sets ({ties} ties)")
         acc[f"knn_set_agreement_{tag}"] = O.knn_set_agreement(got, ref_i)
         acc[f"knn_mismatch_{tag}"] = [int(bad), int(ties)]
+        acc[f"knn_queries_{tag}"] = int(nq)
         L = gpu[f"load_{tag}"]
         acc[f"pca_angle_signal_{tag}"] = O.principal_angle(L[:, :r_sig], load_o[:, :r_sig])
         acc[f"pca_angle_lead20_{tag}"] = O.principal_angle(L[:, :20], load_o[:, :20])
@@ -132,7 +155,6 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
                                                                - var_o / tot_o)))
     acc["eig_gap_p"] = float(var_o[args.pcs - 1] - ev[args.pcs])
     acc["eig_rel_gap_20"] = float(rel[19])
-    acc["queries"] = int(args.acc_queries)
     return base, acc
 
 
@@ -182,24 +204,63 @@ def screen_traffic(prec: str, n: int, p: int):
 
 def streamed_leg(args) -> dict:
     """The out-of-core configuration: the CSR stays in pinned host memory and every
-    pass streams it through two device slots (crest_pipeline_run_streamed)."""
+    pass streams it through two device slots (crest_pipeline_run_streamed), at the
+    headline precisions; checked against the oracle (FP64 Gram PCA over every cell,
+    the FP64 exact kNN of sampled cells) unless --no-cpu."""
     from crest_amd import synth
     from crest_amd.pipeline import HostCSR, Pipeline
     t0 = time.perf_counter()
     ip, ix, dv, info = synth.make_counts(args.streamed_cells, args.genes, args.clusters, density=args.density,
                                          seed=args.seed + 1)
     host = HostCSR.from_arrays(ip, ix, dv, args.genes, pinned=True)
-    del ip, ix, dv
     log(f"streamed: generated {args.streamed_cells} x {args.genes}, nnz {info['nnz']} "
         f"in {time.perf_counter() - t0:.1f}s")
-    pipe = Pipeline(host, np.arange(args.genes, dtype=np.int32), n_comps=args.pcs, k=args.k, prec_gram="bf16x3",
-                    prec_knn=args.knn_prec, chunk_rows=args.stream_rows, stream_rows=args.stream_rows)
+    pipe = Pipeline(host, np.arange(args.genes, dtype=np.int32), n_comps=args.pcs, k=args.k,
+                    prec_gram=args.gram_prec, prec_knn=args.knn_prec, chunk_rows=args.stream_rows,
+                    stream_rows=args.stream_rows)
     ms, stages = timed_steps(pipe, 1, 1)
     log(f"streamed ({args.streamed_cells} cells, chunks of {args.stream_rows}): {ms:.1f} ms/step, "
         f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
-    return {"n_cells": args.streamed_cells, "nnz": info["nnz"], "stream_rows": args.stream_rows,
-            "ms_per_step": ms, "value": args.streamed_cells / (ms * 1e-3), "unit": "cells/s",
-            "stages_ms": {k: round(v, 3) for k, v in stages.items()}}
+    out = {"n_cells": args.streamed_cells, "nnz": info["nnz"], "stream_rows": args.stream_rows,
+           "prec_gram": args.gram_prec, "prec_knn": args.knn_prec, "ms_per_step": ms,
+           "value": args.streamed_cells / (ms * 1e-3), "unit": "cells/s",
+           "stages_ms": {k: round(v, 3) for k, v in stages.items()}}
+    if args.no_cpu:
+        return out
+    from oracle import oracle as O
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+    res = pipe.run()
+    X, (idx, _) = pipe.x_pca(), pipe.knn()
+    del pipe, host
+    t0 = time.perf_counter()
+    n, g = args.streamed_cells, args.genes
+    cmap, gmap = np.arange(n, dtype=np.int64), np.arange(g, dtype=np.int32)
+    st = np.zeros((g, 5))
+    step = 1 << 17
+
+    def ctx(lo, hi):
+        a, b = int(ip[lo]), int(ip[hi])
+        return (ix[a:b], dv[a:b], cmap, gmap, 1e4, True, None, ip[lo:hi + 1].copy(), lo)
+
+    for lo in range(0, n, step):
+        O.gene_stats(*ctx(lo, min(n, lo + step)), st)
+    mean = st[:, 0] / n
+    sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+    sd[sd == 0] = 1
+    gram, colsum = np.zeros((g, g)), np.zeros(g)
+    for lo in range(0, n, step):
+        O.gram_accumulate(*ctx(lo, min(n, lo + step)), gmap, gram, colsum, True, mean, sd, 10.0, threads=threads)
+    var_o, load_o, tot_o = O.pca_from_gram(gram, colsum, n, args.pcs)
+    qs = np.random.default_rng(0).choice(n, args.alt_acc_queries, replace=False)
+    ref_i, ref_d2 = O.knn_exact_f64(X, args.k, qs, threads=threads)
+    bad, ties = O.knn_mismatch_ties(idx[qs], X, qs, ref_i, ref_d2)
+    out["accuracy"] = {"pca_angle_all": O.principal_angle(res.loadings, load_o),
+                       "variance_ratio_err": float(np.max(np.abs(res.variance / res.total_variance
+                                                                 - var_o / tot_o))),
+                       "knn_set_agreement": O.knn_set_agreement(idx[qs], ref_i), "knn_mismatch": [int(bad), int(ties)],
+                       "knn_queries": len(qs)}
+    log(f"streamed accuracy against the oracle ({time.perf_counter() - t0:.1f}s): {out['accuracy']}")
+    return out
 
 
 def batched_leg(args) -> dict:
@@ -433,16 +494,19 @@ def main() -> int:
     ap.add_argument("--chunk-rows", type=int, default=65536)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--cpu-knn-queries", type=int, default=512)
-    ap.add_argument("--acc-queries", type=int, default=1000)
+    ap.add_argument("--acc-queries", type=int, default=65536,
+                    help="sampled queries of the headline's kNN agreement check against the FP64 exact search")
+    ap.add_argument("--alt-acc-queries", type=int, default=4096,
+                    help="sampled queries of the other legs' kNN agreement checks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
-    ap.add_argument("--gram-prec", default="fp16", choices=["fp64", "fp32", "bf16x3", "fp16"],
-                    help="Gram of the headline leg (fp16: split-FP16 hi+lo, three MFMA products)")
+    ap.add_argument("--gram-prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16"],
+                    help="Gram of the headline leg (fp64: FP64 MFMA; fp16: split-FP16 hi+lo, three MFMA products)")
     ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
                     help="kNN screen of the headline leg")
-    ap.add_argument("--alt", default="bf16x3", choices=["none", "fp32", "bf16x3", "fp16"],
-                    help="also time one step with this precision for both Gram and screen (reported, not the "
-                         "headline; bf16x3 = the all-split-BF16 configuration)")
+    ap.add_argument("--alt", default="bf16x3:bf16x3,fp16:fp16",
+                    help="comma-separated GRAM:SCREEN precision pairs also timed (one step each) and checked; "
+                         "'none' for none (bf16x3:bf16x3 = the all-split-BF16 configuration)")
     ap.add_argument("--streamed-cells", type=int, default=2_000_000, metavar="N",
                     help="also time the out-of-core form: N x genes NB counts in pinned host memory, streamed in "
                          "chunks of --stream-rows cells (reported under 'streamed', not the headline)")
@@ -492,10 +556,11 @@ def main() -> int:
     hvg = np.arange(args.genes, dtype=np.int32)  # the 2,000 genes are the pre-selected HVGs
     pipe = Pipeline(csr, hvg, n_comps=args.pcs, k=args.k, prec_gram=args.gram_prec, prec_knn=args.knn_prec,
                     chunk_rows=args.chunk_rows)
-    gpu = {}
+    gpu = {"tags": []}
 
     def capture(tag):
         out_ = pipe.run()
+        gpu["tags"].append(tag)
         gpu.update({f"x_{tag}": pipe.x_pca(), f"idx_{tag}": pipe.knn()[0], f"var_{tag}": out_.variance,
                     f"load_{tag}": out_.loadings, f"tot_{tag}": out_.total_variance})
 
@@ -503,21 +568,43 @@ def main() -> int:
     log(f"{args.gram_prec} Gram + {args.knn_prec} screen: {ms:.1f} ms/step, "
         f"stages {json.dumps({k: round(v, 2) for k, v in stages.items()})}")
     capture("head")
-    alt = None
-    if args.alt != "none" and (args.alt, args.alt) != (args.gram_prec, args.knn_prec):
-        pipe.set_precision(args.alt, args.alt)
-        ms_alt, stages_alt = timed_steps(pipe, 1, 0)
-        log(f"{args.alt} Gram + {args.alt} screen: {ms_alt:.1f} ms/step, "
+    alts = []
+    for pair in ([] if args.alt == "none" else args.alt.split(",")):
+        pg, pk = pair.split(":")
+        if (pg, pk) == (args.gram_prec, args.knn_prec):
+            continue
+        tag = f"{pg}_{pk}"
+        pipe.set_precision(pg, pk)
+        ms_alt, stages_alt = timed_steps(pipe, 1, 1)
+        log(f"{pg} Gram + {pk} screen: {ms_alt:.1f} ms/step, "
             f"stages {json.dumps({k: round(v, 2) for k, v in stages_alt.items()})}")
-        capture("alt")
-        alt = {"prec_gram": args.alt, "prec_knn": args.alt, "ms_per_step": ms_alt, "value": args.cells / (ms_alt * 1e-3),
-               "stages_ms": {k: round(v, 3) for k, v in stages_alt.items()}}
+        capture(tag)
+        alts.append({"tag": tag, "prec_gram": pg, "prec_knn": pk, "ms_per_step": ms_alt,
+                     "value": args.cells / (ms_alt * 1e-3), "stages_ms": {k: round(v, 3) for k, v in stages_alt.items()}})
 
     if args.fp32_steps > 0:
         pipe.set_precision("fp32", "fp32")
         ms32, stages32 = timed_steps(pipe, args.fp32_steps, 1)
         log(f"FP32: {ms32:.1f} ms/step, stages {json.dumps({k: round(v, 2) for k, v in stages32.items()})}")
         capture("fp32")
+    pipe.set_precision(args.gram_prec, args.knn_prec)
+    # T from the first upload: the CSR copied from pinned host memory into the device
+    # buffers, then one step (the same pipeline), timed together
+    from crest_amd._lib import check, lib
+    from crest_amd.pipeline import PinnedArray
+    host = [PinnedArray(a.shape, a.dtype) for a in (indptr, indices, data)]
+    for h_, a in zip(host, (indptr, indices, data)):
+        h_.array[...] = a
+    check(lib().crest_synchronize())
+    t0 = time.perf_counter()
+    for h_, dbuf in zip(host, (csr.indptr, csr.indices, csr.data)):
+        check(lib().crest_memcpy_htod(dbuf.ptr, h_.ptr, h_.array.nbytes))
+    pipe.run()
+    check(lib().crest_synchronize())
+    ms_upload = (time.perf_counter() - t0) * 1e3
+    upload_bytes = sum(h_.array.nbytes for h_ in host)
+    del host
+    log(f"upload ({upload_bytes / 1e9:.2f} GB from pinned host memory) + one step: {ms_upload:.1f} ms")
     if args.calibrate_copy:
         # a plain copy of known size: the reference reading for the HBM byte counters
         from crest_amd._lib import check, lib
@@ -556,9 +643,8 @@ def main() -> int:
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (f"{ {'fp16': 'split-fp16', 'bf16x3': 'split-bf16', 'fp32': 'fp32', 'fp64': 'fp64'}[args.gram_prec] } Gram + "
-                  f"{args.knn_prec} kNN screen (MFMA, f32 accumulate); f64 centring, eigensolve, projection and kNN "
-                  "rerank"),
+        "dtype": (f"{PREC_LABEL[args.gram_prec]} Gram + {PREC_LABEL[args.knn_prec]} kNN screen (MFMA); f64 "
+                  "centring, eigensolve, projection and kNN rerank"),
         "data": "synthetic (seeded NB counts, crest_amd.synth)",
         "config": {"workload": (f"{n} cells x {args.genes} pre-selected HVGs, NB counts density "
                                 f"{info['density']:.4f}, {args.clusters} clusters, seed {args.seed}; normalize_total 1e4,"
@@ -567,6 +653,9 @@ def main() -> int:
                    "chunk_rows": args.chunk_rows, "prec_gram": args.gram_prec, "prec_knn": args.knn_prec},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "ms_per_step_with_d2h": ms + d2h_ms,
+        "ms_from_upload": ms_upload,
+        "value_from_upload": n / (ms_upload * 1e-3),
+        "upload_bytes": upload_bytes,
         "hbm_copy_GBps": copy_gbps,
         "roofline": {"kernel": {"fp16": "k_knn_screen16<1,...>", "bf16x3": "k_knn_screen16<0,...>",
                                 "fp32": "k_knn_screen_f32"}[args.knn_prec],
@@ -576,12 +665,14 @@ def main() -> int:
                      "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
                      "launch_ms": stages["knn_screen"]},
     }
-    if alt is not None:
-        line["alt"] = alt
+    if alts:
+        line["alt"] = alts
     if args.fp32_steps > 0:
         line["fp32"] = {"ms_per_step": ms32, "value": n / (ms32 * 1e-3),
                         "stages_ms": {k: round(v, 3) for k, v in stages32.items()}}
         line["speedup_vs_fp32"] = ms32 / ms
+        for a in alts:
+            a["speedup_vs_fp32"] = ms32 / a["ms_per_step"]
     if args.streamed_cells:
         line["streamed"] = streamed_leg(args)
     if args.batch_subsets:

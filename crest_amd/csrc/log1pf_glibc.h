@@ -30,16 +30,15 @@
 #define CREST_LOG1PF_GLIBC_H
 
 #include <stdint.h>
-#include <string.h>
 
 CREST_LOG1PF_FN float crest_f32_from_bits(uint32_t u) {
     float f;
-    memcpy(&f, &u, 4);
+    __builtin_memcpy(&f, &u, 4);
     return f;
 }
 CREST_LOG1PF_FN uint32_t crest_bits_of_f32(float f) {
     uint32_t u;
-    memcpy(&u, &f, 4);
+    __builtin_memcpy(&u, &f, 4);
     return u;
 }
 

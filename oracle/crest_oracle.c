@@ -283,6 +283,32 @@ void oc_gram_accumulate(const oc_chunk *v, const int32_t *sub, double *gram, dou
     free(s);
 }
 
+/* The colsum and the Gram diagonal of gram_accumulate alone (kernels.rs:258-260 and
+ * the a == b terms of :291-308): s1[m] += s, s2[m] += s * s in entry order, for a
+ * checker that needs only the covariance trace. */
+void oc_scaled_moments(const oc_chunk *v, const int32_t *sub, double *s1, double *s2, int scale, const double *mean,
+                       const double *sd, double mv) {
+    span *s;
+    const size_t ns = oc_spans(v, &s);
+    const size_t mx = max_span(s, ns);
+    uint32_t *bv = malloc((mx ? mx : 1) * sizeof(uint32_t));
+    float *bx = malloc((mx ? mx : 1) * sizeof(float));
+    for (size_t i = 0; i < ns; ++i) {
+        size_t bn;
+        if (oc_cell(v, &s[i], bv, bx, &bn) < 0) continue;
+        for (size_t j = 0; j < bn; ++j) {
+            const int32_t m = sub[bv[j]];
+            if (m < 0) continue;
+            const double x = apply_t(scale, mean, sd, mv, (size_t)m, bx[j]);
+            s1[m] += x;
+            s2[m] += x * x;
+        }
+    }
+    free(bv);
+    free(bx);
+    free(s);
+}
+
 /* kernels.rs:319-323: cov = (G[min,max] - c_a c_b / n) / (n - 1); total = trace. */
 void oc_cov_from_gram(const double *gram, const double *colsum, size_t d, size_t n, double *cov, double *total) {
     const double nf = (double)n;

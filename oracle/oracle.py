@@ -68,6 +68,9 @@ def lib():
         _LIB.oc_topk_update.restype = None
         _LIB.oc_topk_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int64,
                                         C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]
+        _LIB.oc_scaled_moments.restype = None
+        _LIB.oc_scaled_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_double]
         _LIB.oc_knn_screen_f64.restype = None
         _LIB.oc_knn_screen_f64.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
                                            C.c_void_p, C.c_void_p, C.c_int]
@@ -135,6 +138,15 @@ def project(*args):
     loadings = np.ascontiguousarray(loadings, np.float64)
     lib().oc_project(C.byref(ch), _p(sub_map), _p(loadings), loadings.shape[1], _p(out), int(bool(scale)),
                      _p(mean), _p(std), float(max_value))
+
+
+def scaled_moments(*args):
+    """(*ctx, sub_map, s1, s2, scale, mean, std, max_value): the colsum and the Gram
+    diagonal of gram_accumulate, accumulated in place (crest_oracle.c oc_scaled_moments)."""
+    ctx, (sub_map, s1, s2, scale, mean, std, max_value) = args[:9], args[9:]
+    ch, _k = _chunk(*ctx)
+    lib().oc_scaled_moments(C.byref(ch), _p(sub_map), _p(s1), _p(s2), int(bool(scale)), _p(mean), _p(std),
+                            float(max_value))
 
 
 def cov_from_gram(gram, colsum, n):

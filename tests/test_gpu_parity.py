@@ -541,3 +541,31 @@ def test_pca_from_gram_top_p_eigensolver(case, monkeypatch):
     assert tot == tot2
     if case == "dense_distinct":
         np.testing.assert_allclose(load, load2, atol=1e-9)
+
+
+def test_log1pf_device_matches_libm(oracle):
+    """The device transform's log1pf (log1pf_glibc.h) against the oracle's, which calls
+    the C library's log1pf (the reference's f32::ln_1p, kernels.rs:81): every 97th
+    float of [-1, +inf] and every float within 2^16 of each branch threshold, bit for
+    bit (no normalisation, so the transform is log1pf of the raw value)."""
+    pos = np.arange(0, 0x7F800001, 97, dtype=np.int64)
+    neg = np.arange(0x80000000, 0xBF800001, 97, dtype=np.int64)
+    edges = [0x3ED413D7, 0x3504F7 | 0x3F800000, 0xBE95F61F, 0x31000000, 0x24800000, 0x5A000000, 0xB1000000,
+             0x3F800000, 0xBF800000 - 65536]
+    near = np.concatenate([np.arange(e - 65536, e + 65536, dtype=np.int64) for e in edges])
+    bits = np.unique(np.concatenate([pos, neg, near, [0x7F800000, 0xBF800000]]))
+    x = bits.astype(np.uint32).view(np.float32)
+    x = x[(x >= -1.0) | ~np.isfinite(x)]
+    x = x[~np.isnan(x)]
+    per = 64
+    n = len(x) // per * per
+    x = np.ascontiguousarray(x[:n])
+    cells = n // per
+    genes = np.tile(np.arange(per, dtype=np.uint32), cells)
+    ip = np.arange(0, n + 1, per, dtype=np.int64)
+    ctx = (genes, x, np.arange(cells, dtype=np.int64), np.arange(per, dtype=np.int32), 0.0, True, None, ip, 0)
+    print(f"[log1pf] {n} floats through crest_materialize", flush=True)
+    got = N.materialize(*ctx)[2]
+    ref = oracle.materialize(*ctx)[2]
+    assert len(got) == len(ref) == n
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
