@@ -88,12 +88,16 @@ _SIGS = {
     "crest_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "crest_synchronize": (C.c_int, []),
     "crest_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "crest_stream_synchronize": (C.c_int, [C.c_void_p]),
     "crest_gene_stats": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t]),
     "crest_qc": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                            C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
     "crest_gram_accumulate": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                         C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p, C.c_size_t, C.c_void_p,
                                         C.c_size_t, C.c_double]),
+    "crest_gram_accumulate_noalloc": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                                C.c_size_t, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,
+                                                C.c_void_p]),
     "crest_pca_from_gram": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]),
     "crest_project": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,

@@ -1589,6 +1589,15 @@ int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     });
 }
 
+int crest_stream_synchronize(void *stream) {
+    return guarded([&] {
+        const int dev = g_device;
+        if (dev < 0 || dev >= 64) value_error("device index out of range");
+        HIPCHK(hipSetDevice(dev));
+        HIPCHK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
 int crest_dev_copy(void *dst, const void *src, size_t bytes) {
     return guarded([&] {
         if (bytes % 16) value_error("bytes must be a multiple of 16");
@@ -1706,6 +1715,125 @@ int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size
             colsum[a] += hc[a];
             for (size_t b = a; b < d; ++b) gram[a * d + b] += hg[a * d_pad + b];
         }
+    });
+}
+
+// ---------------------------------------------------------------- gram_accumulate without allocation
+// The scratch of crest_gram_accumulate_noalloc: the d_pad x d_pad Gram and colsum it
+// accumulates into, one split-K partial per Gram block and the densified tile(s).
+// Every GPU holds its own copy of this array from module load to process end; host
+// code reaches it only through the per-device address from hipGetSymbolAddress.
+namespace crest {
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+constexpr int kNoAllocMaxD = 2048;
+}  // namespace crest
+static __device__ __align__(256) unsigned char scratch[crest::kNoAllocScratchBytes];
+
+namespace crest {
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[64] = {};
+
+// Address of this device's scratch (looked up once per device, with it current).
+static unsigned char *noalloc_scratch(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) fail(CREST_E_DEVICE, "no device address for the scratch array");
+        g_scratch_dev[dev] = ptr;
+    }
+    return static_cast<unsigned char *>(g_scratch_dev[dev]);
+}
+
+// dev_gram[a * d + b] += g[a * d_pad + b] (a <= b < d), dev_colsum[a] += c[a] (a < d).
+__global__ void k_noalloc_flush(const double *g, const double *c, int d, int d_pad, double *dev_gram,
+                                double *dev_colsum) {
+    const int64_t total = (int64_t)d * d;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(e / d), b = (int)(e % d);
+        if (b >= a) dev_gram[e] += g[(int64_t)a * d_pad + b];
+        if (b == 0) dev_colsum[a] += c[a];
+    }
+}
+}  // namespace crest
+
+int crest_gram_accumulate_noalloc(const crest_chunk *dev_chunk, const int32_t *dev_sub_map, size_t n_sub_map,
+                                  double *dev_gram, double *dev_colsum, size_t d, int32_t scale,
+                                  const double *dev_mean, const double *dev_std, double max_value, void *stream) {
+    return guarded([&] {
+        const crest_chunk *c = dev_chunk;
+        if (!c) value_error("chunk is NULL");
+        if (c->cells || !c->indptr || c->n_indptr < 1) value_error("the no-allocation form takes indptr chunks only");
+        if ((c->nnz && (!c->genes || !c->values)) || (c->n_cell_map && !c->cell_map) ||
+            (c->n_gene_map && !c->gene_map) || (n_sub_map && !dev_sub_map))
+            value_error("chunk arrays must not be NULL");
+        if (d && (!dev_gram || !dev_colsum)) value_error("gram must be (d, d)");
+        if (scale && (!dev_mean || !dev_std)) value_error("scale=True needs mean and std");
+        if (d > (size_t)kNoAllocMaxD)
+            value_error("d exceeds the no-allocation scratch (at most " + std::to_string(kNoAllocMaxD) + " HVGs)");
+        const int prec = g_prec;
+        check_gram_prec(prec, scale, max_value);
+        if (d == 0 || c->n_indptr < 2) return;
+        const int dev = g_device;
+        if (dev < 0 || dev >= 64) value_error("device index out of range");
+        HIPCHK(hipSetDevice(dev));
+        unsigned char *base = noalloc_scratch(dev);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int fmt = tile_fmt(prec);
+        const int d_pad = (int)round_up((int64_t)d, kGramBlock);
+        const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
+        // scratch layout: Gram, colsum, partials, tile(s), each 256-byte aligned
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            unsigned char *p = base + off;
+            off += (size_t)round_up((int64_t)bytes, 256);
+            return p;
+        };
+        double *g = reinterpret_cast<double *>(carve((size_t)d_pad * d_pad * sizeof(double)));
+        double *cs = reinterpret_cast<double *>(carve((size_t)d_pad * sizeof(double)));
+        double *partial = reinterpret_cast<double *>(carve((size_t)nblk * kGramBlock * kGramBlock * sizeof(double)));
+        const bool x2 = fmt == TILE_BF16X2 || fmt == TILE_F16X2;
+        const size_t row_bytes = (size_t)d_pad * (x2 ? 4 : tile_elem_bytes(fmt));
+        if (off + 512 > kNoAllocScratchBytes) value_error("d exceeds the no-allocation scratch");
+        const int64_t tile_rows = (int64_t)((kNoAllocScratchBytes - off - 512) / row_bytes) / 16 * 16;
+        if (tile_rows < 16) value_error("d exceeds the no-allocation scratch");
+        void *ta = carve((size_t)tile_rows * d_pad * (x2 ? 2 : tile_elem_bytes(fmt)));
+        void *tb = x2 ? carve((size_t)tile_rows * d_pad * 2) : nullptr;
+        DevChunk ch{};
+        ch.span_off = c->indptr;
+        ch.span_cell = nullptr;
+        ch.first_cell = c->first_cell;
+        ch.n_spans = (int64_t)c->n_indptr - 1;
+        ch.genes = c->genes;
+        ch.values = c->values;
+        ch.cell_map = c->cell_map;
+        ch.n_cell_map = (int64_t)c->n_cell_map;
+        ch.gene_var = c->gene_map;
+        ch.n_gene_map = (int64_t)c->n_gene_map;
+        ch.target_sum = c->target_sum;
+        ch.log1p = c->log1p ? 1 : 0;
+        ch.base = 0;
+        DevScale sc{};
+        sc.sub = dev_sub_map;
+        sc.scale = scale ? 1 : 0;
+        sc.max_value = max_value;
+        sc.d = (int)d;
+        sc.mean = dev_mean;
+        sc.sd = dev_std;
+        // the Gram and colsum of this call start at zero (fill kernels, no memset)
+        launch_fill_f64(g, (int64_t)d_pad * d_pad, 0.0, st);
+        launch_fill_f64(cs, d_pad, 0.0, st);
+        for (int64_t r0 = 0; r0 < ch.n_spans; r0 += tile_rows) {
+            const int64_t r1 = std::min<int64_t>(ch.n_spans, r0 + tile_rows);
+            const int64_t rows_pad = round_up(r1 - r0, 16);
+            launch_densify(sub_chunk(ch, r0, r1), sc, fmt, ta, tb, rows_pad, d_pad, cs, g, st);
+            launch_gram(fmt, ta, tb, rows_pad, d_pad, 1, partial, st);
+            launch_gram_reduce(partial, 1, d_pad, g, st);
+        }
+        const int blocks = (int)std::min<int64_t>(ceil_div((int64_t)d * d, 256), 4096);
+        hipLaunchKernelGGL(k_noalloc_flush, dim3(blocks), dim3(256), 0, st, g, cs, (int)d, d_pad, dev_gram,
+                           dev_colsum);
+        HIPCHK(hipGetLastError());
     });
 }
 

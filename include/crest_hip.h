@@ -112,6 +112,22 @@ int crest_gram_accumulate(const crest_chunk *chunk, const int32_t *sub_map, size
                           const double *mean, size_t n_mean, const double *std, size_t n_std,
                           double max_value);
 
+/* No counterpart in the reference: crest_gram_accumulate for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The same densify + Gram kernels over a chunk whose arrays all
+ * live in device memory, launched on `stream` (a hipStream_t; NULL = the default
+ * stream) with no allocation, no memset and no synchronisation: the caller waits on
+ * the stream.  dev_chunk->indptr is device memory with indptr[0] == 0 (genes/values
+ * start at that entry), cells must be NULL; the arrays are not validated entry by
+ * entry (sub_map values must be < d).  dev_gram (+=) is [d, d] row-major (upper
+ * triangle written), dev_colsum (+=) [d].  Precision: crest_set_precision's.
+ * Scratch: one 128 MiB device array per GPU (the d_pad x d_pad Gram, the partials
+ * and the densified tiles), so d <= 2048 (larger d: CREST_E_VALUE).  Calls that
+ * overlap on one GPU share that array and race; calls on one stream do not.      */
+int crest_gram_accumulate_noalloc(const crest_chunk *dev_chunk, const int32_t *dev_sub_map, size_t n_sub_map,
+                                  double *dev_gram, double *dev_colsum, size_t d, int32_t scale,
+                                  const double *dev_mean, const double *dev_std, double max_value, void *stream);
+
 /* Replaces py.rs:386-399 `pca_from_gram(gram, colsum, n, k)` (kernels.rs:311-348).
  * Writes variance [k'] (descending, floored at 0), loadings [d, k'] row-major with
  * the sklearn svd_flip sign rule, *total = trace of the covariance and *k_out = k'
@@ -275,6 +291,9 @@ int crest_dev_free(void *ptr);
 int crest_memcpy_htod(void *dst, const void *src, size_t bytes);
 int crest_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int crest_synchronize(void);
+/* Wait for the work queued on `stream` (a hipStream_t; NULL = the default stream) of
+ * the current device, e.g. after crest_gram_accumulate_noalloc.                   */
+int crest_stream_synchronize(void *stream);
 /* Device-to-device copy by a plain vectorised kernel (k_copy16): the reference point
  * for calibrating HBM byte counters.  bytes must be a multiple of 16.            */
 int crest_dev_copy(void *dst, const void *src, size_t bytes);
