@@ -541,6 +541,22 @@ using namespace crest;
 
 // ================================================================ C ABI
 // ---------------------------------------------------------------- fused pipeline
+// HVG index -> store gene from hvg_of_gene (store gene -> HVG index or -1): every
+// index 0..d-1 used by exactly one gene.
+static std::vector<int32_t> hvg_genes(const int32_t *hvg_of_gene, size_t n_genes, int d) {
+    std::vector<int32_t> gene_of_hvg(d, -1);
+    for (size_t g = 0; g < n_genes; ++g) {
+        const int32_t h = hvg_of_gene[g];
+        if (h >= d) value_error("hvg_of_gene values must be < d");
+        if (h < 0) continue;
+        if (gene_of_hvg[h] >= 0) value_error("hvg_of_gene maps two genes to HVG index " + std::to_string(h));
+        gene_of_hvg[h] = (int32_t)g;
+    }
+    for (int h = 0; h < d; ++h)
+        if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+    return gene_of_hvg;
+}
+
 static void check_pipeline_args(const crest_pipeline_args *a, double *variance, double *loadings,
                                 crest_pipeline_result *res) {
     if (!a || !res || !variance || !loadings) value_error("NULL argument");
@@ -550,6 +566,7 @@ static void check_pipeline_args(const crest_pipeline_args *a, double *variance, 
         a->prec_gram != CREST_PREC_FP16)
         value_error("unknown Gram precision");
     check_gram_prec(a->prec_gram, a->scale, a->max_value);
+    hvg_genes(a->hvg_of_gene, a->n_genes, (int)a->d);
     if (a->prec_knn != CREST_PREC_FP64 && a->prec_knn != CREST_PREC_FP32 && a->prec_knn != CREST_PREC_BF16X3 &&
         a->prec_knn != CREST_PREC_FP16)
         value_error("unknown kNN screen precision");
@@ -566,14 +583,7 @@ static void run_pipeline(Ctx &x, const crest_pipeline_args *a, Each &&each, doub
     const int kp = (int)std::min<int64_t>({(int64_t)a->n_comps, (int64_t)d, n - 1});
     if (kp > 256) value_error("n_comps > 256 is not supported");
     const int kn = (int)std::max<int64_t>(std::min<int64_t>((int64_t)a->k, n - 1), 1);
-    std::vector<int32_t> gene_of_hvg(d, -1);
-    for (size_t g = 0; g < a->n_genes; ++g) {
-        const int32_t h = a->hvg_of_gene[g];
-        if (h >= d) value_error("hvg_of_gene values must be < d");
-        if (h >= 0) gene_of_hvg[h] = (int32_t)g;
-    }
-    for (int h = 0; h < d; ++h)
-        if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+    const std::vector<int32_t> gene_of_hvg = hvg_genes(a->hvg_of_gene, a->n_genes, d);
     const int d_pad = (int)round_up(d, kGramBlock);
     const int64_t tile_rows = a->chunk_rows ? (int64_t)round_up((int64_t)a->chunk_rows, 16) : 65536;
     // constant maps: identity cell map and gene map (every cell and gene kept)
@@ -827,6 +837,7 @@ static void check_batch_args(const crest_batch_args *b, double *variance, double
         a->prec_knn != CREST_PREC_FP16)
         value_error("unknown kNN screen precision");
     if (a->k < 1 || a->k > 64) value_error("the batched kNN needs 1 <= k <= 64");
+    hvg_genes(a->hvg_of_gene, a->n_genes, (int)a->d);
     const int64_t kp = (int64_t)std::min<size_t>(a->n_comps, a->d);
     if (kp < 1 || kp > 256) value_error("n_comps must be in 1 .. 256");
     const int64_t need = std::max<int64_t>(kp, (int64_t)a->k) + 1;
@@ -1358,14 +1369,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
     P.tiles = plan_tiles(P.off, P.tile_rows);
     for (const BatchTile &t : P.tiles) P.max_tile_rows = std::max(P.max_tile_rows, t.rows_pad);
 
-    std::vector<int32_t> gene_of_hvg(P.d, -1);
-    for (size_t g = 0; g < a->n_genes; ++g) {
-        const int32_t h = a->hvg_of_gene[g];
-        if (h >= P.d) value_error("hvg_of_gene values must be < d");
-        if (h >= 0) gene_of_hvg[h] = (int32_t)g;
-    }
-    for (int h = 0; h < P.d; ++h)
-        if (gene_of_hvg[h] < 0) value_error("every HVG index 0..d-1 must be used");
+    const std::vector<int32_t> gene_of_hvg = hvg_genes(a->hvg_of_gene, a->n_genes, P.d);
 
     // Gram row ranges: given, cached by the autotuner for this shape, or tuned now (the
     // tuner needs the first tile's tables and buffers sized for the smallest candidate)
@@ -2055,11 +2059,13 @@ int crest_pipeline_run_streamed(const crest_pipeline_args *a, const int64_t *hos
         const int64_t n = (int64_t)a->n_cells;
         if (host_indptr[0] != 0) value_error("indptr must start at 0");
         const int64_t rows = stream_rows ? (int64_t)stream_rows : 32768;
-        // largest chunk, and the CSR's own consistency (non-decreasing offsets)
+        // the CSR's own consistency: every offset non-decreasing (a chunk's entries then
+        // fit the device slots sized from its end offsets), then the largest chunk
+        for (int64_t i = 1; i <= n; ++i)
+            if (host_indptr[i] < host_indptr[i - 1]) value_error("indptr must be non-decreasing");
         int64_t max_ne = 0;
         for (int64_t lo = 0; lo < n; lo += rows) {
             const int64_t hi = std::min(n, lo + rows);
-            if (host_indptr[hi] < host_indptr[lo]) value_error("indptr must be non-decreasing");
             max_ne = std::max(max_ne, host_indptr[hi] - host_indptr[lo]);
         }
         Ctx &x = ctx();

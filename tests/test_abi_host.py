@@ -206,3 +206,56 @@ def test_batch_plan_of_subsets_is_contiguous():
     a, b, c, _ = synth.make_counts(7, 50, 3, density=0.1, seed=4 + 1000 + 1)
     np.testing.assert_array_equal(ix[ip[40]:ip[47]], b)
     np.testing.assert_array_equal(np.diff(ip[40:48]), np.diff(a))
+
+
+def _pipe_args(n_cells, n_genes, hvg, d):
+    fake = 256
+    return _lib.PipelineArgs(fake, fake, fake, n_cells, n_genes, hvg.ctypes.data, d, 1e4, 1, 1, 10.0, 10, 15,
+                             _lib.PREC_FP64, _lib.PREC_FP16, 0, fake, fake, fake)
+
+
+def test_streamed_rejects_decreasing_offsets_inside_a_chunk():
+    """crest_pipeline_run_streamed checks every offset on the host before any upload: an
+    offset inside a chunk that decreases (or overshoots the next) is a ValueError."""
+    import ctypes as C
+    n, g = 10, 20
+    hvg = np.arange(g, dtype=np.int32)
+    args = _pipe_args(n, g, hvg, g)
+    ip = np.arange(0, 11 * 3, 3, dtype=np.int64)[:n + 1].copy()
+    ip[4] = 40  # overshoots, then decreases at 5; the chunk boundaries 0 and 10 look fine
+    genes = np.zeros(int(ip[-1]) + 64, np.uint32)
+    vals = np.ones(int(ip[-1]) + 64, np.float32)
+    var, load = np.zeros(10), np.zeros(g * 10)
+    res = _lib.PipelineResult()
+    rc = _lib.lib().crest_pipeline_run_streamed(C.byref(args), ip.ctypes.data, genes.ctypes.data, vals.ctypes.data,
+                                                0, 10, var.ctypes.data, load.ctypes.data, C.byref(res))
+    with pytest.raises(ValueError, match="indptr must be non-decreasing"):
+        _lib.check(rc)
+
+
+@pytest.mark.parametrize("entry", ["run", "streamed", "batched"])
+def test_duplicate_hvg_index_is_rejected(entry):
+    """hvg_of_gene may map each HVG index to one gene only (else the last gene would
+    silently win mean/std while all of them fed the same Gram column)."""
+    import ctypes as C
+    n, g, d = 40, 20, 19
+    hvg = np.arange(g, dtype=np.int32)
+    hvg[19] = 3  # genes 3 and 19 both -> HVG 3
+    args = _pipe_args(n, g, hvg, d)
+    var, load, tot = np.zeros(64), np.zeros(g * 64), np.zeros(64)
+    if entry == "run":
+        rc = _lib.lib().crest_pipeline_run(C.byref(args), var.ctypes.data, load.ctypes.data,
+                                           C.byref(_lib.PipelineResult()))
+    elif entry == "streamed":
+        ip = np.zeros(n + 1, np.int64)
+        z = np.zeros(1, np.uint32)
+        rc = _lib.lib().crest_pipeline_run_streamed(C.byref(args), ip.ctypes.data, z.ctypes.data, z.ctypes.data, 0,
+                                                    0, var.ctypes.data, load.ctypes.data,
+                                                    C.byref(_lib.PipelineResult()))
+    else:
+        off = np.array([0, n], np.int64)
+        b = _lib.BatchArgs(args, 1, off.ctypes.data, 0, 0, 0, 0)
+        rc = _lib.lib().crest_pipeline_run_batched(C.byref(b), var.ctypes.data, load.ctypes.data, tot.ctypes.data,
+                                                   C.byref(_lib.BatchResult()))
+    with pytest.raises(ValueError, match="maps two genes to HVG index 3"):
+        _lib.check(rc)
