@@ -402,7 +402,6 @@ def c2_cpu_baseline(ip, ix, dv, G, chunk_nnz, bf, idx, args) -> dict:
     statistics, HVG selection, FP64 Gram over all cells, numpy eigh, projection of
     20,000 cells and exact kNN of 2,048 queries (both scaled to all cells); and the
     GPU results checked against it (HVG index set, variances, subspace, kNN sets)."""
-    from crest_amd.pp import hvg_from_stats
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
@@ -423,13 +422,17 @@ def c2_cpu_baseline(ip, ix, dv, G, chunk_nnz, bf, idx, args) -> dict:
     st = np.zeros((G, 5))
     for _, _, ctx in chunks():
         O.gene_stats(*ctx, st)
-    cols = hvg_from_stats(st, n, 2000)
+    m1 = st[:, 2] / n
+    v1 = np.maximum((st[:, 3] - n * m1 * m1) / (n - 1), 0.0)
+    hv, _ = O.hvg_seurat_pandas(m1, v1, n_top_genes=2000)
     t["hvg"] = time.perf_counter() - t0
-    hv = cols["highly_variable"]
     d = int(hv.sum())
     gmap = np.full(G, -1, np.int32)
     gmap[hv] = np.arange(d, dtype=np.int32)
-    mean, sd = cols["mean_log"][hv].copy(), cols["std_log"][hv].copy()
+    ml = st[:, 0] / n
+    sdl = np.sqrt(np.maximum((st[:, 1] - n * ml * ml) / (n - 1), 0.0))
+    sdl[sdl == 0] = 1.0
+    mean, sd = ml[hv].copy(), sdl[hv].copy()
     t0 = time.perf_counter()
     gram, colsum = np.zeros((d, d)), np.zeros(d)
     for _, _, ctx in chunks():

@@ -117,6 +117,19 @@ _SIGS = {
                                               C.POINTER(PipelineResult)]),
     "crest_pipeline_run_batched": (C.c_int, [C.POINTER(BatchArgs), C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.POINTER(BatchResult)]),
+    "crest_session_create": (C.c_int, [C.c_size_t, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "crest_session_destroy": (C.c_int, [C.c_void_p]),
+    "crest_session_push_stats": (C.c_int, [C.c_void_p, C.POINTER(Chunk)]),
+    "crest_session_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "crest_session_hvg_seurat": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                           C.c_double] + [C.c_void_p] * 6),
+    "crest_session_begin_gram": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int32, C.c_void_p,
+                                           C.c_void_p, C.c_double]),
+    "crest_session_push_gram": (C.c_int, [C.c_void_p, C.POINTER(Chunk)]),
+    "crest_session_solve": (C.c_int, [C.c_void_p, C.c_size_t] + [C.c_void_p] * 6),
+    "crest_session_begin_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "crest_session_push_project": (C.c_int, [C.c_void_p, C.POINTER(Chunk)]),
+    "crest_session_x_pca": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "crest_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "crest_host_free": (C.c_int, [C.c_void_p]),
 }

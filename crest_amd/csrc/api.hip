@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -82,6 +83,15 @@ struct Ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // autotuner: shape key -> chosen Gram row range
     std::map<std::string, int64_t> tuned;
+    // staging of pushed chunks (sessions): two pinned host buffers and their device
+    // twins, grow-only for the process; a slot is reused once its consumer finished
+    struct StageSlot {
+        uint8_t *pin = nullptr, *dev = nullptr;
+        size_t pin_bytes = 0, dev_bytes = 0;
+        hipEvent_t copied = nullptr, consumed = nullptr;
+        bool used = false;
+    } stage[2];
+    int stage_next = 0;
 
     void init(int d) {
         dev = d;
@@ -103,6 +113,8 @@ struct Ctx {
         for (int i = 0; i < 2; ++i) {
             HIPCHK(hipEventCreateWithFlags(&copied[i], hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&consumed[i], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&stage[i].copied, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&stage[i].consumed, hipEventDisableTiming));
         }
     }
     // Pinned host buffer `name` of at least `bytes` (grow-only).
@@ -159,6 +171,13 @@ struct Ctx {
         for (auto &kv : pinned)
             if (kv.second.first) hipHostFree(kv.second.first);
         pinned.clear();
+        for (auto &sl : stage) {
+            if (sl.pin) hipHostFree(sl.pin);
+            if (sl.dev) hipFree(sl.dev);
+            sl.pin = sl.dev = nullptr;
+            sl.pin_bytes = sl.dev_bytes = 0;
+            sl.used = false;
+        }
     }
 };
 
@@ -193,7 +212,15 @@ struct HostChunk {
     int64_t n_vars = 0;  // max(gene_map) + 1
 };
 
-static HostChunk check_chunk(const crest_chunk *c) {
+// max(map) + 1 over a host array (0 when empty or all negative)
+template <class T> static int64_t map_extent(const T *m, size_t n) {
+    int64_t mx = -1;
+    for (size_t i = 0; i < n; ++i) mx = std::max<int64_t>(mx, (int64_t)m[i]);
+    return mx + 1 > 0 ? mx + 1 : 0;
+}
+
+// Geometry of a chunk (spans, base) without the map extents.
+static HostChunk check_chunk_geometry(const crest_chunk *c) {
     if (!c) value_error("chunk is NULL");
     HostChunk h;
     h.c = c;
@@ -224,12 +251,13 @@ static HostChunk check_chunk(const crest_chunk *c) {
         h.n_spans = (int64_t)m - 1;
         h.base = p[0];
     }
-    int64_t mr = -1;
-    for (size_t i = 0; i < c->n_cell_map; ++i) mr = std::max<int64_t>(mr, c->cell_map[i]);
-    h.n_rows = mr + 1 > 0 ? mr + 1 : 0;
-    int64_t mv = -1;
-    for (size_t i = 0; i < c->n_gene_map; ++i) mv = std::max<int64_t>(mv, c->gene_map[i]);
-    h.n_vars = mv + 1 > 0 ? mv + 1 : 0;
+    return h;
+}
+
+static HostChunk check_chunk(const crest_chunk *c) {
+    HostChunk h = check_chunk_geometry(c);
+    h.n_rows = map_extent(c->cell_map, c->n_cell_map);
+    h.n_vars = map_extent(c->gene_map, c->n_gene_map);
     return h;
 }
 
@@ -2155,3 +2183,388 @@ int crest_pipeline_run_batched(const crest_batch_args *b, double *variance, doub
 }
 
 }  // extern "C"
+
+// ================================================================ sessions
+// The device state of one tl.pca / highly_variable_genes run over a chunk source:
+// gene statistics, scale parameters, Gram, colsum, eigenpairs and scores stay in
+// device memory between pushes; each push stages its chunk through one of two pinned
+// slots and the copy stream, so the upload of chunk i+1 overlaps the kernels of chunk i.
+struct crest_session {
+    int dev = 0;
+    int64_t n_vars = 0, n_rows = 0;
+    std::vector<void *> owned;  // device allocations freed with the session
+    // chunk maps, cached by the host array (pointer, length) they came from
+    const int64_t *cmap_src = nullptr;
+    size_t cmap_len = 0;
+    int64_t *d_cmap = nullptr;
+    size_t d_cmap_cap = 0;
+    const int32_t *gmap_src = nullptr;
+    size_t gmap_len = 0;
+    int32_t *d_gmap = nullptr;
+    size_t d_gmap_cap = 0;
+    double *d_stats = nullptr;
+    int64_t stats_pushed = 0;
+    // the Gram pass
+    bool gram_ready = false;
+    int d = 0, d_pad = 0, prec = 0, scale = 0;
+    double max_value = 0.0;
+    int32_t *d_sub = nullptr;
+    double *d_mean = nullptr, *d_sd = nullptr, *d_gram = nullptr, *d_colsum = nullptr;
+    // the eigensolve and projection
+    bool solved = false, projecting = false;
+    int k = 0;
+    double *d_var = nullptr, *d_load = nullptr, *d_total = nullptr;
+    float *d_shift = nullptr, *d_x = nullptr;
+};
+
+namespace crest {
+
+static void *session_alloc(crest_session *S, size_t bytes) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    S->owned.push_back(p);
+    return p;
+}
+static void session_free(crest_session *S, void *p) {
+    if (!p) return;
+    auto it = std::find(S->owned.begin(), S->owned.end(), p);
+    if (it != S->owned.end()) S->owned.erase(it);
+    HIPCHK(hipFree(p));
+}
+static Ctx &session_ctx(crest_session *S) {
+    if (!S) value_error("session is NULL");
+    const int keep = g_device;
+    g_device = S->dev;
+    Ctx &x = ctx();
+    g_device = keep;
+    return x;
+}
+
+// The chunk's maps in device memory (uploaded when the host array differs from the
+// one cached), with the session's extents checked.
+static void session_maps(Ctx &x, crest_session *S, const crest_chunk *c) {
+    if (c->cell_map != S->cmap_src || c->n_cell_map != S->cmap_len) {
+        if (map_extent(c->cell_map, c->n_cell_map) > S->n_rows)
+            value_error("cell_map maps cells beyond the session's n_rows");
+        x.sync();  // kernels in flight may read the previous map
+        if (S->d_cmap_cap < c->n_cell_map) {
+            session_free(S, S->d_cmap);
+            S->d_cmap = (int64_t *)session_alloc(S, c->n_cell_map * sizeof(int64_t));
+            S->d_cmap_cap = c->n_cell_map;
+        }
+        if (c->n_cell_map) HIPCHK(hipMemcpy(S->d_cmap, c->cell_map, c->n_cell_map * 8, hipMemcpyHostToDevice));
+        S->cmap_src = c->cell_map;
+        S->cmap_len = c->n_cell_map;
+    }
+    if (c->gene_map != S->gmap_src || c->n_gene_map != S->gmap_len) {
+        if (map_extent(c->gene_map, c->n_gene_map) != S->n_vars)
+            value_error("gene_map must map onto the session's n_vars genes");
+        x.sync();
+        if (S->d_gmap_cap < c->n_gene_map) {
+            session_free(S, S->d_gmap);
+            S->d_gmap = (int32_t *)session_alloc(S, c->n_gene_map * sizeof(int32_t));
+            S->d_gmap_cap = c->n_gene_map;
+        }
+        if (c->n_gene_map) HIPCHK(hipMemcpy(S->d_gmap, c->gene_map, c->n_gene_map * 4, hipMemcpyHostToDevice));
+        S->gmap_src = c->gene_map;
+        S->gmap_len = c->n_gene_map;
+    }
+}
+
+// Stage one chunk into the next slot and hand back its device view; the caller
+// launches its kernels on x.st and then calls session_consumed.
+static DevChunk session_stage(Ctx &x, crest_session *S, const crest_chunk *c, int &slot_index) {
+    const HostChunk h = check_chunk_geometry(c);
+    session_maps(x, S, c);
+    const int si = x.stage_next;
+    x.stage_next ^= 1;
+    Ctx::StageSlot &sl = x.stage[si];
+    if (sl.used) HIPCHK(hipEventSynchronize(sl.consumed));  // its last kernels and copy are done
+    const size_t nnz = c->nnz;
+    const bool cells = c->cells != nullptr;
+    const size_t n_off = cells ? h.span_off.size() : c->n_indptr;
+    auto up = [](size_t b) { return (size_t)round_up((int64_t)b, 256); };
+    const size_t o_g = 0, o_v = o_g + up(nnz * 4), o_s = o_v + up(nnz * 4), o_c = o_s + up(n_off * 8);
+    const size_t total = o_c + (cells ? up(h.span_cell.size() * 4) : 0);
+    if (sl.pin_bytes < total) {
+        if (sl.pin) HIPCHK(hipHostFree(sl.pin));
+        sl.pin = nullptr;
+        sl.pin_bytes = 0;
+        HIPCHK(hipHostMalloc((void **)&sl.pin, total, hipHostMallocDefault));
+        sl.pin_bytes = total;
+    }
+    if (sl.dev_bytes < total) {
+        if (sl.dev) HIPCHK(hipFree(sl.dev));
+        sl.dev = nullptr;
+        sl.dev_bytes = 0;
+        HIPCHK(hipMalloc((void **)&sl.dev, total));
+        sl.dev_bytes = total;
+    }
+    if (nnz) {
+        par_copy(sl.pin + o_g, c->genes, nnz * 4);
+        par_copy(sl.pin + o_v, c->values, nnz * 4);
+    }
+    if (cells) {
+        std::memcpy(sl.pin + o_s, h.span_off.data(), n_off * 8);
+        std::memcpy(sl.pin + o_c, h.span_cell.data(), h.span_cell.size() * 4);
+    } else {
+        std::memcpy(sl.pin + o_s, c->indptr, n_off * 8);
+    }
+    HIPCHK(hipMemcpyAsync(sl.dev, sl.pin, total, hipMemcpyHostToDevice, x.copy));
+    HIPCHK(hipEventRecord(sl.copied, x.copy));
+    HIPCHK(hipStreamWaitEvent(x.st, sl.copied, 0));
+    DevChunk d{};
+    d.genes = reinterpret_cast<const uint32_t *>(sl.dev + o_g);
+    d.values = reinterpret_cast<const float *>(sl.dev + o_v);
+    d.span_off = reinterpret_cast<const int64_t *>(sl.dev + o_s);
+    d.span_cell = cells ? reinterpret_cast<const uint32_t *>(sl.dev + o_c) : nullptr;
+    d.first_cell = cells ? 0 : c->first_cell;
+    d.n_spans = h.n_spans;
+    d.base = h.base;
+    d.cell_map = S->d_cmap;
+    d.n_cell_map = (int64_t)c->n_cell_map;
+    d.gene_var = S->d_gmap;
+    d.n_gene_map = (int64_t)c->n_gene_map;
+    d.target_sum = c->target_sum;
+    d.log1p = c->log1p ? 1 : 0;
+    slot_index = si;
+    return d;
+}
+static void session_consumed(Ctx &x, int si) {
+    x.check_launch();
+    HIPCHK(hipEventRecord(x.stage[si].consumed, x.st));
+    x.stage[si].used = true;
+}
+static DevScale session_scale(const crest_session *S) {
+    DevScale sc{};
+    sc.sub = S->d_sub;
+    sc.scale = S->scale;
+    sc.max_value = S->max_value;
+    sc.d = S->d;
+    sc.mean = S->d_mean;
+    sc.sd = S->d_sd;
+    return sc;
+}
+
+}  // namespace crest
+
+int crest_session_create(size_t n_vars, size_t n_rows, crest_session **out) {
+    return guarded([&] {
+        if (!out) value_error("out is NULL");
+        *out = nullptr;
+        if (n_vars == 0 || n_vars > (size_t)INT32_MAX) value_error("n_vars must be in 1 .. 2^31 - 1");
+        if (n_rows > (size_t)UINT32_MAX) value_error("too many rows for a session");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        std::unique_ptr<crest_session> S(new crest_session());
+        S->dev = x.dev;
+        S->n_vars = (int64_t)n_vars;
+        S->n_rows = (int64_t)n_rows;
+        try {
+            S->d_stats = (double *)session_alloc(S.get(), n_vars * 5 * sizeof(double));
+            launch_fill_f64(S->d_stats, (int64_t)n_vars * 5, 0.0, x.st);
+            x.check_launch();
+        } catch (...) {
+            for (void *p : S->owned) hipFree(p);
+            throw;
+        }
+        *out = S.release();
+    });
+}
+
+int crest_session_destroy(crest_session *S) {
+    return guarded([&] {
+        if (!S) return;
+        Ctx &x = session_ctx(S);
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.sync();
+        HIPCHK(hipStreamSynchronize(x.copy));
+        for (void *p : S->owned) hipFree(p);
+        delete S;
+    });
+}
+
+int crest_session_push_stats(crest_session *S, const crest_chunk *chunk) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        std::lock_guard<std::mutex> lk(x.mu);
+        int si = 0;
+        const DevChunk c = session_stage(x, S, chunk, si);
+        launch_gene_stats(c, S->d_stats, (int)S->n_vars, x.st);
+        session_consumed(x, si);
+        ++S->stats_pushed;
+    });
+}
+
+int crest_session_stats(crest_session *S, double *out, size_t out_len) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!out || out_len != (size_t)S->n_vars * 5) value_error("out must be (n_vars, 5)");
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.download(out, S->d_stats, out_len);
+        x.sync();
+    });
+}
+
+int crest_session_hvg_seurat(crest_session *S, int64_t n_top, int32_t n_bins, double min_mean, double max_mean,
+                             double min_disp, double max_disp, double *means, double *dispersions,
+                             double *dispersions_norm, uint8_t *highly_variable, double *mean_log,
+                             double *std_log) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!means || !dispersions || !dispersions_norm || !highly_variable || !mean_log || !std_log)
+            value_error("NULL argument");
+        if (!hvg_bins_supported(n_bins)) value_error("n_bins must be in 1 .. 256");
+        std::lock_guard<std::mutex> lk(x.mu);
+        const int64_t G = S->n_vars;
+        int32_t *codes = x.get<int32_t>("hvg_codes", G);
+        double *work = x.get<double>("hvg_work", (size_t)G * 6);
+        uint8_t *hv = x.get<uint8_t>("hvg_flag", G);
+        double *o_means = work + G, *o_disp = work + 2 * G, *o_dn = work + 3 * G, *o_ml = work + 4 * G,
+               *o_sl = work + 5 * G;
+        launch_hvg_seurat(S->d_stats, G, (double)S->n_rows, n_bins, n_top, min_mean, max_mean, min_disp, max_disp,
+                          codes, work, o_means, o_disp, o_dn, hv, o_ml, o_sl, x.st);
+        x.check_launch();
+        x.download(means, o_means, G);
+        x.download(dispersions, o_disp, G);
+        x.download(dispersions_norm, o_dn, G);
+        x.download(highly_variable, hv, G);
+        x.download(mean_log, o_ml, G);
+        x.download(std_log, o_sl, G);
+        x.sync();
+    });
+}
+
+int crest_session_begin_gram(crest_session *S, const int32_t *sub_map, size_t n_sub_map, size_t d, int32_t scale,
+                             const double *mean, const double *std_, double max_value) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!sub_map || (int64_t)n_sub_map != S->n_vars) value_error("sub_map must have length n_vars and values < d");
+        if (d == 0) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
+        for (size_t i = 0; i < n_sub_map; ++i)
+            if (sub_map[i] >= (int64_t)d) value_error("sub_map must have length n_vars and values < d");
+        const bool derive = scale && !mean && !std_;
+        if (scale && !derive && (!mean || !std_)) value_error("scale=True needs mean and std");
+        if (derive && S->stats_pushed == 0) value_error("no statistics were pushed to derive mean and std from");
+        const int prec = g_prec;
+        check_gram_prec(prec, scale, max_value);
+        std::vector<int32_t> goh;
+        if (derive) goh = hvg_genes(sub_map, n_sub_map, (int)d);
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.sync();  // kernels of an earlier Gram pass may still read these buffers
+        for (void *p : {(void *)S->d_sub, (void *)S->d_mean, (void *)S->d_sd, (void *)S->d_gram, (void *)S->d_colsum})
+            session_free(S, p);
+        S->gram_ready = S->solved = S->projecting = false;
+        S->cmap_src = nullptr;  // each pass reads its maps afresh
+        S->gmap_src = nullptr;
+        S->d = (int)d;
+        S->d_pad = (int)round_up((int64_t)d, kGramBlock);
+        S->prec = prec;
+        S->scale = scale ? 1 : 0;
+        S->max_value = max_value;
+        S->d_sub = (int32_t *)session_alloc(S, n_sub_map * 4);
+        S->d_mean = (double *)session_alloc(S, d * 8);
+        S->d_sd = (double *)session_alloc(S, d * 8);
+        S->d_gram = (double *)session_alloc(S, (size_t)S->d_pad * S->d_pad * 8);
+        S->d_colsum = (double *)session_alloc(S, (size_t)S->d_pad * 8);
+        HIPCHK(hipMemcpy(S->d_sub, sub_map, n_sub_map * 4, hipMemcpyHostToDevice));
+        if (scale && !derive) {
+            HIPCHK(hipMemcpy(S->d_mean, mean, d * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S->d_sd, std_, d * 8, hipMemcpyHostToDevice));
+        } else if (derive) {
+            int32_t *d_goh = x.get<int32_t>("sess_goh", d);
+            HIPCHK(hipMemcpyAsync(d_goh, goh.data(), d * 4, hipMemcpyHostToDevice, x.st));
+            hipLaunchKernelGGL(k_mean_std, dim3((unsigned)ceil_div((int64_t)d, 256)), dim3(256), 0, x.st, S->d_stats,
+                               d_goh, (int)d, (double)S->n_rows, S->d_mean, S->d_sd, (int64_t)0,
+                               (const int64_t *)nullptr);
+        }
+        launch_fill_f64(S->d_gram, (int64_t)S->d_pad * S->d_pad, 0.0, x.st);
+        launch_fill_f64(S->d_colsum, S->d_pad, 0.0, x.st);
+        x.check_launch();
+        S->gram_ready = true;
+    });
+}
+
+int crest_session_push_gram(crest_session *S, const crest_chunk *chunk) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!S->gram_ready) value_error("crest_session_begin_gram first");
+        std::lock_guard<std::mutex> lk(x.mu);
+        int si = 0;
+        const DevChunk c = session_stage(x, S, chunk, si);
+        gram_rows(x, c, session_scale(S), S->prec, S->d_pad, 65536, S->d_gram, S->d_colsum);
+        session_consumed(x, si);
+    });
+}
+
+int crest_session_solve(crest_session *S, size_t k, double *variance, double *loadings, double *total,
+                        double *colsum, double *mean, double *std_) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!S->gram_ready) value_error("crest_session_begin_gram first");
+        if (S->n_rows < 2) fail(CREST_E_RUNTIME, "need at least 2 cells and 1 gene");
+        if (k < 1 || k > (size_t)S->d || k > 256) value_error("k must be in 1 .. min(d, 256)");
+        if (!variance || !loadings || !total || !colsum) value_error("NULL argument");
+        std::lock_guard<std::mutex> lk(x.mu);
+        const int d = S->d, kk = (int)k;
+        if (S->k != kk || !S->d_var) {
+            for (void *p : {(void *)S->d_var, (void *)S->d_load, (void *)S->d_total, (void *)S->d_shift,
+                            (void *)S->d_x})
+                session_free(S, p);
+            S->d_var = (double *)session_alloc(S, k * 8);
+            S->d_load = (double *)session_alloc(S, (size_t)d * k * 8);
+            S->d_total = (double *)session_alloc(S, 8);
+            S->d_shift = (float *)session_alloc(S, k * 4);
+            S->d_x = (float *)session_alloc(S, (size_t)S->n_rows * k * 4);
+        }
+        S->k = kk;
+        S->solved = false;
+        pca_device(x, S->d_gram, S->d_pad, S->d_colsum, d, (double)S->n_rows, kk, S->d_var, S->d_load, S->d_total);
+        x.download(variance, S->d_var, k);
+        x.download(loadings, S->d_load, (size_t)d * k);
+        x.download(total, S->d_total, 1);
+        x.download(colsum, S->d_colsum, d);
+        if (mean) x.download(mean, S->d_mean, d);
+        if (std_) x.download(std_, S->d_sd, d);
+        x.sync();
+        S->solved = true;
+    });
+}
+
+int crest_session_begin_project(crest_session *S, const float *prefill, size_t k) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!S->solved) value_error("crest_session_solve first");
+        if (!prefill || k != (size_t)S->k) value_error("prefill must have k entries");
+        std::lock_guard<std::mutex> lk(x.mu);
+        HIPCHK(hipMemcpy(S->d_shift, prefill, k * 4, hipMemcpyHostToDevice));
+        launch_fill_rows(S->d_shift, S->k, S->n_rows, S->d_x, x.st);
+        x.check_launch();
+        S->projecting = true;
+        S->cmap_src = nullptr;  // each pass reads its maps afresh
+        S->gmap_src = nullptr;
+    });
+}
+
+int crest_session_push_project(crest_session *S, const crest_chunk *chunk) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!S->projecting) value_error("crest_session_begin_project first");
+        std::lock_guard<std::mutex> lk(x.mu);
+        int si = 0;
+        const DevChunk c = session_stage(x, S, chunk, si);
+        launch_project(c, session_scale(S), S->d_load, S->k, S->d_x, 0, x.st);
+        session_consumed(x, si);
+    });
+}
+
+int crest_session_x_pca(crest_session *S, float *out, size_t out_len) {
+    return guarded([&] {
+        Ctx &x = session_ctx(S);
+        if (!S->projecting) value_error("crest_session_begin_project first");
+        if (!out || out_len != (size_t)S->n_rows * S->k) value_error("out must be (n_rows, k)");
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.download(out, S->d_x, out_len);
+        x.sync();
+    });
+}

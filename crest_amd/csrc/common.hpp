@@ -241,6 +241,14 @@ void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, in
                            uint64_t *keys, uint64_t *keys2, float *w, float *w2, float *wu, uint32_t *flag,
                            uint32_t *pos, double *part, void *temp, size_t temp_bytes, uint32_t *rows,
                            uint32_t *cols, float *wout, uint32_t *count, hipStream_t s);
+// kernels_hvg.hip: seurat HVG selection from [n_vars, 5] gene statistics (one
+// workgroup); work: codes int32 [n_vars], compact double [n_vars].
+bool hvg_bins_supported(int n_bins);
+void launch_hvg_seurat(const double *stats, int64_t n_vars, double n_cells, int n_bins, int64_t n_top,
+                       double min_mean, double max_mean, double min_disp, double max_disp, int32_t *codes,
+                       double *compact, double *means, double *disp, double *disp_norm, uint8_t *hv,
+                       double *mean_log, double *std_log, hipStream_t s);
+void launch_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x, hipStream_t s);
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);

@@ -951,6 +951,12 @@ void launch_shift_fill(const double *colsum, const double *V, int d, int k, doub
     hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x, (const int64_t *)nullptr);
 }
 
+void launch_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(ceil_div(n_rows * k, 256), 8192);
+    if (n_rows * k > 0)
+        hipLaunchKernelGGL(k_fill_rows, dim3(g), dim3(256), 0, s, shift_f, k, n_rows, x, (const int64_t *)nullptr);
+}
+
 void launch_shift_fill_batch(const double *colsum, int ld_c, const double *V, int d, int k,
                              const int64_t *dev_cell_off, int n_sub, int64_t max_rows, float *x, float *shift_f,
                              hipStream_t s) {

@@ -1,77 +1,91 @@
-"""Streamed PCA (crest/tl.py:24-91): per-gene statistics, the Gram pass, the FP64
-eigensolve and the projection pass, each chunk through the gfx950 library."""
+"""crest.tl.pca on the GPU library (crest/tl.py:24-91).
+
+The PCA of the (optionally scaled and clipped) matrix through its d x d covariance,
+without forming the dense matrix: one device session takes the store's chunks through
+up to three passes -- gene statistics for the scale parameters (skipped when
+highly_variable_genes cached them under the same ops), the Gram, and the projection --
+with the Gram, colsum, eigensolve and scores resident on the GPU.  Arguments,
+results (obsm['X_pca'], varm['PCs'], uns['pca']) and errors are the reference's.
+"""
 from __future__ import annotations
 
 from typing import Optional
 
 import numpy as np
 
-from . import _native
 from .core import BioFrame
-from .pp import _mean_var, gene_stats
+from .session import Session
 
 __all__ = ["pca"]
 
+_MAX_GRAM_GENES = 20_000  # the reference's bound on the d x d Gram (crest/tl.py:55-56)
 
-def _scale_params(bf: BioFrame, mask: np.ndarray):
-    """Per-gene log-scale mean/std (ddof = 1), cached by HVG selection for the same
-    ops (crest/tl.py:24-34)."""
-    cached = bf.uns.get("hvg", {}).get("ops") == bf.ops and "mean_log" in bf.var
-    if cached:
-        mean, std = bf.var["mean_log"], bf.var["std_log"]
-    else:
-        st = gene_stats(bf)
-        mean, var = _mean_var(st[:, 0], st[:, 1], bf.n_obs)
-        std = np.sqrt(var)
-        std[std == 0] = 1.0
-    return np.ascontiguousarray(mean[mask], np.float64), np.ascontiguousarray(std[mask], np.float64)
+
+def _hvg_mask(bf: BioFrame, use_highly_variable: Optional[bool]):
+    if use_highly_variable is None:
+        use_highly_variable = "highly_variable" in bf.var
+    if use_highly_variable:
+        return np.asarray(bf.var["highly_variable"], bool), True
+    return np.ones(bf.n_vars, bool), False
+
+
+def _cached_scale(bf: BioFrame, mask: np.ndarray):
+    """mean_log / std_log of the selected genes when highly_variable_genes computed them
+    under the current ops (crest/tl.py:26-28); otherwise None (derived on the device)."""
+    if bf.uns.get("hvg", {}).get("ops") != bf.ops or "mean_log" not in bf.var:
+        return None, None
+    return (np.ascontiguousarray(np.asarray(bf.var["mean_log"])[mask], np.float64),
+            np.ascontiguousarray(np.asarray(bf.var["std_log"])[mask], np.float64))
 
 
 def pca(bf: BioFrame, n_comps: int = 50, use_highly_variable: Optional[bool] = None,
         scale: Optional[bool] = None, max_value: Optional[float] = None) -> BioFrame:
-    """Exact PCA of the (optionally scaled) matrix through its covariance, streamed
-    chunk by chunk; equals sc.pp.scale(max_value) + sc.tl.pca on the dense matrix
-    without forming it.  Adds obsm['X_pca'], varm['PCs'] and uns['pca']."""
-    if use_highly_variable is None:
-        use_highly_variable = "highly_variable" in bf.var
-    mask = bf.var["highly_variable"].astype(bool) if use_highly_variable else np.ones(bf.n_vars, bool)
-    d = int(mask.sum())
+    """Exact PCA of the (scaled) matrix via its covariance, streamed chunk by chunk;
+    with scaling it equals sc.pp.scale(max_value) + sc.tl.pca on the dense matrix.
+    Adds obsm['X_pca'] (f32 [n_obs, k]), varm['PCs'] (f32 [n_vars, k]) and uns['pca']
+    (variance, variance_ratio, params, projection)."""
+    mask, used_hvg = _hvg_mask(bf, use_highly_variable)
+    d = int(np.count_nonzero(mask))
     if d < 2:
         raise ValueError("need at least 2 genes for PCA")
-    if d > 20_000:
+    if d > _MAX_GRAM_GENES:
         raise ValueError(f"{d} genes: select highly variable genes first (Gram PCA is O(genes^2) memory)")
-    if scale is None:
-        scale = "scale" in bf.uns
+    do_scale = ("scale" in bf.uns) if scale is None else bool(scale)
     if max_value is None:
-        max_value = bf.uns.get("scale", {}).get("max_value", 10.0) if scale else 0.0
-    gmap = np.full(bf.n_vars, -1, np.int32)
-    gmap[mask] = np.arange(d, dtype=np.int32)
-    mean = std = None
-    if scale:
-        mean, std = _scale_params(bf, mask)
-
+        max_value = bf.uns.get("scale", {}).get("max_value", 10.0) if do_scale else 0.0
+    clip = float(max_value or 0.0)
+    sub = np.full(bf.n_vars, -1, np.int32)
+    sub[np.flatnonzero(mask)] = np.arange(d, dtype=np.int32)
     n = bf.n_obs
-    gram = np.zeros((d, d), np.float64)
-    colsum = np.zeros(d, np.float64)
-    for ctx in bf.iter_ctx():
-        _native.gram_accumulate(*ctx, gmap, gram, colsum, scale, mean, std, float(max_value or 0.0))
     k = min(n_comps, d, n - 1)
-    variance, loadings, total = _native.pca_from_gram(gram, colsum, n, k)
-    del gram
 
-    shift = (colsum / n) @ loadings
-    X = np.empty((n, k), np.float32)
-    X[:] = -shift.astype(np.float32)
-    for ctx in bf.iter_ctx():
-        _native.project(*ctx, gmap, loadings, X, scale, mean, std, float(max_value or 0.0))
+    with Session(bf.n_vars, n) as sess:
+        mean = std = None
+        if do_scale:
+            mean, std = _cached_scale(bf, mask)
+            if mean is None:  # statistics pass; mean / std derived on the device
+                for ctx in bf.iter_ctx():
+                    sess.push_stats(ctx)
+        sess.begin_gram(sub, d, do_scale, mean, std, clip)
+        for ctx in bf.iter_ctx():
+            sess.push_gram(ctx)
+        variance, loadings, total, colsum, mean_used, std_used = sess.solve(k)
+        shift = (colsum / n) @ loadings
+        sess.begin_project(-shift.astype(np.float32))
+        for ctx in bf.iter_ctx():
+            sess.push_project(ctx)
+        scores = sess.x_pca()
 
     pcs = np.zeros((bf.n_vars, k), np.float32)
     pcs[mask] = loadings
-    bf.obsm["X_pca"] = X
+    bf.obsm["X_pca"] = scores
     bf.varm["PCs"] = pcs
-    bf.uns["pca"] = {"variance": variance, "variance_ratio": variance / total,
-                     "params": {"scale": bool(scale), "max_value": max_value,
-                                "use_highly_variable": bool(use_highly_variable)},
-                     "projection": {"genes": [g for g, m in zip(bf.var_names, mask) if m], "loadings": loadings,
-                                    "mean": mean, "std": std, "shift": shift, "ops": list(bf.ops)}}
+    bf.uns["pca"] = {
+        "variance": variance,
+        "variance_ratio": variance / total,
+        "params": {"scale": do_scale, "max_value": max_value, "use_highly_variable": used_hvg},
+        "projection": {"genes": [g for g, m in zip(bf.var_names, mask) if m], "loadings": loadings,
+                       "mean": mean_used if do_scale else None, "std": std_used if do_scale else None,
+                       "shift": shift, "ops": list(bf.ops)},
+    }
     return bf

@@ -180,6 +180,61 @@ int crest_connectivities(const uint32_t *indices, const float *distances, size_t
 int crest_materialize(const crest_chunk *chunk, uint32_t *rows, uint32_t *vars, float *values,
                       size_t cap, size_t *n_out);
 
+/* Sessions (no reference counterpart; the explicit device context of one tl.pca or
+ * highly_variable_genes run).  The reference keeps its accumulators in numpy arrays
+ * between per-chunk native calls (crest/tl.py:24-91, crest/pp.py:115-120); a session
+ * keeps them in device memory instead: chunks of any chunk source (the 9-tuple of
+ * crest/core.py:265-278, host memory) are pushed one at a time through two pinned
+ * staging slots and a copy stream, so the upload of one chunk overlaps the kernels of
+ * the previous one, and nothing is downloaded until a result is asked for.  A call
+ * returns once its chunk is staged; kernels run asynchronously.  The cell_map and
+ * gene_map arrays are read on their first use in each pass (statistics, Gram,
+ * projection) and again only when another array (pointer or length) is passed; do
+ * not modify them within a pass.
+ * Calls on one session must not overlap; sessions are independent.               */
+typedef struct crest_session crest_session;
+
+/* A session over n_vars kept genes (max(gene_map)+1 of its chunks) and n_rows kept
+ * cells (max(cell_map)+1); gene statistics start at zero.                         */
+int crest_session_create(size_t n_vars, size_t n_rows, crest_session **out);
+int crest_session_destroy(crest_session *s);
+
+/* Gene statistics of one chunk accumulated on the device (kernels.rs:141-160).     */
+int crest_session_push_stats(crest_session *s, const crest_chunk *chunk);
+/* The accumulated [n_vars, 5] statistics (Σx, Σx², Σexpm1(x), Σexpm1(x)², nnz).  */
+int crest_session_stats(crest_session *s, double *out, size_t out_len);
+
+/* Seurat-flavour highly variable genes (crest/pp.py:201-292) from the session's
+ * statistics of n_rows cells, on the device.  n_top < 0: the mean / dispersion
+ * window instead of the top n_top.  Each output has n_vars entries: means (log1p of
+ * the expm1 mean), dispersions (log; NaN where zero), dispersions_norm,
+ * highly_variable (0/1), mean_log, std_log (0 -> 1).  1 <= n_bins <= 256.      */
+int crest_session_hvg_seurat(crest_session *s, int64_t n_top, int32_t n_bins, double min_mean, double max_mean,
+                             double min_disp, double max_disp, double *means, double *dispersions,
+                             double *dispersions_norm, uint8_t *highly_variable, double *mean_log,
+                             double *std_log);
+
+/* Start the Gram pass of tl.pca (crest/tl.py:61-71): sub_map [n_vars] -> HVG index
+ * 0..d-1 or -1; with scale, mean/std [d] given, or both NULL to derive them on the
+ * device from the session's statistics (tl.py:30-34).  Precision: crest_set_precision's
+ * (FP64 by default).  Resets the session's Gram and colsum.                       */
+int crest_session_begin_gram(crest_session *s, const int32_t *sub_map, size_t n_sub_map, size_t d, int32_t scale,
+                             const double *mean, const double *std, double max_value);
+/* gram_accumulate of one chunk (kernels.rs:221-309) into the session's device Gram. */
+int crest_session_push_gram(crest_session *s, const crest_chunk *chunk);
+/* pca_from_gram (kernels.rs:311-348) on the session's Gram and colsum with n = n_rows
+ * and k components (tl.py:72-73): variance [k], loadings [d, k] row-major, *total,
+ * colsum [d] (for the shift, tl.py:76); mean/std [d] (may be NULL) receive the scale
+ * parameters in use.                                                                */
+int crest_session_solve(crest_session *s, size_t k, double *variance, double *loadings, double *total,
+                        double *colsum, double *mean, double *std);
+/* Start the projection pass: every score row set to prefill [k] (tl.py:77-78).      */
+int crest_session_begin_project(crest_session *s, const float *prefill, size_t k);
+/* project of one chunk (kernels.rs:350-371) into the session's device scores.       */
+int crest_session_push_project(crest_session *s, const crest_chunk *chunk);
+/* The scores [n_rows, k] (f32).                                                     */
+int crest_session_x_pca(crest_session *s, float *out, size_t out_len);
+
 /* Device-resident pipeline (no reference counterpart: the fused form of tl.pca +
  * pp.neighbors over a CSR already resident in HBM) -------------------------------- */
 

@@ -338,6 +338,39 @@ def hvg_seurat_pandas(mean_expm1, var_expm1, n_top_genes=None, n_bins=20, min_me
     return hv, dn
 
 
+def hvg_cell_ranger_pandas(mean_log_unused, stats, n, n_top_genes=None, min_mean=0.0125, max_mean=3.0,
+                           min_disp=0.5, max_disp=np.inf):
+    """scanpy's cell_ranger flavour with pandas: the log-scale mean and variance, bins
+    cut at the 10th..100th percentiles of the mean (pandas.cut), per bin the median
+    and the MAD / 0.6745 of the dispersions.  Returns (highly_variable, dispersions_norm)."""
+    import pandas as pd
+    m = stats[:, 0] / n
+    v = np.maximum((stats[:, 1] - n * m * m) / (n - 1), 0.0)
+    m = m.copy()
+    m[m == 0] = 1e-12
+    with np.errstate(divide="ignore", invalid="ignore"):
+        disp = v / m
+    edges = np.r_[-np.inf, np.percentile(m, np.arange(10, 105, 5)), np.inf]
+    df = pd.DataFrame({"means": m, "dispersions": disp})
+    df["mean_bin"] = pd.cut(df["means"], edges)
+    grp = df.groupby("mean_bin", observed=False)["dispersions"]
+    med = grp.median()
+    mad = grp.apply(lambda x: np.median(np.abs(x.dropna() - np.median(x.dropna()))) if x.notna().any()
+                    else np.nan) / 0.6744897501960817
+    codes = df["mean_bin"].cat.codes.to_numpy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dn = (disp - med.to_numpy()[codes]) / mad.to_numpy()[codes]
+    if n_top_genes is not None:
+        x = dn[~np.isnan(dn)]
+        k = min(n_top_genes, x.size, len(dn))
+        cutoff = np.sort(x)[::-1][k - 1]
+        hv = np.nan_to_num(dn, nan=-np.inf) >= cutoff
+    else:
+        d0 = np.nan_to_num(dn)
+        hv = (m > min_mean) & (m < max_mean) & (d0 > min_disp) & (d0 < max_disp)
+    return hv, dn
+
+
 # ------------------------------------------------------------------ metrics
 def principal_angle(V1, V2):
     """Largest principal angle (rad) between the column spans of V1 and V2, from the

@@ -268,31 +268,19 @@ def test_principal_angle(oracle):
     assert oracle.principal_angle(a, a[:, ::-1]) < 1e-15
 
 
-def test_pd_cut_codes_match_pandas():
-    import pandas as pd
-    from crest_amd.pp import _pd_cut_codes
-    rng = np.random.default_rng(0)
-    for _ in range(5):
-        x = rng.gamma(0.5, 1.0, 3000)
-        codes = _pd_cut_codes(x, 20)
-        np.testing.assert_array_equal(codes, pd.cut(x, 20).codes)
-
-
-def test_hvg_restatement_matches_pandas_oracle(oracle):
-    from crest_amd.pp import hvg_from_stats
+def test_cell_ranger_columns_match_pandas_oracle(oracle):
+    """The cell_ranger flavour (host numpy from the device statistics) against a pandas
+    restatement: percentile bins (pandas.cut), groupby median and MAD."""
+    from crest_amd.pp import _cell_ranger_columns
     Xb = make_counts(1500, 600, 5, seed=4)
     st = np.zeros((600, 5))
     oracle.gene_stats(*ctx_of(Xb), st)
     n = Xb.shape[0]
-    cols = hvg_from_stats(st, n, n_top_genes=200)
-    m = st[:, 2] / n
-    v = np.maximum((st[:, 3] - n * m * m) / (n - 1), 0)
-    hv, dn = oracle.hvg_seurat_pandas(m, v, n_top_genes=200)
-    np.testing.assert_array_equal(cols["highly_variable"], hv)
-    np.testing.assert_allclose(cols["dispersions_norm"], dn, rtol=1e-10, equal_nan=True)
-    cols2 = hvg_from_stats(st, n)
-    hv2, _ = oracle.hvg_seurat_pandas(m, v)
-    np.testing.assert_array_equal(cols2["highly_variable"], hv2)
+    for n_top in (200, None):
+        cols = _cell_ranger_columns(st, n, n_top, 0.0125, 3.0, 0.5, np.inf)
+        hv, dn = oracle.hvg_cell_ranger_pandas(st[:, 0] / n, st, n, n_top_genes=n_top)
+        np.testing.assert_allclose(cols["dispersions_norm"], dn, rtol=1e-12, equal_nan=True)
+        np.testing.assert_array_equal(cols["highly_variable"], hv)
 
 
 def test_connectivities_pins(oracle):
