@@ -495,15 +495,15 @@ static void knn_device_large(Ctx &x, const float *dev_xc, int64_t nc, const floa
 
 static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
                        int prec, uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
-    if (k > 64) return knn_device_large(x, dev_xc, nc, dev_xq, nq, p, k, dev_idx, dev_dist, ev3);
+    const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
+    const int kpad = knn_kpad(kp, p);
+    // k above the register lists, or rows wider than the register screens hold (134
+    // columns at FP32, 137 split-BF16, 252 FP16): the any-width FP32 GEMM screen of
+    // k + max(32, k/8) candidates per query, then the same exact rerank
+    if (k > 64 || kpad < 0) return knn_device_large(x, dev_xc, nc, dev_xq, nq, p, k, dev_idx, dev_dist, ev3);
     const bool graph = dev_xq == nullptr;
     if (graph) nq = nc;
-    const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
     const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
-    const int kpad = knn_kpad(kp, p);
-    if (kpad < 0)
-        value_error("data has too many columns for the GPU kNN (at most 134 for fp32/fp64, 137 for bf16x3, "
-                    "252 for fp16)");
     const int64_t nc_pad = knn_rows_padded(nc), nq_pad = knn_rows_padded(nq);
     const size_t eb = kp == CREST_PREC_FP32 ? 4 : 2;
     void *xa = x.get<uint8_t>("knn_xa", (size_t)nc_pad * kpad * eb);

@@ -149,8 +149,10 @@ int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub
  * *k_out; indices [n, k'] u32 and distances [n, k'] f32 (Euclidean, ascending, ties
  * by index, self excluded) must hold n*min(k, max(n-1,1)) entries.  The search is
  * exact for every n: `exact` (-1 = None, 0, 1) and `seed` are accepted and ignored.
- * k' <= 64 uses the screen precision of crest_set_knn_precision; 64 < k' <= 3584
- * (Scrublet's k, doublets.py:131-137) screens in FP32; larger k' is a ValueError.  */
+ * Any dim.  k' <= 64 uses the screen precision of crest_set_knn_precision; 64 < k'
+ * <= 3584 (Scrublet's k, doublets.py:131-137), or rows wider than the register
+ * screens hold (134 columns FP32/FP64, 137 split-BF16, 252 FP16), screen in FP32 on
+ * GEMM dot products; larger k' is a ValueError.                                   */
 int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact,
                     uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
 

@@ -211,12 +211,11 @@ def _clustered(n, d, seed):
 @pytest.mark.parametrize("n,d,k,scale,offset", [(600, 7, 10, 1.0, 0.0), (5000, 50, 15, 1.0, 0.0),
                                                 (2049, 30, 32, 1.0, 0.0), (300, 3, 1, 1.0, 0.0),
                                                 (1500, 200, 15, 1.0, 0.0), (2000, 50, 15, 3e4, 0.0),
-                                                (2000, 20, 15, 1e-3, 50.0), (1000, 12, 64, 1.0, 0.0)])
+                                                (2000, 20, 15, 1e-3, 50.0), (1000, 12, 64, 1.0, 0.0),
+                                                (1200, 512, 15, 1.0, 0.0), (900, 300, 64, 1.0, 0.0)])
 def test_knn_graph(oracle, prec, n, d, k, scale, offset):
-    if prec == "bf16x3" and d > 137:
-        pytest.skip("split-BF16 screen holds at most 137 columns")
-    if prec == "fp64" and d > 134:
-        pytest.skip("FP32 screen holds at most 134 columns")
+    """Any width (knn.rs:337-369): rows wider than the register screens go through the
+    GEMM screen and the same exact rerank."""
     x = (_clustered(n, d, n + d) * np.float32(scale) + np.float32(offset)).astype(np.float32)
     ca.set_knn_precision(prec)
     try:
@@ -327,13 +326,17 @@ def test_knn_precision_knob():
     assert ca.get_knn_precision() == "fp64"
 
 
-def test_knn_too_wide():
+def test_knn_wide_ties_by_index():
+    """300 identical columns-wide rows: every distance is 0, so each row's neighbours
+    are the lowest other indices (knn.rs:68-87 keeps the earlier candidate on ties)."""
     ca.set_knn_precision("fp16")
     try:
-        with pytest.raises(ValueError):
-            N.knn_graph(np.zeros((10, 300), np.float32), 3)
+        idx, dist = N.knn_graph(np.zeros((10, 300), np.float32), 3)
     finally:
         ca.set_knn_precision(None)
+    want = np.array([[j for j in range(10) if j != i][:3] for i in range(10)])
+    np.testing.assert_array_equal(idx, want)
+    assert (dist == 0).all()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
@@ -354,7 +357,8 @@ def test_knn_query_exact(prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
 @pytest.mark.parametrize("nr,nq,d,k,shift", [(5000, 3000, 50, 15, 0.0), (700, 1300, 7, 30, 0.0),
-                                             (40, 600, 3, 64, 0.0), (2000, 900, 20, 15, 4.0)])
+                                             (40, 600, 3, 64, 0.0), (2000, 900, 20, 15, 4.0),
+                                             (3000, 500, 400, 15, 0.0)])
 def test_knn_query_clustered(oracle, prec, nr, nq, d, k, shift):
     R = _clustered(nr, d, nr + d)
     Q = (_clustered(nq, d, nq + 7 * d) + np.float32(shift)).astype(np.float32)  # shift: query set offset

@@ -152,3 +152,23 @@ def test_many_subsets(oracle):
         ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
         bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
         assert bad == ties, (i, n, bad, ties)
+
+
+def test_pipeline_wide_scores(oracle):
+    """150 components (wider than the FP32 register screen's 134 columns): the kNN of
+    the fused pipeline runs the GEMM screen and exact rerank; sets equal the FP64
+    exact search except ties, distances the reference's f32 formula."""
+    from crest_amd.pipeline import DeviceCSR, Pipeline
+    ip, ix, dv, _ = synth.make_counts(6000, 400, 8, density=0.1, seed=21)
+    p = Pipeline(DeviceCSR.upload(ip, ix, dv, 400), np.arange(400, dtype=np.int32), n_comps=150, k=15,
+                 prec_gram="fp64", prec_knn="fp64")
+    out = p.run()
+    assert out.variance.shape == (150,)
+    X = p.x_pca()
+    idx, dist = p.knn()
+    q = np.arange(0, len(X), 7)
+    ref_i, ref_d2 = oracle.knn_exact_f64(X, 15, q)
+    bad, ties = oracle.knn_mismatch_ties(idx[q], X, q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    pd = oracle.pair_dist(X, np.repeat(q, 15), idx[q].ravel()).reshape(len(q), 15)
+    np.testing.assert_array_equal(dist[q], pd)
