@@ -158,6 +158,37 @@ def cpu_baseline_leg(indptr, indices, data, n_genes, gpu, args):
     return base, acc
 
 
+def stage_only(args) -> int:
+    """One chunk stage alone on the 1M x 2k workload, for hardware-counter runs that
+    must not include the eigensolve's dispatches: 'stats' (gene statistics) or 'gram'
+    (statistics, then the Gram in the headline's Gram precision), through a device
+    session over the whole CSR as one chunk."""
+    import crest_amd as ca
+    from crest_amd import synth
+    from crest_amd.session import Session
+    ip, ix, dv, info = synth.make_counts(args.cells, args.genes, args.clusters, density=args.density, seed=args.seed)
+    ctx = (ix, dv, np.arange(args.cells, dtype=np.int64), np.arange(args.genes, dtype=np.int32), 1e4, True, None, ip, 0)
+    ca.set_precision(args.gram_prec)
+    with Session(args.genes, args.cells) as sess:
+        for _ in range(args.steps):
+            sess.push_stats(ctx)
+            if args.stage_only == "gram":
+                sess.begin_gram(np.arange(args.genes, dtype=np.int32), args.genes, True, None, None, 10.0)
+                sess.push_gram(ctx)
+            sess.stats()
+    if args.calibrate_copy:
+        from crest_amd._lib import check, lib
+        from crest_amd.pipeline import DeviceBuffer
+        nb = args.calibrate_copy << 20
+        a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+        check(lib().crest_dev_copy(b.ptr, a.ptr, nb))
+        check(lib().crest_synchronize())
+        log(f"copy kernel: {nb} bytes read, {nb} bytes written")
+    print(json.dumps({"stage_only": args.stage_only, "n": args.cells, "nnz": info["nnz"],
+                      "prec_gram": args.gram_prec}), flush=True)
+    return 0
+
+
 def knn_only(args) -> int:
     """The kNN stage alone (knn_graph through the C ABI) on clustered synthetic scores of
     the workload's shape, plus the optional copy calibration: a short program for
@@ -503,6 +534,8 @@ def main() -> int:
                     help="sampled queries of the other legs' kNN agreement checks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
+    ap.add_argument("--stage-only", choices=["stats", "gram"], default=None,
+                    help="run only the gene statistics or statistics + Gram over the workload (profiling)")
     ap.add_argument("--gram-prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16"],
                     help="Gram of the headline leg (fp64: FP64 MFMA; fp16: split-FP16 hi+lo, three MFMA products)")
     ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
@@ -543,6 +576,8 @@ def main() -> int:
 
     if args.knn_only:
         return knn_only(args)
+    if args.stage_only:
+        return stage_only(args)
     if args.batch_only:
         print(json.dumps({"batch_only": True, "batch": batched_leg(args)}), flush=True)
         return 0

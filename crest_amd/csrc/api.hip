@@ -282,6 +282,7 @@ static DevChunk upload_chunk(Ctx &x, const HostChunk &h) {
     d.log1p = c->log1p ? 1 : 0;
     d.n_spans = h.n_spans;
     d.base = h.base;
+    d.nnz = (int64_t)c->nnz;
     if (c->cells) {
         d.span_off = x.upload("span_off", h.span_off.data(), h.span_off.size());
         d.span_cell = x.upload("span_cell", h.span_cell.data(), h.span_cell.size());
@@ -348,8 +349,35 @@ static int choose_split(int nblk, int64_t rows_pad, int fmt) {
 // its tile writes, the Gram by L2 and MFMA, so the two share the CUs well).  Densify's
 // exact lo^2 diagonal then accumulates into a buffer of its own, added to the Gram's
 // diagonal at the end, since the split-K reduce adds into dev_gram without atomics.
+// The FP64 Gram by sparse pair products (kernels_sparse.hip): the default for
+// CREST_PREC_FP64 up to 4,096 HVGs; CREST_GRAM_SPARSE=0 selects the dense FP64 MFMA form.
+static bool gram_sparse_enabled(int prec, int d_pad) {
+    if (prec != CREST_PREC_FP64 || !sparse_gram_supported(d_pad)) return false;
+    const char *e = std::getenv("CREST_GRAM_SPARSE");
+    return !(e && std::string(e) == "0");
+}
+
+static void gram_rows_sparse(Ctx &x, const DevChunk &c, const DevScale &sc, int d_pad, double *dev_gram,
+                             double *dev_colsum) {
+    if (c.n_spans <= 0) return;
+    const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
+    const int n_split = 16;
+    const size_t ne = (size_t)std::max<int64_t>(c.nnz, 1);
+    double *vals = x.get<double>("sp_vals", ne);
+    uint8_t *cols = x.get<uint8_t>("sp_cols", ne);
+    uint32_t *meta = x.get<uint32_t>("sp_meta", sparse_meta_words(c.n_spans, d_pad));
+    double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
+    double *cs_partial = x.get<double>("sp_cs_partial", (size_t)nb * n_split * kGramBlock);
+    launch_sparse_pack(c, sc, d_pad, vals, cols, meta, x.st);
+    launch_gram_sparse(vals, cols, meta, c.span_off, c.base, c.n_spans, d_pad, n_split, partial, cs_partial, dev_colsum,
+                       x.st);
+    launch_gram_reduce(partial, n_split, d_pad, dev_gram, x.st);
+    x.check_launch();
+}
+
 static void gram_rows(Ctx &x, const DevChunk &c, const DevScale &sc, int prec, int d_pad, int64_t tile_rows,
                       double *dev_gram, double *dev_colsum) {
+    if (gram_sparse_enabled(prec, d_pad)) return gram_rows_sparse(x, c, sc, d_pad, dev_gram, dev_colsum);
     const int fmt = tile_fmt(prec);
     const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
     const int64_t n_tiles = ceil_div(c.n_spans, tile_rows);
@@ -882,7 +910,7 @@ static void check_batch_args(const crest_batch_args *b, double *variance, double
 struct BatchPlan {
     std::string pre;  // buffer-name prefix: graph-mode calls own their buffers ("bg_"), eager ones "bt_"
     int n_sub = 0, d = 0, d_pad = 0, kp = 0, kn = 0, fmt = 0, kprec = 0, L = 0, kpad = 0, nblk = 0, grid = 0;
-    int64_t n = 0, max_rows = 0, n_pad = 0, max_pad = 0, tile_rows = 0, gram_rows = 0, eig_batch = 0;
+    int64_t n = 0, max_rows = 0, n_pad = 0, max_pad = 0, tile_rows = 0, gram_rows = 0, eig_batch = 0, nnz = 0;
     std::vector<int64_t> off, row0, nrow;
     std::vector<BatchTile> tiles;
     // device tables
@@ -1035,6 +1063,7 @@ static DevChunk batch_chunk(const crest_batch_args *b, const BatchPlan &P, const
     c.target_sum = a->target_sum;
     c.log1p = a->log1p ? 1 : 0;
     c.base = 0;
+    c.nnz = P.nnz;
     return c;
 }
 
@@ -1368,6 +1397,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
     BatchPlan P;
     P.n_sub = (int)b->n_subsets;
     P.n = (int64_t)a->n_cells;
+    HIPCHK(hipMemcpy(&P.nnz, a->dev_indptr + P.n, 8, hipMemcpyDeviceToHost));
     P.d = (int)a->d;
     P.d_pad = (int)round_up(P.d, kGramBlock);
     P.kp = (int)std::min<size_t>(a->n_comps, a->d);
@@ -1845,6 +1875,7 @@ int crest_gram_accumulate_noalloc(const crest_chunk *dev_chunk, const int32_t *d
         ch.target_sum = c->target_sum;
         ch.log1p = c->log1p ? 1 : 0;
         ch.base = 0;
+        ch.nnz = (int64_t)c->nnz;
         DevScale sc{};
         sc.sub = dev_sub_map;
         sc.scale = scale ? 1 : 0;
@@ -2063,6 +2094,8 @@ int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *l
         Ctx &x = ctx();
         std::lock_guard<std::mutex> lk(x.mu);
         const int64_t n = (int64_t)a->n_cells;
+        int64_t nnz = 0;
+        HIPCHK(hipMemcpy(&nnz, a->dev_indptr + n, 8, hipMemcpyDeviceToHost));
         auto resident = [&](const DevChunk &tmpl, auto &&fn) {
             DevChunk c = tmpl;
             c.span_off = a->dev_indptr;
@@ -2072,6 +2105,7 @@ int crest_pipeline_run(const crest_pipeline_args *a, double *variance, double *l
             c.genes = a->dev_genes;
             c.values = a->dev_values;
             c.base = 0;
+            c.nnz = nnz;
             fn(c);
         };
         run_pipeline(x, a, resident, variance, loadings, res);
@@ -2147,6 +2181,7 @@ int crest_pipeline_run_streamed(const crest_pipeline_args *a, const int64_t *hos
                 c.genes = d_g[b];
                 c.values = d_v[b];
                 c.base = e0;
+                c.nnz = ne;
                 fn(c);
                 HIPCHK(hipEventRecord(x.consumed[b], x.st));
             }
@@ -2321,6 +2356,7 @@ static DevChunk session_stage(Ctx &x, crest_session *S, const crest_chunk *c, in
     d.first_cell = cells ? 0 : c->first_cell;
     d.n_spans = h.n_spans;
     d.base = h.base;
+    d.nnz = (int64_t)nnz;
     d.cell_map = S->d_cmap;
     d.n_cell_map = (int64_t)c->n_cell_map;
     d.gene_var = S->d_gmap;

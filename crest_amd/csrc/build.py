@@ -16,7 +16,8 @@ PKG = HERE.parent
 ROOT = PKG.parent
 OUT = PKG / "libcrest_hip.so"
 OBJ = HERE / "_build"
-SOURCES = ["api.hip", "kernels_chunk.hip", "kernels_gram.hip", "kernels_knn.hip", "kernels_graph.hip", "kernels_eig.hip", "kernels_sytrd.hip", "kernels_hvg.hip"]
+SOURCES = ["api.hip", "kernels_chunk.hip", "kernels_gram.hip", "kernels_knn.hip", "kernels_graph.hip", "kernels_eig.hip", "kernels_sytrd.hip", "kernels_hvg.hip",
+           "kernels_sparse.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 CFLAGS = [

@@ -60,6 +60,7 @@ struct DevChunk {
     double target_sum;
     int log1p;
     int64_t base;              // absolute offset of genes[0] / values[0]
+    int64_t nnz;               // entries of genes / values (span_off[n_spans] - base for a whole chunk)
 };
 
 __device__ inline int64_t chunk_row(const DevChunk &c, int64_t i) {
@@ -249,6 +250,17 @@ void launch_hvg_seurat(const double *stats, int64_t n_vars, double n_cells, int 
                        double *compact, double *means, double *disp, double *disp_norm, uint8_t *hv,
                        double *mean_log, double *std_log, hipStream_t s);
 void launch_fill_rows(const float *shift_f, int k, int64_t n_rows, float *x, hipStream_t s);
+// kernels_sparse.hip: the FP64 Gram by sparse pair products (d_pad <= 4096): pack the
+// scaled HVG entries of a chunk by 128-column bucket, then one workgroup per
+// (Gram block, row split) into partial [blk][split][128 x 128] and colsum.
+bool sparse_gram_supported(int d_pad);
+size_t sparse_meta_words(int64_t n_rows, int d_pad);
+size_t sparse_gram_lds();
+void launch_sparse_pack(const DevChunk &c, const DevScale &sc, int d_pad, double *vals, uint8_t *cols, uint32_t *meta,
+                        hipStream_t s);
+void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                        int64_t base, int64_t n_rows, int d_pad, int n_split, double *partial, double *cs_partial,
+                        double *colsum, hipStream_t s);
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);

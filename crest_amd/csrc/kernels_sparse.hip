@@ -1,0 +1,335 @@
+// The FP64 Gram of the scaled entries by sparse pair products (kernels.rs:221-309).
+//
+// The scaled matrix S is sparse: its entries are the non-zero counts of HVGs, each
+// s = clip((x - mu)/sd) - clip(-mu/sd) (kernels.rs:211-218), and zeros stay zero.
+// The reference accumulates the upper triangle G[a][b] += s_a s_b row by row over
+// each row's (HVG, s) list; so does this path, exactly in FP64 (one rounding per
+// product, FP64 sums), instead of a dense MFMA product over a tile that is 92% zeros.
+//
+// k_sparse_pack (one workgroup per group of 64 rows): the row's HVG entries, scaled,
+// written grouped by 128-column bucket (bucket = hvg >> 7): within a group of 64 rows
+// all entries of bucket 0 (row by row, each row's entries in entry order), then
+// bucket 1, ...  The group's region starts at the group's own CSR offset (it holds at
+// most that many entries).  Per group a record of bucket starts (nb + 1 words) and
+// per-bucket row offsets (65 u16 per bucket).  Entry = FP64 value + the column
+// within its bucket (one byte).
+//
+// k_gram_sparse (one 1024-thread workgroup per (128 x 128 Gram block, row split)): the
+// block lives in LDS as FP64 (row stride 129, so the rows of one add fall in different
+// banks).  For each group of its split, each wave takes 4 rows, one per 16-lane row of
+// the wave: lane t loads a-entry t and b-entry t of its row straight into registers
+// (the next group's rows loading while this group's pairs run), and step i broadcasts
+// a-entry i across the row's lanes (DPP row_newbcast), every lane adding s_a s_b into
+// the block with an LDS FP64 atomic add (diagonal blocks: pairs with col_a <= col_b,
+// each entry's s into the colsum).  The adds are the only LDS traffic and no result
+// feeds another, so they issue back to back.  The block and colsum go
+// to a per-split partial slot, reduced over splits in order afterwards.  The order of
+// the adds within one workgroup is the hardware's, so results agree with any other
+// order to FP64 rounding (not bitwise between runs).
+#include "common.hpp"
+
+namespace crest {
+
+constexpr int kSpGroup = 64;      // rows per group
+constexpr int kSpMaxNb = 32;      // buckets (d_pad <= 4096)
+constexpr int kSpThreads = 1024;  // block kernel (16 waves: 4 rows each of a 64-row group)
+constexpr int kSpWaves = kSpThreads / kWave;
+constexpr int kSpLd = 129;        // LDS row stride of the Gram block (doubles)
+
+__device__ inline double sp_cell_total(const DevChunk &c, int64_t lo, int64_t hi, int lane) {
+    double t = 0.0;
+    for (int64_t e = lo + lane; e < hi; e += kWave)
+        if (chunk_var(c, c.genes[e]) >= 0) t += (double)c.values[e];
+    return wave_sum_f64(t);
+}
+
+// words of one group record: bucket starts [nb + 1] (u32), then row offsets
+// [nb][65] (u16, two per word), padded to a multiple of 16 words
+__host__ __device__ inline int sp_rec_words(int nb) { return ((nb + 1) + (nb * 65 + 1) / 2 + 15) / 16 * 16; }
+
+__global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc, int nb, int64_t n_groups,
+                                                     double *vals, uint8_t *cols, uint32_t *meta) {
+    __shared__ int cnt[kSpGroup][kSpMaxNb];
+    __shared__ int bstart[kSpMaxNb + 1];
+    __shared__ int run[4][kSpMaxNb];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int rw = sp_rec_words(nb);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+        const int64_t r0 = g * kSpGroup;
+        const int nr = (int)((c.n_spans - r0) < kSpGroup ? (c.n_spans - r0) : kSpGroup);
+        const int64_t gbase = c.span_off[r0] - c.base;
+        for (int i = threadIdx.x; i < kSpGroup * kSpMaxNb; i += blockDim.x) cnt[i / kSpMaxNb][i % kSpMaxNb] = 0;
+        __syncthreads();
+        // entries per (row, bucket)
+        for (int rl = wave; rl < nr; rl += 4) {
+            if (chunk_row(c, r0 + rl) < 0) continue;
+            const int64_t lo = c.span_off[r0 + rl] - c.base, hi = c.span_off[r0 + rl + 1] - c.base;
+            for (int64_t e = lo + lane; e < hi; e += kWave) {
+                const int32_t m = chunk_var(c, c.genes[e]);
+                if (m < 0) continue;
+                const int32_t h = sc.sub[m];
+                if (h >= 0 && h < sc.d) atomicAdd(&cnt[rl][h >> 7], 1);
+            }
+        }
+        __syncthreads();
+        // bucket starts; per bucket the exclusive prefix over rows (in place)
+        if (threadIdx.x == 0) {
+            int o = 0;
+            for (int b = 0; b < nb; ++b) {
+                bstart[b] = o;
+                for (int rl = 0; rl < kSpGroup; ++rl) o += cnt[rl][b];
+            }
+            bstart[nb] = o;
+        }
+        __syncthreads();
+        uint32_t *rec = meta + g * rw;
+        uint16_t *roff = reinterpret_cast<uint16_t *>(rec + nb + 1);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            int o = 0;
+            for (int rl = 0; rl < kSpGroup; ++rl) {
+                const int t = cnt[rl][b];
+                cnt[rl][b] = o;
+                roff[b * 65 + rl] = (uint16_t)o;
+                o += t;
+            }
+            roff[b * 65 + kSpGroup] = (uint16_t)o;
+        }
+        for (int b = threadIdx.x; b <= nb; b += blockDim.x) rec[b] = (uint32_t)bstart[b];
+        __syncthreads();
+        // scaled entries at their positions (entry order within each (row, bucket))
+        for (int rl = wave; rl < nr; rl += 4) {
+            if (chunk_row(c, r0 + rl) < 0) continue;
+            const int64_t lo = c.span_off[r0 + rl] - c.base, hi = c.span_off[r0 + rl + 1] - c.base;
+            const double ns = norm_factor(sp_cell_total(c, lo, hi, lane), c.target_sum);
+            for (int b = lane; b < nb; b += kWave) run[wave][b] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int64_t e0 = lo; e0 < hi; e0 += kWave) {
+                const int64_t e = e0 + lane;
+                int h = -1;
+                double s = 0.0;
+                if (e < hi) {
+                    const int32_t m = chunk_var(c, c.genes[e]);
+                    if (m >= 0) {
+                        h = sc.sub[m];
+                        if (h >= sc.d) h = -1;
+                        if (h >= 0) {
+                            const float v = lazy_transform(c.values[e], ns, c.target_sum, c.log1p);
+                            s = sc.scale ? scale_value(v, sc.mean[h], sc.sd[h], sc.max_value, 1) : (double)v;
+                        }
+                    }
+                }
+                const int b = h >= 0 ? (h >> 7) : -1;
+                int pos = -1;
+                for (int bb = 0; bb < nb; ++bb) {
+                    const uint64_t msk = __ballot(b == bb);
+                    if (msk == 0) continue;
+                    const int base_bb = run[wave][bb];
+                    if (b == bb) pos = base_bb + __popcll(msk & lt);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) run[wave][bb] = base_bb + __popcll(msk);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (b >= 0) {
+                    const int64_t at = gbase + bstart[b] + cnt[rl][b] + pos;
+                    vals[at] = s;
+                    cols[at] = (uint8_t)(h & 127);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// lane n of each row of 16 lanes, to every lane of that row (DPP row_newbcast)
+template <int N> __device__ inline int row_bcast(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x150 + N, 0xf, 0xf, false); }
+template <int N> __device__ inline double row_bcast_f64(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = row_bcast<N>((int)(u & 0xffffffffll)), hi = row_bcast<N>((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Pairs of one 16 x 16 chunk: lane t of row group r holds a-entry (sa, ca) and
+// b-entry (sb, cb) t of its row; step i adds a-entry i (broadcast over the row's 16
+// lanes) times every b-entry of the row.  na: the row's a-entries in this chunk.
+template <bool DIAG, int I>
+__device__ inline void sp_step(double *G, double *cs, double sa, int ca, double sb, int cb, bool ok_b, int na) {
+    const double ai = row_bcast_f64<I>(sa);
+    const int ci = row_bcast<I>(ca);
+    if (ok_b && I < na && (!DIAG || ci <= cb)) {
+        atomicAdd(&G[ci * kSpLd + cb], ai * sb);
+        if (DIAG && ci == cb) atomicAdd(&cs[cb], sb);
+    }
+}
+template <bool DIAG>
+__device__ inline void sp_chunk(double *G, double *cs, double sa, int ca, double sb, int cb, bool ok_b, int na,
+                                int n_i) {
+    sp_step<DIAG, 0>(G, cs, sa, ca, sb, cb, ok_b, na);
+#define CREST_SP_STEP(I) \
+    if (n_i > I) sp_step<DIAG, I>(G, cs, sa, ca, sb, cb, ok_b, na);
+    CREST_SP_STEP(1) CREST_SP_STEP(2) CREST_SP_STEP(3) CREST_SP_STEP(4) CREST_SP_STEP(5) CREST_SP_STEP(6)
+    CREST_SP_STEP(7) CREST_SP_STEP(8) CREST_SP_STEP(9) CREST_SP_STEP(10) CREST_SP_STEP(11) CREST_SP_STEP(12)
+    CREST_SP_STEP(13) CREST_SP_STEP(14) CREST_SP_STEP(15)
+#undef CREST_SP_STEP
+}
+
+// wave-uniform maximum of a per-row value held by every lane of each 16-lane row
+__device__ inline int rows_max(int v) {
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
+}
+
+// One wave's view of its 4 rows of a group: row r = lane / 16 of the wave, its
+// a-range and b-range (absolute entry positions) and the first 16 entries of each.
+struct SpRows {
+    int64_t a_at, b_at;  // first a- / b-entry of this lane's row
+    int na, nbr;
+    double sa, sb;
+    int ca, cb;
+};
+
+template <bool DIAG>
+__device__ inline void sp_load_rows(SpRows &R, const double *vals, const uint8_t *cols, const uint32_t *meta,
+                                    const int64_t *span_off, int64_t base, int rw, int nb, int bi, int bj, int64_t g,
+                                    int row, int t) {
+    const uint32_t *rec = meta + g * rw;
+    const uint16_t *roff = reinterpret_cast<const uint16_t *>(rec + nb + 1);
+    const int64_t gbase = span_off[g * kSpGroup] - base;
+    const int a0 = roff[bi * 65 + row], a1 = roff[bi * 65 + row + 1];
+    R.a_at = gbase + rec[bi] + a0;
+    R.na = a1 - a0;
+    if (DIAG) {
+        R.b_at = R.a_at;
+        R.nbr = R.na;
+    } else {
+        const int b0 = roff[bj * 65 + row], b1 = roff[bj * 65 + row + 1];
+        R.b_at = gbase + rec[bj] + b0;
+        R.nbr = b1 - b0;
+    }
+    R.sa = t < R.na ? vals[R.a_at + t] : 0.0;
+    R.ca = t < R.na ? (int)cols[R.a_at + t] : 0;
+    R.sb = t < R.nbr ? vals[R.b_at + t] : 0.0;
+    R.cb = t < R.nbr ? (int)cols[R.b_at + t] : 0;
+}
+
+template <bool DIAG>
+__device__ inline void sp_pairs(double *G, double *cs, const SpRows &R, const double *vals, const uint8_t *cols,
+                                int t) {
+    const int mx_a = rows_max(R.na), mx_b = rows_max(R.nbr);
+    for (int ac = 0; ac < mx_a; ac += 16) {
+        double sa = R.sa;
+        int ca = R.ca;
+        if (ac > 0) {  // further a-entries of rows with more than 16 in this bucket
+            const bool h = ac + t < R.na;
+            sa = h ? vals[R.a_at + ac + t] : 0.0;
+            ca = h ? (int)cols[R.a_at + ac + t] : 0;
+        }
+        const int na_c = R.na - ac, n_i = min(mx_a - ac, 16);
+        for (int bc = 0; bc < mx_b; bc += 16) {
+            double sb = R.sb;
+            int cb = R.cb;
+            if (bc > 0) {
+                const bool h = bc + t < R.nbr;
+                sb = h ? vals[R.b_at + bc + t] : 0.0;
+                cb = h ? (int)cols[R.b_at + bc + t] : 0;
+            }
+            sp_chunk<DIAG>(G, cs, sa, ca, sb, cb, bc + t < R.nbr, na_c, n_i);
+        }
+    }
+}
+
+template <bool DIAG>
+__device__ void sp_block(double *G, double *cs, const double *vals, const uint8_t *cols, const uint32_t *meta,
+                         const int64_t *span_off, int64_t base, int rw, int nb, int bi, int bj, int64_t g0, int64_t g1,
+                         int wave, int lane) {
+    const int row = 4 * wave + (lane >> 4), t = lane & 15;
+    if (g0 >= g1) return;
+    SpRows cur, nxt;
+    sp_load_rows<DIAG>(cur, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, row, t);
+    for (int64_t g = g0; g < g1; ++g) {
+        if (g + 1 < g1) sp_load_rows<DIAG>(nxt, vals, cols, meta, span_off, base, rw, nb, bi, bj, g + 1, row, t);
+        sp_pairs<DIAG>(G, cs, cur, vals, cols, t);
+        cur = nxt;
+    }
+}
+
+__global__ __launch_bounds__(kSpThreads) void k_gram_sparse(const double *__restrict__ vals,
+                                                          const uint8_t *__restrict__ cols,
+                                                          const uint32_t *__restrict__ meta,
+                                                          const int64_t *__restrict__ span_off, int64_t base,
+                                                          int64_t n_groups, int nb, int n_split,
+                                                          double *partial, double *cs_partial) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double *G = reinterpret_cast<double *>(lds_raw);  // [128][kSpLd]
+    double *cs = G + 128 * kSpLd;                       // [128]
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int split = blockIdx.x % n_split, blk = blockIdx.x / n_split;
+    int bi = 0, bj = 0;
+    {
+        int r = 0, t = blk;
+        while (t >= nb - r) {
+            t -= nb - r;
+            ++r;
+        }
+        bi = r;
+        bj = r + t;
+    }
+    const int rw = sp_rec_words(nb);
+    for (int i = tid; i < 128 * kSpLd; i += kSpThreads) G[i] = 0.0;
+    for (int i = tid; i < 128; i += kSpThreads) cs[i] = 0.0;
+    __syncthreads();
+    const int64_t g0 = split * n_groups / n_split, g1 = (split + 1) * n_groups / n_split;
+    if (bi == bj)
+        sp_block<true>(G, cs, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, g1, wave, lane);
+    else
+        sp_block<false>(G, cs, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, g1, wave, lane);
+    __syncthreads();
+    double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
+    for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads)
+        dst[i] = G[(i / kGramBlock) * kSpLd + (i % kGramBlock)];
+    if (bi == bj)
+        for (int i = tid; i < 128; i += kSpThreads) cs_partial[((int64_t)bi * n_split + split) * 128 + i] = cs[i];
+}
+
+// colsum[h] += sum over splits (in order) of the diagonal blocks' entry sums
+__global__ void k_sparse_colsum(const double *cs_partial, int nb, int n_split, double *colsum) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nb * 128) return;
+    const int b = h / 128, i = h % 128;
+    double s = 0.0;
+    for (int sp = 0; sp < n_split; ++sp) s += cs_partial[((int64_t)b * n_split + sp) * 128 + i];
+    colsum[h] += s;
+}
+
+bool sparse_gram_supported(int d_pad) { return d_pad > 0 && d_pad <= kSpMaxNb * kGramBlock; }
+size_t sparse_meta_words(int64_t n_rows, int d_pad) {
+    return (size_t)ceil_div(n_rows, kSpGroup) * sp_rec_words(d_pad / kGramBlock);
+}
+size_t sparse_gram_lds() { return (size_t)128 * kSpLd * 8 + 128 * 8; }
+
+void launch_sparse_pack(const DevChunk &c, const DevScale &sc, int d_pad, double *vals, uint8_t *cols, uint32_t *meta,
+                        hipStream_t s) {
+    const int64_t n_groups = ceil_div(c.n_spans, kSpGroup);
+    if (n_groups <= 0) return;
+    const int g = (int)std::min<int64_t>(n_groups, 8192);
+    hipLaunchKernelGGL(k_sparse_pack, dim3(g), dim3(256), 0, s, c, sc, d_pad / kGramBlock, n_groups, vals, cols, meta);
+}
+
+void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                        int64_t base, int64_t n_rows, int d_pad, int n_split, double *partial, double *cs_partial,
+                        double *colsum, hipStream_t s) {
+    const int64_t n_groups = ceil_div(n_rows, kSpGroup);
+    const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
+    if (n_groups <= 0) return;
+    const size_t lds = sparse_gram_lds();
+    hipFuncSetAttribute((const void *)k_gram_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_gram_sparse, dim3(nblk * n_split), dim3(kSpThreads), lds, s, vals, cols, meta, span_off, base,
+                       n_groups, nb, n_split, partial, cs_partial);
+    hipLaunchKernelGGL(k_sparse_colsum, dim3((unsigned)ceil_div(d_pad, 256)), dim3(256), 0, s, cs_partial, nb, n_split,
+                       colsum);
+}
+
+}  // namespace crest

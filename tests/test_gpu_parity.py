@@ -573,3 +573,49 @@ def test_log1pf_device_matches_libm(oracle):
     ref = oracle.materialize(*ctx)[2]
     assert len(got) == len(ref) == n
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("dense_rows", [False, True])
+def test_sparse_fp64_gram(oracle, dense_rows, monkeypatch):
+    """The FP64 Gram by sparse pair products (the default FP64 form) against the oracle
+    (kernels.rs:221-309) and against the dense FP64 MFMA form (CREST_GRAM_SPARSE=0), to
+    FP64 rounding; with rows whose every gene is non-zero, a group's bucket exceeds the
+    staged capacity and the unstaged branch runs."""
+    ip, ix, dv, _ = synth.make_counts(5000, 700, 6, density=0.08, seed=31)
+    if dense_rows:  # cells 100..169 express every gene
+        rows = [np.arange(ip[r], ip[r + 1]) for r in range(len(ip) - 1)]
+        new_ix, new_dv, new_ip = [], [], [0]
+        rng = np.random.default_rng(5)
+        for r, sl in enumerate(rows):
+            if 100 <= r < 170:
+                g = np.arange(700, dtype=np.uint32)
+                v = rng.integers(1, 20, 700).astype(np.float32)
+            else:
+                g, v = ix[sl], dv[sl]
+            new_ix.append(g)
+            new_dv.append(v)
+            new_ip.append(new_ip[-1] + len(g))
+        ip, ix, dv = np.asarray(new_ip, np.int64), np.concatenate(new_ix), np.concatenate(new_dv)
+    bf = ca.BioFrame.from_csr(ip, ix, dv, 700)
+    bf.chunk_nnz = len(dv) // 3
+    rng = np.random.default_rng(0)
+    bf = bf.filter_cells(rng.random(bf.n_obs) > 0.05).filter_genes(rng.random(bf.n_vars) > 0.1)
+    ca.pp.normalize_total(bf, 1e4)
+    ca.pp.log1p(bf)
+    gmap, mean, sd, d = _gram_inputs(bf, oracle, d_top=600)
+    got = [np.zeros((d, d)), np.zeros(d)]
+    ref = [np.zeros((d, d)), np.zeros(d)]
+    dense = [np.zeros((d, d)), np.zeros(d)]
+    ca.set_precision("fp64")
+    for ctx in bf.iter_ctx():
+        N.gram_accumulate(*ctx, gmap, got[0], got[1], True, mean, sd, 10.0)
+        oracle.gram_accumulate(*ctx, gmap, ref[0], ref[1], True, mean, sd, 10.0)
+    monkeypatch.setenv("CREST_GRAM_SPARSE", "0")
+    for ctx in bf.iter_ctx():
+        N.gram_accumulate(*ctx, gmap, dense[0], dense[1], True, mean, sd, 10.0)
+    iu = np.triu_indices(d)
+    scale = np.abs(ref[0][iu]).max()
+    assert np.abs(got[0][iu] - ref[0][iu]).max() <= 1e-12 * scale
+    assert np.abs(got[0][iu] - dense[0][iu]).max() <= 1e-12 * scale
+    assert np.all(np.tril(got[0], -1) == 0)
+    np.testing.assert_allclose(got[1], ref[1], rtol=1e-12, atol=1e-9)
