@@ -432,6 +432,238 @@ static bool use_topp(int d, int k, int n_mat) {
     return !(e && std::string(e) == "dsyevd");
 }
 
+// Block size of the filtered subspace iteration for the top p of order d: p + 16
+// rounded up to a multiple of 8, at most chefsi_max_block() and d / 2; 0 when it does
+// not apply.  It runs by default from order 512 (below that the tridiagonal path takes
+// a few milliseconds); CREST_EIGENSOLVER=chefsi forces it wherever the block fits,
+// =topp or =dsyevd switches it off.
+static int chefsi_block(int d, int p) {
+    const int k = (int)round_up(p + 16, 8);
+    if (p < 1 || k > chefsi_max_block() || 2 * k > d) return 0;
+    const char *e = std::getenv("CREST_EIGENSOLVER");
+    const std::string mode = e ? e : "";
+    if (mode == "chefsi") return k;
+    return (mode.empty() && d >= 512) ? k : 0;
+}
+
+// Degree of the next filter: the largest m <= 40 whose T_m at the top of the active
+// block (x_top in the filter's variable) stays below kappa, so the filtered block
+// is no worse conditioned than kappa from the spread of its directions' gains.
+static int chefsi_degree(double x_top, double kappa) {
+    if (!(x_top > 1.0 + 1e-12)) return 40;
+    const int m = (int)std::floor(std::acosh(kappa) / std::acosh(x_top));
+    return std::max(1, std::min(40, m));
+}
+
+// The p largest eigenpairs of the symmetric d x d matrix C (column-major, ld d) by
+// Chebyshev-filtered subspace iteration, into dev_var / dev_load with the sign rule
+// of launch_eig_select.  The work happens in a copy A of C padded with zeros to
+// d_pad = d rounded up to 256 (the padding rows of every block stay zero).  Per
+// iteration: filter the active block with T_m((A - c) / e), where [0, b] (b = the
+// block's smallest Ritz value) is the damped interval; remove the locked directions;
+// CholeskyQR2 with the columns in descending Ritz order (so each column is
+// orthogonalised against those above it, whose contamination the filter amplified);
+// Rayleigh-Ritz against A, the small eigenproblem by one-workgroup Jacobi limited to
+// two sweeps (the basis it leaves is still orthonormal and spans the same subspace;
+// the rotation it misses shows up in the residuals and is finished in a later
+// iteration); residual norms.  Ritz pairs converged to 1e-12 of the largest Ritz value
+// so far are locked from the top down and deflated out of A (A -= q theta q^T), which
+// keeps the filter degree up as the top of the spectrum leaves.  A final
+// Rayleigh-Ritz against C over the locked vectors, with Jacobi to convergence, gives
+// the result.  Products with A run on k_ch_gemm split over K (kSplit partials summed
+// by the kernel that consumes them), the Gram-like products as rocBLAS dgemm split the
+// same way (their partials summed by a grid-wide kernel before the one-workgroup
+// kernels read them).  Returns false -- with dev_var / dev_load unwritten -- when the block loses
+// rank, the final Jacobi solve does not converge, the spectrum is too flat to filter
+// (b below 1e-8 of the top) or 60 iterations do not converge; the caller then takes
+// the tridiagonal path.  C is left as it was.
+static bool chefsi_topp(Ctx &x, const double *C, int d, int p, double *dev_var, double *dev_load) {
+    const int k = chefsi_block(d, p);
+    if (!k) return false;
+    const int dp = (int)round_up(d, 256);
+    const int kSplit = 8;                       // K chunks of the d_pad x d_pad products (multiples of 32)
+    const int kSkinny = std::min(32, dp / 64);  // K chunks of the k x k Gram products
+    const int kSweeps = 2;                      // Jacobi sweeps of an intermediate Rayleigh-Ritz
+    const int64_t dk = (int64_t)dp * k;
+    double *A = x.get<double>("ch_a", (size_t)dp * dp);
+    double *buf[4];
+    for (int i = 0; i < 4; ++i) buf[i] = x.get<double>("ch_b" + std::to_string(i), (size_t)dk);
+    double *W = x.get<double>("ch_w", (size_t)dk);
+    double *WZ = x.get<double>("ch_wz", (size_t)dk);
+    double *P = x.get<double>("ch_part", (size_t)kSplit * dk);         // partials of A Y
+    double *Ps = x.get<double>("ch_part_s", (size_t)kSkinny * k * k);  // partials of Y^T Y, Y^T W, QL^T Y
+    double *G = x.get<double>("ch_g", (size_t)k * k);  // summed Gram-like product
+    double *Ri = x.get<double>("ch_ri", (size_t)k * k);
+    double *Z = x.get<double>("ch_z", (size_t)k * k);
+    double *QL = x.get<double>("ch_ql", (size_t)dp * p);
+    double *thL = x.get<double>("ch_thl", p);
+    double *T = x.get<double>("ch_t", (size_t)p * k);
+    double *Vout = x.get<double>("ch_vout", (size_t)d * p);
+    // stat: Ritz values [k] | residual norms [k] | block lost rank | Jacobi failed
+    double *stat = x.get<double>("ch_stat", (size_t)2 * k + 2);
+    double *hs = static_cast<double *>(x.host("ch_stat", (2 * (size_t)k + 2) * sizeof(double)));
+    const bool trace = std::getenv("CREST_CHEFSI_TRACE") != nullptr;
+    chefsi_set_attrs();
+    const auto N = rocblas_operation_none, Tr = rocblas_operation_transpose;
+    auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int m, int n, int kk, double alpha, const double *a,
+                    int lda, const double *b, int ldb, double beta, double *c, int ldc) {
+        BLASCHK(rocblas_dgemm(x.blas, ta, tb, m, n, kk, &alpha, a, lda, b, ldb, &beta, c, ldc));
+    };
+    // partials of a^T b for a (dp x m), b (dp x n): Ps + s m n, s < kSkinny
+    auto inner = [&](const double *a, int m, const double *b, int n) {
+        const double one = 1.0, zero = 0.0;
+        const int kc = dp / kSkinny;
+        BLASCHK(rocblas_dgemm_strided_batched(x.blas, Tr, N, m, n, kc, &one, a, dp, kc, b, dp, kc, &zero, Ps, m,
+                                              (int64_t)m * n, kSkinny));
+    };
+    auto other = [&](std::initializer_list<int> used) {
+        for (int i = 0; i < 4; ++i)
+            if (std::find(used.begin(), used.end(), i) == used.end()) return i;
+        return -1;
+    };
+    // CholeskyQR2 of buf[y] (dp x ka), leaving buf[keep] alone; returns the buffer that
+    // holds the result
+    auto cholqr2 = [&](int y, int ka, int keep) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int o = other({y, keep});
+            inner(buf[y], ka, buf[y], ka);
+            launch_ch_sum(G, Ps, kSkinny, (int64_t)ka * ka, (int64_t)ka * ka, x.st);
+            launch_ch_chol_rinv(G, 1, ka, Ri, stat + 2 * k, x.st);
+            gemm(N, N, dp, ka, ka, 1.0, buf[y], dp, Ri, ka, 0.0, buf[o], dp);
+            y = o;
+        }
+        return y;
+    };
+    // Rayleigh-Ritz of the orthonormal buf[y] (dp x ka) against op: Ritz vectors into
+    // the returned buffer (not buf[keep]; descending unless final), op times them into
+    // WZ, values and residual norms into stat, then downloaded into hs
+    auto rayleigh_ritz = [&](const double *op, int y, int ka, int keep, bool final) {
+        launch_ch_gemm(op, buf[y], dp, ka, kSplit, P, x.st);
+        launch_ch_sum(W, P, kSplit, (int64_t)dp * ka, (int64_t)dp * ka, x.st);
+        inner(buf[y], ka, W, ka);
+        launch_ch_sum(G, Ps, kSkinny, (int64_t)ka * ka, (int64_t)ka * ka, x.st);
+        launch_ch_jacobi(G, 1, ka, final ? 40 : kSweeps, final, !final, stat, Z, stat + 2 * k + 1, x.st);
+        const int v = other({y, keep});
+        gemm(N, N, dp, ka, ka, 1.0, buf[y], dp, Z, ka, 0.0, buf[v], dp);
+        gemm(N, N, dp, ka, ka, 1.0, W, dp, Z, ka, 0.0, WZ, dp);
+        launch_ch_resid(WZ, buf[v], stat, d, dp, ka, stat + k, x.st);
+        x.check_launch();
+        x.download(hs, stat, 2 * (size_t)k + 2);
+        x.sync();
+        return v;
+    };
+    auto give_up = [&](const char *why) {
+        if (trace) std::fprintf(stderr, "[chefsi] fallback: %s\n", why);
+        return false;
+    };
+    auto finite = [&](int ka) {
+        for (int j = 0; j < ka; ++j)
+            if (!std::isfinite(hs[j]) || !std::isfinite(hs[k + j])) return false;
+        return true;
+    };
+    auto load_padded = [&](const double *src) {
+        launch_fill_f64(A, (int64_t)dp * dp, 0.0, x.st);
+        HIPCHK(hipMemcpy2DAsync(A, (size_t)dp * sizeof(double), src, (size_t)d * sizeof(double),
+                                (size_t)d * sizeof(double), d, hipMemcpyDeviceToDevice, x.st));
+    };
+
+    load_padded(C);
+    launch_fill_f64(stat, 2 * (int64_t)k + 2, 0.0, x.st);
+    launch_ch_rand(buf[0], d, dp, k, 0x5eed0f5ca1edull + (uint64_t)d * 1000003u + (uint64_t)p, x.st);
+    int ka = k, L = 0, iters = 0, products = 1;
+    int v = rayleigh_ritz(A, cholqr2(0, ka, -1), ka, -1, false);
+    if (hs[2 * k] != 0.0 || !finite(ka) || !(hs[0] > 0.0)) return give_up("start block");
+    // the active block: ka Ritz vectors (descending) at vcol, A times them at wzcol, their
+    // values and residual norms at hs[j0 ..] and hs[k + j0 ..]
+    const double *vcol = buf[v], *wzcol = WZ;
+    int j0 = 0;
+    double top = 0.0, tol = 0.0;  // the largest Ritz value so far; the locking threshold
+    std::vector<double> prev(2 * (size_t)k);
+    double kappa = 1e3;  // the first filter acts on a random block: keep its gains close
+    bool done = false;
+    for (int it = 0, retries = 0; it < 60; ++it) {
+        top = std::max(top, hs[0]);
+        tol = 1e-12 * top;
+        int J = 0;
+        while (J < p - L && hs[k + J] <= tol) ++J;
+        if (J) {  // lock the J largest (descending into QL) and deflate them out of A
+            launch_ch_lock(vcol, stat, dp, J, QL + (int64_t)L * dp, thL + L, W, x.st);
+            gemm(N, Tr, dp, dp, J, -1.0, W, dp, QL + (int64_t)L * dp, dp, 1.0, A, dp);
+            L += J;
+            ka -= J;
+        }
+        j0 = J;
+        if (L >= p) {
+            done = true;
+            break;
+        }
+        vcol += (int64_t)J * dp;
+        wzcol += (int64_t)J * dp;
+        std::copy(hs, hs + 2 * k, prev.begin());
+        const double a = 0.0, b = prev[j0 + ka - 1], theta_top = prev[j0];
+        if (!(b > 1e-8 * top)) return give_up("flat spectrum");
+        const double c = 0.5 * (a + b), e = 0.5 * (b - a);
+        const int64_t n_el = (int64_t)dp * ka;
+        for (;;) {
+            const int m = chefsi_degree((theta_top - c) / e, kappa);
+            if (trace)
+                std::fprintf(stderr, "[chefsi] it %d locked %d active %d degree %d top-resid %.3e b/top %.6e\n", it, L,
+                             ka, m, prev[k + j0] / top, b / top);
+            // y1 = (A y0 - c y0) / e with A y0 = wzcol; y_{j+1} = 2 (A y_j - c y_j) / e - y_{j-1};
+            // the Ritz vectors (buf[v]) stay for a retry
+            int y1 = other({v});
+            const double *y0p = vcol;
+            launch_ch_combine(buf[y1], wzcol, 1, 0, vcol, nullptr, n_el, 1.0 / e, c, 0.0, x.st);
+            int y0 = -1;  // buffer of y0 once it is no longer the Ritz block
+            for (int j = 1; j < m; ++j) {
+                launch_ch_gemm(A, buf[y1], dp, ka, kSplit, P, x.st);
+                const int y2 = other({y0, y1, v});
+                launch_ch_combine(buf[y2], P, kSplit, n_el, buf[y1], y0p, n_el, 2.0 / e, c, 1.0, x.st);
+                y0 = y1;
+                y0p = buf[y1];
+                y1 = y2;
+            }
+            if (L) {  // y -= QL (QL^T y)
+                inner(QL, L, buf[y1], ka);
+                launch_ch_sum(T, Ps, kSkinny, (int64_t)L * ka, (int64_t)L * ka, x.st);
+                gemm(N, N, dp, ka, L, -1.0, QL, dp, T, L, 1.0, buf[y1], dp);
+            }
+            x.check_launch();
+            const int nv = rayleigh_ritz(A, cholqr2(y1, ka, v), ka, v, false);
+            ++iters;
+            products += m;
+            if (hs[2 * k] == 0.0 && finite(ka)) {
+                v = nv;
+                vcol = buf[v];
+                wzcol = WZ;
+                kappa = std::min(kappa * 100.0, 1e8);
+                break;
+            }
+            // the filtered block lost rank: filter the same Ritz vectors with a lower degree
+            if (++retries > 8 || kappa < 10.0) return give_up("rank loss");
+            kappa = std::sqrt(kappa);
+            launch_fill_f64(stat + 2 * k, 1, 0.0, x.st);
+            launch_ch_gemm(A, vcol, dp, ka, kSplit, P, x.st);
+            launch_ch_sum(WZ, P, kSplit, n_el, n_el, x.st);
+            wzcol = WZ;
+        }
+    }
+    if (!done) return give_up("no convergence");
+    // final Rayleigh-Ritz of the locked vectors against C
+    load_padded(C);
+    HIPCHK(hipMemcpyAsync(buf[0], QL, (size_t)dp * p * sizeof(double), hipMemcpyDeviceToDevice, x.st));
+    v = rayleigh_ritz(A, cholqr2(0, p, -1), p, -1, true);
+    if (hs[2 * k] != 0.0 || hs[2 * k + 1] != 0.0 || !finite(p)) return give_up("final Rayleigh-Ritz");
+    for (int j = 0; j < p; ++j)
+        if (!(hs[k + j] <= 100.0 * tol)) return give_up("final residual");
+    if (trace) std::fprintf(stderr, "[chefsi] converged: %d iterations, %d products with A\n", iters, products);
+    HIPCHK(hipMemcpy2DAsync(Vout, (size_t)d * sizeof(double), buf[v], (size_t)dp * sizeof(double),
+                            (size_t)d * sizeof(double), p, hipMemcpyDeviceToDevice, x.st));
+    launch_eig_select(Vout, stat, d, p, dev_var, dev_load, x.st, p);
+    x.check_launch();
+    return true;
+}
+
 static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *dev_colsum, int d, double n, int k,
                        double *dev_var, double *dev_load, double *dev_total) {
     double *cov = x.get<double>("cov", (size_t)d * d);
@@ -440,6 +672,7 @@ static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *d
     rocblas_int *info = x.get<rocblas_int>("eig_info", 1);
     launch_cov(dev_gram, dev_colsum, d, ld_g, n, cov, dev_total, x.st);
     x.check_launch();
+    if (chefsi_topp(x, cov, d, k, dev_var, dev_load)) return;
     if (use_topp(d, k, 1)) {
         double *tau = x.get<double>("eig_tau", d);
         double *W = x.get<double>("eig_topw", k);

@@ -17,7 +17,7 @@ ROOT = PKG.parent
 OUT = PKG / "libcrest_hip.so"
 OBJ = HERE / "_build"
 SOURCES = ["api.hip", "kernels_chunk.hip", "kernels_gram.hip", "kernels_knn.hip", "kernels_graph.hip", "kernels_eig.hip", "kernels_sytrd.hip", "kernels_hvg.hip",
-           "kernels_sparse.hip"]
+           "kernels_sparse.hip", "kernels_chefsi.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 CFLAGS = [

@@ -185,6 +185,22 @@ void tri_set_attrs();
 void launch_tri_topp(const double *A, int64_t sA, int lda, const double *tau, int64_t sTau, const double *D,
                      int64_t sD, const double *E, int64_t sE, int n, int p, int n_mat, double *W, double *Y,
                      double *X, double *fac, uint8_t *swp, int *info, hipStream_t s);
+// kernels_chefsi.hip: the pieces of the Chebyshev-filtered subspace iteration around
+// its dgemm calls (blocks of at most chefsi_max_block() vectors).
+int chefsi_max_block();
+void chefsi_set_attrs();
+void launch_ch_rand(double *v, int rows, int ld, int cols, uint64_t seed, hipStream_t s);
+void launch_ch_sum(double *out, const double *part, int S, int64_t stride, int64_t n, hipStream_t s);
+void launch_ch_combine(double *y_next, const double *w, int S, int64_t stride, const double *y, const double *y_prev,
+                       int64_t n, double a1, double c, double a2, hipStream_t s);
+void launch_ch_chol_rinv(const double *Gp, int S, int k, double *Rinv, double *bad, hipStream_t s);
+void launch_ch_jacobi(const double *Hp, int S, int n, int max_sweeps, bool strict, bool desc, double *theta,
+                      double *Z, double *bad, hipStream_t s);
+void launch_ch_gemm(const double *A, const double *Y, int dp, int n, int S, double *out, hipStream_t s);
+void launch_ch_resid(const double *cv, const double *v, const double *theta, int d, int ld, int k, double *out,
+                     hipStream_t s);
+void launch_ch_lock(const double *src, const double *src_val, int ld, int n, double *dst, double *dst_val,
+                    double *dst_scaled, hipStream_t s);
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
                        float *x, float *shift_scratch, hipStream_t s);
 // kNN screen operands: rows are padded to knn_rows_padded(n) and stored in a

@@ -547,6 +547,66 @@ def test_pca_from_gram_top_p_eigensolver(case, monkeypatch):
         np.testing.assert_allclose(load, load2, atol=1e-9)
 
 
+@pytest.mark.parametrize("case,d,k", [
+    ("flat_tail", 1200, 50), ("dominant", 1200, 50), ("cluster_at_cut", 1000, 40), ("dense_distinct", 600, 64),
+    ("rank_deficient", 800, 50), ("small_forced", 300, 20)])
+def test_pca_from_gram_filtered_subspace(case, d, k, monkeypatch, capfd):
+    """The Chebyshev-filtered subspace iteration (the default eigensolver from order
+    512) on spectra shaped like a scaled expression matrix's (one dominant value and a
+    flat tail with gaps of 1e-3 around the cut), with a dominant block, with an exactly
+    repeated cluster across the cut, fully distinct with the largest block, and rank
+    deficient below p (the iteration gives up and the tridiagonal path answers): the
+    eigenvalues of numpy's eigh, orthonormal loadings, residuals, the determined
+    subspaces and the sign rule, and the tridiagonal path's results."""
+    rng = np.random.default_rng(11)
+    if case == "flat_tail":
+        lam = np.concatenate([[17.4, 4.7, 4.5], np.sort(rng.uniform(1.3, 3.0, 40))[::-1],
+                              np.linspace(1.28, 1.20, 60), np.sort(rng.uniform(0.0, 1.2, d - 103))[::-1]])
+    elif case == "dominant":
+        lam = np.concatenate([np.geomspace(1e4, 50, 10), np.sort(rng.uniform(1, 40, 60))[::-1],
+                              np.sort(rng.uniform(0, 1, d - 70))[::-1]])
+    elif case == "cluster_at_cut":
+        lam = np.concatenate([np.linspace(20, 5, 35), np.full(12, 3.0), np.sort(rng.uniform(0, 2.5, d - 47))[::-1]])
+    elif case == "dense_distinct":
+        lam = np.sort(rng.uniform(0.01, 20, d))[::-1]
+    elif case == "rank_deficient":
+        lam = np.concatenate([np.sort(rng.uniform(1, 5, 30))[::-1], np.zeros(d - 30)])
+    else:
+        lam = np.sort(rng.uniform(0.01, 20, d))[::-1]
+    g, c, C = _gram_with_spectrum(lam, 5000, 5)
+    monkeypatch.setenv("CREST_CHEFSI_TRACE", "1")
+    if case == "small_forced":
+        monkeypatch.setenv("CREST_EIGENSOLVER", "chefsi")
+    else:
+        monkeypatch.delenv("CREST_EIGENSOLVER", raising=False)
+    print(f"[chefsi] {case}: d {d}, p {k}", flush=True)
+    var, load, tot = N.pca_from_gram(g, c, 5000, k)
+    err = capfd.readouterr().err
+    if case == "rank_deficient":
+        assert "[chefsi] fallback" in err
+    else:
+        assert "[chefsi] converged" in err, err[-2000:]
+    ev, V = np.linalg.eigh(C)
+    ev, V = ev[::-1], V[:, ::-1]
+    np.testing.assert_allclose(var, np.maximum(ev[:k], 0), atol=1e-12 * ev[0], rtol=1e-10)
+    assert abs(tot - np.trace(C)) <= 1e-12 * abs(np.trace(C))
+    assert np.abs(load.T @ load - np.eye(k)).max() < 1e-12
+    assert np.abs(C @ load - load * var).max() < 1e-10 * ev[0]
+    # every leading subspace that a relative gap of 1e-4 separates
+    for r in range(1, k + 1):
+        gap = ev[r - 1] - ev[r]
+        if gap > 1e-4 * ev[0]:
+            assert _subspace_angle(V[:, :r], load[:, :r]) < 1e-9 * ev[0] / gap, r
+    idx = np.argmax(np.abs(load), axis=0)
+    assert (load[idx, np.arange(k)] > 0).all()
+    monkeypatch.setenv("CREST_EIGENSOLVER", "topp")
+    var2, load2, _ = N.pca_from_gram(g, c, 5000, k)
+    np.testing.assert_allclose(var, var2, atol=1e-12 * ev[0], rtol=1e-10)
+    iso = np.minimum(np.abs(np.diff(np.r_[np.inf, ev[:k]])), np.abs(np.diff(ev[:k + 1]))) > 1e-2 * ev[0]
+    for j in np.nonzero(iso)[0]:  # isolated components: the same vector, the same sign
+        assert np.abs(load[:, j] - load2[:, j]).max() < 1e-8
+
+
 def test_log1pf_device_matches_libm(oracle):
     """The device transform's log1pf (log1pf_glibc.h) against the oracle's, which calls
     the C library's log1pf (the reference's f32::ln_1p, kernels.rs:81): every 97th
