@@ -1157,7 +1157,58 @@ struct BatchPlan {
     std::vector<const GramItem *> d_tile_items;
     std::vector<const RedItem *> d_tile_red;
     int64_t max_slots = 0, max_tile_rows = 0;
+    // the FP64 Gram by sparse pair products: 64-row groups (never straddling two
+    // subsets), items over group ranges, reduce items for subsets of several ranges
+    bool sparse = false;
+    std::vector<SegDesc> sp_groups;
+    std::vector<GramItem> sp_items;
+    std::vector<RedItem> sp_red;
+    int64_t sp_slots = 0;
+    const SegDesc *d_sp_groups = nullptr;
+    const GramItem *d_sp_items = nullptr;
+    const RedItem *d_sp_red = nullptr;
 };
+
+// Sparse Gram work of a batch: each subset's rows in groups of 64, its groups cut
+// into ceil(groups / range_groups) ranges of equal length; one item per (range,
+// block), the items of the longest ranges first.  A subset of one range adds its
+// blocks straight into its Gram; the ranges of a longer one write partial slots
+// that a reduce item sums in range order.
+static void plan_sparse(BatchPlan &P, int64_t range_groups) {
+    struct Range {
+        int32_t subset, n_split, j;
+        int64_t g0, g1, slot0;
+    };
+    P.sp_groups.clear();
+    P.sp_items.clear();
+    P.sp_red.clear();
+    P.sp_slots = 0;
+    std::vector<Range> ranges;
+    for (int y = 0; y < P.n_sub; ++y) {
+        const int64_t ny = P.off[y + 1] - P.off[y], gy0 = (int64_t)P.sp_groups.size();
+        for (int64_t r = 0; r < ny; r += kRowGroupSparse)
+            P.sp_groups.push_back(SegDesc{P.off[y] + r, std::min<int64_t>(kRowGroupSparse, ny - r), 0, y, 0});
+        const int64_t G = (int64_t)P.sp_groups.size() - gy0;
+        if (G == 0) continue;
+        const int n_split = (int)ceil_div(G, range_groups);
+        const int64_t per = ceil_div(G, n_split);
+        int64_t slot0 = -1;
+        if (n_split > 1) {
+            slot0 = P.sp_slots;
+            P.sp_slots += (int64_t)P.nblk * n_split;
+            for (int b = 0; b < P.nblk; ++b)
+                P.sp_red.push_back(RedItem{b, y, slot0 + (int64_t)b * n_split, n_split, 0});
+        }
+        for (int j = 0; j < n_split; ++j)
+            ranges.push_back(Range{y, n_split, j, gy0 + j * per, gy0 + std::min(G, (j + 1) * per), slot0});
+    }
+    std::stable_sort(ranges.begin(), ranges.end(),
+                     [](const Range &a, const Range &b) { return a.g1 - a.g0 > b.g1 - b.g0; });
+    for (const Range &R : ranges)
+        for (int b = 0; b < P.nblk; ++b)
+            P.sp_items.push_back(
+                GramItem{b, R.subset, R.g0, R.g1, R.slot0 < 0 ? -1 : R.slot0 + (int64_t)b * R.n_split + R.j});
+}
 
 // Builds every host table and uploads them into one buffer (synchronously).
 static void batch_tables(Ctx &x, BatchPlan &P) {
@@ -1186,6 +1237,7 @@ static void batch_tables(Ctx &x, BatchPlan &P) {
     for (int y : by_size)
         for (int64_t r = P.row0[y]; r < P.row0[y + 1]; r += 512) order.push_back((int32_t)(r / 512));
     const size_t o_bs = tp.add(blk_seg), o_ord = tp.add(order);
+    const size_t o_spg = tp.add(P.sp_groups), o_spi = tp.add(P.sp_items), o_spr = tp.add(P.sp_red);
     std::vector<size_t> o_ts, o_td, o_ti, o_tr;
     for (const BatchTile &t : P.tiles) {
         o_ts.push_back(tp.add(t.segs));
@@ -1204,6 +1256,9 @@ static void batch_tables(Ctx &x, BatchPlan &P) {
     P.d_proj_ranges = (const RowRange *)(dev + o_pr);
     P.d_blk_seg = (const int32_t *)(dev + o_bs);
     P.d_order = (const int32_t *)(dev + o_ord);
+    P.d_sp_groups = (const SegDesc *)(dev + o_spg);
+    P.d_sp_items = (const GramItem *)(dev + o_spi);
+    P.d_sp_red = (const RedItem *)(dev + o_spr);
     P.d_tile_segs.clear();
     P.d_tile_dens.clear();
     P.d_tile_items.clear();
@@ -1230,6 +1285,9 @@ struct BatchBufs {
     float *shift;
     void *ta, *tb;
     double *partial;
+    double *sp_vals;  // sparse Gram: packed scaled entries, their columns, group records
+    uint8_t *sp_cols;
+    uint32_t *sp_meta;
     void *xa, *xb, *kwork;
     float *cd;
     uint32_t *ci;
@@ -1267,10 +1325,23 @@ static BatchBufs batch_buffers(Ctx &x, const crest_batch_args *b, const BatchPla
     B.load = x.get<double>(P.pre + "load", S * d * P.kp);
     B.tot = x.get<double>(P.pre + "total", S);
     B.shift = x.get<float>(P.pre + "shift", S * P.kp);
-    const size_t tile_elems = (size_t)P.max_tile_rows * dp;
-    B.ta = x.get<uint8_t>(P.pre + "tile_a", tile_elems * tile_elem_bytes(P.fmt));
-    B.tb = (P.fmt == TILE_BF16X2 || P.fmt == TILE_F16X2) ? x.get<uint8_t>(P.pre + "tile_b", tile_elems * 2) : nullptr;
-    B.partial = x.get<double>(P.pre + "partial", (size_t)std::max<int64_t>(P.max_slots, 1) * kGramBlock * kGramBlock);
+    if (P.sparse) {
+        B.ta = B.tb = nullptr;
+        B.sp_vals = x.get<double>(P.pre + "sp_vals", (size_t)std::max<int64_t>(P.nnz, 1));
+        B.sp_cols = x.get<uint8_t>(P.pre + "sp_cols", (size_t)std::max<int64_t>(P.nnz, 1));
+        B.sp_meta = x.get<uint32_t>(P.pre + "sp_meta", sparse_meta_words((int64_t)P.sp_groups.size() * kRowGroupSparse,
+                                                                          P.d_pad));
+    } else {
+        const size_t tile_elems = (size_t)P.max_tile_rows * dp;
+        B.ta = x.get<uint8_t>(P.pre + "tile_a", tile_elems * tile_elem_bytes(P.fmt));
+        B.tb = (P.fmt == TILE_BF16X2 || P.fmt == TILE_F16X2) ? x.get<uint8_t>(P.pre + "tile_b", tile_elems * 2)
+                                                             : nullptr;
+        B.sp_vals = nullptr;
+        B.sp_cols = nullptr;
+        B.sp_meta = nullptr;
+    }
+    B.partial = x.get<double>(P.pre + "partial", (size_t)std::max<int64_t>(std::max(P.max_slots, P.sp_slots), 1) *
+                                                     kGramBlock * kGramBlock);
     const size_t kb = P.kprec == CREST_PREC_FP32 ? 4 : 2;
     B.xa = x.get<uint8_t>(P.pre + "knn_xa", (size_t)P.n_pad * P.kpad * kb);
     B.xb = x.get<uint8_t>(P.pre + "knn_xb", (size_t)P.n_pad * P.kpad * kb);
@@ -1326,6 +1397,14 @@ static void batch_gram(Ctx &x, const crest_batch_args *b, const BatchPlan &P, co
     launch_fill_f64(B.colsum, (int64_t)P.n_sub * P.d_pad, 0.0, x.st);
     const DevChunk c = batch_chunk(b, P, B);
     const DevScale sc = batch_scale(b, P, B);
+    if (P.sparse) {
+        launch_sparse_pack_groups(c, sc, P.d_pad, P.d_sp_groups, (int64_t)P.sp_groups.size(), B.sp_vals, B.sp_cols,
+                                  B.sp_meta, x.st);
+        launch_gram_sparse_items(B.sp_vals, B.sp_cols, B.sp_meta, c.span_off, c.base, P.d_sp_groups, P.d_sp_items,
+                                 (int64_t)P.sp_items.size(), P.d_pad, B.partial, B.gram, B.colsum, x.st);
+        launch_gram_reduce_items(B.partial, P.d_sp_red, (int64_t)P.sp_red.size(), P.d_pad, B.gram, x.st);
+        return;
+    }
     for (size_t i = 0; i < P.tiles.size(); ++i) {
         const BatchTile &t = P.tiles[i];
         launch_densify_ranges(c, sc, P.fmt, B.ta, B.tb, P.d_pad, P.d_tile_segs[i], P.d_tile_dens[i],
@@ -1619,7 +1698,7 @@ static std::string batch_key(const crest_batch_args *b, const BatchPlan &P) {
     put(&b->pipe, sizeof(crest_pipeline_args));
     put(b->pipe.hvg_of_gene, b->pipe.n_genes * sizeof(int32_t));
     put(P.off.data(), P.off.size() * sizeof(int64_t));
-    const int64_t extra[4] = {P.tile_rows, P.gram_rows, P.eig_batch, (int64_t)P.grid};
+    const int64_t extra[5] = {P.tile_rows, P.gram_rows, P.eig_batch, (int64_t)P.grid, (int64_t)P.sparse};
     put(extra, sizeof(extra));
     return k;
 }
@@ -1674,7 +1753,13 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
             P.max_slots = std::max(P.max_slots, t.n_slots);
         }
     };
-    if (!P.gram_rows && !tune_lookup(tune_key(P), P.gram_rows)) {
+    P.sparse = gram_sparse_enabled(a->prec_gram, P.d_pad);
+    if (P.sparse) {  // gram_rows: rows per sparse Gram range (no tuning: one range per subset up to 16,384 rows)
+        const int64_t rg = P.gram_rows ? std::max<int64_t>(1, P.gram_rows / kRowGroupSparse) : 256;
+        P.gram_rows = rg * kRowGroupSparse;
+        plan_sparse(P, rg);
+        sparse_set_attrs();
+    } else if (!P.gram_rows && !tune_lookup(tune_key(P), P.gram_rows)) {
         // a new shape: time the candidates (a graph captured for other arguments goes,
         // since the buffers it reads may be resized here)
         if (b->use_graph) x.drop_graphs();
@@ -1684,7 +1769,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
         BatchBufs B0 = batch_buffers(x, b, P, gene_of_hvg);
         P.gram_rows = tune_gram_rows(x, b, P, B0, tuned_now);
     }
-    plan_all(P.gram_rows);
+    if (!P.sparse) plan_all(P.gram_rows);
     densify_set_attrs();
     const std::string key = batch_key(b, P);
     // graphs are replayed only by graph-mode calls with identical arguments; eager calls

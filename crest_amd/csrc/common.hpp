@@ -9,6 +9,7 @@ namespace crest {
 
 constexpr int kWave = 64;
 constexpr int kRowGroup = 8;      // rows interleaved per (row-group, gene) unit of a tile
+constexpr int kRowGroupSparse = 64;  // rows per group of the sparse FP64 Gram's packed entries
 constexpr int kGramBlock = 128;   // Gram output block (per workgroup)
 constexpr int kStatsWindow = 2048;  // vars accumulated in LDS per gene_stats window
 
@@ -277,6 +278,15 @@ void launch_sparse_pack(const DevChunk &c, const DevScale &sc, int d_pad, double
 void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
                         int64_t base, int64_t n_rows, int d_pad, int n_split, double *partial, double *cs_partial,
                         double *colsum, hipStream_t s);
+// the batch forms: 64-row groups from a table (each within one subset, scale
+// parameters of that subset), Gram items over group ranges (GramItem.g0/g1 index
+// the group table)
+void launch_sparse_pack_groups(const DevChunk &c, const DevScale &sc, int d_pad, const SegDesc *groups,
+                               int64_t n_groups, double *vals, uint8_t *cols, uint32_t *meta, hipStream_t s);
+void launch_gram_sparse_items(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                              int64_t base, const SegDesc *groups, const GramItem *items, int64_t n_items, int d_pad,
+                              double *partial, double *gram, double *colsum, hipStream_t s);
+void sparse_set_attrs();
 void launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float *p, int64_t n, float v, hipStream_t s);

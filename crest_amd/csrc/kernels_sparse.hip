@@ -30,7 +30,7 @@
 
 namespace crest {
 
-constexpr int kSpGroup = 64;      // rows per group
+constexpr int kSpGroup = kRowGroupSparse;  // rows per group
 constexpr int kSpMaxNb = 32;      // buckets (d_pad <= 4096)
 constexpr int kSpThreads = 1024;  // block kernel (16 waves: 4 rows each of a 64-row group)
 constexpr int kSpWaves = kSpThreads / kWave;
@@ -47,8 +47,8 @@ __device__ inline double sp_cell_total(const DevChunk &c, int64_t lo, int64_t hi
 // [nb][65] (u16, two per word), padded to a multiple of 16 words
 __host__ __device__ inline int sp_rec_words(int nb) { return ((nb + 1) + (nb * 65 + 1) / 2 + 15) / 16 * 16; }
 
-__global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc, int nb, int64_t n_groups,
-                                                     double *vals, uint8_t *cols, uint32_t *meta) {
+__global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc0, int nb, int64_t n_groups,
+                                                     const SegDesc *grp, double *vals, uint8_t *cols, uint32_t *meta) {
     __shared__ int cnt[kSpGroup][kSpMaxNb];
     __shared__ int bstart[kSpMaxNb + 1];
     __shared__ int run[4][kSpMaxNb];
@@ -56,8 +56,16 @@ __global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc, in
     const int rw = sp_rec_words(nb);
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
-        const int64_t r0 = g * kSpGroup;
-        const int nr = (int)((c.n_spans - r0) < kSpGroup ? (c.n_spans - r0) : kSpGroup);
+        int64_t r0 = g * kSpGroup;
+        int nr = (int)((c.n_spans - r0) < kSpGroup ? (c.n_spans - r0) : kSpGroup);
+        DevScale sc = sc0;
+        if (grp) {  // a batch: the group's own rows and its subset's scale parameters
+            const SegDesc gd = grp[g];
+            r0 = gd.cell0;
+            nr = (int)gd.n;
+            sc.mean += (int64_t)gd.subset * sc.d;
+            sc.sd += (int64_t)gd.subset * sc.d;
+        }
         const int64_t gbase = c.span_off[r0] - c.base;
         for (int i = threadIdx.x; i < kSpGroup * kSpMaxNb; i += blockDim.x) cnt[i / kSpMaxNb][i % kSpMaxNb] = 0;
         __syncthreads();
@@ -193,11 +201,11 @@ struct SpRows {
 
 template <bool DIAG>
 __device__ inline void sp_load_rows(SpRows &R, const double *vals, const uint8_t *cols, const uint32_t *meta,
-                                    const int64_t *span_off, int64_t base, int rw, int nb, int bi, int bj, int64_t g,
-                                    int row, int t) {
+                                    const int64_t *span_off, int64_t base, const SegDesc *grp, int rw, int nb, int bi,
+                                    int bj, int64_t g, int row, int t) {
     const uint32_t *rec = meta + g * rw;
     const uint16_t *roff = reinterpret_cast<const uint16_t *>(rec + nb + 1);
-    const int64_t gbase = span_off[g * kSpGroup] - base;
+    const int64_t gbase = span_off[grp ? grp[g].cell0 : g * kSpGroup] - base;
     const int a0 = roff[bi * 65 + row], a1 = roff[bi * 65 + row + 1];
     R.a_at = gbase + rec[bi] + a0;
     R.na = a1 - a0;
@@ -243,14 +251,15 @@ __device__ inline void sp_pairs(double *G, double *cs, const SpRows &R, const do
 
 template <bool DIAG>
 __device__ void sp_block(double *G, double *cs, const double *vals, const uint8_t *cols, const uint32_t *meta,
-                         const int64_t *span_off, int64_t base, int rw, int nb, int bi, int bj, int64_t g0, int64_t g1,
-                         int wave, int lane) {
+                         const int64_t *span_off, int64_t base, const SegDesc *grp, int rw, int nb, int bi, int bj,
+                         int64_t g0, int64_t g1, int wave, int lane) {
     const int row = 4 * wave + (lane >> 4), t = lane & 15;
     if (g0 >= g1) return;
     SpRows cur, nxt;
-    sp_load_rows<DIAG>(cur, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, row, t);
+    sp_load_rows<DIAG>(cur, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g0, row, t);
     for (int64_t g = g0; g < g1; ++g) {
-        if (g + 1 < g1) sp_load_rows<DIAG>(nxt, vals, cols, meta, span_off, base, rw, nb, bi, bj, g + 1, row, t);
+        if (g + 1 < g1)
+            sp_load_rows<DIAG>(nxt, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g + 1, row, t);
         sp_pairs<DIAG>(G, cs, cur, vals, cols, t);
         cur = nxt;
     }
@@ -283,15 +292,70 @@ __global__ __launch_bounds__(kSpThreads) void k_gram_sparse(const double *__rest
     __syncthreads();
     const int64_t g0 = split * n_groups / n_split, g1 = (split + 1) * n_groups / n_split;
     if (bi == bj)
-        sp_block<true>(G, cs, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, g1, wave, lane);
+        sp_block<true>(G, cs, vals, cols, meta, span_off, base, nullptr, rw, nb, bi, bj, g0, g1, wave, lane);
     else
-        sp_block<false>(G, cs, vals, cols, meta, span_off, base, rw, nb, bi, bj, g0, g1, wave, lane);
+        sp_block<false>(G, cs, vals, cols, meta, span_off, base, nullptr, rw, nb, bi, bj, g0, g1, wave, lane);
     __syncthreads();
     double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
     for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads)
         dst[i] = G[(i / kGramBlock) * kSpLd + (i % kGramBlock)];
     if (bi == bj)
         for (int i = tid; i < 128; i += kSpThreads) cs_partial[((int64_t)bi * n_split + split) * 128 + i] = cs[i];
+}
+
+// Batch form: workgroup b takes item b -- one 128 x 128 block of one subset's Gram
+// over that subset's groups [g0, g1) of the group table (64-row groups that never
+// straddle two subsets).  An item with slot < 0 is the only one for its (subset,
+// block) and adds the block into the subset's Gram; the others write partial slot
+// `slot`, which a reduce item sums into the Gram in range order afterwards.  The
+// diagonal blocks' entry sums go into the subset's colsum with FP64 atomics (one
+// per column and item).
+__global__ __launch_bounds__(kSpThreads) void k_gram_sparse_items(const double *__restrict__ vals,
+                                                                const uint8_t *__restrict__ cols,
+                                                                const uint32_t *__restrict__ meta,
+                                                                const int64_t *__restrict__ span_off, int64_t base,
+                                                                const SegDesc *__restrict__ grp, const GramItem *items,
+                                                                int nb, int d_pad, double *partial, double *gram,
+                                                                double *colsum) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double *G = reinterpret_cast<double *>(lds_raw);  // [128][kSpLd]
+    double *cs = G + 128 * kSpLd;                       // [128]
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const GramItem it = items[blockIdx.x];
+    if (it.blk < 0) return;
+    int bi = 0, bj = 0;
+    {
+        int r = 0, t = it.blk;
+        while (t >= nb - r) {
+            t -= nb - r;
+            ++r;
+        }
+        bi = r;
+        bj = r + t;
+    }
+    const int rw = sp_rec_words(nb);
+    for (int i = tid; i < 128 * kSpLd; i += kSpThreads) G[i] = 0.0;
+    for (int i = tid; i < 128; i += kSpThreads) cs[i] = 0.0;
+    __syncthreads();
+    if (bi == bj)
+        sp_block<true>(G, cs, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, it.g0, it.g1, wave, lane);
+    else
+        sp_block<false>(G, cs, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, it.g0, it.g1, wave, lane);
+    __syncthreads();
+    if (it.slot < 0) {
+        double *dst = gram + (int64_t)it.subset * d_pad * d_pad + (int64_t)bi * kGramBlock * d_pad + bj * kGramBlock;
+        for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads) {
+            const int m = i / kGramBlock, n = i % kGramBlock;
+            dst[(int64_t)m * d_pad + n] += G[m * kSpLd + n];
+        }
+    } else {
+        double *dst = partial + it.slot * kGramBlock * kGramBlock;
+        for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads)
+            dst[i] = G[(i / kGramBlock) * kSpLd + (i % kGramBlock)];
+    }
+    if (bi == bj)
+        for (int i = tid; i < 128; i += kSpThreads)
+            atomicAdd(&colsum[(int64_t)it.subset * d_pad + bi * kGramBlock + i], cs[i]);
 }
 
 // colsum[h] += sum over splits (in order) of the diagonal blocks' entry sums
@@ -315,7 +379,31 @@ void launch_sparse_pack(const DevChunk &c, const DevScale &sc, int d_pad, double
     const int64_t n_groups = ceil_div(c.n_spans, kSpGroup);
     if (n_groups <= 0) return;
     const int g = (int)std::min<int64_t>(n_groups, 8192);
-    hipLaunchKernelGGL(k_sparse_pack, dim3(g), dim3(256), 0, s, c, sc, d_pad / kGramBlock, n_groups, vals, cols, meta);
+    const SegDesc *no_groups = nullptr;
+    hipLaunchKernelGGL(k_sparse_pack, dim3(g), dim3(256), 0, s, c, sc, d_pad / kGramBlock, n_groups, no_groups, vals,
+                       cols, meta);
+}
+
+void launch_sparse_pack_groups(const DevChunk &c, const DevScale &sc, int d_pad, const SegDesc *groups,
+                               int64_t n_groups, double *vals, uint8_t *cols, uint32_t *meta, hipStream_t s) {
+    if (n_groups <= 0) return;
+    const int g = (int)std::min<int64_t>(n_groups, 8192);
+    hipLaunchKernelGGL(k_sparse_pack, dim3(g), dim3(256), 0, s, c, sc, d_pad / kGramBlock, n_groups, groups, vals, cols,
+                       meta);
+}
+
+void sparse_set_attrs() {
+    const int lds = (int)sparse_gram_lds();
+    hipFuncSetAttribute((const void *)k_gram_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    hipFuncSetAttribute((const void *)k_gram_sparse_items, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+void launch_gram_sparse_items(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                              int64_t base, const SegDesc *groups, const GramItem *items, int64_t n_items, int d_pad,
+                              double *partial, double *gram, double *colsum, hipStream_t s) {
+    if (n_items <= 0) return;
+    hipLaunchKernelGGL(k_gram_sparse_items, dim3((unsigned)n_items), dim3(kSpThreads), sparse_gram_lds(), s, vals, cols,
+                       meta, span_off, base, groups, items, d_pad / kGramBlock, d_pad, partial, gram, colsum);
 }
 
 void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,

@@ -125,6 +125,25 @@ def test_batch_tiles_ranges_and_eig_groups(batch, oracle):
     assert (np.sort(idx, 1) == np.sort(i_def, 1)).all(1).mean() > 0.999
 
 
+def test_batch_sparse_fp64_gram_matches_dense(batch, oracle, monkeypatch):
+    """The batch's FP64 Gram by sparse pair products (the default at FP64: 64-row
+    groups per subset, one item per block and range, ranges of 4 groups with gram_rows
+    = 256 so that every subset above 256 rows goes through partial slots) against the
+    dense FP64 MFMA form (CREST_GRAM_SPARSE=0): both exact FP64 sums, so the
+    eigenpairs agree to FP64 rounding."""
+    _, o_sp, X_sp, i_sp, _ = _run(batch, prec_gram="fp64", prec_knn="fp32")
+    _, o_sr, _, _, _ = _run(batch, prec_gram="fp64", prec_knn="fp32", gram_rows=256)
+    monkeypatch.setenv("CREST_GRAM_SPARSE", "0")
+    _, o_d, X_d, i_d, _ = _run(batch, prec_gram="fp64", prec_knn="fp32")
+    for o in (o_sp, o_sr):
+        np.testing.assert_allclose(o.variance, o_d.variance, rtol=1e-11)
+        np.testing.assert_allclose(o.total_variance, o_d.total_variance, rtol=1e-12)
+        for i in range(len(SIZES)):
+            assert _angle(oracle, o.loadings[i], o_d.loadings[i], 9) < 1e-9, i
+    np.testing.assert_allclose(X_sp, X_d, rtol=1e-5, atol=1e-5)
+    assert (np.sort(i_sp, 1) == np.sort(i_d, 1)).all(1).mean() > 0.999
+
+
 def test_batch_autotuner_reports_its_choice(batch):
     _, o1, _, _, _ = _run(batch, prec_gram="fp16", prec_knn="fp16", tile_rows=4096)
     assert o1.gram_rows in (2048, 4096, 8192, 16384, 1 << 20)
