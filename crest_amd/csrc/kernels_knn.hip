@@ -650,7 +650,7 @@ __device__ inline uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 template <int L>
-__device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, uint32_t id) {
+__device__ __attribute__((always_inline)) inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, uint32_t id) {
     const uint32_t db = __float_as_uint(d);
     bool c[L];
 #pragma unroll
@@ -671,7 +671,7 @@ __device__ inline void topk_insert(float (&ld)[L], uint32_t (&li)[L], float d, u
 // visit inserts into the lists of the lanes whose value is still below their
 // threshold.
 template <int L>
-__device__ inline void topk_update(float (&ld)[L], uint32_t (&li)[L], const f32x16 &a, uint32_t cb, int half) {
+__device__ __attribute__((always_inline)) inline void topk_update(float (&ld)[L], uint32_t (&li)[L], const f32x16 &a, uint32_t cb, int half) {
     // the minimum on the bit patterns as signed integers: non-negative floats order as
     // their patterns, negative ones fall below any threshold (which is >= 0), so
     // "some value < thr" is decided without canonicalising the MFMA outputs
@@ -688,6 +688,58 @@ __device__ inline void topk_update(float (&ld)[L], uint32_t (&li)[L], const f32x
         u &= u - 1;
         const float v = a[r];
         if (v < ld[L - 1]) topk_insert<L>(ld, li, v > 0.0f ? v : 0.0f, cb + 8 * (r >> 2) + 4 * half + (r & 3));
+    }
+}
+
+// Insertion buffer slots per lane and list in the 16-bit screens (measured at 1M x 50,
+// FP16, k = 15: 129.1 ms with direct insertions, 122.9 ms with buffers of 4); 0 (direct
+// insertions) for the shapes whose registers already spill without the buffers.
+template <int KC, int L> constexpr int screen_buf() {
+    return ((L == 16 && (KC == 12 || KC == 16)) || KC == 52 || (L == 64 && KC >= 16)) ? 0 : 4;
+}
+
+// Deferred insertions: a lane's entries below its list's threshold are appended to
+// a per-lane buffer of B slots in LDS (slot s of the lane at buf[64 s]), and the
+// buffers are merged into the lists together once some lane's buffer is full.  A
+// list insertion costs the whole wave's issue time whichever lanes need it, and
+// lanes insert at different fragments, so merging every lane's buffer in one
+// wave-uniform pass turns about one insertion pass per inserted entry into one per
+// B entries of the fullest lane.  The threshold of the append test is the list's at
+// the last merge (entries that stop qualifying are dropped by the merge).
+template <int L, int B>
+__device__ __attribute__((always_inline)) inline void topk_flush(float (&ld)[L], uint32_t (&li)[L], const uint2 *buf, uint32_t &cnt) {
+    // slots in a uniform loop up to the fullest lane's count
+    uint32_t s = 0;
+    while (__builtin_amdgcn_ballot_w64(s < cnt) != 0) {
+        if (s < cnt) {
+            const uint2 e = buf[64 * s];
+            if (__uint_as_float(e.x) < ld[L - 1]) topk_insert<L>(ld, li, __uint_as_float(e.x), e.y);
+        }
+        ++s;
+    }
+    cnt = 0;
+}
+
+template <int L, int B>
+__device__ __attribute__((always_inline)) inline void topk_update_buf(float (&ld)[L], uint32_t (&li)[L], const f32x16 &a, uint32_t cb, int half,
+                                       uint2 *buf, uint32_t &cnt) {
+    int32_t m = __float_as_int(a[0]);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) m = min(m, __float_as_int(a[r]));
+    if (__builtin_amdgcn_ballot_w64(m < __float_as_int(ld[L - 1])) == 0) return;
+    const float thr = ld[L - 1];
+    uint32_t u = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) u |= (__builtin_amdgcn_ballot_w64(a[r] < thr) != 0 ? 1u : 0u) << r;
+    while (u) {
+        const int r = __builtin_ctz(u);
+        u &= u - 1;
+        const float v = a[r];
+        if (v < ld[L - 1]) {
+            buf[64 * cnt] = make_uint2(__float_as_uint(v > 0.0f ? v : 0.0f), cb + 8 * (r >> 2) + 4 * half + (r & 3));
+            ++cnt;
+        }
+        if (__builtin_amdgcn_ballot_w64(cnt == (uint32_t)B) != 0) topk_flush<L, B>(ld, li, buf, cnt);
     }
 }
 
@@ -742,7 +794,7 @@ template <int KC> constexpr int screen_tb() { return KC <= 8 ? 4 : KC <= 16 ? 2 
 // fragments alternate between two accumulator pairs, and each fragment's list
 // update is issued after the MFMAs of the next one, so the matrix core works while
 // the wave's VALU merges the previous results.
-template <int TY, int KC, int L, int NW, int TB>
+template <int TY, int KC, int L, int NW, int TB, int BUF>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_waves<KC, L>()))) void k_knn_screen16(
     const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb, int64_t nc, int64_t nq, const uint32_t *keys_c,
     const uint32_t *keys_q, int self, float *cand_d, uint32_t *cand_i, KnnSeg sg) {
@@ -754,7 +806,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     static_assert(TU % NT == 0, "stage must split evenly over the workgroup");
     constexpr int PER = TU / NT;
     __shared__ u32x4 lds[2][TU];
+    // insertion buffers (BUF > 0): [wave][list][slot][lane]
+    __shared__ uint2 qbuf[BUF > 0 ? NW * 2 * BUF * 64 : 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
+    uint2 *buf0 = qbuf + (BUF > 0 ? (wave * 2) * BUF * 64 + lane : 0);
+    uint2 *buf1 = qbuf + (BUF > 0 ? (wave * 2 + 1) * BUF * 64 + lane : 0);
+    uint32_t cnt0 = 0, cnt1 = 0;
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
     // contiguous range of query blocks, so the workgroups sharing an XCD's L2 have
     // neighbouring home stages and walk overlapping candidate streams.
@@ -807,13 +864,18 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
             c1 = mfma16<TY>(a[s], b1[s], c1);
         }
     };
-    auto update = [&](f32x16 &c0, f32x16 &c1, uint32_t cb) {
+    auto update = [&](f32x16 &c0, f32x16 &c1, uint32_t cb) __attribute__((always_inline)) {
         if (self && (cb == (uint32_t)q0 || cb == (uint32_t)q0 + 32)) {
             self_mask<L>(c0, cb, half, qa);
             self_mask<L>(c1, cb, half, qb);
         }
-        topk_update<L>(d0, i0, c0, cb, half);
-        topk_update<L>(d1, i1, c1, cb, half);
+        if constexpr (BUF > 0) {
+            topk_update_buf<L, BUF>(d0, i0, c0, cb, half, buf0, cnt0);
+            topk_update_buf<L, BUF>(d1, i1, c1, cb, half, buf1, cnt1);
+        } else {
+            topk_update<L>(d0, i0, c0, cb, half);
+            topk_update<L>(d1, i1, c1, cb, half);
+        }
     };
     int64_t n_stages = (nc + kPadRows - 1) / kPadRows * (kPadRows / (kTC * TB));
     // Stages are visited outward from the one holding the middle of this
@@ -854,21 +916,9 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
         if (k <= 2 * m) return (k & 1) ? h + (k + 1) / 2 : h - k / 2;
         return above > below ? h + (k - m) : h - (k - m);
     };
-    // Pipeline: LDS holds visit t (being computed) and receives visit t+1 from
-    // registers loaded one iteration earlier; the loads for visit t+2 are issued after
-    // that store, so their latency overlaps the whole next iteration.  (The final
-    // iterations re-load the last stage; those copies are never read.)
-    {
-        const u32x4 *first = xa + stage_at(0) * TU;
-        for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = first[i];
-    }
-    Regs<u32x4, PER> pre;
-    regs_load<u32x4, PER, NT>(pre, xa + stage_at(n_stages > 1 ? 1 : 0) * TU, threadIdx.x);
-    __syncthreads();
     f32x16 x0, x1, y0, y1;  // even fragments in x, odd in y
     uint32_t cy = 0;        // first candidate of the fragment held in y
-    for (int64_t t = 0; t < n_stages; ++t) {
-        const u32x4 *stage = lds[t & 1];
+    auto compute = [&](const u32x4 *stage, int64_t t) __attribute__((always_inline)) {
         const uint32_t c0 = (uint32_t)(stage_at(t) * kTC * TB);
         if constexpr (screen_pipe<KC, L>()) {
             u32x4 ae[KS], ao[KS];  // operands of the even / odd fragment
@@ -890,12 +940,30 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
                 update(x0, x1, c0 + (uint32_t)(f * 32));
             }
         }
+    };
+    // LDS holds visit t (being computed) and receives visit t+1 from registers loaded
+    // one iteration earlier; the loads for visit t+2 are issued after that store, so
+    // their latency overlaps the whole next iteration.  (The final iterations re-load
+    // the last stage; those copies are never read.)
+    {
+        const u32x4 *first = xa + stage_at(0) * TU;
+        for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = first[i];
+    }
+    Regs<u32x4, PER> pre;
+    regs_load<u32x4, PER, NT>(pre, xa + stage_at(n_stages > 1 ? 1 : 0) * TU, threadIdx.x);
+    __syncthreads();
+    for (int64_t t = 0; t < n_stages; ++t) {
+        compute(lds[t & 1], t);
         regs_store<u32x4, PER, NT>(pre, lds[(t & 1) ^ 1], threadIdx.x);
         const int64_t tn = t + 2 < n_stages ? t + 2 : n_stages - 1;
         regs_load<u32x4, PER, NT>(pre, xa + stage_at(tn) * TU, threadIdx.x);
         __syncthreads();
     }
     if constexpr (screen_pipe<KC, L>()) update(y0, y1, cy);
+    if constexpr (BUF > 0) {
+        topk_flush<L, BUF>(d0, i0, buf0, cnt0);
+        topk_flush<L, BUF>(d1, i1, buf1, cnt1);
+    }
     write_lists<L>(d0, i0, qa, half, cand_d, cand_i);
     write_lists<L>(d1, i1, qb, half, cand_d, cand_i);
 }
@@ -980,9 +1048,9 @@ static bool launch_screen_L(int prec, const void *xa, const void *xb, int64_t nc
 #define CREST_X16(TY, KCV)                                                                                   \
     case KCV: {                                                                                             \
         constexpr int NW = screen_nw<TY, KCV, L>();                                                         \
-        hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW, screen_tb<KCV>()>), dim3((unsigned)(nq_pad / (NW * 64))), \
-                           dim3(NW * 64), 0, s, (const u32x4 *)xa, (const u32x4 *)xb, nc, nq, keys_c, keys_q, \
-                           self, cd, ci, sg);                                                               \
+        hipLaunchKernelGGL((k_knn_screen16<TY, KCV, L, NW, screen_tb<KCV>(), screen_buf<KCV, L>()>),         \
+                           dim3((unsigned)(nq_pad / (NW * 64))), dim3(NW * 64), 0, s, (const u32x4 *)xa,     \
+                           (const u32x4 *)xb, nc, nq, keys_c, keys_q, self, cd, ci, sg);                    \
         return true;                                                                                        \
     }
 #define CREST_F(KCV)                                                                                          \
