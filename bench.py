@@ -162,20 +162,32 @@ def stage_only(args) -> int:
     """One chunk stage alone on the 1M x 2k workload, for hardware-counter runs that
     must not include the eigensolve's dispatches: 'stats' (gene statistics) or 'gram'
     (statistics, then the Gram in the headline's Gram precision), through a device
-    session over the whole CSR as one chunk."""
+    session over the whole CSR as one chunk, or 'project' (the operator path's
+    projection onto 50 fixed orthonormal loadings)."""
     import crest_amd as ca
     from crest_amd import synth
     from crest_amd.session import Session
     ip, ix, dv, info = synth.make_counts(args.cells, args.genes, args.clusters, density=args.density, seed=args.seed)
     ctx = (ix, dv, np.arange(args.cells, dtype=np.int64), np.arange(args.genes, dtype=np.int32), 1e4, True, None, ip, 0)
     ca.set_precision(args.gram_prec)
-    with Session(args.genes, args.cells) as sess:
+    if args.stage_only == "project":
+        # the projection of every cell onto 50 orthonormal FP64 loadings (the operator
+        # path's crest_project over the whole CSR as one chunk)
+        from crest_amd import _native as N
+        rng = np.random.default_rng(0)
+        V = np.linalg.qr(rng.normal(size=(args.genes, args.pcs)))[0]
+        mean, sd = np.full(args.genes, 0.5), np.full(args.genes, 1.5)
+        out = np.zeros((args.cells, args.pcs), np.float32)
         for _ in range(args.steps):
-            sess.push_stats(ctx)
-            if args.stage_only == "gram":
-                sess.begin_gram(np.arange(args.genes, dtype=np.int32), args.genes, True, None, None, 10.0)
-                sess.push_gram(ctx)
-            sess.stats()
+            N.project(*ctx, np.arange(args.genes, dtype=np.int32), V, out, True, mean, sd, 10.0)
+    else:
+        with Session(args.genes, args.cells) as sess:
+            for _ in range(args.steps):
+                sess.push_stats(ctx)
+                if args.stage_only == "gram":
+                    sess.begin_gram(np.arange(args.genes, dtype=np.int32), args.genes, True, None, None, 10.0)
+                    sess.push_gram(ctx)
+                sess.stats()
     if args.calibrate_copy:
         from crest_amd._lib import check, lib
         from crest_amd.pipeline import DeviceBuffer
@@ -534,8 +546,8 @@ def main() -> int:
                     help="sampled queries of the other legs' kNN agreement checks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline / accuracy leg")
     ap.add_argument("--knn-only", action="store_true", help="run only knn_graph on synthetic scores (profiling)")
-    ap.add_argument("--stage-only", choices=["stats", "gram"], default=None,
-                    help="run only the gene statistics or statistics + Gram over the workload (profiling)")
+    ap.add_argument("--stage-only", choices=["stats", "gram", "project"], default=None,
+                    help="run only the gene statistics, statistics + Gram, or the projection over the workload (profiling)")
     ap.add_argument("--gram-prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16"],
                     help="Gram of the headline leg (fp64: FP64 MFMA; fp16: split-FP16 hi+lo, three MFMA products)")
     ap.add_argument("--knn-prec", default="fp16", choices=["fp32", "bf16x3", "fp16"],
