@@ -664,6 +664,254 @@ static bool chefsi_topp(Ctx &x, const double *C, int d, int p, double *dev_var, 
     return true;
 }
 
+// The iteration of chefsi_topp for nm matrices of order d side by side (the covariances
+// C, d x d column-major, d^2 apart): the same start block, filter, CholeskyQR2,
+// Rayleigh-Ritz, locking and final solve per matrix, with every product of all the
+// matrices in one launch (the big products on k_chb_gemm, the skinny ones as strided
+// batched dgemm), one host round trip per iteration for all of them, and per-matrix
+// decisions (degree, locking, retries) as in chefsi_topp.  A matrix that would make
+// chefsi_topp give up -- rank loss after its retries, a flat spectrum, no convergence
+// in 60 iterations, a failed final solve -- drops out of the batch; solved[z] tells
+// which matrices got their variance and loadings (subset ids[z] of var / load).
+// Buffers are named with `pre`.  C is left as it was.
+static std::vector<char> chefsi_topp_batch(Ctx &x, const std::string &pre, const double *C, int nm, int d, int p,
+                                           const int32_t *ids_dev, double *var, double *load) {
+    std::vector<char> solved(nm, 0);
+    const int K = chefsi_block(d, p);
+    if (!K || nm <= 0) return solved;
+    const int dp = (int)round_up(d, 256);
+    const int64_t dk = (int64_t)dp * K, dd = (int64_t)dp * dp, kk = (int64_t)K * K, ns = 2 * (int64_t)K + 2;
+    double *A = x.get<double>(pre + "chb_a", (size_t)nm * dd);
+    double *buf[4];
+    for (int i = 0; i < 4; ++i) buf[i] = x.get<double>(pre + "chb_b" + std::to_string(i), (size_t)nm * dk);
+    double *W = x.get<double>(pre + "chb_w", (size_t)nm * dk);
+    double *WZ = x.get<double>(pre + "chb_wz", (size_t)nm * dk);
+    double *Wl = x.get<double>(pre + "chb_wl", (size_t)nm * dk);
+    double *Qn = x.get<double>(pre + "chb_qn", (size_t)nm * dk);
+    double *QL = x.get<double>(pre + "chb_ql", (size_t)nm * dp * p);
+    double *thL = x.get<double>(pre + "chb_thl", (size_t)nm * p);
+    double *T = x.get<double>(pre + "chb_t", (size_t)nm * p * K);
+    double *G = x.get<double>(pre + "chb_g", (size_t)nm * kk);
+    double *Ri = x.get<double>(pre + "chb_ri", (size_t)nm * kk);
+    double *Z = x.get<double>(pre + "chb_z", (size_t)nm * kk);
+    double *stat = x.get<double>(pre + "chb_stat", (size_t)nm * ns);
+    double *Vout = x.get<double>(pre + "chb_vout", (size_t)nm * d * p);
+    double *evals = x.get<double>(pre + "chb_evals", (size_t)nm * p);
+    ChbMat *Pd = x.get<ChbMat>(pre + "chb_p", (size_t)nm);
+    ChbMat *Ph = static_cast<ChbMat *>(x.host(pre + "chb_p", (size_t)nm * sizeof(ChbMat)));
+    double *hs = static_cast<double *>(x.host(pre + "chb_stat", (size_t)nm * ns * sizeof(double)));
+    const bool trace = std::getenv("CREST_CHEFSI_TRACE") != nullptr;
+    chefsi_batch_set_attrs();
+    const auto N = rocblas_operation_none, Tr = rocblas_operation_transpose;
+    auto bgemm = [&](rocblas_operation ta, rocblas_operation tb, int m, int n, int kq, double alpha, const double *a,
+                     int lda, int64_t sa, const double *b, int ldb, int64_t sb, double beta, double *c, int ldc,
+                     int64_t sc) {
+        BLASCHK(rocblas_dgemm_strided_batched(x.blas, ta, tb, m, n, kq, &alpha, a, lda, sa, b, ldb, sb, &beta, c, ldc,
+                                              sc, nm));
+    };
+    auto other = [&](std::initializer_list<int> used) {
+        for (int i = 0; i < 4; ++i)
+            if (std::find(used.begin(), used.end(), i) == used.end()) return i;
+        return -1;
+    };
+    auto upload = [&]() { HIPCHK(hipMemcpyAsync(Pd, Ph, (size_t)nm * sizeof(ChbMat), hipMemcpyHostToDevice, x.st)); };
+    auto cholqr2 = [&](int y, int keep) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int o = other({y, keep});
+            bgemm(Tr, N, K, K, dp, 1.0, buf[y], dp, dk, buf[y], dp, dk, 0.0, G, K, kk);
+            launch_chb_chol_rinv(G, K, nm, Pd, Ri, stat, x.st);
+            bgemm(N, N, dp, K, K, 1.0, buf[y], dp, dk, Ri, K, kk, 0.0, buf[o], dp, dk);
+            y = o;
+        }
+        return y;
+    };
+    auto rayleigh_ritz = [&](int y, int keep, bool final) {
+        launch_chb_gemm(A, buf[y], dp, K, nm, Pd, -1, false, W, x.st);
+        bgemm(Tr, N, K, K, dp, 1.0, buf[y], dp, dk, W, dp, dk, 0.0, G, K, kk);
+        launch_chb_jacobi(G, K, nm, Pd, final ? 40 : 2, final, !final, Z, stat, x.st);
+        const int v = other({y, keep});
+        bgemm(N, N, dp, K, K, 1.0, buf[y], dp, dk, Z, K, kk, 0.0, buf[v], dp, dk);
+        bgemm(N, N, dp, K, K, 1.0, W, dp, dk, Z, K, kk, 0.0, WZ, dp, dk);
+        launch_chb_resid(WZ, buf[v], d, dp, K, nm, Pd, stat, x.st);
+        x.check_launch();
+        x.download(hs, stat, (size_t)nm * ns);
+        x.sync();
+        return v;
+    };
+    auto finite = [&](int z, int ka) {
+        const double *h = hs + z * ns;
+        for (int j = 0; j < ka; ++j)
+            if (!std::isfinite(h[j]) || !std::isfinite(h[K + j])) return false;
+        return true;
+    };
+    enum { ACTIVE, DONE, OUT };
+    struct Mat {
+        int state = ACTIVE, ka = 0, L = 0, j0 = 0, retries = 0;
+        bool retry = false;
+        double top = 0.0, tol = 0.0, kappa = 1e3;
+        std::vector<double> prev;
+    };
+    std::vector<Mat> M(nm);
+    for (auto &m : M) {
+        m.ka = K;
+        m.prev.assign(2 * K, 0.0);
+    }
+    // start: every matrix's padded operator, the start block, zeroed buffers
+    for (int z = 0; z < nm; ++z) Ph[z] = ChbMat{K, 0, 0, 0, 0, 0, 0.0, 1.0};
+    upload();
+    launch_chb_pad(C, d, dp, nm, Pd, A, x.st);
+    for (double *b : {buf[1], buf[2], buf[3], W, WZ}) launch_fill_f64(b, (int64_t)nm * dk, 0.0, x.st);
+    launch_fill_f64(QL, (int64_t)nm * dp * p, 0.0, x.st);
+    launch_fill_f64(stat, (int64_t)nm * ns, 0.0, x.st);
+    launch_chb_rand(buf[0], d, dp, K, nm, 0x5eed0f5ca1edull + (uint64_t)d * 1000003u + (uint64_t)p, x.st);
+    int v = rayleigh_ritz(cholqr2(0, -1), -1, false);
+    for (int z = 0; z < nm; ++z)
+        if (hs[z * ns + 2 * K] != 0.0 || !finite(z, K) || !(hs[z * ns] > 0.0)) M[z].state = OUT;
+    int iters = 0;
+    for (int it = 0; it < 60; ++it) {
+        // per matrix: locking from the last Rayleigh-Ritz, then the next filter
+        bool any_lock = false, any_active = false, any_locked = false;
+        int deg = 0;
+        for (int z = 0; z < nm; ++z) {
+            Mat &m = M[z];
+            ChbMat &q = Ph[z];
+            q = ChbMat{0, 0, 0, m.L, 0, 0, 0.0, 1.0};
+            if (m.state != ACTIVE) continue;
+            const double *h = hs + z * ns;
+            if (!m.retry) {
+                m.top = std::max(m.top, h[0]);
+                m.tol = 1e-12 * m.top;
+                int J = 0;
+                while (J < p - m.L && h[K + J] <= m.tol) ++J;
+                q.J = J;
+                any_lock = any_lock || J > 0;
+                m.L += J;
+                m.ka -= J;
+                m.j0 = J;
+                std::copy(h, h + 2 * K, m.prev.begin());
+                if (m.L >= p) {
+                    m.state = DONE;
+                    continue;
+                }
+            }
+            const double b = m.prev[m.j0 + m.ka - 1], theta_top = m.prev[m.j0];
+            if (!(b > 1e-8 * m.top)) {
+                m.state = OUT;
+                if (trace) std::fprintf(stderr, "[chefsi-batch] matrix %d: flat spectrum\n", z);
+                continue;
+            }
+            const double c = 0.5 * b, e = 0.5 * b;
+            q.ka = m.ka;
+            q.m = chefsi_degree((theta_top - c) / e, m.kappa);
+            q.voff = m.j0;
+            q.c = c;
+            q.e = e;
+            deg = std::max(deg, q.m);
+            any_active = true;
+            any_locked = any_locked || m.L > 0;
+        }
+        upload();
+        if (any_lock) {  // lock the converged leading columns and deflate them out of A
+            launch_chb_lock(buf[v], stat, dp, K, p, nm, Pd, QL, thL, Wl, Qn, x.st);
+            bgemm(N, Tr, dp, dp, K, -1.0, Wl, dp, dk, Qn, dp, dk, 1.0, A, dp, dd);
+        }
+        if (!any_active) break;
+        if (trace)
+            std::fprintf(stderr, "[chefsi-batch] it %d: max degree %d, active %d\n", it, deg,
+                         (int)std::count_if(M.begin(), M.end(), [](const Mat &m) { return m.state == ACTIVE; }));
+        launch_fill_f64(stat, (int64_t)nm * ns, 0.0, x.st);
+        // the filter, deg steps for all; a matrix of lower degree carries its result
+        int y1 = other({v});
+        launch_chb_combine(buf[y1], WZ, buf[v], nullptr, dp, K, nm, Pd, 0, false, x.st);
+        int y0 = -1;
+        for (int j = 1; j < deg; ++j) {
+            launch_chb_gemm(A, buf[y1], dp, K, nm, Pd, j, false, W, x.st);
+            const int y2 = other({y0, y1, v});
+            launch_chb_combine(buf[y2], W, buf[y1], j == 1 ? buf[v] : buf[y0], dp, K, nm, Pd, j, j == 1, x.st);
+            y0 = y1;
+            y1 = y2;
+        }
+        if (any_locked) {  // y -= QL (QL^T y)
+            bgemm(Tr, N, p, K, dp, 1.0, QL, dp, (int64_t)dp * p, buf[y1], dp, dk, 0.0, T, p, (int64_t)p * K);
+            bgemm(N, N, dp, K, p, -1.0, QL, dp, (int64_t)dp * p, T, p, (int64_t)p * K, 1.0, buf[y1], dp, dk);
+        }
+        x.check_launch();
+        const int nv = rayleigh_ritz(cholqr2(y1, v), v, false);
+        ++iters;
+        // failures: filter the same Ritz vectors again at a lower degree (as chefsi_topp)
+        bool any_retry = false;
+        for (int z = 0; z < nm; ++z) {
+            Mat &m = M[z];
+            ChbMat &q = Ph[z];
+            q = ChbMat{0, 0, 0, m.L, 0, 0, 0.0, 1.0};
+            if (m.state != ACTIVE) continue;
+            if (hs[z * ns + 2 * K] == 0.0 && finite(z, m.ka)) {
+                m.kappa = std::min(m.kappa * 100.0, 1e8);
+                m.retry = false;
+                continue;
+            }
+            if (++m.retries > 8 || m.kappa < 10.0) {
+                m.state = OUT;
+                if (trace) std::fprintf(stderr, "[chefsi-batch] matrix %d: rank loss\n", z);
+                continue;
+            }
+            m.kappa = std::sqrt(m.kappa);
+            m.retry = true;
+            // its Ritz block moves to the front of buf[nv]; A times it into WZ
+            q.ka = m.ka;
+            q.voff = m.j0;
+            for (int i = 0; i < m.ka; ++i) {
+                m.prev[i] = m.prev[m.j0 + i];
+                m.prev[K + i] = m.prev[K + m.j0 + i];
+            }
+            m.j0 = 0;
+            any_retry = true;
+        }
+        if (any_retry) {
+            upload();
+            launch_chb_copy(buf[nv], buf[v], K, dp, K, nm, Pd, true, x.st);
+            x.check_launch();
+            x.sync();  // the upload and the copy are done before the host changes Ph
+            for (int z = 0; z < nm; ++z) Ph[z].voff = 0;
+            upload();
+            launch_chb_gemm(A, buf[nv], dp, K, nm, Pd, -1, false, WZ, x.st);
+            x.check_launch();
+            x.sync();
+        }
+        v = nv;
+    }
+    // the final Rayleigh-Ritz of each finished matrix's locked vectors against C
+    x.sync();  // the last upload of Ph is done
+    bool any_done = false;
+    for (int z = 0; z < nm; ++z) {
+        if (M[z].state == ACTIVE) M[z].state = OUT;
+        Ph[z] = ChbMat{M[z].state == DONE ? p : 0, 0, 0, 0, 0, 0, 0.0, 1.0};
+        any_done = any_done || M[z].state == DONE;
+    }
+    if (any_done) {
+        upload();
+        launch_chb_pad(C, d, dp, nm, Pd, A, x.st);
+        launch_chb_copy(buf[0], QL, p, dp, K, nm, Pd, false, x.st);
+        launch_chb_zero_tail(buf[0], dp, K, nm, Pd, x.st);
+        launch_fill_f64(stat, (int64_t)nm * ns, 0.0, x.st);
+        v = rayleigh_ritz(cholqr2(0, -1), -1, true);
+        for (int z = 0; z < nm; ++z) {
+            if (M[z].state != DONE) continue;
+            const double *h = hs + z * ns;
+            bool ok = h[2 * K] == 0.0 && h[2 * K + 1] == 0.0 && finite(z, p);
+            for (int j = 0; ok && j < p; ++j) ok = h[K + j] <= 100.0 * M[z].tol;
+            solved[z] = ok ? 1 : 0;
+        }
+        launch_chb_final(buf[v], stat, d, dp, K, p, nm, Vout, evals, x.st);
+        launch_eig_select_batch(Vout, evals, d, p, nm, var, load, x.st, p, ids_dev);
+        x.check_launch();
+    }
+    if (trace)
+        std::fprintf(stderr, "[chefsi-batch] %d matrices: %d solved after %d iterations\n", nm,
+                     (int)std::count(solved.begin(), solved.end(), 1), iters);
+    return solved;
+}
+
 static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *dev_colsum, int d, double n, int k,
                        double *dev_var, double *dev_load, double *dev_total) {
     double *cov = x.get<double>("cov", (size_t)d * d);
@@ -1478,6 +1726,15 @@ static void batch_eig(Ctx &x, const crest_batch_args *b, const BatchPlan &P, con
     }
 }
 
+// The batch's covariances by the filtered subspace iteration side by side (default
+// wherever chefsi_topp would solve one of them; CREST_BATCH_EIG=tri keeps the
+// tridiagonalization of k_sytrd_lower / rocSOLVER and the top-p kernels).
+static bool batch_chefsi_enabled(int d, int kp) {
+    const char *e = std::getenv("CREST_BATCH_EIG");
+    if (e && std::string(e) == "tri") return false;
+    return chefsi_block(d, kp) > 0;
+}
+
 static void batch_eig_impl(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B) {
     const bool topp = use_topp(P.d, P.kp, (int)std::min<int64_t>(P.eig_batch, P.n_sub));
     if (topp) tri_set_attrs();
@@ -1588,11 +1845,19 @@ static void batch_eig_impl(Ctx &x, const crest_batch_args *b, const BatchPlan &P
         HIPCHK(hipEventRecord(x.join, x.side));
     }
     // ---- the d x d covariances
+    const bool chb = batch_chefsi_enabled(P.d, P.kp);
+    if (chb) launch_fill_u32(reinterpret_cast<uint32_t *>(B.info), P.n_sub, 0u, x.st);
     for (int g0 = 0; g0 < (int)primal.size(); g0 += (int)P.eig_batch) {
         const int ng = (int)std::min<int64_t>(P.eig_batch, (int64_t)primal.size() - g0);
         const int32_t *ids = ids_dev + g0;
         launch_cov_batch(B.gram, B.colsum, P.d, P.d_pad, P.d_off, ng, B.cov, B.tot, x.st, ids);
         x.check_launch();
+        if (chb) {  // the filtered subspace iteration, all ng side by side
+            const std::vector<char> ok = chefsi_topp_batch(x, P.pre, B.cov, ng, P.d, P.kp, ids, B.var, B.load);
+            for (int i = 0; i < ng; ++i)
+                if (!ok[i]) fallback(primal[g0 + i]);
+            continue;
+        }
         if (topp) {
             batch_sytrd(x, P.pre, B.cov, P.d, ng, B.w, B.e, B.tau, x.st, x.blas);
             launch_fill_u32(reinterpret_cast<uint32_t *>(B.tinfo), ng, 0u, x.st);

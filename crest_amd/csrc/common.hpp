@@ -202,6 +202,30 @@ void launch_ch_resid(const double *cv, const double *v, const double *theta, int
                      hipStream_t s);
 void launch_ch_lock(const double *src, const double *src_val, int ld, int n, double *dst, double *dst_val,
                     double *dst_scaled, hipStream_t s);
+// batches (chefsi_topp_batch): per-matrix state of one iteration
+struct ChbMat {
+    int32_t ka, m, voff, L, J, pad_;
+    double c, e;
+};
+void chefsi_batch_set_attrs();
+void launch_chb_gemm(const double *A, const double *Y, int dp, int K, int nm, const ChbMat *P, int step, bool use_voff,
+                     double *out, hipStream_t s);
+void launch_chb_combine(double *y_next, const double *w, const double *y, const double *y_prev, int dp, int K, int nm,
+                        const ChbMat *P, int step, bool prev_voff, hipStream_t s);
+void launch_chb_zero_tail(double *y, int dp, int K, int nm, const ChbMat *P, hipStream_t s);
+void launch_chb_chol_rinv(const double *G, int K, int nm, const ChbMat *P, double *Rinv, double *stat, hipStream_t s);
+void launch_chb_jacobi(const double *H, int K, int nm, const ChbMat *P, int max_sweeps, bool strict, bool desc,
+                       double *Z, double *stat, hipStream_t s);
+void launch_chb_resid(const double *cv, const double *v, int d, int dp, int K, int nm, const ChbMat *P, double *stat,
+                      hipStream_t s);
+void launch_chb_lock(const double *v, const double *stat, int dp, int K, int p, int nm, const ChbMat *P, double *QL,
+                     double *thL, double *wl, double *ql, hipStream_t s);
+void launch_chb_pad(const double *C, int d, int dp, int nm, const ChbMat *P, double *A, hipStream_t s);
+void launch_chb_copy(double *dst, const double *src, int src_cols, int dp, int K, int nm, const ChbMat *P,
+                     bool use_voff, hipStream_t s);
+void launch_chb_final(const double *v, const double *stat, int d, int dp, int K, int p, int nm, double *vout,
+                      double *evals, hipStream_t s);
+void launch_chb_rand(double *v, int rows, int ld, int cols, int nm, uint64_t seed, hipStream_t s);
 void launch_shift_fill(const double *colsum, const double *V, int d, int k, double n, int64_t n_rows,
                        float *x, float *shift_scratch, hipStream_t s);
 // kNN screen operands: rows are padded to knn_rows_padded(n) and stored in a

@@ -71,12 +71,13 @@ typedef double f64x4_ch __attribute__((ext_vector_type(4)));
 // consecutive.  Wave w takes rows r0 + 16 w .. + 15 on v_mfma_f64_16x16x4f64 (lane l
 // supplies P[l & 15][l >> 4] = A(row, k) and Q[l >> 4][l & 15] = Y(k, col), and holds
 // D[4 v + (l >> 4)][l & 15] in register v).
-__global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, const double *__restrict__ Y, int dp,
-                                                 int n, int kc, double *__restrict__ out) {
+// One workgroup's tile: rows r0 .. r0 + 63, columns c0 .. c0 + 31 of A Y over k0 .. k0 +
+// kc - 1, into o (column-major, dp rows).
+__device__ inline void ch_gemm_tile(const double *__restrict__ A, const double *__restrict__ Y, int dp, int n, int r0,
+                                    int k0, int kc, int c0, double *__restrict__ o) {
     constexpr int KS = 32, NC = 32, LA = 64 + 4, LY = NC + 4;  // k per stage, columns; LDS row lengths
     __shared__ double As[2][KS * LA], Ys[2][KS * LY];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
-    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * kc, c0 = blockIdx.z * NC;
     // this thread's share of a stage: A row ar, k offsets 8 ap .. 8 ap + 7; Y column
     // c0 + yc, k offsets 8 yp .. 8 yp + 7 for the first 128 threads... four double2 each
     const int ar = tid >> 2, ap = tid & 3;
@@ -125,7 +126,6 @@ __global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, c
             cur ^= 1;
         }
     }
-    double *o = out + (int64_t)blockIdx.y * dp * n;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int col = c0 + 16 * t + li;
@@ -133,6 +133,11 @@ __global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, c
 #pragma unroll
             for (int v = 0; v < 4; ++v) o[(int64_t)col * dp + r0 + 16 * w + 4 * v + lk] = acc[t][v];
     }
+}
+
+__global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, const double *__restrict__ Y, int dp,
+                                                 int n, int kc, double *__restrict__ out) {
+    ch_gemm_tile(A, Y, dp, n, blockIdx.x * 64, blockIdx.y * kc, kc, blockIdx.z * 32, out + (int64_t)blockIdx.y * dp * n);
 }
 
 // 1 / x and 1 / sqrt(x) in FP64 from the hardware estimates and one Newton step
@@ -160,21 +165,25 @@ constexpr int kChCholThreads = 256;
 // barrier per column of the right-looking factorization and per row of the inversion
 // (Rt^-1 from the last row up); lanes run along the contiguous index and each thread
 // loads a batch of operands before it stores.  LDS 2 k^2 doubles.
-__global__ __launch_bounds__(kChCholThreads) void k_ch_chol_rinv(const double *Gp, int S, int k, double *Rinv,
-                                                                double *bad) {
+// G is read with leading dimension ld (partials pstride apart) and Rinv written with
+// leading dimension ld, zero in rows and columns k .. ld - 1 (ld = k for one matrix).
+__device__ inline void ch_chol_rinv(const double *Gp, int S, int k, int64_t pstride, int ld, double *Rinv,
+                                    double *bad) {
     extern __shared__ double lds[];
     double *L = lds;          // [k][k] column-major: Schur complements; then X = Rt^-1 (row-major, upper)
     double *F = lds + k * k;  // [k][k] column-major: the lower factor Rt^T
     __shared__ double dsc[kChMaxK], rdg[kChMaxK];  // column scales; pivots of Rt
     __shared__ int far_from_identity;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t kk = (int64_t)k * k;
     if (tid == 0) far_from_identity = 0;
     for (int e = tid; e < k * k; e += blockDim.x) {
-        double g = Gp[e];
-        for (int s = 1; s < S; ++s) g += Gp[s * kk + e];
+        const int64_t ge = (int64_t)(e / k) * ld + e % k;
+        double g = Gp[ge];
+        for (int s = 1; s < S; ++s) g += Gp[s * pstride + ge];
         L[e] = g;
     }
+    for (int e = tid; e < ld * ld; e += blockDim.x)  // the padding of Rinv
+        if (e / ld >= k || e % ld >= k) Rinv[e] = 0.0;
     __syncthreads();
     for (int i = tid; i < k; i += blockDim.x) {
         const double g = L[i * k + i];
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(kChCholThreads) void k_ch_chol_rinv(const double *G
         for (int e = tid; e < k * k; e += blockDim.x) {
             const int c = e / k, i = e % k;
             const double x = i < c ? -L[e] : i == c ? 1.0 - 0.5 * (L[e] - 1.0) : 0.0;
-            Rinv[e] = dsc[i] * x;
+            Rinv[(int64_t)c * ld + i] = dsc[i] * x;
         }
         return;
     }
@@ -267,8 +276,13 @@ __global__ __launch_bounds__(kChCholThreads) void k_ch_chol_rinv(const double *G
     }
     for (int e = tid; e < k * k; e += blockDim.x) {
         const int c = e / k, i = e % k;
-        Rinv[e] = i <= c ? dsc[i] * F[e] : 0.0;
+        Rinv[(int64_t)c * ld + i] = i <= c ? dsc[i] * F[e] : 0.0;
     }
+}
+
+__global__ __launch_bounds__(kChCholThreads) void k_ch_chol_rinv(const double *Gp, int S, int k, double *Rinv,
+                                                                double *bad) {
+    ch_chol_rinv(Gp, S, k, (int64_t)k * k, k, Rinv, bad);
 }
 
 // Eigenpairs of the symmetric n x n matrix H = sym(sum of S partials, column-major,
@@ -282,8 +296,10 @@ __global__ __launch_bounds__(kChCholThreads) void k_ch_chol_rinv(const double *G
 // set, not converging within them sets *bad = 1.  Output: theta (descending when desc,
 // else ascending) and Z [n x n] column-major, column j the eigenvector of theta[j].
 // One workgroup; LDS 2 m (m + 1) doubles.
-__global__ __launch_bounds__(kChThreads) void k_ch_jacobi(const double *Hp, int S, int n, int max_sweeps, int strict,
-                                                         int desc, double *theta, double *Z, double *bad) {
+// H is read with leading dimension ld and Z written with leading dimension ld, zero in
+// rows and columns n .. ld - 1 (ld = n for one matrix).
+__device__ inline void ch_jacobi(const double *Hp, int S, int n, int ld_h, int max_sweeps, int strict, int desc,
+                                 double *theta, double *Z, double *bad) {
     extern __shared__ double lds[];
     const int m = n + (n & 1), ld = m + 1, half = m / 2, m1 = m - 1;
     double *A = lds;           // [m][ld], row-major
@@ -299,10 +315,10 @@ __global__ __launch_bounds__(kChThreads) void k_ch_jacobi(const double *Hp, int 
         const int i = e / m, j = e % m;
         double h = 0.0;
         if (i < n && j < n) {
-            double a = Hp[(int64_t)j * n + i], b = Hp[(int64_t)i * n + j];
+            double a = Hp[(int64_t)j * ld_h + i], b = Hp[(int64_t)i * ld_h + j];
             for (int s = 1; s < S; ++s) {
-                a += Hp[s * nn + (int64_t)j * n + i];
-                b += Hp[s * nn + (int64_t)i * n + j];
+                a += Hp[s * nn + (int64_t)j * ld_h + i];
+                b += Hp[s * nn + (int64_t)i * ld_h + j];
             }
             h = 0.5 * (a + b);
         }
@@ -432,8 +448,14 @@ __global__ __launch_bounds__(kChThreads) void k_ch_jacobi(const double *Hp, int 
             rank += (desc ? dj > di : dj < di) || (dj == di && j < i);
         }
         theta[rank] = di;
-        for (int r = 0; r < n; ++r) Z[(int64_t)rank * n + r] = V[r * ld + i];
+        for (int r = 0; r < ld_h; ++r) Z[(int64_t)rank * ld_h + r] = r < n ? V[r * ld + i] : 0.0;
     }
+    for (int e = tid; e < (ld_h - n) * ld_h; e += blockDim.x) Z[(int64_t)n * ld_h + e] = 0.0;
+}
+
+__global__ __launch_bounds__(kChThreads) void k_ch_jacobi(const double *Hp, int S, int n, int max_sweeps, int strict,
+                                                         int desc, double *theta, double *Z, double *bad) {
+    ch_jacobi(Hp, S, n, n, max_sweeps, strict, desc, theta, Z, bad);
 }
 
 // out[j] = || cv[:, j] - theta[j] v[:, j] ||_2 over d rows (leading dimension ld), one
@@ -468,6 +490,193 @@ __global__ void k_ch_lock(const double *src, const double *src_val, int ld, int 
         dst[e] = v;
         dst_scaled[e] = v * t;
         if (i == 0) dst_val[j] = t;
+    }
+}
+
+// ---------------------------------------------------------------- batches
+// The same iteration for many matrices side by side (chefsi_topp_batch): matrix z's
+// operator at A + z dp^2, its blocks at buf + z dp K (K = the block size, column-major,
+// dp rows), its small matrices at G + z K^2, its statistics at stat + z (2K + 2).  The
+// per-matrix state of the current iteration is a ChbMat: ka active columns (0: the
+// matrix takes no part), the filter degree m, the first active column voff of the
+// Ritz block after locking, the locked count L, the columns J locked at the start of
+// this iteration, and the filter's centre c and half-width e.
+
+// rows r0.. x columns c0.. of A_z Y_z[:, yoff ..] for the matrices whose product is
+// live at this step (step < m; step < 0: any active matrix)
+__global__ __launch_bounds__(256) void k_chb_gemm(const double *__restrict__ A, const double *__restrict__ Y,
+                                                  int dp, int K, const ChbMat *P, int step, int use_voff,
+                                                  double *__restrict__ out) {
+    const int z = blockIdx.z;
+    const ChbMat m = P[z];
+    const int n = (m.ka > 0 && (step < 0 || step < m.m)) ? m.ka : 0;
+    const int c0 = blockIdx.y * 32;
+    if (c0 >= n) return;
+    const int64_t dk = (int64_t)dp * K;
+    ch_gemm_tile(A + (int64_t)z * dp * dp, Y + z * dk + (use_voff ? (int64_t)m.voff * dp : 0), dp, n, blockIdx.x * 64, 0,
+                 dp, c0, out + z * dk);
+}
+
+// The three-term recurrence for every matrix: step 0: y1 = (w - c y) / e (w = A y,
+// both at column voff); step j >= 1 while j < m: y_next = 2 (w - c y) / e - y_prev
+// (y_prev at column voff when prev_voff); j >= m: y_next = y (the matrix's filter is
+// done; its result moves along with the buffer rotation).  Columns at or past ka are
+// set to 0.
+__global__ void k_chb_combine(double *y_next, const double *w, const double *y, const double *y_prev, int dp, int K,
+                              int nm, const ChbMat *P, int step, int prev_voff) {
+    const int64_t dk = (int64_t)dp * K, total = dk * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dk);
+        const int64_t r = e % dk;
+        const int j = (int)(r / dp);
+        const ChbMat m = P[z];
+        double v = 0.0;
+        if (j < m.ka) {
+            const int64_t base = z * dk;
+            if (step == 0) {
+                const int64_t o = base + r + (int64_t)m.voff * dp;
+                v = (w[o] - m.c * y[o]) * (1.0 / m.e);
+            } else if (step < m.m) {
+                const double yp = y_prev[base + r + (prev_voff ? (int64_t)m.voff * dp : 0)];
+                v = (2.0 / m.e) * (w[base + r] - m.c * y[base + r]) - yp;
+            } else {
+                v = y[base + r];
+            }
+        }
+        y_next[e] = v;
+    }
+}
+
+// the columns of an active matrix at or past ka set to 0 (so that no stale value meets
+// a padded factor in a product)
+__global__ void k_chb_zero_tail(double *y, int dp, int K, int nm, const ChbMat *P) {
+    const int64_t dk = (int64_t)dp * K, total = dk * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dk);
+        if ((int)((e % dk) / dp) >= P[z].ka) y[e] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kChCholThreads) void k_chb_chol_rinv(const double *G, int K, const ChbMat *P,
+                                                                 double *Rinv, double *stat) {
+    const int z = blockIdx.x, ka = P[z].ka;
+    if (ka == 0) return;
+    const int64_t kk = (int64_t)K * K;
+    ch_chol_rinv(G + z * kk, 1, ka, kk, K, Rinv + z * kk, stat + (int64_t)z * (2 * K + 2) + 2 * K);
+}
+
+__global__ __launch_bounds__(kChThreads) void k_chb_jacobi(const double *H, int K, const ChbMat *P, int max_sweeps,
+                                                          int strict, int desc, double *Z, double *stat) {
+    const int z = blockIdx.x, ka = P[z].ka;
+    if (ka == 0) return;
+    const int64_t kk = (int64_t)K * K;
+    double *st = stat + (int64_t)z * (2 * K + 2);
+    ch_jacobi(H + z * kk, 1, ka, K, max_sweeps, strict, desc, st, Z + z * kk, st + 2 * K + 1);
+}
+
+// residual norms || cv[:, j] - theta_j v[:, j] || over d rows, j < ka
+__global__ __launch_bounds__(256) void k_chb_resid(const double *cv, const double *v, int d, int dp, int K,
+                                                  const ChbMat *P, double *stat) {
+    __shared__ double part[256];
+    const int j = blockIdx.x, z = blockIdx.y;
+    if (j >= P[z].ka) return;
+    double *st = stat + (int64_t)z * (2 * K + 2);
+    const int64_t o = (int64_t)z * dp * K + (int64_t)j * dp;
+    const double t = st[j];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        const double r = cv[o + i] - t * v[o + i];
+        s += r * r;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st[K + j] = sqrt(part[0]);
+}
+
+// Locking of the first J columns of the Ritz block v (values in stat): QL[:, L + j] =
+// v[:, j], thL[L + j] = theta_j; the deflation's factors wl[:, j] = theta_j v[:, j] and
+// ql[:, j] = v[:, j] for j < J, zero elsewhere (so A -= wl ql^T over all K columns
+// removes exactly the J new directions).
+__global__ void k_chb_lock(const double *v, const double *stat, int dp, int K, int p, int nm, const ChbMat *P,
+                           double *QL, double *thL, double *wl, double *ql) {
+    const int64_t dk = (int64_t)dp * K, total = dk * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dk);
+        const int64_t r = e % dk;
+        const int j = (int)(r / dp), i = (int)(r % dp);
+        const ChbMat m = P[z];
+        double a = 0.0, b = 0.0;
+        if (j < m.J) {
+            const double t = stat[(int64_t)z * (2 * K + 2) + j], x = v[e];
+            a = x * t;
+            b = x;
+            QL[(int64_t)z * dp * p + (int64_t)(m.L + j) * dp + i] = x;
+            if (i == 0) thL[(int64_t)z * p + m.L + j] = t;
+        }
+        wl[e] = a;
+        ql[e] = b;
+    }
+}
+
+// A_z = C_z (d x d, column-major) padded with zeros to dp x dp, for the matrices with
+// ka > 0
+__global__ void k_chb_pad(const double *C, int d, int dp, int nm, const ChbMat *P, double *A) {
+    const int64_t dd = (int64_t)dp * dp, total = dd * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dd);
+        if (P[z].ka == 0) continue;
+        const int64_t r = e % dd;
+        const int c = (int)(r / dp), i = (int)(r % dp);
+        A[e] = (c < d && i < d) ? C[(int64_t)z * d * d + (int64_t)c * d + i] : 0.0;
+    }
+}
+
+// dst[:, j] = src[:, off + j] for j < ka of the matrices with ka > 0 (off = voff when
+// use_voff; src leading dimension per matrix: src_cols columns of dp rows)
+__global__ void k_chb_copy(double *dst, const double *src, int src_cols, int dp, int K, int nm, const ChbMat *P,
+                           int use_voff) {
+    const int64_t dk = (int64_t)dp * K, total = dk * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dk);
+        const int64_t r = e % dk;
+        const int j = (int)(r / dp);
+        const ChbMat m = P[z];
+        if (m.ka == 0 || j >= m.ka) continue;
+        dst[e] = src[(int64_t)z * dp * src_cols + (int64_t)(j + (use_voff ? m.voff : 0)) * dp + r % dp];
+    }
+}
+
+// the final p vectors (d rows) and values of each matrix, compact for the selection
+__global__ void k_chb_final(const double *v, const double *stat, int d, int dp, int K, int p, int nm, double *vout,
+                            double *evals) {
+    const int64_t dpp = (int64_t)d * p, total = dpp * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / dpp);
+        const int64_t r = e % dpp;
+        const int j = (int)(r / d), i = (int)(r % d);
+        vout[e] = v[(int64_t)z * dp * K + (int64_t)j * dp + i];
+        if (i == 0) evals[(int64_t)z * p + j] = stat[(int64_t)z * (2 * K + 2) + j];
+    }
+}
+
+// the deterministic start block of chefsi_topp, the same for every matrix
+__global__ void k_chb_rand(double *v, int rows, int ld, int cols, int nm, uint64_t seed) {
+    const int64_t n = (int64_t)ld * cols, total = n * nm;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = e % n, r = q % ld, c = q / ld;
+        double out = 0.0;
+        if (r < rows) {
+            uint64_t x = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(c * rows + r + 1);
+            x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+            x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+            x ^= x >> 31;
+            out = (double)(x >> 11) * (1.0 / 4503599627370496.0) - 1.0;
+        }
+        v[e] = out;
     }
 }
 
@@ -509,6 +718,60 @@ void launch_ch_gemm(const double *A, const double *Y, int dp, int n, int S, doub
 void launch_ch_resid(const double *cv, const double *v, const double *theta, int d, int ld, int k, double *out,
                      hipStream_t s) {
     hipLaunchKernelGGL(k_ch_resid, dim3(k), dim3(256), 0, s, cv, v, theta, d, ld, out);
+}
+static int chb_grid(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 16384); }
+void chefsi_batch_set_attrs() {
+    hipFuncSetAttribute((const void *)k_chb_chol_rinv, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)ch_chol_lds(kChMaxK));
+    hipFuncSetAttribute((const void *)k_chb_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)ch_jacobi_lds(kChMaxK));
+}
+void launch_chb_gemm(const double *A, const double *Y, int dp, int K, int nm, const ChbMat *P, int step, bool use_voff,
+                     double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_gemm, dim3(dp / 64, (K + 31) / 32, nm), dim3(256), 0, s, A, Y, dp, K, P, step,
+                       (int)use_voff, out);
+}
+void launch_chb_combine(double *y_next, const double *w, const double *y, const double *y_prev, int dp, int K, int nm,
+                        const ChbMat *P, int step, bool prev_voff, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_combine, dim3(chb_grid((int64_t)dp * K * nm)), dim3(256), 0, s, y_next, w, y, y_prev, dp,
+                       K, nm, P, step, (int)prev_voff);
+}
+void launch_chb_zero_tail(double *y, int dp, int K, int nm, const ChbMat *P, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_zero_tail, dim3(chb_grid((int64_t)dp * K * nm)), dim3(256), 0, s, y, dp, K, nm, P);
+}
+void launch_chb_chol_rinv(const double *G, int K, int nm, const ChbMat *P, double *Rinv, double *stat, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_chol_rinv, dim3(nm), dim3(kChCholThreads), ch_chol_lds(K), s, G, K, P, Rinv, stat);
+}
+void launch_chb_jacobi(const double *H, int K, int nm, const ChbMat *P, int max_sweeps, bool strict, bool desc,
+                       double *Z, double *stat, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_jacobi, dim3(nm), dim3(kChThreads), ch_jacobi_lds(K), s, H, K, P, max_sweeps,
+                       (int)strict, (int)desc, Z, stat);
+}
+void launch_chb_resid(const double *cv, const double *v, int d, int dp, int K, int nm, const ChbMat *P, double *stat,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_resid, dim3(K, nm), dim3(256), 0, s, cv, v, d, dp, K, P, stat);
+}
+void launch_chb_lock(const double *v, const double *stat, int dp, int K, int p, int nm, const ChbMat *P, double *QL,
+                     double *thL, double *wl, double *ql, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_lock, dim3(chb_grid((int64_t)dp * K * nm)), dim3(256), 0, s, v, stat, dp, K, p, nm, P, QL,
+                       thL, wl, ql);
+}
+void launch_chb_pad(const double *C, int d, int dp, int nm, const ChbMat *P, double *A, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_pad, dim3(chb_grid((int64_t)dp * dp * nm)), dim3(256), 0, s, C, d, dp, nm, P, A);
+}
+void launch_chb_copy(double *dst, const double *src, int src_cols, int dp, int K, int nm, const ChbMat *P,
+                     bool use_voff, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_copy, dim3(chb_grid((int64_t)dp * K * nm)), dim3(256), 0, s, dst, src, src_cols, dp, K,
+                       nm, P, (int)use_voff);
+}
+void launch_chb_final(const double *v, const double *stat, int d, int dp, int K, int p, int nm, double *vout,
+                      double *evals, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_final, dim3(chb_grid((int64_t)d * p * nm)), dim3(256), 0, s, v, stat, d, dp, K, p, nm, vout,
+                       evals);
+}
+void launch_chb_rand(double *v, int rows, int ld, int cols, int nm, uint64_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_chb_rand, dim3(chb_grid((int64_t)ld * cols * nm)), dim3(256), 0, s, v, rows, ld, cols, nm,
+                       seed);
 }
 void launch_ch_lock(const double *src, const double *src_val, int ld, int n, double *dst, double *dst_val,
                     double *dst_scaled, hipStream_t s) {

@@ -241,6 +241,7 @@ def test_batch_tridiagonalization_kernel_matches_rocsolver(oracle, monkeypatch, 
     indptr, indices, data, off = synth.make_subsets(sizes, genes, 10, density=0.07, seed=seed)
     csr = DeviceCSR.upload(indptr, indices, data, genes)
     res = {}
+    monkeypatch.setenv("CREST_BATCH_EIG", "tri")  # the covariances through the tridiagonalization too
     for mode in ("rocsolver", "wg"):
         monkeypatch.setenv("CREST_SYTRD", mode)
         p = BatchPipeline(csr, off, np.arange(genes, dtype=np.int32), n_comps=30, k=15, prec_gram="fp64",
@@ -254,3 +255,66 @@ def test_batch_tridiagonalization_kernel_matches_rocsolver(oracle, monkeypatch, 
         assert _angle(oracle, o.loadings[i], r.loadings[i], 9) < 1e-9, i
         V = o.loadings[i]
         np.testing.assert_allclose(V.T @ V, np.eye(V.shape[1]), atol=1e-10)
+
+
+# Orders from 512 up take the filtered subspace iteration, all covariances of a batch
+# side by side; these subsets are all above the dual buckets below 640 genes.
+G3 = 640
+SIZES3 = [600, 900, 1500, 2200, 3100, 700, 1300, 4000]
+
+
+@pytest.mark.parametrize("prec,pcs", [("fp64", 30), ("fp16", 30), ("fp64", 50)])
+def test_batch_filtered_subspace_matches_tridiagonal(oracle, monkeypatch, capfd, prec, pcs):
+    """The batched filtered subspace iteration (default from order 512) against the
+    tridiagonal path of the same batch (CREST_BATCH_EIG=tri) and, in FP64, the oracle's
+    eigensolve of a subset's Gram: every matrix solved in the batch, variances, totals,
+    orthonormal loadings and the leading subspaces."""
+    from crest_amd.pipeline import BatchPipeline, DeviceCSR
+    indptr, indices, data, off = synth.make_subsets(SIZES3, G3, 10, density=0.06, seed=13)
+    csr = DeviceCSR.upload(indptr, indices, data, G3)
+    res = {}
+    monkeypatch.setenv("CREST_CHEFSI_TRACE", "1")
+    for mode in ("chefsi", "tri"):
+        if mode == "tri":
+            monkeypatch.setenv("CREST_BATCH_EIG", "tri")
+        else:
+            monkeypatch.delenv("CREST_BATCH_EIG", raising=False)
+        print(f"[batch-chefsi] {prec} p={pcs} {mode}", flush=True)
+        p = BatchPipeline(csr, off, np.arange(G3, dtype=np.int32), n_comps=pcs, k=15, prec_gram=prec,
+                          prec_knn="fp32")
+        res[mode] = (p.run(), p.x_pca())
+        if mode == "chefsi":
+            err = capfd.readouterr().err
+            assert f"[chefsi-batch] {len(SIZES3)} matrices: {len(SIZES3)} solved" in err, err[-3000:]
+    o, X = res["chefsi"]
+    r, Xr = res["tri"]
+    assert np.isfinite(o.variance).all() and np.isfinite(o.loadings).all()
+    np.testing.assert_allclose(o.total_variance, r.total_variance, rtol=1e-14)
+    np.testing.assert_allclose(o.variance, r.variance, rtol=1e-10, atol=1e-12 * r.variance.max())
+    for i in range(len(SIZES3)):
+        V = o.loadings[i]
+        np.testing.assert_allclose(V.T @ V, np.eye(pcs), atol=1e-11)
+        assert _angle(oracle, V, r.loadings[i], 9) < 1e-9, i
+        a, b = int(off[i]), int(off[i + 1])
+        sign = np.sign((X[a:b, :9] * Xr[a:b, :9]).sum(0))
+        np.testing.assert_allclose(X[a:b, :9] * sign, Xr[a:b, :9], atol=1e-4 * np.abs(Xr[a:b, :9]).max())
+    if prec != "fp64":
+        return
+    gmap = np.arange(G3, dtype=np.int32)
+    for i in (0, 4, 7):
+        a, b = int(off[i]), int(off[i + 1])
+        n = b - a
+        ip = indptr[a:b + 1] - indptr[a]
+        ctx = (indices[indptr[a]:indptr[b]], data[indptr[a]:indptr[b]], np.arange(n, dtype=np.int64), gmap, 1e4,
+               True, None, ip, 0)
+        st = np.zeros((G3, 5))
+        oracle.gene_stats(*ctx, st)
+        mean = st[:, 0] / n
+        sd = np.sqrt(np.maximum((st[:, 1] - n * mean * mean) / (n - 1), 0))
+        sd[sd == 0] = 1
+        gram, colsum = np.zeros((G3, G3)), np.zeros(G3)
+        oracle.gram_accumulate(*ctx, gmap, gram, colsum, True, mean, sd, 10.0, threads=4)
+        var_o, load_o, tot_o = oracle.pca_from_gram(gram, colsum, n, pcs)
+        np.testing.assert_allclose(o.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
+        assert abs(o.total_variance[i] - tot_o) <= 1e-11 * tot_o
+        assert _angle(oracle, o.loadings[i], load_o, 9) < 1e-9, i
