@@ -135,6 +135,80 @@ __device__ inline void ch_gemm_tile(const double *__restrict__ A, const double *
     }
 }
 
+// The same product for a whole block of up to 16 NG columns per workgroup (rows r0 ..
+// r0 + 63, all K = dp): each row panel of A is read once for all the block's columns.
+// Stage loads: Y(k0 .. k0 + 31, c) for the 16 NG columns as 16 NG x 16 double2, thread
+// t taking double2 u 256 + t (16 threads per column, its 256 contiguous bytes).
+template <int NG>
+__device__ inline void ch_gemm_panel(const double *__restrict__ A, const double *__restrict__ Y, int dp, int n,
+                                     int r0, double *__restrict__ o) {
+    constexpr int KS = 32, NC = 16 * NG, LA = 64 + 4, LY = NC + 4;
+    __shared__ double As[2][KS * LA], Ys[2][KS * LY];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+    const int ar = tid >> 2, ap = tid & 3;
+    const double2 *ga = reinterpret_cast<const double2 *>(A + (int64_t)(r0 + ar) * dp + 8 * ap);
+    const double2 *gy[NG];
+    bool yok[NG];
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
+        const int q = u * 256 + tid, yc = q >> 4, kp = q & 15;
+        yok[u] = yc < n;
+        gy[u] = reinterpret_cast<const double2 *>(Y + (int64_t)(yok[u] ? yc : 0) * dp + 2 * kp);
+    }
+    double2 ra[4], ry[NG];
+    auto fetch = [&](int kb) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ra[u] = ga[kb / 2 + u];
+#pragma unroll
+        for (int u = 0; u < NG; ++u) ry[u] = yok[u] ? gy[u][kb / 2] : make_double2(0.0, 0.0);
+    };
+    auto stash = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            As[buf][(8 * ap + 2 * u) * LA + ar] = ra[u].x;
+            As[buf][(8 * ap + 2 * u + 1) * LA + ar] = ra[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < NG; ++u) {
+            const int q = u * 256 + tid, yc = q >> 4, kp = q & 15;
+            Ys[buf][(2 * kp) * LY + yc] = ry[u].x;
+            Ys[buf][(2 * kp + 1) * LY + yc] = ry[u].y;
+        }
+    };
+    f64x4_ch acc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) acc[g] = (f64x4_ch){0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    int cur = 0;
+    for (int kb = 0; kb < dp; kb += KS) {
+        const bool more = kb + KS < dp;
+        if (more) fetch(kb + KS);
+        const double *as = As[cur], *ys = Ys[cur];
+#pragma unroll
+        for (int st = 0; st < KS / 4; ++st) {
+            const int kr = 4 * st + lk;
+            const double a = as[kr * LA + 16 * w + li];
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ys[kr * LY + 16 * g + li], acc[g], 0, 0, 0);
+        }
+        if (more) {
+            stash(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int col = 16 * g + li;
+        if (col < n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[(int64_t)col * dp + r0 + 16 * w + 4 * v + lk] = acc[g][v];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, const double *__restrict__ Y, int dp,
                                                  int n, int kc, double *__restrict__ out) {
     ch_gemm_tile(A, Y, dp, n, blockIdx.x * 64, blockIdx.y * kc, kc, blockIdx.z * 32, out + (int64_t)blockIdx.y * dp * n);
@@ -504,17 +578,18 @@ __global__ void k_ch_lock(const double *src, const double *src_val, int ld, int 
 
 // rows r0.. x columns c0.. of A_z Y_z[:, yoff ..] for the matrices whose product is
 // live at this step (step < m; step < 0: any active matrix)
+// (one workgroup per 64-row panel of each matrix, all its active columns)
+template <int NG>
 __global__ __launch_bounds__(256) void k_chb_gemm(const double *__restrict__ A, const double *__restrict__ Y,
                                                   int dp, int K, const ChbMat *P, int step, int use_voff,
                                                   double *__restrict__ out) {
-    const int z = blockIdx.z;
+    const int z = blockIdx.y;
     const ChbMat m = P[z];
     const int n = (m.ka > 0 && (step < 0 || step < m.m)) ? m.ka : 0;
-    const int c0 = blockIdx.y * 32;
-    if (c0 >= n) return;
+    if (n == 0) return;
     const int64_t dk = (int64_t)dp * K;
-    ch_gemm_tile(A + (int64_t)z * dp * dp, Y + z * dk + (use_voff ? (int64_t)m.voff * dp : 0), dp, n, blockIdx.x * 64, 0,
-                 dp, c0, out + z * dk);
+    ch_gemm_panel<NG>(A + (int64_t)z * dp * dp, Y + z * dk + (use_voff ? (int64_t)m.voff * dp : 0), dp, n,
+                      blockIdx.x * 64, out + z * dk);
 }
 
 // The three-term recurrence for every matrix: step 0: y1 = (w - c y) / e (w = A y,
@@ -728,8 +803,14 @@ void chefsi_batch_set_attrs() {
 }
 void launch_chb_gemm(const double *A, const double *Y, int dp, int K, int nm, const ChbMat *P, int step, bool use_voff,
                      double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_chb_gemm, dim3(dp / 64, (K + 31) / 32, nm), dim3(256), 0, s, A, Y, dp, K, P, step,
-                       (int)use_voff, out);
+    const dim3 g(dp / 64, nm), b(256);
+    switch ((K + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_chb_gemm<1>, g, b, 0, s, A, Y, dp, K, P, step, (int)use_voff, out); break;
+    case 2: hipLaunchKernelGGL(k_chb_gemm<2>, g, b, 0, s, A, Y, dp, K, P, step, (int)use_voff, out); break;
+    case 3: hipLaunchKernelGGL(k_chb_gemm<3>, g, b, 0, s, A, Y, dp, K, P, step, (int)use_voff, out); break;
+    case 4: hipLaunchKernelGGL(k_chb_gemm<4>, g, b, 0, s, A, Y, dp, K, P, step, (int)use_voff, out); break;
+    default: hipLaunchKernelGGL(k_chb_gemm<5>, g, b, 0, s, A, Y, dp, K, P, step, (int)use_voff, out); break;
+    }
 }
 void launch_chb_combine(double *y_next, const double *w, const double *y, const double *y_prev, int dp, int K, int nm,
                         const ChbMat *P, int step, bool prev_voff, hipStream_t s) {
