@@ -459,30 +459,38 @@ __global__ __launch_bounds__(256) void k_project(DevChunk c, DevScale sc, const 
                 }
             }
             // Four entries at a time: their V rows are loaded before any of the four
-            // multiply-adds.  An entry outside the Gram contributes an explicit +0.0,
-            // which leaves the sum unchanged (it starts at +0.0 and, rounding to
-            // nearest, never becomes -0.0), so the result equals skipping it.
-            const int cnt = hi - e0 < kWave ? (int)(hi - e0) : kWave;
+            // multiply-adds.  An entry's HVG index and scaled value are read from its
+            // lane into scalar registers (wave-uniform), so the V row address is
+            // scalar and the lanes only load and multiply-add; lanes past k load
+            // column k - 1 and are never stored.  An entry outside the Gram is given
+            // the value 0, so it adds a signed zero, which leaves the sum unchanged
+            // (the sum starts at +0.0 and, rounding to nearest, never becomes -0.0),
+            // so the result equals skipping it.
+            // (cnt is wave-uniform: the span is the wave's)
+            const int cnt = __builtin_amdgcn_readfirstlane(hi - e0 < kWave ? (int)(hi - e0) : kWave);
+            const int s_lo = (int)(__double_as_longlong(s) & 0xffffffffll), s_hi = (int)(__double_as_longlong(s) >> 32);
             for (int j0 = 0; j0 < cnt; j0 += 4) {
-                int32_t hj[4];
                 double sj[4], vj[4][NC];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int j = j0 + u < cnt ? j0 + u : cnt - 1;
-                    hj[u] = j0 + u < cnt ? __shfl(h, j, kWave) : -1;
-                    sj[u] = __shfl(s, j, kWave);
-                    const double *vr = V + (int64_t)(hj[u] >= 0 ? hj[u] : 0) * k;
+                    const int32_t hr = __builtin_amdgcn_readlane(h, j);
+                    const bool use = j0 + u < cnt && hr >= 0;
+                    const double sr = __longlong_as_double(((long long)__builtin_amdgcn_readlane(s_hi, j) << 32) |
+                                                           (unsigned int)__builtin_amdgcn_readlane(s_lo, j));
+                    sj[u] = use ? sr : 0.0;
+                    const double *vr = V + (int64_t)(use ? hr : 0) * k;
 #pragma unroll
                     for (int q = 0; q < NC; ++q) {
                         const int col = lane + q * kWave;
-                        vj[u][q] = col < k ? vr[col] : 0.0;
+                        vj[u][q] = vr[col < k ? col : k - 1];
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
                     for (int q = 0; q < NC; ++q) {
-                        const double prod = hj[u] >= 0 ? sj[u] * vj[u][q] : 0.0;
+                        const double prod = sj[u] * vj[u][q];
                         acc[q] = acc[q] + prod;
                     }
             }
