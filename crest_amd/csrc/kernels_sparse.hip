@@ -19,11 +19,13 @@
 // banks).  For each group of its split, each wave takes 4 rows, one per 16-lane row of
 // the wave: lane t loads a-entry t and b-entry t of its row straight into registers
 // (the next group's rows loading while this group's pairs run), and step i broadcasts
-// a-entry i across the row's lanes (DPP row_newbcast), every lane adding s_a s_b into
-// the block with an LDS FP64 atomic add (diagonal blocks: pairs with col_a <= col_b,
-// each entry's s into the colsum).  The adds are the only LDS traffic and no result
-// feeds another, so they issue back to back.  The block and colsum go
-// to a per-split partial slot, reduced over splits in order afterwards.  The order of
+// a-entry i (its value and its block-row offset) across the row's lanes (DPP
+// row_newbcast), every lane adding s_a s_b into the block with an LDS FP64 atomic add.
+// A diagonal block adds both orders of every pair and drops its lower triangle when
+// it is written out, so no lane compares columns; its entries' s go into the colsum
+// once each.  The adds are the only LDS traffic and no result feeds another, so they
+// issue back to back.  The block (upper triangle, as the reference accumulates it) and
+// colsum go to a per-split partial slot, reduced over splits in order afterwards.  The order of
 // the adds within one workgroup is the hardware's, so results agree with any other
 // order to FP64 rounding (not bitwise between runs).
 #include "common.hpp"
@@ -151,32 +153,31 @@ __global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc0, i
     }
 }
 
-// lane n of each row of 16 lanes, to every lane of that row (DPP row_newbcast)
-template <int N> __device__ inline int row_bcast(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x150 + N, 0xf, 0xf, false); }
+// lane n of each row of 16 lanes, to every lane of that row (DPP row_newbcast; every
+// lane is written, so no previous value is needed)
+template <int N> __device__ inline int row_bcast(int v) { return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xf, 0xf, false); }
 template <int N> __device__ inline double row_bcast_f64(double v) {
     const long long u = __double_as_longlong(v);
     const int lo = row_bcast<N>((int)(u & 0xffffffffll)), hi = row_bcast<N>((int)(u >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Pairs of one 16 x 16 chunk: lane t of row group r holds a-entry (sa, ca) and
-// b-entry (sb, cb) t of its row; step i adds a-entry i (broadcast over the row's 16
-// lanes) times every b-entry of the row.  na: the row's a-entries in this chunk.
-template <bool DIAG, int I>
-__device__ inline void sp_step(double *G, double *cs, double sa, int ca, double sb, int cb, bool ok_b, int na) {
+// Pairs of one 16 x 16 chunk: lane t of row group r holds a-entry t (value sa, byte
+// offset ao = 8 kSpLd column of its block row) and b-entry t (value sb; gb = its
+// column's element of block row 0) of its row; step i adds a-entry i (broadcast over
+// the row's 16 lanes) times every b-entry of the row.  na: the row's a-entries in this
+// chunk.  In a diagonal block both orders of every pair are added (the block's lower
+// triangle is dropped when the block is written out), so no lane tests its columns.
+template <int I>
+__device__ inline void sp_step(double *gb, double sa, int ao, double sb, bool ok_b, int na) {
     const double ai = row_bcast_f64<I>(sa);
-    const int ci = row_bcast<I>(ca);
-    if (ok_b && I < na && (!DIAG || ci <= cb)) {
-        atomicAdd(&G[ci * kSpLd + cb], ai * sb);
-        if (DIAG && ci == cb) atomicAdd(&cs[cb], sb);
-    }
+    const int ai_off = row_bcast<I>(ao);
+    if (ok_b && I < na) atomicAdd(reinterpret_cast<double *>(reinterpret_cast<char *>(gb) + ai_off), ai * sb);
 }
-template <bool DIAG>
-__device__ inline void sp_chunk(double *G, double *cs, double sa, int ca, double sb, int cb, bool ok_b, int na,
-                                int n_i) {
-    sp_step<DIAG, 0>(G, cs, sa, ca, sb, cb, ok_b, na);
+__device__ inline void sp_chunk(double *gb, double sa, int ao, double sb, bool ok_b, int na, int n_i) {
+    sp_step<0>(gb, sa, ao, sb, ok_b, na);
 #define CREST_SP_STEP(I) \
-    if (n_i > I) sp_step<DIAG, I>(G, cs, sa, ca, sb, cb, ok_b, na);
+    if (n_i > I) sp_step<I>(gb, sa, ao, sb, ok_b, na);
     CREST_SP_STEP(1) CREST_SP_STEP(2) CREST_SP_STEP(3) CREST_SP_STEP(4) CREST_SP_STEP(5) CREST_SP_STEP(6)
     CREST_SP_STEP(7) CREST_SP_STEP(8) CREST_SP_STEP(9) CREST_SP_STEP(10) CREST_SP_STEP(11) CREST_SP_STEP(12)
     CREST_SP_STEP(13) CREST_SP_STEP(14) CREST_SP_STEP(15)
@@ -235,6 +236,7 @@ __device__ inline void sp_pairs(double *G, double *cs, const SpRows &R, const do
             sa = h ? vals[R.a_at + ac + t] : 0.0;
             ca = h ? (int)cols[R.a_at + ac + t] : 0;
         }
+        const int ao = ca * (kSpLd * (int)sizeof(double));
         const int na_c = R.na - ac, n_i = min(mx_a - ac, 16);
         for (int bc = 0; bc < mx_b; bc += 16) {
             double sb = R.sb;
@@ -244,13 +246,16 @@ __device__ inline void sp_pairs(double *G, double *cs, const SpRows &R, const do
                 sb = h ? vals[R.b_at + bc + t] : 0.0;
                 cb = h ? (int)cols[R.b_at + bc + t] : 0;
             }
-            sp_chunk<DIAG>(G, cs, sa, ca, sb, cb, bc + t < R.nbr, na_c, n_i);
+            const bool ok_b = bc + t < R.nbr;
+            // a diagonal block's entries are its column sums' terms, each once
+            if (DIAG && ac == 0 && ok_b) atomicAdd(&cs[cb], sb);
+            sp_chunk(G + cb, sa, ao, sb, ok_b, na_c, n_i);
         }
     }
 }
 
 template <bool DIAG>
-__device__ void sp_block(double *G, double *cs, const double *vals, const uint8_t *cols, const uint32_t *meta,
+__device__ __attribute__((always_inline)) inline void sp_block(double *G, double *cs, const double *vals, const uint8_t *cols, const uint32_t *meta,
                          const int64_t *span_off, int64_t base, const SegDesc *grp, int rw, int nb, int bi, int bj,
                          int64_t g0, int64_t g1, int wave, int lane) {
     const int row = 4 * wave + (lane >> 4), t = lane & 15;
@@ -297,8 +302,10 @@ __global__ __launch_bounds__(kSpThreads) void k_gram_sparse(const double *__rest
         sp_block<false>(G, cs, vals, cols, meta, span_off, base, nullptr, rw, nb, bi, bj, g0, g1, wave, lane);
     __syncthreads();
     double *dst = partial + ((int64_t)blk * n_split + split) * kGramBlock * kGramBlock;
-    for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads)
-        dst[i] = G[(i / kGramBlock) * kSpLd + (i % kGramBlock)];
+    for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads) {
+        const int m = i / kGramBlock, n = i % kGramBlock;
+        dst[i] = (bi == bj && m > n) ? 0.0 : G[m * kSpLd + n];  // upper triangle only, as the reference's
+    }
     if (bi == bj)
         for (int i = tid; i < 128; i += kSpThreads) cs_partial[((int64_t)bi * n_split + split) * 128 + i] = cs[i];
 }
@@ -346,12 +353,14 @@ __global__ __launch_bounds__(kSpThreads) void k_gram_sparse_items(const double *
         double *dst = gram + (int64_t)it.subset * d_pad * d_pad + (int64_t)bi * kGramBlock * d_pad + bj * kGramBlock;
         for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads) {
             const int m = i / kGramBlock, n = i % kGramBlock;
-            dst[(int64_t)m * d_pad + n] += G[m * kSpLd + n];
+            if (bi != bj || m <= n) dst[(int64_t)m * d_pad + n] += G[m * kSpLd + n];
         }
     } else {
         double *dst = partial + it.slot * kGramBlock * kGramBlock;
-        for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads)
-            dst[i] = G[(i / kGramBlock) * kSpLd + (i % kGramBlock)];
+        for (int i = tid; i < kGramBlock * kGramBlock; i += kSpThreads) {
+            const int m = i / kGramBlock, n = i % kGramBlock;
+            dst[i] = (bi == bj && m > n) ? 0.0 : G[m * kSpLd + n];
+        }
     }
     if (bi == bj)
         for (int i = tid; i < 128; i += kSpThreads)
