@@ -162,12 +162,13 @@ template <int N> __device__ inline double row_bcast_f64(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Pairs of one 16 x 16 chunk: lane t of row group r holds a-entry t (value sa, byte
-// offset ao = 8 kSpLd column of its block row) and b-entry t (value sb; gb = its
-// column's element of block row 0) of its row; step i adds a-entry i (broadcast over
-// the row's 16 lanes) times every b-entry of the row.  na: the row's a-entries in this
-// chunk.  In a diagonal block both orders of every pair are added (the block's lower
-// triangle is dropped when the block is written out), so no lane tests its columns.
+// Pairs of one 16 x 16 chunk: lane t of row group r holds broadcast entry t (value sa,
+// byte offset ao of its part of the block address) and lane entry t (value sb; gb =
+// the block element its own part addresses) of its row; step i adds broadcast entry i
+// (over the row's 16 lanes) times every lane entry of the row.  na: the row's broadcast
+// entries in this chunk.  In a diagonal block both orders of every pair are added (the
+// block's lower triangle is dropped when the block is written out), so no lane tests
+// its columns.
 template <int I>
 __device__ inline void sp_step(double *gb, double sa, int ao, double sb, bool ok_b, int na) {
     const double ai = row_bcast_f64<I>(sa);
@@ -191,13 +192,20 @@ __device__ inline int rows_max(int v) {
     return max(max(a, b), max(c, d));
 }
 
-// One wave's view of its 4 rows of a group: row r = lane / 16 of the wave, its
-// a-range and b-range (absolute entry positions) and the first 16 entries of each.
+// One wave's view of its 4 rows of a group: row r = lane / 16 of the wave.  Of the
+// row's a-list (entries of bucket bi: the block's rows) and b-list (bucket bj: its
+// columns) the shorter one is broadcast entry by entry and the longer one spread over
+// the row's 16 lanes (a diagonal block: the a-list both ways), so a row takes
+// min(na, nb) steps with max(na, nb) lanes busy; each row chooses for itself, as the
+// broadcast stays within its 16 lanes.  Per lane: the broadcast list's start, length
+// and first 16 entries, the lane list's the same, and whether the roles are swapped
+// (the broadcast list is the b-list).
 struct SpRows {
-    int64_t a_at, b_at;  // first a- / b-entry of this lane's row
-    int na, nbr;
-    double sa, sb;
-    int ca, cb;
+    int64_t k_at, l_at;  // first broadcast / lane entry of this lane's row
+    int nk, nl;
+    double vk, vl;
+    int ck, cl;
+    bool swap;
 };
 
 template <bool DIAG>
@@ -208,48 +216,56 @@ __device__ inline void sp_load_rows(SpRows &R, const double *vals, const uint8_t
     const uint16_t *roff = reinterpret_cast<const uint16_t *>(rec + nb + 1);
     const int64_t gbase = span_off[grp ? grp[g].cell0 : g * kSpGroup] - base;
     const int a0 = roff[bi * 65 + row], a1 = roff[bi * 65 + row + 1];
-    R.a_at = gbase + rec[bi] + a0;
-    R.na = a1 - a0;
-    if (DIAG) {
-        R.b_at = R.a_at;
-        R.nbr = R.na;
-    } else {
+    const int64_t a_at = gbase + rec[bi] + a0;
+    const int na = a1 - a0;
+    int64_t b_at = a_at;
+    int nbr = na;
+    if (!DIAG) {
         const int b0 = roff[bj * 65 + row], b1 = roff[bj * 65 + row + 1];
-        R.b_at = gbase + rec[bj] + b0;
-        R.nbr = b1 - b0;
+        b_at = gbase + rec[bj] + b0;
+        nbr = b1 - b0;
     }
-    R.sa = t < R.na ? vals[R.a_at + t] : 0.0;
-    R.ca = t < R.na ? (int)cols[R.a_at + t] : 0;
-    R.sb = t < R.nbr ? vals[R.b_at + t] : 0.0;
-    R.cb = t < R.nbr ? (int)cols[R.b_at + t] : 0;
+    R.swap = !DIAG && na > nbr;
+    R.k_at = R.swap ? b_at : a_at;
+    R.nk = R.swap ? nbr : na;
+    R.l_at = R.swap ? a_at : b_at;
+    R.nl = R.swap ? na : nbr;
+    R.vk = t < R.nk ? vals[R.k_at + t] : 0.0;
+    R.ck = t < R.nk ? (int)cols[R.k_at + t] : 0;
+    R.vl = t < R.nl ? vals[R.l_at + t] : 0.0;
+    R.cl = t < R.nl ? (int)cols[R.l_at + t] : 0;
 }
 
 template <bool DIAG>
 __device__ inline void sp_pairs(double *G, double *cs, const SpRows &R, const double *vals, const uint8_t *cols,
                                 int t) {
-    const int mx_a = rows_max(R.na), mx_b = rows_max(R.nbr);
-    for (int ac = 0; ac < mx_a; ac += 16) {
-        double sa = R.sa;
-        int ca = R.ca;
-        if (ac > 0) {  // further a-entries of rows with more than 16 in this bucket
-            const bool h = ac + t < R.na;
-            sa = h ? vals[R.a_at + ac + t] : 0.0;
-            ca = h ? (int)cols[R.a_at + ac + t] : 0;
+    const int mx_k = rows_max(R.nk), mx_l = rows_max(R.nl);
+    // the product of a-column ca and b-column cb goes to element ca kSpLd + cb: the
+    // broadcast entry gives its byte offset, the lane entry its element pointer
+    const int k_scale = R.swap ? (int)sizeof(double) : kSpLd * (int)sizeof(double);
+    const int l_scale = R.swap ? kSpLd : 1;
+    for (int kc = 0; kc < mx_k; kc += 16) {
+        double vk = R.vk;
+        int ck = R.ck;
+        if (kc > 0) {  // further broadcast entries of rows with more than 16 in this bucket
+            const bool h = kc + t < R.nk;
+            vk = h ? vals[R.k_at + kc + t] : 0.0;
+            ck = h ? (int)cols[R.k_at + kc + t] : 0;
         }
-        const int ao = ca * (kSpLd * (int)sizeof(double));
-        const int na_c = R.na - ac, n_i = min(mx_a - ac, 16);
-        for (int bc = 0; bc < mx_b; bc += 16) {
-            double sb = R.sb;
-            int cb = R.cb;
-            if (bc > 0) {
-                const bool h = bc + t < R.nbr;
-                sb = h ? vals[R.b_at + bc + t] : 0.0;
-                cb = h ? (int)cols[R.b_at + bc + t] : 0;
+        const int ko = ck * k_scale;
+        const int nk_c = R.nk - kc, n_i = min(mx_k - kc, 16);
+        for (int lc = 0; lc < mx_l; lc += 16) {
+            double vl = R.vl;
+            int cl = R.cl;
+            if (lc > 0) {
+                const bool h = lc + t < R.nl;
+                vl = h ? vals[R.l_at + lc + t] : 0.0;
+                cl = h ? (int)cols[R.l_at + lc + t] : 0;
             }
-            const bool ok_b = bc + t < R.nbr;
+            const bool ok_l = lc + t < R.nl;
             // a diagonal block's entries are its column sums' terms, each once
-            if (DIAG && ac == 0 && ok_b) atomicAdd(&cs[cb], sb);
-            sp_chunk(G + cb, sa, ao, sb, ok_b, na_c, n_i);
+            if (DIAG && kc == 0 && ok_l) atomicAdd(&cs[cl], vl);
+            sp_chunk(G + cl * l_scale, vk, ko, vl, ok_l, nk_c, n_i);
         }
     }
 }
