@@ -141,7 +141,10 @@ __device__ inline void ch_gemm_tile(const double *__restrict__ A, const double *
 // t taking double2 u 256 + t (16 threads per column, its 256 contiguous bytes).
 template <int NG>
 __device__ inline void ch_gemm_panel(const double *__restrict__ A, const double *__restrict__ Y, int dp, int n,
-                                     int r0, double *__restrict__ o) {
+                                     int r0, double *__restrict__ o, int k0 = 0, int kc = -1) {
+    if (kc < 0) kc = dp;
+    A += k0;
+    Y += k0;
     constexpr int KS = 32, NC = 16 * NG, LA = 64 + 4, LY = NC + 4;
     __shared__ double As[2][KS * LA], Ys[2][KS * LY];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
@@ -182,8 +185,8 @@ __device__ inline void ch_gemm_panel(const double *__restrict__ A, const double 
     stash(0);
     __syncthreads();
     int cur = 0;
-    for (int kb = 0; kb < dp; kb += KS) {
-        const bool more = kb + KS < dp;
+    for (int kb = 0; kb < kc; kb += KS) {
+        const bool more = kb + KS < kc;
         if (more) fetch(kb + KS);
         const double *as = As[cur], *ys = Ys[cur];
 #pragma unroll
@@ -207,6 +210,14 @@ __device__ inline void ch_gemm_panel(const double *__restrict__ A, const double 
 #pragma unroll
             for (int v = 0; v < 4; ++v) o[(int64_t)col * dp + r0 + 16 * w + 4 * v + lk] = acc[g][v];
     }
+}
+
+// One matrix: one workgroup per (64-row panel, K part) over all n <= 16 NG columns;
+// part s writes its partial product to out + s dp n.
+template <int NG>
+__global__ __launch_bounds__(256) void k_ch_gemm_panel(const double *__restrict__ A, const double *__restrict__ Y,
+                                                       int dp, int n, int kc, double *__restrict__ out) {
+    ch_gemm_panel<NG>(A, Y, dp, n, blockIdx.x * 64, out + (int64_t)blockIdx.y * dp * n, blockIdx.y * kc, kc);
 }
 
 __global__ __launch_bounds__(256) void k_ch_gemm(const double *__restrict__ A, const double *__restrict__ Y, int dp,
@@ -788,7 +799,16 @@ void launch_ch_jacobi(const double *Hp, int S, int n, int max_sweeps, bool stric
                        (int)desc, theta, Z, bad);
 }
 void launch_ch_gemm(const double *A, const double *Y, int dp, int n, int S, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_ch_gemm, dim3(dp / 64, S, (n + 31) / 32), dim3(256), 0, s, A, Y, dp, n, dp / S, out);
+    const dim3 g(dp / 64, S), b(256);
+    const int kc = dp / S;
+    switch ((n + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_ch_gemm_panel<1>, g, b, 0, s, A, Y, dp, n, kc, out); break;
+    case 2: hipLaunchKernelGGL(k_ch_gemm_panel<2>, g, b, 0, s, A, Y, dp, n, kc, out); break;
+    case 3: hipLaunchKernelGGL(k_ch_gemm_panel<3>, g, b, 0, s, A, Y, dp, n, kc, out); break;
+    case 4: hipLaunchKernelGGL(k_ch_gemm_panel<4>, g, b, 0, s, A, Y, dp, n, kc, out); break;
+    case 5: hipLaunchKernelGGL(k_ch_gemm_panel<5>, g, b, 0, s, A, Y, dp, n, kc, out); break;
+    default: hipLaunchKernelGGL(k_ch_gemm, dim3(dp / 64, S, (n + 31) / 32), b, 0, s, A, Y, dp, n, kc, out); break;
+    }
 }
 void launch_ch_resid(const double *cv, const double *v, const double *theta, int d, int ld, int k, double *out,
                      hipStream_t s) {
