@@ -9,10 +9,11 @@
 // k_sparse_pack (one workgroup per group of 64 rows): the row's HVG entries, scaled,
 // written grouped by 128-column bucket (bucket = hvg >> 7): within a group of 64 rows
 // all entries of bucket 0 (row by row, each row's entries in entry order), then
-// bucket 1, ...  The group's region starts at the group's own CSR offset (it holds at
+// bucket 1, ...  The rows take their places in the group by descending HVG entry
+// count (the Gram is a sum over rows, so their order is free).  The group's region starts at the group's own CSR offset (it holds at
 // most that many entries).  Per group a record of bucket starts (nb + 1 words) and
-// per-bucket row offsets (65 u16 per bucket).  Entry = FP64 value + the column
-// within its bucket (one byte).
+// per-bucket row offsets (65 u16 per bucket, by place).  Entry = FP64 value + the
+// column within its bucket (one byte).
 //
 // k_gram_sparse (one 1024-thread workgroup per (128 x 128 Gram block, row split)): the
 // block lives in LDS as FP64 (row stride 129, so the rows of one add fall in different
@@ -51,9 +52,10 @@ __host__ __device__ inline int sp_rec_words(int nb) { return ((nb + 1) + (nb * 6
 
 __global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc0, int nb, int64_t n_groups,
                                                      const SegDesc *grp, double *vals, uint8_t *cols, uint32_t *meta) {
-    __shared__ int cnt[kSpGroup][kSpMaxNb];
+    __shared__ int cnt[kSpGroup][kSpMaxNb + 1];  // padded: a wave reads one bucket of 64 rows
     __shared__ int bstart[kSpMaxNb + 1];
     __shared__ int run[4][kSpMaxNb];
+    __shared__ int tot[kSpGroup], inv[kSpGroup], btot[kSpMaxNb];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int rw = sp_rec_words(nb);
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -70,46 +72,81 @@ __global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc0, i
         }
         const int64_t gbase = c.span_off[r0] - c.base;
         for (int i = threadIdx.x; i < kSpGroup * kSpMaxNb; i += blockDim.x) cnt[i / kSpMaxNb][i % kSpMaxNb] = 0;
+        if (threadIdx.x < kSpGroup) tot[threadIdx.x] = 0;
         __syncthreads();
-        // entries per (row, bucket)
+        // entries per (row, bucket), and per row
         for (int rl = wave; rl < nr; rl += 4) {
             if (chunk_row(c, r0 + rl) < 0) continue;
             const int64_t lo = c.span_off[r0 + rl] - c.base, hi = c.span_off[r0 + rl + 1] - c.base;
-            for (int64_t e = lo + lane; e < hi; e += kWave) {
-                const int32_t m = chunk_var(c, c.genes[e]);
-                if (m < 0) continue;
-                const int32_t h = sc.sub[m];
-                if (h >= 0 && h < sc.d) atomicAdd(&cnt[rl][h >> 7], 1);
+            int n_row = 0;
+            for (int64_t e0 = lo; e0 < hi; e0 += kWave) {
+                const int64_t e = e0 + lane;
+                bool in = false;
+                if (e < hi) {
+                    const int32_t m = chunk_var(c, c.genes[e]);
+                    const int32_t h = m >= 0 ? sc.sub[m] : -1;
+                    in = h >= 0 && h < sc.d;
+                    if (in) atomicAdd(&cnt[rl][h >> 7], 1);
+                }
+                n_row += __popcll(__ballot(in));
             }
+            if (lane == 0) tot[rl] = n_row;
         }
         __syncthreads();
-        // bucket starts; per bucket the exclusive prefix over rows (in place)
-        if (threadIdx.x == 0) {
-            int o = 0;
-            for (int b = 0; b < nb; ++b) {
-                bstart[b] = o;
-                for (int rl = 0; rl < kSpGroup; ++rl) o += cnt[rl][b];
+        // the group's rows take their places by descending entry count (ties in row
+        // order), so the four rows a wave of the block kernel takes have similar lists:
+        // wave 0, lane r = row r, ranks its row against every other by register reads
+        if (wave == 0) {
+            const int t = tot[lane];
+            int k = 0;
+#pragma unroll
+            for (int q = 0; q < kSpGroup; ++q) {
+                const int tq = __builtin_amdgcn_readlane(t, q);
+                k += (tq > t || (tq == t && q < lane)) ? 1 : 0;
             }
-            bstart[nb] = o;
+            inv[k] = lane;
         }
         __syncthreads();
+        // per bucket (one wave each), lane k = the row at place k: the exclusive prefix of
+        // the bucket's counts over the places, the row's offset within the bucket
         uint32_t *rec = meta + g * rw;
         uint16_t *roff = reinterpret_cast<uint16_t *>(rec + nb + 1);
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            int o = 0;
-            for (int rl = 0; rl < kSpGroup; ++rl) {
-                const int t = cnt[rl][b];
-                cnt[rl][b] = o;
-                roff[b * 65 + rl] = (uint16_t)o;
-                o += t;
+        for (int b = wave; b < nb; b += 4) {
+            const int rl = inv[lane], t = cnt[rl][b];
+            int x = t;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(x, o, kWave);
+                if (lane >= o) x += y;
             }
-            roff[b * 65 + kSpGroup] = (uint16_t)o;
+            cnt[rl][b] = x - t;
+            roff[b * 65 + lane] = (uint16_t)(x - t);
+            if (lane == kWave - 1) {
+                roff[b * 65 + kSpGroup] = (uint16_t)x;
+                btot[b] = x;
+            }
         }
-        for (int b = threadIdx.x; b <= nb; b += blockDim.x) rec[b] = (uint32_t)bstart[b];
         __syncthreads();
-        // scaled entries at their positions (entry order within each (row, bucket))
-        for (int rl = wave; rl < nr; rl += 4) {
-            if (chunk_row(c, r0 + rl) < 0) continue;
+        // bucket starts: the exclusive prefix of the bucket totals
+        if (wave == 0) {
+            const int t = lane < nb ? btot[lane] : 0;
+            int x = t;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(x, o, kWave);
+                if (lane >= o) x += y;
+            }
+            if (lane <= nb) {
+                bstart[lane] = x - t;
+                rec[lane] = (uint32_t)(x - t);
+            }
+        }
+        __syncthreads();
+        // scaled entries at their positions (entry order within each (row, bucket)); the
+        // waves take the rows in place order, so their stores fill neighbouring lines
+        for (int k = wave; k < kSpGroup; k += 4) {
+            const int rl = inv[k];
+            if (rl >= nr || chunk_row(c, r0 + rl) < 0) continue;
             const int64_t lo = c.span_off[r0 + rl] - c.base, hi = c.span_off[r0 + rl + 1] - c.base;
             const double ns = norm_factor(sp_cell_total(c, lo, hi, lane), c.target_sum);
             for (int b = lane; b < nb; b += kWave) run[wave][b] = 0;
@@ -153,14 +190,9 @@ __global__ __launch_bounds__(256) void k_sparse_pack(DevChunk c, DevScale sc0, i
     }
 }
 
-// lane n of each row of 16 lanes, to every lane of that row (DPP row_newbcast; every
-// lane is written, so no previous value is needed)
-template <int N> __device__ inline int row_bcast(int v) { return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xf, 0xf, false); }
-template <int N> __device__ inline double row_bcast_f64(double v) {
-    const long long u = __double_as_longlong(v);
-    const int lo = row_bcast<N>((int)(u & 0xffffffffll)), hi = row_bcast<N>((int)(u >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
+// lane N of each row of 16 lanes, to every lane of that row (DPP row_newbcast; every
+// lane is written, so no previous value is needed; a double moves in one v_mov_b64_dpp)
+template <int N, class T> __device__ inline T row_bcast(T v) { return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xf, 0xf, false); }
 
 // Pairs of one 16 x 16 chunk: lane t of row group r holds broadcast entry t (value sa,
 // byte offset ao of its part of the block address) and lane entry t (value sb; gb =
@@ -171,7 +203,7 @@ template <int N> __device__ inline double row_bcast_f64(double v) {
 // its columns.
 template <int I>
 __device__ inline void sp_step(double *gb, double sa, int ao, double sb, bool ok_b, int na) {
-    const double ai = row_bcast_f64<I>(sa);
+    const double ai = row_bcast<I>(sa);
     const int ai_off = row_bcast<I>(ao);
     if (ok_b && I < na) atomicAdd(reinterpret_cast<double *>(reinterpret_cast<char *>(gb) + ai_off), ai * sb);
 }
@@ -274,13 +306,17 @@ template <bool DIAG>
 __device__ __attribute__((always_inline)) inline void sp_block(double *G, double *cs, const double *vals, const uint8_t *cols, const uint32_t *meta,
                          const int64_t *span_off, int64_t base, const SegDesc *grp, int rw, int nb, int bi, int bj,
                          int64_t g0, int64_t g1, int wave, int lane) {
-    const int row = 4 * wave + (lane >> 4), t = lane & 15;
+    const int t = lane & 15;
+    // a group's rows sit by descending entry count, so wave w takes places
+    // 4((w + g) mod 16) .. +3 of group g: rows of similar lists together, and every
+    // wave meets every size class in turn
+    auto row_of = [&](int64_t g) { return 4 * (int)((wave + g) % kSpWaves) + (lane >> 4); };
     if (g0 >= g1) return;
     SpRows cur, nxt;
-    sp_load_rows<DIAG>(cur, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g0, row, t);
+    sp_load_rows<DIAG>(cur, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g0, row_of(g0), t);
     for (int64_t g = g0; g < g1; ++g) {
         if (g + 1 < g1)
-            sp_load_rows<DIAG>(nxt, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g + 1, row, t);
+            sp_load_rows<DIAG>(nxt, vals, cols, meta, span_off, base, grp, rw, nb, bi, bj, g + 1, row_of(g + 1), t);
         sp_pairs<DIAG>(G, cs, cur, vals, cols, t);
         cur = nxt;
     }
