@@ -679,7 +679,7 @@ __device__ __attribute__((always_inline)) inline void topk_update(float (&ld)[L]
 #pragma unroll
     for (int r = 1; r < 16; ++r) m = min(m, __float_as_int(a[r]));
     const float thr = ld[L - 1];
-    if (__builtin_amdgcn_ballot_w64(m < __float_as_int(thr)) == 0) return;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(m < __float_as_int(thr)) == 0, 1)) return;
     uint32_t u = 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) u |= (__builtin_amdgcn_ballot_w64(a[r] < thr) != 0 ? 1u : 0u) << r;
@@ -726,7 +726,8 @@ __device__ __attribute__((always_inline)) inline void topk_update_buf(float (&ld
     int32_t m = __float_as_int(a[0]);
 #pragma unroll
     for (int r = 1; r < 16; ++r) m = min(m, __float_as_int(a[r]));
-    if (__builtin_amdgcn_ballot_w64(m < __float_as_int(ld[L - 1])) == 0) return;
+    // the slow path is rare after the first stages: laid out away from the MFMA stream
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(m < __float_as_int(ld[L - 1])) == 0, 1)) return;
     const float thr = ld[L - 1];
     uint32_t u = 0;
 #pragma unroll
@@ -918,7 +919,21 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
     };
     f32x16 x0, x1, y0, y1;  // even fragments in x, odd in y
     uint32_t cy = 0;        // first candidate of the fragment held in y
-    auto compute = [&](const u32x4 *stage, int64_t t) __attribute__((always_inline)) {
+    // LDS holds visit t (being computed) and receives visit t+1 from registers loaded
+    // one iteration earlier: after the MFMAs of each fragment the workgroup stores its
+    // share of those registers (PER / NF units per thread) and reloads them with visit
+    // t+2, so the stage traffic runs beside the matrix work instead of in a burst at
+    // the barrier, and each load has a whole iteration to arrive.  (The final
+    // iterations re-load the last stage; those copies are never read.)
+    u32x4 pre[PER];
+    auto move = [&](u32x4 *dst, const u32x4 *src, int f) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = f * PER / NF; m < (f + 1) * PER / NF; ++m) {
+            dst[threadIdx.x + m * NT] = pre[m];
+            pre[m] = src[threadIdx.x + m * NT];
+        }
+    };
+    auto compute = [&](const u32x4 *stage, int64_t t, u32x4 *dst, const u32x4 *src) __attribute__((always_inline)) {
         const uint32_t c0 = (uint32_t)(stage_at(t) * kTC * TB);
         if constexpr (screen_pipe<KC, L>()) {
             u32x4 ae[KS], ao[KS];  // operands of the even / odd fragment
@@ -927,9 +942,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
             for (int f = 0; f < NF; f += 2) {
                 lda(stage, f + 1, ao);
                 mmaa(ae, x0, x1);
+                move(dst, src, f);
                 if (f > 0 || t > 0) update(y0, y1, cy);
                 if (f + 2 < NF) lda(stage, f + 2, ae);
                 mmaa(ao, y0, y1);
+                move(dst, src, f + 1);
                 cy = c0 + (uint32_t)((f + 1) * 32);
                 update(x0, x1, c0 + (uint32_t)(f * 32));
             }
@@ -937,26 +954,27 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 mma(stage, f, x0, x1);
+                move(dst, src, f);
                 update(x0, x1, c0 + (uint32_t)(f * 32));
             }
         }
     };
-    // LDS holds visit t (being computed) and receives visit t+1 from registers loaded
-    // one iteration earlier; the loads for visit t+2 are issued after that store, so
-    // their latency overlaps the whole next iteration.  (The final iterations re-load
-    // the last stage; those copies are never read.)
+    // visit 1 into registers, then visit 0 into LDS: the stores of visit 0 wait for
+    // every load before them, so the loop starts with no load in flight and its waits
+    // count only the loop's own loads (reloaded in unit order, one unit per fragment)
+    {
+        const u32x4 *second = xa + stage_at(n_stages > 1 ? 1 : 0) * TU;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) pre[m] = second[threadIdx.x + m * NT];
+    }
     {
         const u32x4 *first = xa + stage_at(0) * TU;
         for (int i = threadIdx.x; i < TU; i += NT) lds[0][i] = first[i];
     }
-    Regs<u32x4, PER> pre;
-    regs_load<u32x4, PER, NT>(pre, xa + stage_at(n_stages > 1 ? 1 : 0) * TU, threadIdx.x);
     __syncthreads();
     for (int64_t t = 0; t < n_stages; ++t) {
-        compute(lds[t & 1], t);
-        regs_store<u32x4, PER, NT>(pre, lds[(t & 1) ^ 1], threadIdx.x);
         const int64_t tn = t + 2 < n_stages ? t + 2 : n_stages - 1;
-        regs_load<u32x4, PER, NT>(pre, xa + stage_at(tn) * TU, threadIdx.x);
+        compute(lds[t & 1], t, lds[(t & 1) ^ 1], xa + stage_at(tn) * TU);
         __syncthreads();
     }
     if constexpr (screen_pipe<KC, L>()) update(y0, y1, cy);
