@@ -446,13 +446,32 @@ static int chefsi_block(int d, int p) {
     return (mode.empty() && d >= 512) ? k : 0;
 }
 
-// Degree of the next filter: the largest m <= 40 whose T_m at the top of the active
+constexpr int kChebMaxDegree = 64;  // longest filter per iteration
+
+// Degree of the next filter: the largest m <= kChebMaxDegree whose T_m at the top of the active
 // block (x_top in the filter's variable) stays below kappa, so the filtered block
 // is no worse conditioned than kappa from the spread of its directions' gains.
 static int chefsi_degree(double x_top, double kappa) {
-    if (!(x_top > 1.0 + 1e-12)) return 40;
+    if (!(x_top > 1.0 + 1e-12)) return kChebMaxDegree;
     const int m = (int)std::floor(std::acosh(kappa) / std::acosh(x_top));
-    return std::max(1, std::min(40, m));
+    return std::max(1, std::min(kChebMaxDegree, m));
+}
+
+// The degree the wanted Ritz vectors still unlocked need: the filter multiplies a
+// vector of value theta_j by T_m(x_j) (x_j = (theta_j - c) / e) against everything in
+// the damped interval, so its residual falls by about that factor; the degree that
+// brings every res_j to `target` (a hundredth of the locking threshold, so one more
+// Rayleigh-Ritz locks them), or the longest filter when some x_j gives no bound.  Locking still tests
+// the residuals themselves, so a degree that falls short costs one more iteration.
+static int chefsi_needed(const double *theta, const double *res, int n_want, double c, double e, double target) {
+    int m = 1;
+    for (int j = 0; j < n_want; ++j) {
+        const double x = (theta[j] - c) / e;
+        if (!(x > 1.0 + 1e-12)) return kChebMaxDegree;
+        const double need = res[j] / target;
+        if (need > 1.0) m = std::max(m, (int)std::ceil(std::acosh(need) / std::acosh(x)));
+    }
+    return std::min(m, kChebMaxDegree);
 }
 
 // The p largest eigenpairs of the symmetric d x d matrix C (column-major, ld d) by
@@ -605,7 +624,8 @@ static bool chefsi_topp(Ctx &x, const double *C, int d, int p, double *dev_var, 
         const double c = 0.5 * (a + b), e = 0.5 * (b - a);
         const int64_t n_el = (int64_t)dp * ka;
         for (;;) {
-            const int m = chefsi_degree((theta_top - c) / e, kappa);
+            const int m = std::min(chefsi_degree((theta_top - c) / e, kappa),
+                                   chefsi_needed(&prev[j0], &prev[k + j0], std::min(p - L, ka), c, e, 1e-2 * tol));
             if (trace)
                 std::fprintf(stderr, "[chefsi] it %d locked %d active %d degree %d top-resid %.3e b/top %.6e\n", it, L,
                              ka, m, prev[k + j0] / top, b / top);
@@ -802,7 +822,8 @@ static std::vector<char> chefsi_topp_batch(Ctx &x, const std::string &pre, const
             }
             const double c = 0.5 * b, e = 0.5 * b;
             q.ka = m.ka;
-            q.m = chefsi_degree((theta_top - c) / e, m.kappa);
+            q.m = std::min(chefsi_degree((theta_top - c) / e, m.kappa),
+                           chefsi_needed(&m.prev[m.j0], &m.prev[K + m.j0], std::min(p - m.L, m.ka), c, e, 1e-2 * m.tol));
             q.voff = m.j0;
             q.c = c;
             q.e = e;
