@@ -368,9 +368,32 @@ static void gram_rows_sparse(Ctx &x, const DevChunk &c, const DevScale &sc, int 
     uint32_t *meta = x.get<uint32_t>("sp_meta", sparse_meta_words(c.n_spans, d_pad));
     double *partial = x.get<double>("gram_partial", (size_t)nblk * n_split * kGramBlock * kGramBlock);
     double *cs_partial = x.get<double>("sp_cs_partial", (size_t)nb * n_split * kGramBlock);
-    launch_sparse_pack(c, sc, d_pad, vals, cols, meta, x.st);
-    launch_gram_sparse(vals, cols, meta, c.span_off, c.base, c.n_spans, d_pad, n_split, partial, cs_partial, dev_colsum,
-                       x.st);
+    const int64_t n_groups = ceil_div(c.n_spans, kRowGroupSparse);
+    const int half = n_split / 2;
+    const int64_t g_half = (int64_t)half * n_groups / n_split;  // the groups of splits 0 .. half-1
+    if (g_half > 0 && g_half < n_groups) {
+        // the rows of the second half are packed on the side stream while the pair
+        // products of the first half run (the pack waits on its loads, the products
+        // on issue and LDS atomics, so they share the CUs); each half's packed entries
+        // sit at their own CSR offsets and its group records at its groups' places
+        const int64_t r_half = g_half * kRowGroupSparse;
+        launch_sparse_pack(seg_rows(c, 0, r_half), sc, d_pad, vals, cols, meta, x.st);
+        HIPCHK(hipEventRecord(x.fork, x.st));
+        HIPCHK(hipStreamWaitEvent(x.side, x.fork, 0));
+        launch_sparse_pack(seg_rows(c, r_half, c.n_spans), sc, d_pad, vals, cols,
+                           meta + sparse_meta_words(r_half, d_pad), x.side);
+        HIPCHK(hipEventRecord(x.join, x.side));
+        launch_gram_sparse_part(vals, cols, meta, c.span_off, c.base, c.n_spans, d_pad, n_split, 0, half, partial,
+                                cs_partial, x.st);
+        HIPCHK(hipStreamWaitEvent(x.st, x.join, 0));
+        launch_gram_sparse_part(vals, cols, meta, c.span_off, c.base, c.n_spans, d_pad, n_split, half, n_split - half,
+                                partial, cs_partial, x.st);
+        launch_sparse_colsum(cs_partial, d_pad, n_split, dev_colsum, x.st);
+    } else {
+        launch_sparse_pack(c, sc, d_pad, vals, cols, meta, x.st);
+        launch_gram_sparse(vals, cols, meta, c.span_off, c.base, c.n_spans, d_pad, n_split, partial, cs_partial,
+                           dev_colsum, x.st);
+    }
     launch_gram_reduce(partial, n_split, d_pad, dev_gram, x.st);
     x.check_launch();
 }

@@ -299,6 +299,12 @@ size_t sparse_meta_words(int64_t n_rows, int d_pad);
 size_t sparse_gram_lds();
 void launch_sparse_pack(const DevChunk &c, const DevScale &sc, int d_pad, double *vals, uint8_t *cols, uint32_t *meta,
                         hipStream_t s);
+// splits split0 .. split0 + n_sl - 1 of n_split (over the row groups) of the sparse Gram;
+// launch_sparse_colsum then sums the column partials of all n_split splits
+void launch_gram_sparse_part(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                             int64_t base, int64_t n_rows, int d_pad, int n_split, int split0, int n_sl, double *partial,
+                             double *cs_partial, hipStream_t s);
+void launch_sparse_colsum(const double *cs_partial, int d_pad, int n_split, double *colsum, hipStream_t s);
 void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
                         int64_t base, int64_t n_rows, int d_pad, int n_split, double *partial, double *cs_partial,
                         double *colsum, hipStream_t s);

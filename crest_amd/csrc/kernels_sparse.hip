@@ -326,13 +326,14 @@ __global__ __launch_bounds__(kSpThreads) void k_gram_sparse(const double *__rest
                                                           const uint8_t *__restrict__ cols,
                                                           const uint32_t *__restrict__ meta,
                                                           const int64_t *__restrict__ span_off, int64_t base,
-                                                          int64_t n_groups, int nb, int n_split,
-                                                          double *partial, double *cs_partial) {
+                                                          int64_t n_groups, int nb, int n_split, int split0,
+                                                          int n_sl, double *partial, double *cs_partial) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double *G = reinterpret_cast<double *>(lds_raw);  // [128][kSpLd]
     double *cs = G + 128 * kSpLd;                       // [128]
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    const int split = blockIdx.x % n_split, blk = blockIdx.x / n_split;
+    // this launch's splits split0 .. split0 + n_sl - 1 of n_split over the groups
+    const int split = split0 + (int)(blockIdx.x % n_sl), blk = (int)(blockIdx.x / n_sl);
     int bi = 0, bj = 0;
     {
         int r = 0, t = blk;
@@ -467,18 +468,28 @@ void launch_gram_sparse_items(const double *vals, const uint8_t *cols, const uin
                        meta, span_off, base, groups, items, d_pad / kGramBlock, d_pad, partial, gram, colsum);
 }
 
+void launch_gram_sparse_part(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
+                             int64_t base, int64_t n_rows, int d_pad, int n_split, int split0, int n_sl, double *partial,
+                             double *cs_partial, hipStream_t s) {
+    const int64_t n_groups = ceil_div(n_rows, kSpGroup);
+    const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
+    if (n_groups <= 0 || n_sl <= 0) return;
+    const size_t lds = sparse_gram_lds();
+    hipFuncSetAttribute((const void *)k_gram_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_gram_sparse, dim3(nblk * n_sl), dim3(kSpThreads), lds, s, vals, cols, meta, span_off, base,
+                       n_groups, nb, n_split, split0, n_sl, partial, cs_partial);
+}
+void launch_sparse_colsum(const double *cs_partial, int d_pad, int n_split, double *colsum, hipStream_t s) {
+    hipLaunchKernelGGL(k_sparse_colsum, dim3((unsigned)ceil_div(d_pad, 256)), dim3(256), 0, s, cs_partial,
+                       d_pad / kGramBlock, n_split, colsum);
+}
 void launch_gram_sparse(const double *vals, const uint8_t *cols, const uint32_t *meta, const int64_t *span_off,
                         int64_t base, int64_t n_rows, int d_pad, int n_split, double *partial, double *cs_partial,
                         double *colsum, hipStream_t s) {
-    const int64_t n_groups = ceil_div(n_rows, kSpGroup);
-    const int nb = d_pad / kGramBlock, nblk = nb * (nb + 1) / 2;
-    if (n_groups <= 0) return;
-    const size_t lds = sparse_gram_lds();
-    hipFuncSetAttribute((const void *)k_gram_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_gram_sparse, dim3(nblk * n_split), dim3(kSpThreads), lds, s, vals, cols, meta, span_off, base,
-                       n_groups, nb, n_split, partial, cs_partial);
-    hipLaunchKernelGGL(k_sparse_colsum, dim3((unsigned)ceil_div(d_pad, 256)), dim3(256), 0, s, cs_partial, nb, n_split,
-                       colsum);
+    if (n_rows <= 0) return;
+    launch_gram_sparse_part(vals, cols, meta, span_off, base, n_rows, d_pad, n_split, 0, n_split, partial, cs_partial,
+                            s);
+    launch_sparse_colsum(cs_partial, d_pad, n_split, colsum, s);
 }
 
 }  // namespace crest
