@@ -417,6 +417,95 @@ __global__ void k_knn_prep_f16(const float *x, const uint32_t *perm, const float
     }
 }
 
+// The same operands for 64 consecutive sorted positions per workgroup (no subsets):
+// their rows, in permuted order, are first read into LDS with the threads walking
+// along each row (a thread per row reading its row element by element touches 64
+// rows' lines per load), then each row's norm and the 16-byte units are built from
+// LDS with exactly the arithmetic of k_knn_prep_f16, so the operands are identical.
+__global__ __launch_bounds__(256) void k_knn_prep_f16_tile(const float *x, const uint32_t *perm, const float *mean,
+                                                           const uint32_t *maxbits, int64_t n, int p, int kc,
+                                                           uint4 *xa, uint4 *xb) {
+    extern __shared__ float tile[];  // [64][p + 1]
+    __shared__ float nnv[64];
+    const int ld = p + 1;
+    const int64_t i0 = (int64_t)blockIdx.x * 64;
+    const float s = fp16_prescale(maxbits);
+    for (int e = threadIdx.x; e < 64 * p; e += blockDim.x) {
+        const int r = e / p, f = e - r * p;
+        const int64_t i = i0 + r;
+        tile[r * ld + f] = i < n ? x[(int64_t)perm[i] * p + f] : 0.0f;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        float nn = 0.0f;
+        for (int f = 0; f < p; ++f) {
+            const float v = (tile[r * ld + f] - mean[f]) * s;
+            const float sq = v * v;
+            nn = nn + sq;
+        }
+        nnv[r] = i0 + r < n ? nn : INFINITY;
+    }
+    __syncthreads();
+    const _Float16 one = (_Float16)1.0f, zero = (_Float16)0.0f;
+    for (int q = threadIdx.x; q < 64 * kc; q += blockDim.x) {
+        const int r = q & 63, j = q >> 6;
+        const bool real = i0 + r < n;
+        const float nn = nnv[r];
+        const _Float16 nh = (_Float16)nn;
+        const _Float16 nl = real ? (_Float16)(nn - (float)nh) : (_Float16)0.0f;
+        _Float16 ea[8], eb[8];
+        for (int e = 0; e < 8; ++e) {
+            const int k = 8 * j + e;
+            ea[e] = eb[e] = zero;
+            if (k < p) {
+                if (real) {
+                    const _Float16 h = (_Float16)((tile[r * ld + k] - mean[k]) * s);
+                    ea[e] = h;
+                    eb[e] = (_Float16)(-2.0f * (float)h);
+                }
+            } else if (k < p + 4) {
+                const int t = k - p;
+                ea[e] = t == 0 ? nh : (t == 1 ? nl : one);
+                eb[e] = real ? (t == 2 ? nh : (t == 3 ? nl : one)) : zero;
+            }
+        }
+        uint4 va, vb;
+        __builtin_memcpy(&va, ea, 16);
+        __builtin_memcpy(&vb, eb, 16);
+        const int64_t u = ((int64_t)blockIdx.x * kc + j) * 64 + r;
+        if (xa) xa[u] = va;
+        if (xb) xb[u] = vb;
+    }
+}
+
+// The largest centred squared norm of rows [0, n) (natural order), 64 rows per
+// workgroup read along the rows into LDS; each row's norm in k_knn_maxnorm's order.
+__global__ __launch_bounds__(256) void k_knn_maxnorm_tile(const float *x, int64_t n, int p, const float *mean,
+                                                          uint32_t *maxbits) {
+    extern __shared__ float tile[];  // [64][p + 1]
+    const int ld = p + 1;
+    const int64_t i0 = (int64_t)blockIdx.x * 64;
+    const int64_t rows = n - i0 < 64 ? n - i0 : 64;
+    for (int64_t e = threadIdx.x; e < rows * p; e += blockDim.x) {
+        const int r = (int)(e / p), f = (int)(e - (int64_t)r * p);
+        tile[r * ld + f] = x[(i0 + r) * p + f];
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        float nn = 0.0f;
+        if (r < rows)
+            for (int f = 0; f < p; ++f) {
+                const float v = (tile[r * ld + f] - mean[f]) * 1.0f;
+                const float sq = v * v;
+                nn = nn + sq;
+            }
+        for (int o = 32; o > 0; o >>= 1) nn = fmaxf(nn, __shfl_xor(nn, o, 64));
+        if (threadIdx.x == 0 && nn > 0.0f) atomicMax(maxbits, __float_as_uint(nn));
+    }
+}
+
 __device__ inline void f32_elem(const float *r, const float *mean, int p, float nn, int k, float &a, float &b) {
     a = 0.0f;
     b = 0.0f;
@@ -475,8 +564,9 @@ static void write_operands(int prec, const float *x, const uint32_t *perm, const
     const int64_t n_pad = knn_rows_padded(n);
     const int g = (int)ceil_div(n_pad, 256);
     if (prec == kPrecFP16)
-        hipLaunchKernelGGL(k_knn_prep_f16, dim3(g), dim3(256), 0, s, x, perm, w.mean, w.maxbits, n, n_pad, p,
-                           kpad / 8, (uint4 *)xa, (uint4 *)xb, PrepSeg{});
+        hipLaunchKernelGGL(k_knn_prep_f16_tile, dim3((unsigned)(n_pad / 64)), dim3(256),
+                           (size_t)64 * (p + 1) * sizeof(float), s, x, perm, w.mean, w.maxbits, n, p, kpad / 8,
+                           (uint4 *)xa, (uint4 *)xb);
     else if (prec == kPrecBF16X3)
         hipLaunchKernelGGL(k_knn_prep_bf16, dim3(g), dim3(256), 0, s, x, perm, w.mean, n, n_pad, p, kpad / 8,
                            (uint4 *)xa, (uint4 *)xb, PrepSeg{});
@@ -501,11 +591,12 @@ KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq,
     sort_rows(w, xc, nc, p, w.keys_c, w.perm_c, s);
     if (!graph) sort_rows(w, xq, nq, p, w.keys_q, w.perm_q, s);
     if (prec == kPrecFP16) {
-        hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, s, xc, nc, p, w.mean,
-                           w.maxbits, no);
+        const size_t lds = (size_t)64 * (p + 1) * sizeof(float);
+        hipLaunchKernelGGL(k_knn_maxnorm_tile, dim3((unsigned)ceil_div(nc, 64)), dim3(256), lds, s, xc, nc, p, w.mean,
+                           w.maxbits);
         if (!graph)
-            hipLaunchKernelGGL(k_knn_maxnorm, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, xq, nq, p, w.mean,
-                               w.maxbits, no);
+            hipLaunchKernelGGL(k_knn_maxnorm_tile, dim3((unsigned)ceil_div(nq, 64)), dim3(256), lds, s, xq, nq, p,
+                               w.mean, w.maxbits);
     }
     if (graph) {
         write_operands(prec, xc, w.perm_c, w, nc, p, kpad, xa, xb, s);
