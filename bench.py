@@ -46,6 +46,12 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def knn_certificate() -> dict:
+    """The last kNN search's certificate counts (checked / rechecked / uncertified)."""
+    from crest_amd._lib import knn_certificate as cert
+    return cert()
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -267,7 +273,7 @@ def streamed_leg(args) -> dict:
     out = {"n_cells": args.streamed_cells, "nnz": info["nnz"], "stream_rows": args.stream_rows,
            "prec_gram": args.gram_prec, "prec_knn": args.knn_prec, "ms_per_step": ms,
            "value": args.streamed_cells / (ms * 1e-3), "unit": "cells/s",
-           "stages_ms": {k: round(v, 3) for k, v in stages.items()}}
+           "stages_ms": {k: round(v, 3) for k, v in stages.items()}, "knn_certificate": knn_certificate()}
     if args.no_cpu:
         return out
     from oracle import oracle as O
@@ -345,6 +351,7 @@ def batched_leg(args) -> dict:
         f"{og.times_ms['eig']:.1f} eig, {og.times_ms['project']:.1f} projection+kNN graph)")
     Xb = pg.x_pca()
     ib, _ = pg.knn()
+    cert_b = knn_certificate()
     base = {"workload": (f"{args.batch_subsets} subsets of log-uniform [{args.batch_min}, {args.batch_max}] cells "
                          f"({n} cells) x {args.genes} HVGs, each its own NB generator ({args.batch_clusters} clusters, "
                          f"density {nnz / n / args.genes:.4f}), PCA {args.batch_pcs}, exact kNN k={args.k}, "
@@ -354,7 +361,7 @@ def batched_leg(args) -> dict:
             "graph_ms": {"stats_gram": round(og.times_ms["gram"], 2), "eig": round(og.times_ms["eig"], 2),
                          "project_knn": round(og.times_ms["project"], 2)},
             "stages_ms_eager": {k: round(v, 2) for k, v in oe.times_ms.items()},
-            "tile_rows": og.tile_rows, "gram_rows": og.gram_rows}
+            "tile_rows": og.tile_rows, "gram_rows": og.gram_rows, "knn_certificate": cert_b}
     if args.batch_no_seq:
         return base
     # sequential: one pipeline per subset (its own CSR upload outside the timed region)
@@ -612,7 +619,8 @@ def main() -> int:
         out_ = pipe.run()
         gpu["tags"].append(tag)
         gpu.update({f"x_{tag}": pipe.x_pca(), f"idx_{tag}": pipe.knn()[0], f"var_{tag}": out_.variance,
-                    f"load_{tag}": out_.loadings, f"tot_{tag}": out_.total_variance})
+                    f"load_{tag}": out_.loadings, f"tot_{tag}": out_.total_variance,
+                    f"cert_{tag}": knn_certificate()})
 
     ms, stages = timed_steps(pipe, args.steps, args.warmup)
     log(f"{args.gram_prec} Gram + {args.knn_prec} screen: {ms:.1f} ms/step, "
@@ -715,6 +723,9 @@ def main() -> int:
                      "algorithmic": "2*n^2*p flops per launch (n=%d, p=%d)" % (n, p),
                      "launch_ms": stages["knn_screen"]},
     }
+    # per precision: queries whose screened kNN set the screen's error bound proved,
+    # rechecked in FP32, and left unproven (near ties)
+    line["knn_certificate"] = {t: gpu[f"cert_{t}"] for t in gpu["tags"]}
     if alts:
         line["alt"] = alts
     if args.fp32_steps > 0:

@@ -107,6 +107,8 @@ _SIGS = {
                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]),
     "crest_knn_query": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
                                   C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]),
+    "crest_knn_certificate": (C.c_int, [C.POINTER(C.c_int64)] * 3),
+    "crest_knn_screen_bound": (C.c_int, [C.c_int32, C.c_size_t, C.c_void_p, C.POINTER(C.c_int32)]),
     "crest_connectivities": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "crest_materialize": (C.c_int, [C.POINTER(Chunk), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
@@ -178,6 +180,16 @@ def set_knn_precision(prec) -> None:
     "bf16x3", "fp16", or None to follow set_precision."""
     p = -1 if prec is None else (PREC_NAMES[prec] if isinstance(prec, str) else int(prec))
     check(lib().crest_set_knn_precision(p))
+
+
+def knn_certificate() -> dict:
+    """The certificate of the last kNN search on this device: `checked` queries,
+    `rechecked` (the screen's error bound could not prove their sets; screened again
+    by the FP32 GEMM screen), `uncertified` (still not proven: a candidate within the
+    FP32 screen's error of the k-th distance, a near tie)."""
+    v = [C.c_int64(0) for _ in range(3)]
+    check(lib().crest_knn_certificate(*[C.byref(t) for t in v]))
+    return {"checked": v[0].value, "rechecked": v[1].value, "uncertified": v[2].value}
 
 
 def get_knn_precision() -> str:

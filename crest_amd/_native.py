@@ -143,8 +143,9 @@ def project(genes, values, cell_map, gene_map, target_sum, log1p, cells, indptr,
 
 def knn_graph(data, k, exact=None, seed=0):
     """py.rs:52-70: k nearest neighbours of each row, self excluded; returns
-    (indices uint32 (n, k'), distances float32 (n, k')).  The GPU search is exact
-    for every n, so `exact` and `seed` do not change the result."""
+    (indices uint32 (n, k'), distances float32 (n, k')).  The GPU search is brute
+    force for every n (its sets certified against the screen's error bound, see
+    knn_certificate), so `exact` and `seed` do not change the result."""
     data = _arr(data, np.float32, "data")
     if data.ndim != 2:
         raise ValueError("data must be a 2-D array")
@@ -163,8 +164,8 @@ def knn_graph(data, k, exact=None, seed=0):
 def knn_query(reference, query, k, exact=None, seed=0):
     """py.rs:635-653: the k' = min(k, n_ref) nearest reference rows of each query row
     (knn.rs:485-546); returns (indices uint32 (n_query, k'), distances float32
-    (n_query, k')) ordered by (distance, index).  The GPU search is exact for every
-    size, so `exact` and `seed` do not change the result."""
+    (n_query, k')) ordered by (distance, index).  The GPU search is brute force with
+    certified sets for every size, so `exact` and `seed` do not change the result."""
     if not isinstance(reference, np.ndarray) or not isinstance(query, np.ndarray):
         raise TypeError("reference and query must be numpy arrays")
     if reference.ndim != 2 or query.ndim != 2:

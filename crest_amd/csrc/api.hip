@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -65,6 +66,8 @@ struct Ctx {
     int dev = 0;
     hipStream_t st = nullptr;
     rocblas_handle blas = nullptr;
+    // certificate of the last kNN: queries checked, rechecked in FP32, left uncertified
+    int64_t knn_checked = 0, knn_rechecked = 0, knn_uncertified = 0;
     std::map<std::string, std::pair<void *, size_t>> bufs;
     std::map<std::string, std::pair<void *, size_t>> pinned;  // host staging (hipHostMalloc)
     hipEvent_t ev[12] = {};
@@ -999,10 +1002,13 @@ static void pca_device(Ctx &x, const double *dev_gram, int ld_g, const double *d
 // dev_idx/dev_dist [nq, k]; dev_xq == nullptr gives the kNN graph of dev_xc (the
 // queries are the candidates, each row's own entry excluded).
 // k > 64 (knn_device): FP32 dot products of a block of queries (<= 2 GiB) with all candidates
-// (rocBLAS), per-row radix selection of k2 = k + max(32, k/8) screened candidates
+// (rocBLAS) on rows centred on the candidates' mean, per-row radix selection of
+// k2 = k + max(32, k/8) screened candidates
 // (rounded up to 64, at most the candidate count), exact rank and output order.
+// With `cert` (a device counter): each query's set certified against the GEMM screen's
+// error bound (knn_cert_gemm), the queries not proven counted.
 static void knn_device_large(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
-                             uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
+                             uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3, uint32_t *cert = nullptr) {
     const bool graph = dev_xq == nullptr;
     if (graph) {
         dev_xq = dev_xc;
@@ -1024,20 +1030,32 @@ static void knn_device_large(Ctx &x, const float *dev_xc, int64_t nc, const floa
     uint32_t *cand = x.get<uint32_t>("knnl_cand", (size_t)qb * k2);
     uint32_t *comp_key = x.get<uint32_t>("knnl_comp_key", (size_t)qb * cap);
     uint32_t *comp_idx = x.get<uint32_t>("knnl_comp_idx", (size_t)qb * cap);
+    float *tau = cert ? x.get<float>("knnl_tau", (size_t)qb) : nullptr;
+    const KnnCert ct = knn_cert_gemm(p, cert);
+    // the screen works on rows centred on the candidates' mean (its dot products then
+    // carry no common offset); the rerank reads the original rows
+    float *mean = x.get<float>("knnl_mean", (size_t)p);
+    double *part = x.get<double>("knnl_part", (size_t)knn_centre_parts(nc) * p);
+    float *sc = x.get<float>("knnl_xc", (size_t)nc * p);
+    float *sq = graph ? sc : x.get<float>("knnl_xq", (size_t)nq * p);
     if (ev3) HIPCHK(hipEventRecord(ev3[0], x.st));
-    launch_knn_sqnorm(dev_xc, nc, p, cn, x.st);
-    if (!graph) launch_knn_sqnorm(dev_xq, nq, p, qn, x.st);
+    launch_knn_centre(dev_xc, nc, p, mean, true, part, sc, x.st);
+    if (!graph) launch_knn_centre(dev_xq, nq, p, mean, false, part, sq, x.st);
+    launch_knn_sqnorm(sc, nc, p, cn, x.st);
+    if (!graph) launch_knn_sqnorm(sq, nq, p, qn, x.st);
     if (ev3) HIPCHK(hipEventRecord(ev3[1], x.st));
     const float one = 1.0f, zero = 0.0f;
     for (int64_t q0 = 0; q0 < nq; q0 += qb) {
         const int64_t rows = std::min(qb, nq - q0);
         // dots[r][c] = x_c . q_(q0+r): column-major (nc x rows) = Xc^T (nc x p) * Q (p x rows)
         BLASCHK(rocblas_sgemm(x.blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nc,
-                              (rocblas_int)rows, p, &one, dev_xc, p, dev_xq + q0 * p, p, &zero, dots,
+                              (rocblas_int)rows, p, &one, sc, p, sq + q0 * p, p, &zero, dots,
                               (rocblas_int)ld));
         launch_knn_select_rows(dots, ld, nc, qn, cn, q0, rows, graph ? 1 : 0, (int)k2, cap, comp_key, comp_idx, cand,
                                x.st);
-        launch_knn_rerank_sort(dev_xq, dev_xc, p, cand, (int)k2, k, q0, rows, dev_idx, dev_dist, x.st);
+        if (cert) launch_knn_cand_max(dots, ld, qn, cn, q0, rows, cand, (int)k2, tau, x.st);
+        launch_knn_rerank_sort(dev_xq, dev_xc, p, cand, (int)k2, k, q0, rows, dev_idx, dev_dist, x.st, tau,
+                               cert ? &ct : nullptr, sq);
     }
     if (ev3) {  // one timed span: screen and rank interleave per query block
         HIPCHK(hipEventRecord(ev3[2], x.st));
@@ -1046,14 +1064,91 @@ static void knn_device_large(Ctx &x, const float *dev_xc, int64_t nc, const floa
     x.check_launch();
 }
 
+static uint32_t read_count(Ctx &x, const uint32_t *dev) {
+    uint32_t m = 0;
+    HIPCHK(hipMemcpyAsync(&m, dev, sizeof(m), hipMemcpyDeviceToHost, x.st));
+    HIPCHK(hipStreamSynchronize(x.st));
+    return m;
+}
+
+// The GEMM recheck of m queries (rows listed on the device): their rows screened again
+// against every candidate by the FP32 GEMM screen (k + max(32, k/8) candidates rounded
+// up to 64, k + 1 in a graph so that the query's own row can be dropped), reranked in
+// FP64 into their output rows, and certified with the GEMM screen's bound; returns how
+// many are still not proven (a candidate within the FP32 screen's error of the k-th
+// distance: a near tie).
+static int64_t knn_recheck_gemm(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int p, int k,
+                                const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist) {
+    const bool graph = dev_xq == nullptr;
+    float *xsub = x.get<float>("knnr_xq", (size_t)m * p);
+    launch_gather_rows(graph ? dev_xc : dev_xq, p, rows, m, xsub, x.st);
+    const int k1 = graph ? (int)std::min<int64_t>(k + 1, nc) : k;
+    uint32_t *ti = x.get<uint32_t>("knnr_ti", (size_t)m * k1);
+    float *td = x.get<float>("knnr_td", (size_t)m * k1);
+    uint32_t *cnt = x.get<uint32_t>("knnr_cnt", 1);
+    launch_fill_u32(cnt, 1, 0u, x.st);
+    knn_device_large(x, dev_xc, nc, xsub, m, p, k1, ti, td, nullptr, cnt);
+    launch_knn_place(ti, td, m, k1, k, rows, graph ? 1 : 0, dev_idx, dev_dist, x.st);
+    x.check_launch();
+    return read_count(x, cnt);
+}
+
+// The queries a screen's certificate did not cover (m rows of the query set, listed
+// on the device).  Their current k-th distance bounds the true one from above, so a
+// threshold scan in FP32 over every candidate (difference form, relative error only;
+// the threshold widened by the roundings of both distances) keeps every candidate
+// that can be among the k nearest; an FP64 rank of those (at most 256 per query)
+// gives the exact k, written into the queries' output rows.  A query with more
+// candidates than that under its threshold (many points at almost the same distance)
+// takes the GEMM recheck.  Returns how many queries stay unproven.
+static int64_t knn_recheck(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int p, int k,
+                           const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist) {
+    const bool graph = dev_xq == nullptr;
+    constexpr int cap = 256;
+    if (p > 256) return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
+    float *xsub = x.get<float>("knnr_xq", (size_t)m * p);
+    float *thr = x.get<float>("knnr_thr", (size_t)m);
+    uint32_t *cnt = x.get<uint32_t>("knnr_scnt", (size_t)m);
+    uint32_t *cand = x.get<uint32_t>("knnr_cand", (size_t)m * cap);
+    uint32_t *ti = x.get<uint32_t>("knnr_si", (size_t)m * k);
+    float *td = x.get<float>("knnr_sd", (size_t)m * k);
+    launch_gather_rows(graph ? dev_xc : dev_xq, p, rows, m, xsub, x.st);
+    launch_fill_u32(cnt, m, 0u, x.st);
+    launch_fill_u32(cand, m * cap, 0xFFFFFFFFu, x.st);
+    launch_knn_thr_scan(dev_xc, nc, p, xsub, m, dev_dist, rows, k, graph ? 1 : 0, thr, cnt, cand, cap, x.st);
+    launch_knn_rerank_sort(xsub, dev_xc, p, cand, cap, k, 0, m, ti, td, x.st);
+    launch_knn_place(ti, td, m, k, k, rows, 0, dev_idx, dev_dist, x.st);
+    x.check_launch();
+    std::vector<uint32_t> hc((size_t)m), hr((size_t)m);
+    x.download(hc.data(), cnt, (size_t)m);
+    x.download(hr.data(), rows, (size_t)m);
+    x.sync();
+    std::vector<uint32_t> over;
+    for (int64_t r = 0; r < m; ++r)
+        if (hc[(size_t)r] > (uint32_t)cap) over.push_back(hr[(size_t)r]);
+    if (over.empty()) return 0;
+    uint32_t *dl = x.get<uint32_t>("knnr_over", over.size());
+    HIPCHK(hipMemcpyAsync(dl, over.data(), over.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
+    x.sync();
+    return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, dl, (int64_t)over.size(), dev_idx, dev_dist);
+}
+
 static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
                        int prec, uint32_t *dev_idx, float *dev_dist, hipEvent_t *ev3) {
     const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
     const int kpad = knn_kpad(kp, p);
+    x.knn_checked = x.knn_rechecked = x.knn_uncertified = 0;
     // k above the register lists, or rows wider than the register screens hold (134
     // columns at FP32, 137 split-BF16, 252 FP16): the any-width FP32 GEMM screen of
-    // k + max(32, k/8) candidates per query, then the same exact rerank
-    if (k > 64 || kpad < 0) return knn_device_large(x, dev_xc, nc, dev_xq, nq, p, k, dev_idx, dev_dist, ev3);
+    // k + max(32, k/8) candidates per query, then the same exact rerank (no certificate)
+    if (k > 64 || kpad < 0) {
+        uint32_t *cnt = x.get<uint32_t>("knn_cert_cnt", 1);
+        launch_fill_u32(cnt, 1, 0u, x.st);
+        knn_device_large(x, dev_xc, nc, dev_xq, nq, p, k, dev_idx, dev_dist, ev3, cnt);
+        x.knn_checked = dev_xq ? nq : nc;
+        x.knn_uncertified = read_count(x, cnt);
+        return;
+    }
     const bool graph = dev_xq == nullptr;
     if (graph) nq = nc;
     const int L = k <= 16 ? 16 : k <= 32 ? 32 : 64;
@@ -1070,8 +1165,17 @@ static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev
     if (!launch_knn_screen(kp, xa, xb, nc, nq, kpad, L, graph ? 1 : 0, o, cd, ci, x.st))
         value_error("no kNN kernel for this width");
     if (ev3) HIPCHK(hipEventRecord(ev3[2], x.st));
+    uint32_t *cnt = x.get<uint32_t>("knn_cert_cnt", 1);
+    uint32_t *unc = x.get<uint32_t>("knn_cert_rows", (size_t)nq);
+    launch_fill_u32(cnt, 1, 0u, x.st);
+    const KnnCert ct = knn_cert(kp, p, kpad, cd, cnt, unc, nq);
     launch_knn_rerank(graph ? dev_xc : dev_xq, nq, dev_xc, nc, p, o, graph ? 1 : 0, ci, 2 * L, k, dev_idx, dev_dist,
-                      x.st);
+                      x.st, nullptr, nullptr, &ct);
+    x.check_launch();
+    const uint32_t m = read_count(x, cnt);
+    x.knn_checked = nq;
+    x.knn_rechecked = m;
+    if (m) x.knn_uncertified = knn_recheck(x, dev_xc, nc, graph ? nullptr : dev_xq, p, k, unc, m, dev_idx, dev_dist);
     if (ev3) HIPCHK(hipEventRecord(ev3[3], x.st));
     x.check_launch();
 }
@@ -1723,8 +1827,43 @@ static void batch_knn_screen(Ctx &x, const BatchPlan &P, const BatchBufs &B) {
 }
 static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &B,
                              const KnnOrder &o) {
+    // the certificate lists uncertified queries by data row (a fill kernel clears the
+    // count, so the launch can be captured)
+    uint32_t *cnt = x.get<uint32_t>(P.pre + "cert_cnt", 1);
+    uint32_t *unc = x.get<uint32_t>(P.pre + "cert_rows", (size_t)P.n);
+    launch_fill_u32(cnt, 1, 0u, x.st);
+    const KnnCert ct = knn_cert(P.kprec, P.kp, P.kpad, B.cd, cnt, unc, P.n);
     launch_knn_rerank(b->pipe.dev_x_pca, P.n_pad, b->pipe.dev_x_pca, P.n_pad, P.kp, o, 1, B.ci, 2 * P.L, P.kn,
-                      b->pipe.dev_knn_idx, b->pipe.dev_knn_dist, x.st, P.d_blk_seg, P.d_off);
+                      b->pipe.dev_knn_idx, b->pipe.dev_knn_dist, x.st, P.d_blk_seg, P.d_off, &ct);
+}
+
+// After the batch's kNN: the queries its certificate did not cover, rechecked subset by
+// subset in FP32 among their own subset's cells (indices within the subset).
+static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan &P) {
+    x.knn_checked = P.n;
+    x.knn_rechecked = x.knn_uncertified = 0;
+    const uint32_t m = read_count(x, x.get<uint32_t>(P.pre + "cert_cnt", 1));
+    if (!m) return;
+    std::vector<uint32_t> rows(m);
+    x.download(rows.data(), x.get<uint32_t>(P.pre + "cert_rows", (size_t)P.n), m);
+    x.sync();
+    std::sort(rows.begin(), rows.end());
+    uint32_t *dl = x.get<uint32_t>("knnr_rows", (size_t)m);
+    size_t a = 0;
+    while (a < rows.size()) {
+        const int y = (int)(std::upper_bound(P.off.begin(), P.off.end(), (int64_t)rows[a]) - P.off.begin()) - 1;
+        const int64_t o0 = P.off[y], o1 = P.off[y + 1];
+        size_t e = a;
+        std::vector<uint32_t> loc;
+        while (e < rows.size() && (int64_t)rows[e] < o1) loc.push_back((uint32_t)(rows[e++] - o0));
+        HIPCHK(hipMemcpyAsync(dl, loc.data(), loc.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
+        x.sync();
+        x.knn_uncertified += knn_recheck(x, b->pipe.dev_x_pca + o0 * P.kp, o1 - o0, nullptr, P.kp, P.kn, dl,
+                                         (int64_t)loc.size(), b->pipe.dev_knn_idx + o0 * P.kn,
+                                         b->pipe.dev_knn_dist + o0 * P.kn);
+        a = e;
+    }
+    x.knn_rechecked = m;
 }
 // Covariance, batched FP64 eigensolve and selection, eig_batch subsets per call
 // (rocSOLVER: eager, never captured).
@@ -2009,6 +2148,8 @@ static std::string batch_key(const crest_batch_args *b, const BatchPlan &P) {
     put(P.off.data(), P.off.size() * sizeof(int64_t));
     const int64_t extra[5] = {P.tile_rows, P.gram_rows, P.eig_batch, (int64_t)P.grid, (int64_t)P.sparse};
     put(extra, sizeof(extra));
+    const double cs = knn_cert_scale();  // the certificate's bound is a kernel argument of the graph
+    put(&cs, sizeof(cs));
     return k;
 }
 
@@ -2087,6 +2228,8 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
     if (b->use_graph && !replay) x.drop_graphs();
     batch_tables(x, P);
     const BatchBufs B = batch_buffers(x, b, P, gene_of_hvg);
+    (void)x.get<uint32_t>(P.pre + "cert_cnt", 1);  // the certificate's buffers exist before any capture
+    (void)x.get<uint32_t>(P.pre + "cert_rows", (size_t)P.n);
     x.sync();
 
     hipEvent_t *ev = x.ev;
@@ -2169,6 +2312,12 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
         HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[7]));
         res->ms_total = tot;
     }
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        batch_knn_recheck(x, b, P);
+        res->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() *
+                         (x.knn_rechecked ? 1.0 : 0.0);
+    }
     res->k_pcs = (size_t)P.kp;
     res->k_nn = (size_t)P.kn;
     res->tile_rows = (size_t)P.tile_rows;
@@ -2235,6 +2384,34 @@ int32_t crest_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int crest_knn_screen_bound(int32_t prec, size_t dim, double *coef, int32_t *kpad) {
+    return guarded([&] {
+        if (!coef || !kpad) value_error("output pointer is NULL");
+        if (prec < -1 || prec > CREST_PREC_FP16) value_error("unknown precision");
+        if (dim > (size_t)INT32_MAX) value_error("dim too large");
+        const int kp = prec == CREST_PREC_FP64 ? CREST_PREC_FP32 : prec;
+        *kpad = prec < 0 ? (int32_t)dim : knn_kpad(kp, (int)dim);
+        const KnnCert c = prec < 0    ? knn_cert_gemm((int)dim, nullptr)
+                          : *kpad < 0 ? KnnCert{}
+                                      : knn_cert(kp, (int)dim, *kpad, nullptr, nullptr, nullptr, 0);
+        coef[0] = c.c1;
+        coef[1] = c.c2;
+        coef[2] = c.c3;
+        coef[3] = c.c4;
+    });
+}
+
+int crest_knn_certificate(int64_t *checked, int64_t *rechecked, int64_t *uncertified) {
+    return guarded([&] {
+        if (!checked || !rechecked || !uncertified) value_error("output pointer is NULL");
+        Ctx &x = ctx();
+        std::lock_guard<std::mutex> lk(x.mu);
+        *checked = x.knn_checked;
+        *rechecked = x.knn_rechecked;
+        *uncertified = x.knn_uncertified;
+    });
 }
 
 int crest_release(void) {

@@ -236,7 +236,26 @@ void launch_shift_fill(const double *colsum, const double *V, int d, int k, doub
 // separate query set (null for a kNN graph).
 struct KnnOrder {
     const uint32_t *perm_c, *perm_q, *keys_c, *keys_q;
+    const float *mean;        // centring of the operands (per subset in a batch: + y*p)
+    const uint32_t *maxbits;  // source of the FP16 prescale (per subset in a batch: + y)
 };
+// Certificate of a screen's candidate lists.  Every candidate a half-list rejected
+// or evicted has a screen value at least the list's final L-th value t_h; the screen
+// value of a pair differs from s^2 |c - q|^2 (s the operands' prescale) by at most
+//   E(a, b) = c1 a b + c2 (a + b)^2 + c3 (a + b) + c4,   a = s|q'|, b = s|c'|
+// (centred norms), and a candidate closer than the k-th reranked distance D_k has
+// |c'| <= |q'| + sqrt(D_k).  So when min(t_0, t_1) - E(a, a + s sqrt(D_k)) > s^2 D_k,
+// no rejected candidate is as close as the k-th: the reported set is the exact one.
+// Queries for which this fails are counted and listed (their output rows).
+struct KnnCert {
+    const float *cand_d;  // the screen's kept values [query position][n_cand], halves ascending
+    double c1, c2, c3, c4;
+    uint32_t *count;      // null: no certificate
+    uint32_t *list;       // the first `cap` uncertified query rows
+    int64_t cap;
+};
+KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count, uint32_t *list, int64_t cap);
+double knn_cert_scale();
 // A batch of independent subsets in one screen: subset y's operand rows are
 // row0[y] .. row0[y] + nrow[y] - 1 (then padding up to a multiple of 512); blk_seg
 // gives the subset of every kSegBlock-row block of operand rows, and order lists the
@@ -256,9 +275,15 @@ KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq,
 int knn_kpad(int prec, int p);
 bool launch_knn_screen(int prec, const void *xa, const void *xb, int64_t nc, int64_t nq, int kpad, int L,
                        int self, const KnnOrder &o, float *cand_d, uint32_t *cand_i, hipStream_t s);
+// qmap (a recheck of a subset of the queries): query row r of xq answers query
+// qmap[r] — its output row, and with `self` the candidate row it must not report.
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o,
                        int self, const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
-                       hipStream_t s, const int32_t *blk_seg = nullptr, const int64_t *cell_off = nullptr);
+                       hipStream_t s, const int32_t *blk_seg = nullptr, const int64_t *cell_off = nullptr,
+                       const KnnCert *cert = nullptr, const uint32_t *qmap = nullptr);
+void launch_gather_rows(const float *x, int p, const uint32_t *rows, int64_t m, float *out, hipStream_t s);
+void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows, int self,
+                      uint32_t *idx_out, float *dist_out, hipStream_t s);
 // Batch forms (subset y = rows cell_off[y] .. cell_off[y+1]-1 of x, operand rows
 // row0[y] .. of n_pad in all; max_rows / max_pad are the largest n_y and padded n_y).
 size_t knn_batch_work_bytes(int64_t n, int64_t n_pad, int n_sub, int p);
@@ -276,8 +301,25 @@ void launch_knn_sqnorm(const float *x, int64_t n, int p, float *out, hipStream_t
 void launch_knn_select_rows(const float *dots, int64_t ld, int64_t nc, const float *qn, const float *cn,
                             int64_t q0, int64_t rows, int self, int k2, int cap, uint32_t *comp_key,
                             uint32_t *comp_idx, uint32_t *cand, hipStream_t s);
+// With tau and cert: the certificate of the GEMM screen (knn_cert_gemm) counted per row.
 void launch_knn_rerank_sort(const float *xq, const float *xc, int p, const uint32_t *cand, int k2, int k,
-                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s);
+                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s,
+                            const float *tau = nullptr, const KnnCert *cert = nullptr,
+                            const float *xq_scr = nullptr);
+// Threshold scan of a recheck (m gathered query rows xq answering output rows rows[]):
+// candidates within the widened current k-th distance appended (at most cap per row,
+// cnt counts all; cand pre-filled with 0xFFFFFFFF, cnt zeroed).
+void launch_knn_thr_scan(const float *xc, int64_t nc, int p, const float *xq, int64_t m, const float *dist,
+                         const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt, uint32_t *cand, int cap,
+                         hipStream_t s);
+// The GEMM screen's operands: x - mean (mean computed from x when compute_mean; part
+// holds knn_centre_parts(n) * p doubles).
+int knn_centre_parts(int64_t n);
+void launch_knn_centre(const float *x, int64_t n, int p, float *mean, bool compute_mean, double *part, float *out,
+                       hipStream_t s);
+void launch_knn_cand_max(const float *dots, int64_t ld, const float *qn, const float *cn, int64_t q0, int64_t rows,
+                         const uint32_t *cand, int k2, float *tau, hipStream_t s);
+KnnCert knn_cert_gemm(int p, uint32_t *count);
 size_t conn_temp_bytes(int64_t m);
 void launch_connectivities(const uint32_t *idx, const float *dist, int64_t n, int k, int n_neighbors,
                            uint64_t *keys, uint64_t *keys2, float *w, float *w2, float *wu, uint32_t *flag,

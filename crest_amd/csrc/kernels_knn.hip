@@ -18,6 +18,9 @@
 // ((i/64)*KC + j)*64 + i%64 (16-bit types; FP32 splits each chunk into two float2
 // halves), so one 64-row tile is one contiguous block that a workgroup copies into
 // LDS and each MFMA fragment read from LDS is a contiguous 512-byte run per half-wave.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -609,6 +612,8 @@ KnnOrder launch_knn_prep(int prec, const float *xc, int64_t nc, const float *xq,
     o.perm_q = w.perm_q;
     o.keys_c = graph ? nullptr : w.keys_c;
     o.keys_q = graph ? nullptr : w.keys_q;
+    o.mean = w.mean;
+    o.maxbits = prec == kPrecFP16 ? w.maxbits : nullptr;
     return o;
 }
 
@@ -703,6 +708,8 @@ KnnOrder launch_knn_prep_batch(int prec, const float *x, int64_t n, const int64_
     o.perm_q = w.perm_pad;
     o.keys_c = nullptr;
     o.keys_q = nullptr;
+    o.mean = w.mean;
+    o.maxbits = prec == kPrecFP16 ? w.maxbits : nullptr;
     return o;
 }
 
@@ -1231,6 +1238,104 @@ __device__ inline float ref_sq_dist(const float *a, const float *b, int p) {
     return s;
 }
 
+// Screen error coefficients (the bound of KnnCert, in the operands' scaled units).
+// K = kpad terms per dot product accumulate in FP32 on MFMA: at most 2 gamma_K of
+// the sum of their magnitudes (allowing a non-IEEE accumulation order), which is at
+// most (a + b)^2; the squared norms are FP32 sums of p squares (gamma_p), and the
+// centring x - mean rounds once (2 u32 (a + b)^2 on the distance); together c2.
+//   FP16: h = fp16(v) = v + e with |e| <= u|v| + eta (u = 2^-11, eta = 2^-25 the
+//     subnormal half-spacing), products exact in FP32: |2 sum(h_c h_q - v_c v_q)|
+//     <= 2(2u + u^2) a b + 2 eta (1 + u) sqrt(p) (a + b) + 2 p eta^2; the norm split
+//     nh + nl loses at most u^2 nn + eta per norm.
+//   split-BF16: v = h + l + r with |r| <= 2^-16 |v|; the missing l l, (h + l) r, r (h + l)
+//     and r r terms total at most 3.01 * 2^-16 |v_c v_q| per coordinate (doubled); the
+//     norm split loses at most 2^-16 nn.
+//   FP32: operands are the centred FP32 values themselves.
+// CREST_KNN_CERT_SCALE (tests) multiplies the bound, so that the recheck paths run on
+// any data.
+double knn_cert_scale() {
+    const char *e = std::getenv("CREST_KNN_CERT_SCALE");
+    const double v = e ? std::atof(e) : 1.0;
+    return v > 0.0 ? v : 1.0;
+}
+
+KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count, uint32_t *list, int64_t cap) {
+    KnnCert c{};
+    c.cand_d = cand_d;
+    c.count = count;
+    c.list = list;
+    c.cap = cap;
+    const double u32 = std::ldexp(1.0, -24);
+    c.c2 = (2.0 * kpad + p + 16) * u32 * 1.01;
+    c.c4 = 1e-30;
+    if (prec == kPrecFP16) {
+        const double u = std::ldexp(1.0, -11), eta = std::ldexp(1.0, -25);
+        c.c1 = 2.0 * (2.0 * u + u * u) * 1.001;
+        c.c2 += u * u;
+        c.c3 = 2.0 * eta * (1.0 + u) * std::sqrt((double)p) * 1.001;
+        c.c4 += 2.0 * p * eta * eta + 2.0 * eta;
+    } else if (prec == kPrecBF16X3) {
+        const double u2 = std::ldexp(1.0, -16);
+        c.c1 = 2.0 * 3.01 * u2;
+        c.c2 += u2;
+    }
+    const double k = knn_cert_scale();
+    c.c1 *= k;
+    c.c2 *= k;
+    c.c3 *= k;
+    c.c4 *= k;
+    return c;
+}
+
+// The GEMM screen of the large-k path: key = fl(fl(|q|^2 + |c|^2) - 2 fl(q.c)) from
+// the rows centred on the candidates' mean (FP32; a = |q'|, b = |c'|, s = 1): the squared norms are sequential FP32
+// sums ((p + 1) u32 each), the SGEMM dot at most 2 gamma_p |q||c| <= gamma_p (a + b)^2 / 2
+// in any accumulation order, and the two final roundings u32 (a + b)^2 each.
+KnnCert knn_cert_gemm(int p, uint32_t *count) {
+    KnnCert c{};
+    c.count = count;
+    const double u32 = std::ldexp(1.0, -24), k = knn_cert_scale();
+    c.c2 = (2.0 * p + 16) * u32 * 1.01 * k;
+    c.c4 = 1e-30 * k;
+    return c;
+}
+
+// The certificate of query position i (one wave; dk = the k-th smallest reranked
+// FP64 squared distance, INFINITY when fewer than k candidates were valid).
+__device__ inline void knn_certify(const KnnCert &ct, const float *xq, int p, const float *mean,
+                                   const uint32_t *maxbits, int64_t i, int n_cand, double dk, uint32_t row,
+                                   int lane) {
+    double part = 0.0;
+    for (int f = lane; f < p; f += 64) {
+        const double v = (double)xq[f] - (double)mean[f];
+        part += v * v;
+    }
+    const double qq = wave_sum_f64(part);
+    if (lane != 0) return;
+    const float *cd = ct.cand_d + i * (int64_t)n_cand;
+    const int L = n_cand / 2;
+    const double t = fmin((double)cd[L - 1], (double)cd[n_cand - 1]);
+    // a half-list that is not full holds every candidate of its half
+    bool ok = !(t < (double)INFINITY);
+    if (!ok && dk < (double)INFINITY) {
+        const double s = maxbits ? (double)fp16_prescale(maxbits) : 1.0;
+        const double a = s * sqrt(qq) * (1.0 + 1e-6);
+        const double b = a + s * sqrt(dk) * (1.0 + 1e-6);
+        const double e = ct.c1 * a * b + ct.c2 * (a + b) * (a + b) + ct.c3 * (a + b) + ct.c4;
+        ok = t - e > s * s * dk * (1.0 + 1e-12);
+    }
+    if (!ok) {
+        const uint32_t at = atomicAdd(ct.count, 1u);
+        if ((int64_t)at < ct.cap) ct.list[at] = row;
+    }
+}
+
+__device__ inline double wave_max_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // One wave per query.  Candidates are ranked by their exact FP64 squared distance
 // (ties by index); the k best are reported with the reference's FP32 distance and
 // ordered by (distance, index) as knn.rs:356-367 does.
@@ -1241,14 +1346,18 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
                                                     const uint32_t *__restrict__ perm, int self,
                                                     const uint32_t *__restrict__ cand_i, int n_cand, int k,
                                                     uint32_t *idx_out, float *dist_out, const int32_t *blk_seg,
-                                                    const int64_t *cell_off) {
+                                                    const int64_t *cell_off, KnnCert ct, const uint32_t *qmap,
+                                                    const float *mean, const uint32_t *maxbits) {
     const int lane = threadIdx.x & 63;
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;  // sorted query position
     if (i >= nq) return;
     const int64_t q = perm_q[i];
     if (q == 0xFFFFFFFFu) return;  // padding row of a batch
     // batch: indices are reported within the query's subset
-    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[blk_seg[i / kSegBlock]] : 0u;
+    const int y = blk_seg ? blk_seg[i / kSegBlock] : 0;
+    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[y] : 0u;
+    // a recheck answers query qmap[q]: its output row, and the row a graph excludes
+    const uint32_t qo = qmap ? qmap[q] : (uint32_t)q;
     const float *xq = xqs + q * (int64_t)p;
     const uint32_t *ci = cand_i + i * (int64_t)n_cand;
     double dd[NCL];
@@ -1262,9 +1371,9 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
     }
     for (int j = 0; j < n_cand; ++j) {
         const uint32_t cpos = ci[j];
-        bool valid = (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+        bool valid = (int64_t)cpos < n && !(self && !qmap && (int64_t)cpos == i);
         const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
-        valid = valid && c != 0xFFFFFFFFu;
+        valid = valid && c != 0xFFFFFFFFu && !(self && qmap && c == qo);
         double part = 0.0;
         if (valid) {
             const float *xc = x + (int64_t)c * p;
@@ -1313,9 +1422,20 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ xq
 #pragma unroll
     for (int s = 0; s < NCL; ++s)
         if (sel[s]) {
-            idx_out[q * k + rank2[s]] = cc[s] - ibase;
-            dist_out[q * k + rank2[s]] = df[s];
+            idx_out[(int64_t)qo * k + rank2[s]] = cc[s] - ibase;
+            dist_out[(int64_t)qo * k + rank2[s]] = df[s];
         }
+    if (ct.count) {
+        double dk = -(double)INFINITY;
+        int ns = 0;
+#pragma unroll
+        for (int s = 0; s < NCL; ++s) {
+            dk = sel[s] ? fmax(dk, dd[s]) : dk;
+            ns += __popcll(__builtin_amdgcn_ballot_w64(sel[s]));
+        }
+        dk = ns < k ? (double)INFINITY : wave_max_f64(dk);
+        knn_certify(ct, xq, p, mean + (int64_t)y * p, maxbits ? maxbits + y : nullptr, i, n_cand, dk, qo, lane);
+    }
 }
 
 // The same selection with LPC lanes per candidate (64/LPC candidate slots, one wave
@@ -1332,7 +1452,8 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
                                                         const uint32_t *__restrict__ perm, int self,
                                                         const uint32_t *__restrict__ cand_i, int n_cand, int k,
                                                         uint32_t *idx_out, float *dist_out, const int32_t *blk_seg,
-                                                        const int64_t *cell_off) {
+                                                        const int64_t *cell_off, KnnCert ct, const uint32_t *qmap,
+                                                        const float *mean, const uint32_t *maxbits) {
     constexpr int W = 8 / LPC;
     constexpr int S = 64 / LPC;
     const int lane = threadIdx.x & 63;
@@ -1341,13 +1462,15 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
     if (i >= nq) return;  // uniform across the wave
     const int64_t q = perm_q[i];
     if (q == 0xFFFFFFFFu) return;  // padding row of a batch
-    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[blk_seg[i / kSegBlock]] : 0u;
+    const int y = blk_seg ? blk_seg[i / kSegBlock] : 0;
+    const uint32_t ibase = blk_seg ? (uint32_t)cell_off[y] : 0u;
+    const uint32_t qo = qmap ? qmap[q] : (uint32_t)q;
     const float *xq = xqs + q * (int64_t)p;
     const bool has = slot < n_cand;
     const uint32_t cpos = has ? cand_i[i * (int64_t)n_cand + slot] : 0xFFFFFFFFu;
-    bool valid = has && (int64_t)cpos < n && !(self && (int64_t)cpos == i);
+    bool valid = has && (int64_t)cpos < n && !(self && !qmap && (int64_t)cpos == i);
     const uint32_t c = valid ? perm[cpos] : 0xFFFFFFFFu;
-    valid = valid && c != 0xFFFFFFFFu;
+    valid = valid && c != 0xFFFFFFFFu && !(self && qmap && c == qo);
     const float *xc = x + (int64_t)(valid ? c : 0u) * p;
     const int m = p - p % 8;
     float acc[W];
@@ -1407,27 +1530,162 @@ __global__ __launch_bounds__(256) void k_knn_rerank_grp(const float *__restrict_
         if (ss && (fs < dfl || (fs == dfl && cs < c))) ++rank2;
     }
     if (sel && sub == 0) {
-        idx_out[q * k + rank2] = c - ibase;
-        dist_out[q * k + rank2] = dfl;
+        idx_out[(int64_t)qo * k + rank2] = c - ibase;
+        dist_out[(int64_t)qo * k + rank2] = dfl;
+    }
+    if (ct.count) {
+        const bool one = sel && sub == 0;
+        const int ns = __popcll(__builtin_amdgcn_ballot_w64(one));
+        const double dk = ns < k ? (double)INFINITY : wave_max_f64(one ? d : -(double)INFINITY);
+        knn_certify(ct, xq, p, mean + (int64_t)y * p, maxbits ? maxbits + y : nullptr, i, n_cand, dk, qo, lane);
     }
 }
 
 void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc, int p, const KnnOrder &o, int self,
                        const uint32_t *cand_i, int n_cand, int k, uint32_t *idx_out, float *dist_out,
-                       hipStream_t s, const int32_t *blk_seg, const int64_t *cell_off) {
+                       hipStream_t s, const int32_t *blk_seg, const int64_t *cell_off, const KnnCert *cert,
+                       const uint32_t *qmap) {
     const int g = (int)ceil_div(nq, 4);  // one wave per query, four per workgroup
+    const KnnCert ct = cert ? *cert : KnnCert{};
+    const float *mean = o.mean;
+    const uint32_t *maxbits = o.maxbits;
     if (n_cand <= 16)
         hipLaunchKernelGGL(k_knn_rerank_grp<4>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off, ct, qmap, mean,
+                           maxbits);
     else if (n_cand <= 32)
         hipLaunchKernelGGL(k_knn_rerank_grp<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off, ct, qmap, mean,
+                           maxbits);
     else if (n_cand <= 64)
         hipLaunchKernelGGL(k_knn_rerank<1>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off, ct, qmap, mean,
+                           maxbits);
     else
         hipLaunchKernelGGL(k_knn_rerank<2>, dim3(g), dim3(256), 0, s, xq, nq, xc, nc, p, o.perm_q, o.perm_c, self,
-                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off);
+                           cand_i, n_cand, k, idx_out, dist_out, blk_seg, cell_off, ct, qmap, mean,
+                           maxbits);
+}
+
+// out[r] = x[rows[r]] (p floats each): the query rows of a recheck.
+__global__ void k_gather_rows(const float *x, int p, const uint32_t *rows, int64_t m, float *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * p) return;
+    const int64_t r = e / p;
+    out[e] = x[(int64_t)rows[r] * p + (e - r * p)];
+}
+
+void launch_gather_rows(const float *x, int p, const uint32_t *rows, int64_t m, float *out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)ceil_div(m * p, 256)), dim3(256), 0, s, x, p, rows, m, out);
+}
+
+// Row r's k1 nearest (ti/td, in output order) written as output row rows[r]: with
+// `self`, the entry of row rows[r] itself is dropped and the first k others kept.
+__global__ void k_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows,
+                            int self, uint32_t *idx_out, float *dist_out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const uint32_t q = rows[r];
+    int o = 0;
+    for (int j = 0; j < k1 && o < k; ++j) {
+        const uint32_t c = ti[r * k1 + j];
+        if (self && c == q) continue;
+        idx_out[(int64_t)q * k + o] = c;
+        dist_out[(int64_t)q * k + o] = td[r * k1 + j];
+        ++o;
+    }
+}
+
+void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows, int self,
+                      uint32_t *idx_out, float *dist_out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_knn_place, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, ti, td, m, k1, k, rows, self,
+                       idx_out, dist_out);
+}
+
+// Threshold scan of a recheck: every candidate whose FP32 squared distance
+// sum_f (q_f - c_f)^2 (difference form: relative error at most (p + 3) u32, no
+// cancellation) is at most thr[q] is appended to query q's list (the query's own row
+// excluded when self_rows is given).  One workgroup per (64-query block, candidate
+// slice); the query rows sit in LDS (read as broadcasts), each thread takes every
+// 256th candidate of the slice and keeps the 64 partial sums in registers.  The LDS
+// rows take 64 * 4 * (p rounded up to 8) bytes: at most 64 KiB for p <= 256.
+constexpr int kScanQ = 64;
+__global__ __launch_bounds__(256) void k_knn_thr_scan(const float *__restrict__ xc, int64_t nc, int p,
+                                                      const float *__restrict__ xq, int64_t m,
+                                                      const float *__restrict__ thr,
+                                                      const uint32_t *__restrict__ self_rows, int64_t slice,
+                                                      uint32_t *cnt, uint32_t *cand, int cap) {
+    extern __shared__ float sq[];  // [kScanQ][pp], rows zero-padded to pp = p rounded up to 8
+    __shared__ float sthr[kScanQ];
+    __shared__ uint32_t sself[kScanQ];
+    const int pp = (p + 7) / 8 * 8;
+    const int64_t q0 = (int64_t)blockIdx.y * kScanQ;
+    const int nqb = (int)(m - q0 < kScanQ ? m - q0 : kScanQ);
+    for (int e = threadIdx.x; e < kScanQ * pp; e += blockDim.x) {
+        const int j = e / pp, f = e - j * pp;
+        sq[e] = (j < nqb && f < p) ? xq[(q0 + j) * p + f] : 0.0f;
+    }
+    if (threadIdx.x < kScanQ) {
+        const int j = threadIdx.x;
+        sthr[j] = j < nqb ? thr[q0 + j] : -1.0f;  // no candidate passes a negative threshold
+        sself[j] = (j < nqb && self_rows) ? self_rows[q0 + j] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const int64_t c0 = (int64_t)blockIdx.x * slice, c1 = c0 + slice < nc ? c0 + slice : nc;
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const float *cr = xc + c * p;
+        float acc[kScanQ];
+#pragma unroll
+        for (int j = 0; j < kScanQ; ++j) acc[j] = 0.0f;
+        for (int f0 = 0; f0 < pp; f0 += 8) {
+            float cv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) cv[t] = f0 + t < p ? cr[f0 + t] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < kScanQ; ++j)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const float d = sq[j * pp + f0 + t] - cv[t];
+                    acc[j] = fmaf(d, d, acc[j]);
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < kScanQ; ++j) {
+            if (!(acc[j] <= sthr[j]) || (uint32_t)c == sself[j]) continue;
+            const uint32_t at = atomicAdd(&cnt[q0 + j], 1u);
+            if ((int64_t)at < cap) cand[(q0 + j) * cap + at] = (uint32_t)c;
+        }
+    }
+}
+
+// thr[r] = the square of output row rows[r]'s current k-th distance (the reference's
+// f32 formula, relative error at most (p/8 + 10) u32 after the square root is undone),
+// widened by (4p + 64) u32 so that it covers the scan's rounding too: every candidate
+// at most as far as the true k-th neighbour passes.
+__global__ void k_knn_scan_thr(const float *dist, const uint32_t *rows, int64_t m, int k, int p, float *thr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const double dk = (double)dist[(int64_t)rows[r] * k + (k - 1)];
+    const double t = dk * dk * (1.0 + (4.0 * p + 64.0) * 0x1p-24) + 1e-37;
+    float tf = (float)t;
+    if ((double)tf < t) tf = nextafterf(tf, INFINITY);
+    thr[r] = tf;
+}
+
+void launch_knn_thr_scan(const float *xc, int64_t nc, int p, const float *xq, int64_t m, const float *dist,
+                         const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt, uint32_t *cand, int cap,
+                         hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_knn_scan_thr, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, dist, rows, m, k, p, thr);
+    // about 2,048 workgroups in all: slices of at least 4,096 candidates
+    const int64_t qblocks = ceil_div(m, kScanQ);
+    const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
+    const int64_t slice = std::max<int64_t>(4096, ceil_div(nc, want));
+    const size_t lds = (size_t)kScanQ * ((p + 7) / 8 * 8) * sizeof(float);
+    hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)qblocks), dim3(256), lds, s, xc,
+                       nc, p, xq, m, thr, self ? rows : nullptr, slice, cnt, cand, cap);
 }
 
 // ---------------------------------------------------------------- large k
@@ -1450,6 +1708,46 @@ __global__ void k_knn_sqnorm(const float *__restrict__ x, int64_t n, int p, floa
         s = s + sq;
     }
     out[i] = s;
+}
+
+// Centring for the GEMM screen (any width): column sums of row parts in FP64 (thread =
+// column), summed in part order, and the centred copy x - mean.
+__global__ void k_col_sum_any(const float *x, int64_t n, int p, int64_t rows_per, double *part) {
+    const int f = blockIdx.y * 256 + threadIdx.x;
+    if (f >= p) return;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    double acc = 0.0;
+    for (int64_t r = r0; r < r1; ++r) acc += (double)x[r * p + f];
+    part[(int64_t)blockIdx.x * p + f] = acc;
+}
+
+__global__ void k_col_mean_any(const double *part, int nparts, int64_t n, int p, float *mean) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= p) return;
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * p + f];
+    mean[f] = (float)(s / (double)n);
+}
+
+__global__ void k_centre_rows(const float *x, int64_t n, int p, const float *mean, float *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * p) return;
+    out[e] = x[e] - mean[e % p];
+}
+
+int knn_centre_parts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
+
+void launch_knn_centre(const float *x, int64_t n, int p, float *mean, bool compute_mean, double *part, float *out,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned cy = (unsigned)ceil_div(p, 256);
+    if (compute_mean) {
+        const int parts = knn_centre_parts(n);
+        const int64_t rows_per = ceil_div(n, parts);
+        hipLaunchKernelGGL(k_col_sum_any, dim3((unsigned)parts, cy), dim3(256), 0, s, x, n, p, rows_per, part);
+        hipLaunchKernelGGL(k_col_mean_any, dim3(cy), dim3(256), 0, s, part, parts, n, p, mean);
+    }
+    hipLaunchKernelGGL(k_centre_rows, dim3((unsigned)ceil_div(n * p, 256)), dim3(256), 0, s, x, n, p, mean, out);
 }
 
 // Non-negative f32 distances order like their bit patterns.
@@ -1736,7 +2034,9 @@ __device__ void lds_bitonic(double *sd, uint32_t *si, int n2) {
 __global__ __launch_bounds__(256) void k_knn_rerank_sort(const float *__restrict__ xq_all,
                                                          const float *__restrict__ xc, int p,
                                                          const uint32_t *__restrict__ cand, int k2, int n2, int k,
-                                                         int64_t q0, uint32_t *idx_out, float *dist_out) {
+                                                         int64_t q0, uint32_t *idx_out, float *dist_out,
+                                                         const float *__restrict__ tau, KnnCert ct,
+                                                         const float *__restrict__ xq_scr) {
     extern __shared__ double lds_d[];
     double *sd = lds_d;
     uint32_t *si = reinterpret_cast<uint32_t *>(lds_d + n2);
@@ -1746,7 +2046,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank_sort(const float *__restrict
     for (int j = threadIdx.x; j < n2; j += blockDim.x) {
         double d = INFINITY;
         uint32_t c = 0xFFFFFFFFu;
-        if (j < k2) {
+        if (j < k2 && ci[j] != 0xFFFFFFFFu) {  // an empty slot of a threshold scan's list
             c = ci[j];
             const float *xr = xc + (int64_t)c * p;
             double acc = 0.0;
@@ -1761,6 +2061,8 @@ __global__ __launch_bounds__(256) void k_knn_rerank_sort(const float *__restrict
     }
     __syncthreads();
     lds_bitonic(sd, si, n2);
+    const double dk = sd[k - 1];  // the k-th smallest FP64 squared distance (INFINITY: fewer valid)
+    __syncthreads();
     int m = 1;
     while (m < k) m <<= 1;
     for (int j = threadIdx.x; j < m; j += blockDim.x) {
@@ -1779,6 +2081,46 @@ __global__ __launch_bounds__(256) void k_knn_rerank_sort(const float *__restrict
         idx_out[q * k + j] = si[j];
         dist_out[q * k + j] = (float)sd[j];
     }
+    // certificate (KnnCert with s = 1, a = |q| of the screen's (centred) row; every
+    // candidate the select dropped has a key >= tau, the largest kept key)
+    if (ct.count && threadIdx.x == 0) {
+        const double t = (double)tau[blockIdx.x];
+        bool ok = !(t < (double)INFINITY);
+        if (!ok && dk < (double)INFINITY) {
+            const float *xs = xq_scr + q * (int64_t)p;
+            double qq = 0.0;
+            for (int f = 0; f < p; ++f) qq += (double)xs[f] * (double)xs[f];
+            const double a = sqrt(qq) * (1.0 + 1e-6);
+            const double b = a + sqrt(dk) * (1.0 + 1e-6);
+            const double e = ct.c1 * a * b + ct.c2 * (a + b) * (a + b) + ct.c3 * (a + b) + ct.c4;
+            ok = t - e > dk * (1.0 + 1e-12);
+        }
+        if (!ok) atomicAdd(ct.count, 1u);
+    }
+}
+
+// tau[r] = the largest screen key among query row r's k2 selected candidates (every
+// candidate the select dropped has a key at least this).
+__global__ void k_knn_cand_max(const float *__restrict__ dots, int64_t ld, const float *__restrict__ qn_all,
+                               const float *__restrict__ cn, int64_t q0, int64_t rows, const uint32_t *cand, int k2,
+                               float *tau) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float qn = qn_all[q0 + r];
+    uint32_t mx = 0;
+    for (int j = 0; j < k2; ++j) {
+        const uint32_t c = cand[r * k2 + j];
+        const uint32_t key = screen_key(dots[r * ld + c], qn, cn[c]);
+        mx = key > mx ? key : mx;
+    }
+    tau[r] = __uint_as_float(mx);
+}
+
+void launch_knn_cand_max(const float *dots, int64_t ld, const float *qn, const float *cn, int64_t q0, int64_t rows,
+                         const uint32_t *cand, int k2, float *tau, hipStream_t s) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_knn_cand_max, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, s, dots, ld, qn, cn, q0,
+                       rows, cand, k2, tau);
 }
 
 void launch_knn_sqnorm(const float *x, int64_t n, int p, float *out, hipStream_t s) {
@@ -1795,13 +2137,15 @@ void launch_knn_select_rows(const float *dots, int64_t ld, int64_t nc, const flo
 }
 
 void launch_knn_rerank_sort(const float *xq, const float *xc, int p, const uint32_t *cand, int k2, int k,
-                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s) {
+                            int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s,
+                            const float *tau, const KnnCert *cert, const float *xq_scr) {
     if (rows <= 0) return;
     int n2 = 1;
     while (n2 < k2) n2 <<= 1;
     const size_t lds = (size_t)n2 * (sizeof(double) + sizeof(uint32_t));
+    const KnnCert ct = (cert && tau && xq_scr) ? *cert : KnnCert{};
     hipLaunchKernelGGL(k_knn_rerank_sort, dim3((unsigned)rows), dim3(256), lds, s, xq, xc, p, cand, k2, n2, k, q0,
-                       idx_out, dist_out);
+                       idx_out, dist_out, tau, ct, xq_scr);
 }
 
 }  // namespace crest

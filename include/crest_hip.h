@@ -148,11 +148,16 @@ int crest_project(const crest_chunk *chunk, const int32_t *sub_map, size_t n_sub
  * data is [n, dim] row-major f32.  k' = min(k, n-1) (>= 1 when n > 1) is written to
  * *k_out; indices [n, k'] u32 and distances [n, k'] f32 (Euclidean, ascending, ties
  * by index, self excluded) must hold n*min(k, max(n-1,1)) entries.  The search is
- * exact for every n: `exact` (-1 = None, 0, 1) and `seed` are accepted and ignored.
- * Any dim.  k' <= 64 uses the screen precision of crest_set_knn_precision; 64 < k'
- * <= 3584 (Scrublet's k, doublets.py:131-137), or rows wider than the register
- * screens hold (134 columns FP32/FP64, 137 split-BF16, 252 FP16), screen in FP32 on
- * GEMM dot products; larger k' is a ValueError.                                   */
+ * brute force for every n (no IVF): `exact` (-1 = None, 0, 1) and `seed` are
+ * accepted and ignored.  Any dim.  k' <= 64 uses the screen precision of
+ * crest_set_knn_precision, and every query's set is certified: the screen's error
+ * bound proves that no candidate it dropped is as close as the k-th reranked
+ * neighbour, and the queries it cannot prove are screened again in FP32 and
+ * certified again (crest_knn_certificate counts them).  64 < k' <= 3584 (Scrublet's
+ * k, doublets.py:131-137), or rows wider than the register screens hold (134 columns
+ * FP32/FP64, 137 split-BF16, 252 FP16), screen in FP32 on GEMM dot products, whose
+ * sets are certified against that screen's bound (no recheck); larger k' is a
+ * ValueError.                                                                     */
 int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t exact,
                     uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
 
@@ -161,10 +166,28 @@ int crest_knn_graph(const float *data, size_t n, size_t dim, size_t k, int32_t e
  * is [n_ref, dim] and query [n_query, dim], both row-major f32.  k' = min(k, n_ref)
  * (0 when n_query == 0) is written to *k_out; indices [n_query, k'] u32 and
  * distances [n_query, k'] f32 (Euclidean, ascending, ties by index) must hold
- * n_query*min(k, n_ref) entries.  Exact for every size: `exact` and `seed` are
- * accepted and ignored.                                                          */
+ * n_query*min(k, n_ref) entries.  Brute force with the certificate of
+ * crest_knn_graph: `exact` and `seed` are accepted and ignored.                  */
 int crest_knn_query(const float *reference, size_t n_ref, const float *query, size_t n_query, size_t dim,
                     size_t k, int32_t exact, uint64_t seed, uint32_t *indices, float *distances, size_t *k_out);
+
+/* The certificate of the last crest_knn_graph / crest_knn_query / pipeline kNN on
+ * this device (no reference counterpart: the reference searches in f32 throughout).
+ * checked = queries whose screened set was certified or rechecked; rechecked = those
+ * the screen's bound could not certify, screened again by the FP32 GEMM screen;
+ * uncertified = those still not proven after that (a candidate within the FP32
+ * screen's error of the k-th distance, i.e. a near tie).  k' > 64 and rows too wide
+ * for the register screens search by the GEMM screen directly: rechecked = 0 and
+ * uncertified = the queries its bound does not prove.                             */
+int crest_knn_certificate(int64_t *checked, int64_t *rechecked, int64_t *uncertified);
+
+/* The certificate's bound for a screen precision and row width (host only, no device
+ * needed): coef[4] = (c1, c2, c3, c4) with |screen(q, c) - s^2 |c - q|^2| <=
+ * c1 a b + c2 (a + b)^2 + c3 (a + b) + c4, a = s|q - mean|, b = s|c - mean| (s the
+ * FP16 power-of-two prescale, 1 otherwise); *kpad = the operand width (-1: no
+ * register screen at this width).  prec = -1: the FP32 GEMM screen of the large-k
+ * path (rows centred the same way; *kpad = dim).  No reference counterpart.       */
+int crest_knn_screen_bound(int32_t prec, size_t dim, double *coef, int32_t *kpad);
 
 /* Replaces py.rs:72-99 `connectivities(indices, distances, n_neighbors)`
  * (umap/graph.rs:29-107, scanpy's fuzzy simplicial set).  indices/distances are

@@ -318,3 +318,28 @@ def test_batch_filtered_subspace_matches_tridiagonal(oracle, monkeypatch, capfd,
         np.testing.assert_allclose(o.variance[i], var_o, rtol=1e-9, err_msg=f"subset {i}")
         assert abs(o.total_variance[i] - tot_o) <= 1e-11 * tot_o
         assert _angle(oracle, o.loadings[i], load_o, 9) < 1e-9, i
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_knn_certificate(batch, oracle, monkeypatch, use_graph):
+    """The batch's kNN certificate, then with a bound scaled by 1e9 its recheck: every
+    query of a full screen list rescreened in FP32 among its own subset's cells (the
+    31-cell subset's lists are not full, so it stays certified), results still exact."""
+    import crest_amd as ca
+    n = int(batch[3][-1])
+    _run(batch, prec_gram="fp64", prec_knn="fp16", use_graph=use_graph)
+    c0 = ca.knn_certificate()
+    assert c0["checked"] == n and c0["uncertified"] == 0, c0
+    monkeypatch.setenv("CREST_KNN_CERT_SCALE", "1e9")
+    pipe = _run(batch, prec_gram="fp64", prec_knn="fp16", use_graph=use_graph)[0]
+    for rep in range(2):  # the second run of the same pipeline replays its graphs
+        if rep:
+            pipe.run()
+        X, idx = pipe.x_pca(), pipe.knn()[0]
+        c1 = ca.knn_certificate()
+        assert c1["checked"] == n and c1["rechecked"] >= n - SIZES[0], c1
+        assert c1["uncertified"] <= c1["rechecked"], c1
+        off = batch[3]
+        for i in range(len(SIZES)):
+            a, b = int(off[i]), int(off[i + 1])
+            _check_knn(oracle, X[a:b], idx[a:b].astype(np.int64), b - a)

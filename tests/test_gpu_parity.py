@@ -391,6 +391,109 @@ def test_knn_query_edges():
         N.knn_query(R.astype(np.float64), R, 2)
 
 
+def _far_clusters(n, p, R, seed):
+    """Two tight clusters at +-R along the first axis: every row far from the mean and
+    its neighbours close, so the screens' rounding (relative to |x'|^2) exceeds the
+    gaps between neighbour distances and the certificate has to recheck."""
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, p))
+    X[:, 0] += np.where(np.arange(n) % 2 == 0, R, -R)
+    return X.astype(np.float32)
+
+
+def _check_graph(oracle, x, idx, dist, k):
+    n = x.shape[0]
+    assert idx.shape == (n, k) and not (idx == np.arange(n)[:, None]).any()
+    ref_i, ref_d2 = oracle.knn_exact_f64(x, k)
+    bad, ties = oracle.knn_mismatch_ties(idx, x, np.arange(n), ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    pd = oracle.pair_dist(x, np.repeat(np.arange(n), k), idx.ravel()).reshape(n, k)
+    np.testing.assert_array_equal(dist, pd)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16"])
+def test_knn_certificate_recheck(oracle, prec):
+    """Rows 225 sd from the mean: the FP16 and split-BF16 screens cannot prove their
+    sets, the FP32 recheck with 128 candidates proves them all, and the sets equal
+    the exact FP64 search."""
+    n, k = 3000, 15
+    x = _far_clusters(n, 20, 225.0, 5)
+    ca.set_knn_precision(prec)
+    try:
+        idx, dist = N.knn_graph(x, k)
+        cert = ca.knn_certificate()
+    finally:
+        ca.set_knn_precision(None)
+    assert cert["checked"] == n and cert["uncertified"] == 0, cert
+    if prec != "fp64":
+        assert cert["rechecked"] > n // 10, cert
+    _check_graph(oracle, x, idx, dist, k)
+
+
+def test_knn_certificate_recheck_wide(oracle):
+    """200 columns: the FP16 screen holds them and rechecks what it cannot prove; the
+    FP32 GEMM screen of the large-k path certifies its own sets (k = 100)."""
+    n, k = 2000, 15
+    x = _far_clusters(n, 200, 225.0, 6)
+    ca.set_knn_precision("fp16")
+    try:
+        idx, dist = N.knn_graph(x, k)
+        cert = ca.knn_certificate()
+        il, dl = N.knn_graph(x, 100)
+        cl = ca.knn_certificate()
+    finally:
+        ca.set_knn_precision(None)
+    assert cert["checked"] == n and cert["rechecked"] > 0 and cert["uncertified"] == 0, cert
+    _check_graph(oracle, x, idx, dist, k)
+    assert cl["checked"] == n and cl["rechecked"] == 0 and cl["uncertified"] <= n // 100, cl
+    _check_graph(oracle, x, il, dl, 100)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
+def test_knn_certificate_forced_recheck(oracle, prec, monkeypatch):
+    """A bound scaled by 1e9 proves nothing: every query is rechecked (graph and query
+    search) by the threshold scan, whose answers are exact, so none stays unproven."""
+    monkeypatch.setenv("CREST_KNN_CERT_SCALE", "1e9")
+    x = _clustered(2500, 30, 11)
+    ca.set_knn_precision(prec)
+    try:
+        idx, dist = N.knn_graph(x, 15)
+        cg = ca.knn_certificate()
+        R, Q = _clustered(3000, 30, 12), (_clustered(700, 30, 13) + np.float32(0.5)).astype(np.float32)
+        qi, qd = N.knn_query(R, Q, 20)
+        cq = ca.knn_certificate()
+    finally:
+        ca.set_knn_precision(None)
+    assert cg == {"checked": 2500, "rechecked": 2500, "uncertified": 0}, cg
+    assert cq == {"checked": 700, "rechecked": 700, "uncertified": 0}, cq
+    _check_graph(oracle, x, idx, dist, 15)
+    ref_i, ref_d2 = oracle.knn_query_exact_f64(R, Q, 20)
+    bad, ties = oracle.knn_query_mismatch_ties(qi, R, Q, ref_i, ref_d2)
+    assert bad == ties, (bad, ties)
+    X = np.vstack([Q, R])
+    pd = oracle.pair_dist(X, np.repeat(np.arange(700), 20), 700 + qi.ravel()).reshape(700, 20)
+    np.testing.assert_array_equal(qd, pd)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_knn_certificate_duplicates(oracle, prec):
+    """700 copies of one row: their lists hold only zero distances, the screen cannot
+    prove them, the threshold scan finds 699 candidates at distance 0 (more than its
+    256 slots) and the GEMM recheck answers them; every set is right up to ties."""
+    x = _clustered(2000, 16, 21)
+    x[:700] = x[0]
+    ca.set_knn_precision(prec)
+    try:
+        idx, dist = N.knn_graph(x, 15)
+        cert = ca.knn_certificate()
+    finally:
+        ca.set_knn_precision(None)
+    assert cert["checked"] == 2000 and cert["rechecked"] >= 700, cert
+    assert cert["uncertified"] <= cert["rechecked"], cert
+    assert (dist[:700] == 0).all()
+    _check_graph(oracle, x, idx, dist, 15)
+
+
 def test_knn_small_edge_cases():
     idx, dist = N.knn_graph(np.zeros((1, 4), np.float32), 5)
     assert idx.shape == (1, 0)
