@@ -1095,42 +1095,145 @@ static int64_t knn_recheck_gemm(Ctx &x, const float *dev_xc, int64_t nc, const f
 
 // The queries a screen's certificate did not cover (m rows of the query set, listed
 // on the device).  Their current k-th distance bounds the true one from above, so a
-// threshold scan in FP32 over every candidate (difference form, relative error only;
-// the threshold widened by the roundings of both distances) keeps every candidate
-// that can be among the k nearest; an FP64 rank of those (at most 256 per query)
+// threshold scan in FP32 over every candidate (rows centred on the candidates' mean,
+// the GEMM screen's key; the threshold widened by the roundings of both distances and
+// the key's bound) keeps every candidate that can be among the k nearest; an FP64
+// rank of those (at most 256 per query)
 // gives the exact k, written into the queries' output rows.  A query with more
 // candidates than that under its threshold (many points at almost the same distance)
 // takes the GEMM recheck.  Returns how many queries stay unproven.
-static int64_t knn_recheck(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int p, int k,
-                           const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist) {
+// The threshold scan's products: FP32 MFMA (0, rows of at most 96 columns), rocBLAS
+// dot products filtered in one pass (1, wider rows), or the VALU kernel (2).
+// CREST_KNN_SCAN=gemm / valu picks the second / third (same thresholds, same
+// candidates).
+static int knn_scan_kind() {
+    const char *e = std::getenv("CREST_KNN_SCAN");
+    if (e && std::strcmp(e, "gemm") == 0) return 1;
+    if (e && std::strcmp(e, "valu") == 0) return 2;
+    return 0;
+}
+
+// Buffers of a recheck of m queries against at most nc_max candidates (per-query arrays
+// are indexed by the query's place in the recheck; the candidate arrays are reused by
+// successive rechecks on the stream).
+constexpr int kRecheckCap = 256;
+struct RecheckBufs {
+    float *xsub, *xsubc, *qn, *thr, *td, *xcc, *cn, *mean;
+    uint32_t *cnt, *cand, *ti;
+    double *part;
+};
+static RecheckBufs recheck_bufs(Ctx &x, int64_t m, int64_t nc_max, int p, int k) {
+    RecheckBufs B;
+    B.xsub = x.get<float>("knnr_xq", (size_t)m * p);
+    B.xsubc = x.get<float>("knnr_xqc", (size_t)m * p);
+    B.qn = x.get<float>("knnr_qn", (size_t)m);
+    B.thr = x.get<float>("knnr_thr", (size_t)m);
+    B.cnt = x.get<uint32_t>("knnr_scnt", (size_t)m);
+    B.cand = x.get<uint32_t>("knnr_cand", (size_t)m * kRecheckCap);
+    B.ti = x.get<uint32_t>("knnr_si", (size_t)m * k);
+    B.td = x.get<float>("knnr_sd", (size_t)m * k);
+    B.xcc = x.get<float>("knnr_xcc", (size_t)nc_max * p);
+    B.cn = x.get<float>("knnr_cn", (size_t)nc_max);
+    B.mean = x.get<float>("knnr_mean", (size_t)p);
+    B.part = x.get<double>("knnr_part", (size_t)knn_centre_parts(nc_max) * p);
+    return B;
+}
+// The same buffers for the queries from place a on.
+static RecheckBufs recheck_slice(RecheckBufs B, int64_t a, int p, int k) {
+    B.xsub += a * p;
+    B.xsubc += a * p;
+    B.qn += a;
+    B.thr += a;
+    B.cnt += a;
+    B.cand += a * kRecheckCap;
+    B.ti += a * k;
+    B.td += a * k;
+    return B;
+}
+
+// The queries a screen's certificate did not cover (m rows of the query set, listed
+// on the device; p <= 256).  Their current k-th distance bounds the true one from
+// above, so a threshold scan in FP32 over every candidate (rows centred on the
+// candidates' mean, the GEMM screen's key; the threshold widened by the roundings of
+// both distances and the key's bound) keeps every candidate that can be among the k
+// nearest; an FP64 rank of those (at most 256 per query) gives the exact k, written
+// into the queries' output rows.  Launches only: cnt (per query, the candidates under
+// its threshold) tells afterwards which queries overflowed their lists.  `mean`: the
+// candidates' column means if known (the screen's centring), else computed.
+static void knn_recheck_launch(Ctx &x, const RecheckBufs &B, const float *dev_xc, int64_t nc, const float *dev_xq,
+                               int p, int k, const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist,
+                               const float *mean_known) {
     const bool graph = dev_xq == nullptr;
-    constexpr int cap = 256;
-    if (p > 256) return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
-    float *xsub = x.get<float>("knnr_xq", (size_t)m * p);
-    float *thr = x.get<float>("knnr_thr", (size_t)m);
-    uint32_t *cnt = x.get<uint32_t>("knnr_scnt", (size_t)m);
-    uint32_t *cand = x.get<uint32_t>("knnr_cand", (size_t)m * cap);
-    uint32_t *ti = x.get<uint32_t>("knnr_si", (size_t)m * k);
-    float *td = x.get<float>("knnr_sd", (size_t)m * k);
-    launch_gather_rows(graph ? dev_xc : dev_xq, p, rows, m, xsub, x.st);
-    launch_fill_u32(cnt, m, 0u, x.st);
-    launch_fill_u32(cand, m * cap, 0xFFFFFFFFu, x.st);
-    launch_knn_thr_scan(dev_xc, nc, p, xsub, m, dev_dist, rows, k, graph ? 1 : 0, thr, cnt, cand, cap, x.st);
-    launch_knn_rerank_sort(xsub, dev_xc, p, cand, cap, k, 0, m, ti, td, x.st);
-    launch_knn_place(ti, td, m, k, k, rows, 0, dev_idx, dev_dist, x.st);
+    constexpr int cap = kRecheckCap;
+    float *mean = mean_known ? const_cast<float *>(mean_known) : B.mean;  // read only when known
+    launch_gather_rows(graph ? dev_xc : dev_xq, p, rows, m, B.xsub, x.st);
+    launch_knn_centre(dev_xc, nc, p, mean, mean_known == nullptr, B.part, B.xcc, x.st);
+    launch_knn_centre(B.xsub, m, p, mean, false, B.part, B.xsubc, x.st);
+    launch_knn_sqnorm(B.xcc, nc, p, B.cn, x.st);
+    launch_knn_sqnorm(B.xsubc, m, p, B.qn, x.st);
+    launch_fill_u32(B.cnt, m, 0u, x.st);
+    launch_fill_u32(B.cand, m * cap, 0xFFFFFFFFu, x.st);
+    const int scan = knn_scan_kind();
+    if (scan == 0 && p <= 96) {
+        launch_knn_scan_thresholds(dev_dist, rows, B.xsubc, m, k, p, B.thr, x.st);
+        launch_knn_thr_mfma(B.xcc, B.cn, nc, p, B.xsubc, B.qn, m, B.thr, rows, graph ? 1 : 0, B.cnt, B.cand, cap,
+                            x.st);
+    } else if (scan != 2) {
+        // the dot products of blocks of the rechecked rows with every candidate by
+        // rocBLAS (<= 2 GiB per block), each block filtered in one pass
+        launch_knn_scan_thresholds(dev_dist, rows, B.xsubc, m, k, p, B.thr, x.st);
+        const int64_t ld = (nc + 3) / 4 * 4;
+        const int64_t qb = std::max<int64_t>(1, std::min<int64_t>(m, ((int64_t)1 << 29) / ld));
+        float *dots = x.get<float>("knnl_dots", (size_t)qb * ld);
+        const float one = 1.0f, zero = 0.0f;
+        for (int64_t q0 = 0; q0 < m; q0 += qb) {
+            const int64_t nr = std::min(qb, m - q0);
+            BLASCHK(rocblas_sgemm(x.blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nc,
+                                  (rocblas_int)nr, p, &one, B.xcc, p, B.xsubc + q0 * p, p, &zero, dots,
+                                  (rocblas_int)ld));
+            launch_knn_thr_filter(dots, ld, nc, B.cn, B.qn, B.thr, rows, graph ? 1 : 0, q0, nr, B.cnt, B.cand, cap,
+                                  x.st);
+        }
+    } else {
+        launch_knn_thr_scan(B.xcc, B.cn, nc, p, B.xsubc, B.qn, m, dev_dist, rows, k, graph ? 1 : 0, B.thr, B.cnt,
+                            B.cand, cap, x.st);
+    }
+    launch_knn_rerank_sort(B.xsub, dev_xc, p, B.cand, cap, k, 0, m, B.ti, B.td, x.st);
+    launch_knn_place(B.ti, B.td, m, k, k, rows, 0, dev_idx, dev_dist, x.st);
     x.check_launch();
-    std::vector<uint32_t> hc((size_t)m), hr((size_t)m);
-    x.download(hc.data(), cnt, (size_t)m);
+}
+
+// The places (0 .. m-1) of a finished recheck whose lists overflowed (more than 256
+// candidates under the threshold: many points at almost one distance).
+static std::vector<int64_t> recheck_overflows(Ctx &x, const RecheckBufs &B, int64_t m) {
+    std::vector<uint32_t> hc((size_t)m);
+    x.download(hc.data(), B.cnt, (size_t)m);
+    x.sync();
+    std::vector<int64_t> over;
+    for (int64_t r = 0; r < m; ++r)
+        if (hc[(size_t)r] > (uint32_t)kRecheckCap) over.push_back(r);
+    return over;
+}
+
+// The whole recheck of one query set: the threshold scan, then the GEMM recheck for
+// the queries whose lists overflowed (and for rows wider than 256 columns).  Returns
+// how many queries stay unproven.
+static int64_t knn_recheck(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int p, int k,
+                           const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist,
+                           const float *mean_known = nullptr) {
+    if (p > 256) return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
+    const RecheckBufs B = recheck_bufs(x, m, nc, p, k);
+    knn_recheck_launch(x, B, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist, mean_known);
+    const std::vector<int64_t> over = recheck_overflows(x, B, m);
+    if (over.empty()) return 0;
+    std::vector<uint32_t> hr((size_t)m), ov;
     x.download(hr.data(), rows, (size_t)m);
     x.sync();
-    std::vector<uint32_t> over;
-    for (int64_t r = 0; r < m; ++r)
-        if (hc[(size_t)r] > (uint32_t)cap) over.push_back(hr[(size_t)r]);
-    if (over.empty()) return 0;
-    uint32_t *dl = x.get<uint32_t>("knnr_over", over.size());
-    HIPCHK(hipMemcpyAsync(dl, over.data(), over.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
+    for (int64_t r : over) ov.push_back(hr[(size_t)r]);
+    uint32_t *dl = x.get<uint32_t>("knnr_over", ov.size());
+    HIPCHK(hipMemcpyAsync(dl, ov.data(), ov.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
     x.sync();
-    return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, dl, (int64_t)over.size(), dev_idx, dev_dist);
+    return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, dl, (int64_t)ov.size(), dev_idx, dev_dist);
 }
 
 static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev_xq, int64_t nq, int p, int k,
@@ -1175,7 +1278,9 @@ static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev
     const uint32_t m = read_count(x, cnt);
     x.knn_checked = nq;
     x.knn_rechecked = m;
-    if (m) x.knn_uncertified = knn_recheck(x, dev_xc, nc, graph ? nullptr : dev_xq, p, k, unc, m, dev_idx, dev_dist);
+    if (m)
+        x.knn_uncertified =
+            knn_recheck(x, dev_xc, nc, graph ? nullptr : dev_xq, p, k, unc, m, dev_idx, dev_dist, o.mean);
     if (ev3) HIPCHK(hipEventRecord(ev3[3], x.st));
     x.check_launch();
 }
@@ -1838,7 +1943,9 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 }
 
 // After the batch's kNN: the queries its certificate did not cover, rechecked subset by
-// subset in FP32 among their own subset's cells (indices within the subset).
+// subset in FP32 among their own subset's cells (indices within the subset): one upload
+// of every subset's local rows, the subsets' threshold scans launched back to back,
+// one read of the list counts, and the GEMM recheck only for overflowed lists.
 static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan &P) {
     x.knn_checked = P.n;
     x.knn_rechecked = x.knn_uncertified = 0;
@@ -1848,20 +1955,56 @@ static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan
     x.download(rows.data(), x.get<uint32_t>(P.pre + "cert_rows", (size_t)P.n), m);
     x.sync();
     std::sort(rows.begin(), rows.end());
-    uint32_t *dl = x.get<uint32_t>("knnr_rows", (size_t)m);
-    size_t a = 0;
-    while (a < rows.size()) {
+    // subsets in order: [a, e) of the sorted rows are subset y's, as local rows
+    struct Part {
+        int y;
+        size_t a, e;
+    };
+    std::vector<Part> parts;
+    std::vector<uint32_t> loc(m);
+    int64_t nc_max = 0;
+    for (size_t a = 0; a < rows.size();) {
         const int y = (int)(std::upper_bound(P.off.begin(), P.off.end(), (int64_t)rows[a]) - P.off.begin()) - 1;
-        const int64_t o0 = P.off[y], o1 = P.off[y + 1];
         size_t e = a;
-        std::vector<uint32_t> loc;
-        while (e < rows.size() && (int64_t)rows[e] < o1) loc.push_back((uint32_t)(rows[e++] - o0));
-        HIPCHK(hipMemcpyAsync(dl, loc.data(), loc.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
-        x.sync();
-        x.knn_uncertified += knn_recheck(x, b->pipe.dev_x_pca + o0 * P.kp, o1 - o0, nullptr, P.kp, P.kn, dl,
-                                         (int64_t)loc.size(), b->pipe.dev_knn_idx + o0 * P.kn,
-                                         b->pipe.dev_knn_dist + o0 * P.kn);
+        while (e < rows.size() && (int64_t)rows[e] < P.off[y + 1]) {
+            loc[e] = (uint32_t)(rows[e] - P.off[y]);
+            ++e;
+        }
+        parts.push_back({y, a, e});
+        nc_max = std::max(nc_max, P.off[y + 1] - P.off[y]);
         a = e;
+    }
+    uint32_t *dl = x.get<uint32_t>("knnr_rows", (size_t)m);
+    HIPCHK(hipMemcpyAsync(dl, loc.data(), (size_t)m * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
+    x.sync();
+    float *xp = b->pipe.dev_x_pca;
+    uint32_t *oi = b->pipe.dev_knn_idx;
+    float *od = b->pipe.dev_knn_dist;
+    const int p = P.kp, k = P.kn;
+    if (p > 256) {  // rows too wide for the threshold scan: the GEMM recheck per subset
+        for (const Part &t : parts)
+            x.knn_uncertified += knn_recheck_gemm(x, xp + P.off[t.y] * p, P.off[t.y + 1] - P.off[t.y], nullptr, p, k,
+                                                  dl + t.a, (int64_t)(t.e - t.a), oi + P.off[t.y] * k,
+                                                  od + P.off[t.y] * k);
+        x.knn_rechecked = m;
+        return;
+    }
+    const RecheckBufs B = recheck_bufs(x, m, nc_max, p, k);
+    for (const Part &t : parts)
+        knn_recheck_launch(x, recheck_slice(B, (int64_t)t.a, p, k), xp + P.off[t.y] * p, P.off[t.y + 1] - P.off[t.y],
+                           nullptr, p, k, dl + t.a, (int64_t)(t.e - t.a), oi + P.off[t.y] * k, od + P.off[t.y] * k,
+                           nullptr);
+    const std::vector<int64_t> over = recheck_overflows(x, B, m);
+    for (const Part &t : parts) {
+        std::vector<uint32_t> ov;
+        for (int64_t r : over)
+            if ((size_t)r >= t.a && (size_t)r < t.e) ov.push_back(loc[(size_t)r]);
+        if (ov.empty()) continue;
+        uint32_t *dv = x.get<uint32_t>("knnr_over", ov.size());
+        HIPCHK(hipMemcpyAsync(dv, ov.data(), ov.size() * sizeof(uint32_t), hipMemcpyHostToDevice, x.st));
+        x.sync();
+        x.knn_uncertified += knn_recheck_gemm(x, xp + P.off[t.y] * p, P.off[t.y + 1] - P.off[t.y], nullptr, p, k, dv,
+                                              (int64_t)ov.size(), oi + P.off[t.y] * k, od + P.off[t.y] * k);
     }
     x.knn_rechecked = m;
 }

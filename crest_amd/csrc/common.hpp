@@ -306,12 +306,24 @@ void launch_knn_rerank_sort(const float *xq, const float *xc, int p, const uint3
                             int64_t q0, int64_t rows, uint32_t *idx_out, float *dist_out, hipStream_t s,
                             const float *tau = nullptr, const KnnCert *cert = nullptr,
                             const float *xq_scr = nullptr);
-// Threshold scan of a recheck (m gathered query rows xq answering output rows rows[]):
+// Threshold scan of a recheck (m gathered query rows answering output rows rows[];
+// xcc / xqc centred on the candidates' mean, cn / qn their FP32 squared norms):
 // candidates within the widened current k-th distance appended (at most cap per row,
 // cnt counts all; cand pre-filled with 0xFFFFFFFF, cnt zeroed).
-void launch_knn_thr_scan(const float *xc, int64_t nc, int p, const float *xq, int64_t m, const float *dist,
-                         const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt, uint32_t *cand, int cap,
-                         hipStream_t s);
+void launch_knn_thr_scan(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
+                         int64_t m, const float *dist, const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt,
+                         uint32_t *cand, int cap, hipStream_t s);
+// The thresholds alone (thr[r] for the m gathered rows), and the threshold test on a
+// block of GEMM dot products (rows q0 .. q0 + nrows - 1 of the gathered set).
+void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const float *xqc, int64_t m, int k, int p,
+                                float *thr, hipStream_t s);
+void launch_knn_thr_filter(const float *dots, int64_t ld, int64_t nc, const float *cn, const float *qn,
+                           const float *thr, const uint32_t *rows, int self, int64_t q0, int64_t nrows, uint32_t *cnt,
+                           uint32_t *cand, int cap, hipStream_t s);
+// The threshold test on FP32 MFMA (p <= 96; thresholds from launch_knn_scan_thresholds).
+void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
+                         int64_t m, const float *thr, const uint32_t *rows, int self, uint32_t *cnt, uint32_t *cand,
+                         int cap, hipStream_t s);
 // The GEMM screen's operands: x - mean (mean computed from x when compute_mean; part
 // holds knn_centre_parts(n) * p doubles).
 int knn_centre_parts(int64_t n);

@@ -1291,11 +1291,13 @@ KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count
 // the rows centred on the candidates' mean (FP32; a = |q'|, b = |c'|, s = 1): the squared norms are sequential FP32
 // sums ((p + 1) u32 each), the SGEMM dot at most 2 gamma_p |q||c| <= gamma_p (a + b)^2 / 2
 // in any accumulation order, and the two final roundings u32 (a + b)^2 each.
+static double knn_gemm_c2(int p) { return (2.0 * p + 16) * std::ldexp(1.0, -24) * 1.01; }
+
 KnnCert knn_cert_gemm(int p, uint32_t *count) {
     KnnCert c{};
     c.count = count;
-    const double u32 = std::ldexp(1.0, -24), k = knn_cert_scale();
-    c.c2 = (2.0 * p + 16) * u32 * 1.01 * k;
+    const double k = knn_cert_scale();
+    c.c2 = knn_gemm_c2(p) * k;
     c.c4 = 1e-30 * k;
     return c;
 }
@@ -1604,38 +1606,41 @@ void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, in
                        idx_out, dist_out);
 }
 
-// Threshold scan of a recheck: every candidate whose FP32 squared distance
-// sum_f (q_f - c_f)^2 (difference form: relative error at most (p + 3) u32, no
-// cancellation) is at most thr[q] is appended to query q's list (the query's own row
+// Threshold scan of a recheck, on rows centred on the candidates' mean: the key of a
+// pair is the GEMM screen's, fl(fl(|q'|^2 + |c'|^2) - 2 fl(q'.c')) with the dot an FP32
+// fma chain (so it is within knn_cert_gemm's bound of |q - c|^2), and every candidate
+// whose key is at most thr[q] is appended to query q's list (the query's own row
 // excluded when self_rows is given).  One workgroup per (64-query block, candidate
 // slice); the query rows sit in LDS (read as broadcasts), each thread takes every
-// 256th candidate of the slice and keeps the 64 partial sums in registers.  The LDS
+// 256th candidate of the slice and keeps the 64 dot products in registers.  The LDS
 // rows take 64 * 4 * (p rounded up to 8) bytes: at most 64 KiB for p <= 256.
 constexpr int kScanQ = 64;
-__global__ __launch_bounds__(256) void k_knn_thr_scan(const float *__restrict__ xc, int64_t nc, int p,
-                                                      const float *__restrict__ xq, int64_t m,
+__global__ __launch_bounds__(256) void k_knn_thr_scan(const float *__restrict__ xcc, const float *__restrict__ cn,
+                                                      int64_t nc, int p, const float *__restrict__ xqc,
+                                                      const float *__restrict__ qn, int64_t m,
                                                       const float *__restrict__ thr,
                                                       const uint32_t *__restrict__ self_rows, int64_t slice,
                                                       uint32_t *cnt, uint32_t *cand, int cap) {
     extern __shared__ float sq[];  // [kScanQ][pp], rows zero-padded to pp = p rounded up to 8
-    __shared__ float sthr[kScanQ];
+    __shared__ float sthr[kScanQ], sqn[kScanQ];
     __shared__ uint32_t sself[kScanQ];
     const int pp = (p + 7) / 8 * 8;
     const int64_t q0 = (int64_t)blockIdx.y * kScanQ;
     const int nqb = (int)(m - q0 < kScanQ ? m - q0 : kScanQ);
     for (int e = threadIdx.x; e < kScanQ * pp; e += blockDim.x) {
         const int j = e / pp, f = e - j * pp;
-        sq[e] = (j < nqb && f < p) ? xq[(q0 + j) * p + f] : 0.0f;
+        sq[e] = (j < nqb && f < p) ? xqc[(q0 + j) * p + f] : 0.0f;
     }
     if (threadIdx.x < kScanQ) {
         const int j = threadIdx.x;
-        sthr[j] = j < nqb ? thr[q0 + j] : -1.0f;  // no candidate passes a negative threshold
+        sthr[j] = j < nqb ? thr[q0 + j] : -1.0f;  // keys are >= 0: nothing passes a negative threshold
+        sqn[j] = j < nqb ? qn[q0 + j] : 0.0f;
         sself[j] = (j < nqb && self_rows) ? self_rows[q0 + j] : 0xFFFFFFFFu;
     }
     __syncthreads();
     const int64_t c0 = (int64_t)blockIdx.x * slice, c1 = c0 + slice < nc ? c0 + slice : nc;
     for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-        const float *cr = xc + c * p;
+        const float *cr = xcc + c * p;
         float acc[kScanQ];
 #pragma unroll
         for (int j = 0; j < kScanQ; ++j) acc[j] = 0.0f;
@@ -1646,46 +1651,215 @@ __global__ __launch_bounds__(256) void k_knn_thr_scan(const float *__restrict__ 
 #pragma unroll
             for (int j = 0; j < kScanQ; ++j)
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const float d = sq[j * pp + f0 + t] - cv[t];
-                    acc[j] = fmaf(d, d, acc[j]);
-                }
+                for (int t = 0; t < 8; ++t) acc[j] = fmaf(sq[j * pp + f0 + t], cv[t], acc[j]);
         }
+        const float cnc = cn[c];
 #pragma unroll
         for (int j = 0; j < kScanQ; ++j) {
-            if (!(acc[j] <= sthr[j]) || (uint32_t)c == sself[j]) continue;
+            const float sn = sqn[j] + cnc;
+            const float t2 = 2.0f * acc[j];
+            const float key = sn - t2;
+            if (!(key <= sthr[j]) || (uint32_t)c == sself[j]) continue;
             const uint32_t at = atomicAdd(&cnt[q0 + j], 1u);
             if ((int64_t)at < cap) cand[(q0 + j) * cap + at] = (uint32_t)c;
         }
     }
 }
 
-// thr[r] = the square of output row rows[r]'s current k-th distance (the reference's
-// f32 formula, relative error at most (p/8 + 10) u32 after the square root is undone),
-// widened by (4p + 64) u32 so that it covers the scan's rounding too: every candidate
-// at most as far as the true k-th neighbour passes.
-__global__ void k_knn_scan_thr(const float *dist, const uint32_t *rows, int64_t m, int k, int p, float *thr) {
+// The same threshold test on a block of FP32 dot products from the GEMM (dots[r][c] =
+// q'_(q0+r) . c', rows padded to ld, a multiple of 4): four candidates per thread.
+__global__ __launch_bounds__(256) void k_knn_thr_filter(const float *__restrict__ dots, int64_t ld, int64_t nc,
+                                                        const float *__restrict__ cn, const float *__restrict__ qn,
+                                                        const float *__restrict__ thr,
+                                                        const uint32_t *__restrict__ self_rows, int64_t q0,
+                                                        uint32_t *cnt, uint32_t *cand, int cap) {
+    const int64_t r = blockIdx.y, q = q0 + r;
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (c >= nc) return;
+    const float4 d = *reinterpret_cast<const float4 *>(dots + r * ld + c);
+    const float dv[4] = {d.x, d.y, d.z, d.w};
+    const float t = thr[q], qv = qn[q];
+    const uint32_t self = self_rows ? self_rows[q] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t cj = c + j;
+        if (cj >= nc) break;
+        const float sn = qv + cn[cj];
+        const float t2 = 2.0f * dv[j];
+        const float key = sn - t2;
+        if (!(key <= t) || (uint32_t)cj == self) continue;
+        const uint32_t at = atomicAdd(&cnt[q], 1u);
+        if ((int64_t)at < cap) cand[q * cap + at] = (uint32_t)cj;
+    }
+}
+
+void launch_knn_thr_filter(const float *dots, int64_t ld, int64_t nc, const float *cn, const float *qn,
+                           const float *thr, const uint32_t *rows, int self, int64_t q0, int64_t nrows, uint32_t *cnt,
+                           uint32_t *cand, int cap, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_knn_thr_filter, dim3((unsigned)ceil_div(nc, 1024), (unsigned)nrows), dim3(256), 0, s, dots, ld,
+                       nc, cn, qn, thr, self ? rows : nullptr, q0, cnt, cand, cap);
+}
+
+void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const float *xqc, int64_t m, int k, int p,
+                                float *thr, hipStream_t s);
+
+// The threshold test on FP32 MFMA (v_mfma_f32_32x32x2_f32), rows centred as above.
+// Workgroup = 4 waves x 32 queries; each wave keeps its 32 query rows as B operands in
+// registers (lane l: query l % 32, K-step s: element 2s + l / 32), so p <= 96 (two
+// LDS tile buffers of 96 x 65 floats).  The
+// workgroup streams its candidate slice in tiles of 64 rows through two LDS buffers
+// laid out [k][65] (A operand of lane l: candidate l % 32 of a 32-row half, element
+// 2s + l / 32); register r of lane l then holds the dot product of candidate
+// 8(r/4) + 4(l/32) + r%4 of the half with query l % 32, and the key
+// fl(fl(|q'|^2 + |c'|^2) - 2 dot) is tested against that query's threshold.  Rows past
+// nc are zero with norm +inf (never appended); query rows past m have threshold -1.
+constexpr int kMfmaScanMaxP = 96;
+template <int KS>  // MFMA K-steps: features 0 .. 2 KS - 1 (zero past p)
+__global__ __launch_bounds__(256) void k_knn_thr_mfma(const float *__restrict__ xcc, const float *__restrict__ cn,
+                                                      int64_t nc, int p, const float *__restrict__ xqc,
+                                                      const float *__restrict__ qn, int64_t m,
+                                                      const float *__restrict__ thr,
+                                                      const uint32_t *__restrict__ self_rows, int64_t slice,
+                                                      uint32_t *cnt, uint32_t *cand, int cap) {
+    constexpr int LD = 65, PP = 2 * KS, PER = PP / 4;  // 64 * PP floats per tile over 256 threads
+    __shared__ float ta[2][PP * LD];
+    __shared__ float tn[2][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
+    const int64_t q = (int64_t)blockIdx.y * 128 + wave * 32 + col;
+    const bool qreal = q < m;
+    float b[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+        const int f = 2 * st + half;
+        b[st] = (qreal && f < p) ? xqc[q * p + f] : 0.0f;
+    }
+    const float qt = qreal ? thr[q] : -1.0f, qv = qreal ? qn[q] : 0.0f;
+    const uint32_t qself = (qreal && self_rows) ? self_rows[q] : 0xFFFFFFFFu;
+    const int64_t c0 = (int64_t)blockIdx.x * slice;
+    const int64_t c1 = c0 + slice < nc ? c0 + slice : nc;
+    const int64_t nt = (c1 - c0 + 63) / 64;
+    const float *slice_base = xcc + c0 * p;
+    const int64_t slice_len = c1 - c0;
+    // element e of a tile: row e / PP, feature e % PP, stored at [feature][row]
+    float pre[PER];
+    auto load = [&](int64_t t) {
+        const float *tb = slice_base + t * 64 * p;
+        const int64_t rows_left = slice_len - t * 64;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = i * 256 + threadIdx.x, r = e / PP, f = e - r * PP;
+            pre[i] = (r < rows_left && f < p) ? tb[r * p + f] : 0.0f;
+        }
+    };
+    auto store = [&](int buf, int64_t t) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = i * 256 + threadIdx.x, r = e / PP, f = e - r * PP;
+            ta[buf][f * LD + r] = pre[i];
+        }
+        if (threadIdx.x < 64) {
+            const int64_t c = c0 + t * 64 + threadIdx.x;
+            tn[buf][threadIdx.x] = c < c1 ? cn[c] : INFINITY;
+        }
+    };
+    if (nt > 0) {
+        load(0);
+        store(0, 0);
+        if (nt > 1) load(1);
+    }
+    __syncthreads();
+    for (int64_t t = 0; t < nt; ++t) {
+        const int buf = (int)(t & 1);
+        f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+            const float *row = &ta[buf][(2 * st + half) * LD + col];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(row[0], b[st], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(row[32], b[st], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = mt * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                const float sn = qv + tn[buf][i];
+                const float t2 = 2.0f * (mt ? acc1[r] : acc0[r]);
+                const float key = sn - t2;
+                const uint32_t c = (uint32_t)(c0 + t * 64 + i);
+                if (key <= qt && c != qself) {
+                    const uint32_t at = atomicAdd(&cnt[q], 1u);
+                    if ((int64_t)at < cap) cand[q * cap + at] = c;
+                }
+            }
+        if (t + 1 < nt) store(buf ^ 1, t + 1);
+        if (t + 2 < nt) load(t + 2);
+        __syncthreads();
+    }
+}
+
+void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
+                         int64_t m, const float *thr, const uint32_t *rows, int self, uint32_t *cnt, uint32_t *cand,
+                         int cap, hipStream_t s) {
+    if (m <= 0) return;
+    // about 2,048 workgroups in all: slices of at least 4,096 candidates, whole tiles
+    const int64_t qblocks = ceil_div(m, 128);
+    const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
+    const int64_t slice = round_up(std::max<int64_t>(4096, ceil_div(nc, want)), 64);
+    const dim3 grid((unsigned)ceil_div(nc, slice), (unsigned)qblocks);
+    const uint32_t *sr = self ? rows : nullptr;
+    if (p <= 32)
+        hipLaunchKernelGGL(k_knn_thr_mfma<16>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap);
+    else if (p <= 64)
+        hipLaunchKernelGGL(k_knn_thr_mfma<32>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap);
+    else
+        hipLaunchKernelGGL(k_knn_thr_mfma<48>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap);
+}
+
+// thr[r]: T = the square of output row rows[r]'s current k-th distance (the
+// reference's f32 formula, relative error at most (p/8 + 10) u32 after the square root
+// is undone) widened by (4p + 64) u32, so T >= the true k-th squared distance; then
+// T + c2 (2a + sqrt(T))^2 (knn_cert_gemm's bound at a = |q'|, b <= a + sqrt(T)) rounded
+// up: every candidate at most as far as the true k-th neighbour has a key below it.
+__global__ void k_knn_scan_thr(const float *dist, const uint32_t *rows, const float *xqc, int64_t m, int k, int p,
+                               double c2, float *thr) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
     const double dk = (double)dist[(int64_t)rows[r] * k + (k - 1)];
-    const double t = dk * dk * (1.0 + (4.0 * p + 64.0) * 0x1p-24) + 1e-37;
+    const double T = dk * dk * (1.0 + (4.0 * p + 64.0) * 0x1p-24) + 1e-37;
+    double qq = 0.0;
+    for (int f = 0; f < p; ++f) qq += (double)xqc[r * p + f] * (double)xqc[r * p + f];
+    const double a = sqrt(qq) * (1.0 + 1e-6);
+    const double w = 2.0 * a + sqrt(T) * (1.0 + 1e-6);
+    const double t = T + c2 * w * w;
     float tf = (float)t;
     if ((double)tf < t) tf = nextafterf(tf, INFINITY);
     thr[r] = tf;
 }
 
-void launch_knn_thr_scan(const float *xc, int64_t nc, int p, const float *xq, int64_t m, const float *dist,
-                         const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt, uint32_t *cand, int cap,
-                         hipStream_t s) {
+void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const float *xqc, int64_t m, int k, int p,
+                                float *thr, hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(k_knn_scan_thr, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, dist, rows, m, k, p, thr);
+    // the key's own bound (never the test scale of the certificate)
+    hipLaunchKernelGGL(k_knn_scan_thr, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, dist, rows, xqc, m, k, p,
+                       knn_gemm_c2(p), thr);
+}
+
+void launch_knn_thr_scan(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
+                         int64_t m, const float *dist, const uint32_t *rows, int k, int self, float *thr, uint32_t *cnt,
+                         uint32_t *cand, int cap, hipStream_t s) {
+    if (m <= 0) return;
+    launch_knn_scan_thresholds(dist, rows, xqc, m, k, p, thr, s);
     // about 2,048 workgroups in all: slices of at least 4,096 candidates
     const int64_t qblocks = ceil_div(m, kScanQ);
     const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
     const int64_t slice = std::max<int64_t>(4096, ceil_div(nc, want));
     const size_t lds = (size_t)kScanQ * ((p + 7) / 8 * 8) * sizeof(float);
-    hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)qblocks), dim3(256), lds, s, xc,
-                       nc, p, xq, m, thr, self ? rows : nullptr, slice, cnt, cand, cap);
+    hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)qblocks), dim3(256), lds, s, xcc,
+                       cn, nc, p, xqc, qn, m, thr, self ? rows : nullptr, slice, cnt, cand, cap);
 }
 
 // ---------------------------------------------------------------- large k

@@ -449,11 +449,15 @@ def test_knn_certificate_recheck_wide(oracle):
     _check_graph(oracle, x, il, dl, 100)
 
 
+@pytest.mark.parametrize("scan", ["mfma", "gemm", "valu"])
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
-def test_knn_certificate_forced_recheck(oracle, prec, monkeypatch):
+def test_knn_certificate_forced_recheck(oracle, prec, scan, monkeypatch):
     """A bound scaled by 1e9 proves nothing: every query is rechecked (graph and query
-    search) by the threshold scan, whose answers are exact, so none stays unproven."""
+    search) by the threshold scan (its products on FP32 MFMA, by rocBLAS or by the
+    VALU kernel),
+    whose answers are exact, so none stays unproven."""
     monkeypatch.setenv("CREST_KNN_CERT_SCALE", "1e9")
+    monkeypatch.setenv("CREST_KNN_SCAN", scan)
     x = _clustered(2500, 30, 11)
     ca.set_knn_precision(prec)
     try:
