@@ -1811,6 +1811,9 @@ void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, c
     if (p <= 32)
         hipLaunchKernelGGL(k_knn_thr_mfma<16>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
                            cand, cap);
+    else if (p <= 52)  // 50 principal components: 26 K-steps, no all-zero step
+        hipLaunchKernelGGL(k_knn_thr_mfma<26>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap);
     else if (p <= 64)
         hipLaunchKernelGGL(k_knn_thr_mfma<32>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
                            cand, cap);

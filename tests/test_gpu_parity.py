@@ -449,21 +449,22 @@ def test_knn_certificate_recheck_wide(oracle):
     _check_graph(oracle, x, il, dl, 100)
 
 
+@pytest.mark.parametrize("d", [30, 50])
 @pytest.mark.parametrize("scan", ["mfma", "gemm", "valu"])
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
-def test_knn_certificate_forced_recheck(oracle, prec, scan, monkeypatch):
+def test_knn_certificate_forced_recheck(oracle, prec, scan, d, monkeypatch):
     """A bound scaled by 1e9 proves nothing: every query is rechecked (graph and query
     search) by the threshold scan (its products on FP32 MFMA, by rocBLAS or by the
     VALU kernel),
     whose answers are exact, so none stays unproven."""
     monkeypatch.setenv("CREST_KNN_CERT_SCALE", "1e9")
     monkeypatch.setenv("CREST_KNN_SCAN", scan)
-    x = _clustered(2500, 30, 11)
+    x = _clustered(2500, d, 11)
     ca.set_knn_precision(prec)
     try:
         idx, dist = N.knn_graph(x, 15)
         cg = ca.knn_certificate()
-        R, Q = _clustered(3000, 30, 12), (_clustered(700, 30, 13) + np.float32(0.5)).astype(np.float32)
+        R, Q = _clustered(3000, d, 12), (_clustered(700, d, 13) + np.float32(0.5)).astype(np.float32)
         qi, qd = N.knn_query(R, Q, 20)
         cq = ca.knn_certificate()
     finally:
