@@ -1946,7 +1946,7 @@ static void batch_knn_rerank(Ctx &x, const crest_batch_args *b, const BatchPlan 
 // subset in FP32 among their own subset's cells (indices within the subset): one upload
 // of every subset's local rows, the subsets' threshold scans launched back to back,
 // one read of the list counts, and the GEMM recheck only for overflowed lists.
-static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan &P) {
+static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan &P, const BatchBufs &BB) {
     x.knn_checked = P.n;
     x.knn_rechecked = x.knn_uncertified = 0;
     const uint32_t m = read_count(x, x.get<uint32_t>(P.pre + "cert_cnt", 1));
@@ -1990,10 +1990,51 @@ static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan
         return;
     }
     const RecheckBufs B = recheck_bufs(x, m, nc_max, p, k);
-    for (const Part &t : parts)
-        knn_recheck_launch(x, recheck_slice(B, (int64_t)t.a, p, k), xp + P.off[t.y] * p, P.off[t.y + 1] - P.off[t.y],
-                           nullptr, p, k, dl + t.a, (int64_t)(t.e - t.a), oi + P.off[t.y] * k, od + P.off[t.y] * k,
-                           nullptr);
+    if (p <= 96 && knn_scan_kind() == 0) {
+        // all subsets at once: every row centred on its subset's mean (the screen's), the
+        // rechecked rows gathered, one MFMA threshold test over a table of (subset, up to
+        // 128 queries, the subset's candidates), one FP64 rank, indices placed within
+        // the subsets
+        std::vector<uint32_t> grow(m), ibase(m);
+        std::vector<KnnScanBlock> blk;
+        int64_t max_slices = 1;
+        for (const Part &t : parts) {
+            const int64_t c0 = P.off[t.y], c1 = P.off[t.y + 1];
+            const int64_t slice = round_up(std::max<int64_t>(4096, ceil_div(c1 - c0, 16)), 64);
+            max_slices = std::max(max_slices, ceil_div(c1 - c0, slice));
+            for (size_t a = t.a; a < t.e; ++a) {
+                grow[a] = rows[a];
+                ibase[a] = (uint32_t)c0;
+            }
+            for (size_t a = t.a; a < t.e; a += 128)
+                blk.push_back({(int64_t)a, c0, c1, slice, (int32_t)std::min<size_t>(128, t.e - a), 0});
+        }
+        uint32_t *dg = x.upload<uint32_t>("knnr_grows", grow.data(), m);
+        uint32_t *db = x.upload<uint32_t>("knnr_ibase", ibase.data(), m);
+        KnnScanBlock *dk = x.upload<KnnScanBlock>("knnr_blocks", blk.data(), blk.size());
+        x.sync();  // the host vectors above are pageable and end with this block
+        float *xall = x.get<float>("knnr_xall", (size_t)P.n * p);
+        float *nall = x.get<float>("knnr_nall", (size_t)P.n);
+        const float *means = knn_batch_means(BB.kwork, P.n, P.n_pad, P.n_sub, p);
+        launch_knn_centre_seg(xp, P.n, P.d_off, P.n_sub, p, means, xall, x.st);
+        launch_knn_sqnorm(xall, P.n, p, nall, x.st);
+        launch_gather_rows(xp, p, dg, m, B.xsub, x.st);
+        launch_gather_rows(xall, p, dg, m, B.xsubc, x.st);
+        launch_gather_rows(nall, 1, dg, m, B.qn, x.st);
+        launch_fill_u32(B.cnt, m, 0u, x.st);
+        launch_fill_u32(B.cand, (int64_t)m * kRecheckCap, 0xFFFFFFFFu, x.st);
+        launch_knn_scan_thresholds(od, dg, B.xsubc, m, k, p, B.thr, x.st);
+        launch_knn_thr_mfma_blocks(xall, nall, p, B.xsubc, B.qn, B.thr, dg, dk, (int)blk.size(), max_slices, B.cnt,
+                                   B.cand, kRecheckCap, x.st);
+        launch_knn_rerank_sort(B.xsub, xp, p, B.cand, kRecheckCap, k, 0, m, B.ti, B.td, x.st);
+        launch_knn_place(B.ti, B.td, m, k, k, dg, 0, oi, od, x.st, db);
+        x.check_launch();
+    } else {
+        for (const Part &t : parts)
+            knn_recheck_launch(x, recheck_slice(B, (int64_t)t.a, p, k), xp + P.off[t.y] * p,
+                               P.off[t.y + 1] - P.off[t.y], nullptr, p, k, dl + t.a, (int64_t)(t.e - t.a),
+                               oi + P.off[t.y] * k, od + P.off[t.y] * k, nullptr);
+    }
     const std::vector<int64_t> over = recheck_overflows(x, B, m);
     for (const Part &t : parts) {
         std::vector<uint32_t> ov;
@@ -2457,7 +2498,7 @@ static void run_batch(Ctx &x, const crest_batch_args *b, double *variance, doubl
     }
     {
         const auto t0 = std::chrono::steady_clock::now();
-        batch_knn_recheck(x, b, P);
+        batch_knn_recheck(x, b, P, B);
         res->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() *
                          (x.knn_rechecked ? 1.0 : 0.0);
     }

@@ -283,7 +283,7 @@ void launch_knn_rerank(const float *xq, int64_t nq, const float *xc, int64_t nc,
                        const KnnCert *cert = nullptr, const uint32_t *qmap = nullptr);
 void launch_gather_rows(const float *x, int p, const uint32_t *rows, int64_t m, float *out, hipStream_t s);
 void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows, int self,
-                      uint32_t *idx_out, float *dist_out, hipStream_t s);
+                      uint32_t *idx_out, float *dist_out, hipStream_t s, const uint32_t *ibase = nullptr);
 // Batch forms (subset y = rows cell_off[y] .. cell_off[y+1]-1 of x, operand rows
 // row0[y] .. of n_pad in all; max_rows / max_pad are the largest n_y and padded n_y).
 size_t knn_batch_work_bytes(int64_t n, int64_t n_pad, int n_sub, int p);
@@ -320,6 +320,18 @@ void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const f
 void launch_knn_thr_filter(const float *dots, int64_t ld, int64_t nc, const float *cn, const float *qn,
                            const float *thr, const uint32_t *rows, int self, int64_t q0, int64_t nrows, uint32_t *cnt,
                            uint32_t *cand, int cap, hipStream_t s);
+// One block of a batch's threshold test: queries q0 .. q0 + nq - 1 (nq <= 128) of one
+// subset against its candidates c0 .. c1 - 1, in slices of `slice` rows.
+struct KnnScanBlock {
+    int64_t q0, c0, c1, slice;
+    int32_t nq, pad_;
+};
+void launch_knn_thr_mfma_blocks(const float *xcc, const float *cn, int p, const float *xqc, const float *qn,
+                                const float *thr, const uint32_t *rows, const KnnScanBlock *blocks, int n_blocks,
+                                int64_t max_slices, uint32_t *cnt, uint32_t *cand, int cap, hipStream_t s);
+void launch_knn_centre_seg(const float *x, int64_t n, const int64_t *cell_off, int n_sub, int p, const float *means,
+                           float *out, hipStream_t s);
+const float *knn_batch_means(void *work, int64_t n, int64_t n_pad, int n_sub, int p);
 // The threshold test on FP32 MFMA (p <= 96; thresholds from launch_knn_scan_thresholds).
 void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
                          int64_t m, const float *thr, const uint32_t *rows, int self, uint32_t *cnt, uint32_t *cand,

@@ -665,6 +665,10 @@ static KnnBatchWork knn_batch_work(void *w, int64_t n, int64_t n_pad, int n_sub,
     return k;
 }
 
+const float *knn_batch_means(void *work, int64_t n, int64_t n_pad, int n_sub, int p) {
+    return knn_batch_work(work, n, n_pad, n_sub, p, nullptr).mean;
+}
+
 size_t knn_batch_work_bytes(int64_t n, int64_t n_pad, int n_sub, int p) {
     size_t t = 0;
     (void)knn_batch_work(nullptr, n, n_pad, n_sub, p, &t);
@@ -1585,25 +1589,26 @@ void launch_gather_rows(const float *x, int p, const uint32_t *rows, int64_t m, 
 // Row r's k1 nearest (ti/td, in output order) written as output row rows[r]: with
 // `self`, the entry of row rows[r] itself is dropped and the first k others kept.
 __global__ void k_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows,
-                            int self, uint32_t *idx_out, float *dist_out) {
+                            int self, uint32_t *idx_out, float *dist_out, const uint32_t *ibase) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
     const uint32_t q = rows[r];
+    const uint32_t base = ibase ? ibase[r] : 0u;  // indices reported within the query's subset
     int o = 0;
     for (int j = 0; j < k1 && o < k; ++j) {
         const uint32_t c = ti[r * k1 + j];
         if (self && c == q) continue;
-        idx_out[(int64_t)q * k + o] = c;
+        idx_out[(int64_t)q * k + o] = c - base;
         dist_out[(int64_t)q * k + o] = td[r * k1 + j];
         ++o;
     }
 }
 
 void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, int k, const uint32_t *rows, int self,
-                      uint32_t *idx_out, float *dist_out, hipStream_t s) {
+                      uint32_t *idx_out, float *dist_out, hipStream_t s, const uint32_t *ibase) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_knn_place, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, ti, td, m, k1, k, rows, self,
-                       idx_out, dist_out);
+                       idx_out, dist_out, ibase);
 }
 
 // Threshold scan of a recheck, on rows centred on the candidates' mean: the key of a
@@ -1715,19 +1720,32 @@ void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const f
 // fl(fl(|q'|^2 + |c'|^2) - 2 dot) is tested against that query's threshold.  Rows past
 // nc are zero with norm +inf (never appended); query rows past m have threshold -1.
 constexpr int kMfmaScanMaxP = 96;
+// With a block table (a batch of subsets): blockIdx.y names block y of the table —
+// queries q0 .. q0 + nq - 1 (at most 128, one subset's) against candidates c0 .. c1 - 1
+// of that subset in slices of `slice` (workgroups past the subset's last slice exit).
 template <int KS>  // MFMA K-steps: features 0 .. 2 KS - 1 (zero past p)
 __global__ __launch_bounds__(256) void k_knn_thr_mfma(const float *__restrict__ xcc, const float *__restrict__ cn,
                                                       int64_t nc, int p, const float *__restrict__ xqc,
                                                       const float *__restrict__ qn, int64_t m,
                                                       const float *__restrict__ thr,
                                                       const uint32_t *__restrict__ self_rows, int64_t slice,
-                                                      uint32_t *cnt, uint32_t *cand, int cap) {
+                                                      uint32_t *cnt, uint32_t *cand, int cap,
+                                                      const KnnScanBlock *__restrict__ blocks) {
     constexpr int LD = 65, PP = 2 * KS, PER = PP / 4;  // 64 * PP floats per tile over 256 threads
     __shared__ float ta[2][PP * LD];
     __shared__ float tn[2][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
-    const int64_t q = (int64_t)blockIdx.y * 128 + wave * 32 + col;
-    const bool qreal = q < m;
+    int64_t q = (int64_t)blockIdx.y * 128 + wave * 32 + col, cb = 0, ce = nc;
+    bool qreal = q < m;
+    if (blocks) {
+        const KnnScanBlock bk = blocks[blockIdx.y];
+        if ((int64_t)blockIdx.x * bk.slice >= bk.c1 - bk.c0) return;  // the whole workgroup
+        q = bk.q0 + wave * 32 + col;
+        qreal = wave * 32 + col < bk.nq;
+        cb = bk.c0;
+        ce = bk.c1;
+        slice = bk.slice;
+    }
     float b[KS];
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
@@ -1736,8 +1754,8 @@ __global__ __launch_bounds__(256) void k_knn_thr_mfma(const float *__restrict__ 
     }
     const float qt = qreal ? thr[q] : -1.0f, qv = qreal ? qn[q] : 0.0f;
     const uint32_t qself = (qreal && self_rows) ? self_rows[q] : 0xFFFFFFFFu;
-    const int64_t c0 = (int64_t)blockIdx.x * slice;
-    const int64_t c1 = c0 + slice < nc ? c0 + slice : nc;
+    const int64_t c0 = cb + (int64_t)blockIdx.x * slice;
+    const int64_t c1 = c0 + slice < ce ? c0 + slice : ce;
     const int64_t nt = (c1 - c0 + 63) / 64;
     const float *slice_base = xcc + c0 * p;
     const int64_t slice_len = c1 - c0;
@@ -1798,6 +1816,23 @@ __global__ __launch_bounds__(256) void k_knn_thr_mfma(const float *__restrict__ 
     }
 }
 
+static void launch_thr_mfma_grid(dim3 grid, const float *xcc, const float *cn, int64_t nc, int p, const float *xqc,
+                                 const float *qn, int64_t m, const float *thr, const uint32_t *sr, int64_t slice,
+                                 uint32_t *cnt, uint32_t *cand, int cap, const KnnScanBlock *blocks, hipStream_t s) {
+    if (p <= 32)
+        hipLaunchKernelGGL(k_knn_thr_mfma<16>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap, blocks);
+    else if (p <= 52)  // 50 principal components: 26 K-steps, no all-zero step
+        hipLaunchKernelGGL(k_knn_thr_mfma<26>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap, blocks);
+    else if (p <= 64)
+        hipLaunchKernelGGL(k_knn_thr_mfma<32>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap, blocks);
+    else
+        hipLaunchKernelGGL(k_knn_thr_mfma<48>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
+                           cand, cap, blocks);
+}
+
 void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, const float *xqc, const float *qn,
                          int64_t m, const float *thr, const uint32_t *rows, int self, uint32_t *cnt, uint32_t *cand,
                          int cap, hipStream_t s) {
@@ -1807,19 +1842,16 @@ void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, c
     const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
     const int64_t slice = round_up(std::max<int64_t>(4096, ceil_div(nc, want)), 64);
     const dim3 grid((unsigned)ceil_div(nc, slice), (unsigned)qblocks);
-    const uint32_t *sr = self ? rows : nullptr;
-    if (p <= 32)
-        hipLaunchKernelGGL(k_knn_thr_mfma<16>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
-                           cand, cap);
-    else if (p <= 52)  // 50 principal components: 26 K-steps, no all-zero step
-        hipLaunchKernelGGL(k_knn_thr_mfma<26>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
-                           cand, cap);
-    else if (p <= 64)
-        hipLaunchKernelGGL(k_knn_thr_mfma<32>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
-                           cand, cap);
-    else
-        hipLaunchKernelGGL(k_knn_thr_mfma<48>, grid, dim3(256), 0, s, xcc, cn, nc, p, xqc, qn, m, thr, sr, slice, cnt,
-                           cand, cap);
+    launch_thr_mfma_grid(grid, xcc, cn, nc, p, xqc, qn, m, thr, self ? rows : nullptr, slice, cnt, cand, cap, nullptr,
+                         s);
+}
+
+void launch_knn_thr_mfma_blocks(const float *xcc, const float *cn, int p, const float *xqc, const float *qn,
+                                const float *thr, const uint32_t *rows, const KnnScanBlock *blocks, int n_blocks,
+                                int64_t max_slices, uint32_t *cnt, uint32_t *cand, int cap, hipStream_t s) {
+    if (n_blocks <= 0) return;
+    const dim3 grid((unsigned)max_slices, (unsigned)n_blocks);
+    launch_thr_mfma_grid(grid, xcc, cn, 0, p, xqc, qn, 0, thr, rows, 0, cnt, cand, cap, blocks, s);
 }
 
 // thr[r]: T = the square of output row rows[r]'s current k-th distance (the
@@ -1910,6 +1942,28 @@ __global__ void k_centre_rows(const float *x, int64_t n, int p, const float *mea
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * p) return;
     out[e] = x[e] - mean[e % p];
+}
+
+// Every row of a batch centred on its own subset's mean (subset y = rows cell_off[y] ..
+// cell_off[y + 1] - 1, means [n_sub][p]).
+__global__ void k_centre_rows_seg(const float *x, const int64_t *cell_off, int n_sub, int p, const float *means,
+                                  float *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cell_off[n_sub] * p) return;
+    const int64_t r = e / p;
+    int lo = 0, hi = n_sub;  // the last y with cell_off[y] <= r
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (cell_off[mid] <= r) lo = mid; else hi = mid;
+    }
+    out[e] = x[e] - means[(int64_t)lo * p + (e - r * p)];
+}
+
+void launch_knn_centre_seg(const float *x, int64_t n, const int64_t *cell_off, int n_sub, int p, const float *means,
+                           float *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_centre_rows_seg, dim3((unsigned)ceil_div(n * p, 256)), dim3(256), 0, s, x, cell_off, n_sub,
+                       p, means, out);
 }
 
 int knn_centre_parts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
