@@ -1102,14 +1102,15 @@ static int64_t knn_recheck_gemm(Ctx &x, const float *dev_xc, int64_t nc, const f
 // gives the exact k, written into the queries' output rows.  A query with more
 // candidates than that under its threshold (many points at almost the same distance)
 // takes the GEMM recheck.  Returns how many queries stay unproven.
-// The threshold scan's products: FP32 MFMA (0, rows of at most 96 columns), rocBLAS
-// dot products filtered in one pass (1, wider rows), or the VALU kernel (2).
-// CREST_KNN_SCAN=gemm / valu picks the second / third (same thresholds, same
-// candidates).
+// The threshold scan's products: FP32 MFMA (0, rows of at most 96 columns; after an
+// FP16 screen the FP16 test on the screen's operands instead), rocBLAS dot products
+// filtered in one pass (1, wider rows), or the VALU kernel (2); CREST_KNN_SCAN=gemm /
+// valu picks the second / third, =fp32 the FP32 MFMA test after an FP16 screen too.
 static int knn_scan_kind() {
     const char *e = std::getenv("CREST_KNN_SCAN");
     if (e && std::strcmp(e, "gemm") == 0) return 1;
     if (e && std::strcmp(e, "valu") == 0) return 2;
+    if (e && std::strcmp(e, "fp32") == 0) return 3;  // the FP32 MFMA test also after an FP16 screen
     return 0;
 }
 
@@ -1174,7 +1175,7 @@ static void knn_recheck_launch(Ctx &x, const RecheckBufs &B, const float *dev_xc
     launch_fill_u32(B.cnt, m, 0u, x.st);
     launch_fill_u32(B.cand, m * cap, 0xFFFFFFFFu, x.st);
     const int scan = knn_scan_kind();
-    if (scan == 0 && p <= 96) {
+    if ((scan == 0 || scan == 3) && p <= 96) {
         launch_knn_scan_thresholds(dev_dist, rows, B.xsubc, m, k, p, B.thr, x.st);
         launch_knn_thr_mfma(B.xcc, B.cn, nc, p, B.xsubc, B.qn, m, B.thr, rows, graph ? 1 : 0, B.cnt, B.cand, cap,
                             x.st);
@@ -1215,6 +1216,29 @@ static std::vector<int64_t> recheck_overflows(Ctx &x, const RecheckBufs &B, int6
     return over;
 }
 
+static int64_t recheck_finish(Ctx &x, const RecheckBufs &B, const float *dev_xc, int64_t nc, const float *dev_xq, int p,
+                              int k, const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist);
+
+// The recheck of an FP16 screen's unproven queries on the screen's own operands: the
+// threshold test in FP16 on MFMA with the FP16 bound folded into each threshold (no
+// centring pass, a quarter of the FP32 test's MFMA K-steps), then the same FP64 rank
+// and placement, and the GEMM recheck for overflowed lists.
+static int64_t knn_recheck16(Ctx &x, const void *xa, const void *xb, int kpad, const KnnOrder &o, const float *dev_xc,
+                             int64_t nc, const float *dev_xq, int p, int k, const uint32_t *rows, const uint32_t *pos,
+                             int64_t m, uint32_t *dev_idx, float *dev_dist) {
+    const bool graph = dev_xq == nullptr;
+    const RecheckBufs B = recheck_bufs(x, m, 1, p, k);
+    launch_gather_rows(graph ? dev_xc : dev_xq, p, rows, m, B.xsub, x.st);
+    launch_fill_u32(B.cnt, m, 0u, x.st);
+    launch_fill_u32(B.cand, m * kRecheckCap, 0xFFFFFFFFu, x.st);
+    launch_knn_thr16(xa, xb, kpad, nc, o, B.xsub, dev_dist, rows, pos, m, k, p, graph ? 1 : 0, B.thr, B.cnt, B.cand,
+                     kRecheckCap, x.st);
+    launch_knn_rerank_sort(B.xsub, dev_xc, p, B.cand, kRecheckCap, k, 0, m, B.ti, B.td, x.st);
+    launch_knn_place(B.ti, B.td, m, k, k, rows, 0, dev_idx, dev_dist, x.st);
+    x.check_launch();
+    return recheck_finish(x, B, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
+}
+
 // The whole recheck of one query set: the threshold scan, then the GEMM recheck for
 // the queries whose lists overflowed (and for rows wider than 256 columns).  Returns
 // how many queries stay unproven.
@@ -1224,6 +1248,13 @@ static int64_t knn_recheck(Ctx &x, const float *dev_xc, int64_t nc, const float 
     if (p > 256) return knn_recheck_gemm(x, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
     const RecheckBufs B = recheck_bufs(x, m, nc, p, k);
     knn_recheck_launch(x, B, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist, mean_known);
+    return recheck_finish(x, B, dev_xc, nc, dev_xq, p, k, rows, m, dev_idx, dev_dist);
+}
+
+// After a threshold-scan recheck: the GEMM recheck for the queries whose lists
+// overflowed; returns how many queries stay unproven.
+static int64_t recheck_finish(Ctx &x, const RecheckBufs &B, const float *dev_xc, int64_t nc, const float *dev_xq, int p,
+                              int k, const uint32_t *rows, int64_t m, uint32_t *dev_idx, float *dev_dist) {
     const std::vector<int64_t> over = recheck_overflows(x, B, m);
     if (over.empty()) return 0;
     std::vector<uint32_t> hr((size_t)m), ov;
@@ -1270,15 +1301,20 @@ static void knn_device(Ctx &x, const float *dev_xc, int64_t nc, const float *dev
     if (ev3) HIPCHK(hipEventRecord(ev3[2], x.st));
     uint32_t *cnt = x.get<uint32_t>("knn_cert_cnt", 1);
     uint32_t *unc = x.get<uint32_t>("knn_cert_rows", (size_t)nq);
+    uint32_t *upos = x.get<uint32_t>("knn_cert_pos", (size_t)nq);
     launch_fill_u32(cnt, 1, 0u, x.st);
-    const KnnCert ct = knn_cert(kp, p, kpad, cd, cnt, unc, nq);
+    KnnCert ct = knn_cert(kp, p, kpad, cd, cnt, unc, nq);
+    ct.pos = upos;
     launch_knn_rerank(graph ? dev_xc : dev_xq, nq, dev_xc, nc, p, o, graph ? 1 : 0, ci, 2 * L, k, dev_idx, dev_dist,
                       x.st, nullptr, nullptr, &ct);
     x.check_launch();
     const uint32_t m = read_count(x, cnt);
     x.knn_checked = nq;
     x.knn_rechecked = m;
-    if (m)
+    if (m && kp == CREST_PREC_FP16 && knn_scan_kind() == 0)
+        x.knn_uncertified = knn_recheck16(x, xa, xb, kpad, o, dev_xc, nc, graph ? nullptr : dev_xq, p, k, unc, upos, m,
+                                          dev_idx, dev_dist);
+    else if (m)
         x.knn_uncertified =
             knn_recheck(x, dev_xc, nc, graph ? nullptr : dev_xq, p, k, unc, m, dev_idx, dev_dist, o.mean);
     if (ev3) HIPCHK(hipEventRecord(ev3[3], x.st));
@@ -1990,7 +2026,7 @@ static void batch_knn_recheck(Ctx &x, const crest_batch_args *b, const BatchPlan
         return;
     }
     const RecheckBufs B = recheck_bufs(x, m, nc_max, p, k);
-    if (p <= 96 && knn_scan_kind() == 0) {
+    if (p <= 96 && (knn_scan_kind() == 0 || knn_scan_kind() == 3)) {
         // all subsets at once: every row centred on its subset's mean (the screen's), the
         // rechecked rows gathered, one MFMA threshold test over a table of (subset, up to
         // 128 queries, the subset's candidates), one FP64 rank, indices placed within

@@ -253,6 +253,7 @@ struct KnnCert {
     uint32_t *count;      // null: no certificate
     uint32_t *list;       // the first `cap` uncertified query rows
     int64_t cap;
+    uint32_t *pos;        // optional: their query positions in the screen's order
 };
 KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count, uint32_t *list, int64_t cap);
 double knn_cert_scale();
@@ -320,6 +321,14 @@ void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const f
 void launch_knn_thr_filter(const float *dots, int64_t ld, int64_t nc, const float *cn, const float *qn,
                            const float *thr, const uint32_t *rows, int self, int64_t q0, int64_t nrows, uint32_t *cnt,
                            uint32_t *cand, int cap, hipStream_t s);
+// The threshold test on the FP16 screen's own operands (xa / xb, kpad halves per row,
+// m queries at screen positions pos answering output rows rows[]): thresholds
+// s^2 T + E16(a, a + s sqrt(T)) from the queries' current k-th distances, then every
+// candidate whose FP16 screen value is at most its query's threshold appended as its
+// data row (perm_c), the query itself excluded in a graph.
+void launch_knn_thr16(const void *xa, const void *xb, int kpad, int64_t nc, const KnnOrder &o, const float *xq_rows,
+                      const float *dist, const uint32_t *rows, const uint32_t *pos, int64_t m, int k, int p, int self,
+                      float *thr, uint32_t *cnt, uint32_t *cand, int cap, hipStream_t s);
 // One block of a batch's threshold test: queries q0 .. q0 + nq - 1 (nq <= 128) of one
 // subset against its candidates c0 .. c1 - 1, in slices of `slice` rows.
 struct KnnScanBlock {

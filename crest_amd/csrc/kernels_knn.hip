@@ -1263,12 +1263,19 @@ double knn_cert_scale() {
     return v > 0.0 ? v : 1.0;
 }
 
+static KnnCert knn_cert_coefs(int prec, int p, int kpad, double scale);
+
 KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count, uint32_t *list, int64_t cap) {
-    KnnCert c{};
+    KnnCert c = knn_cert_coefs(prec, p, kpad, knn_cert_scale());
     c.cand_d = cand_d;
     c.count = count;
     c.list = list;
     c.cap = cap;
+    return c;
+}
+
+static KnnCert knn_cert_coefs(int prec, int p, int kpad, double scale) {
+    KnnCert c{};
     const double u32 = std::ldexp(1.0, -24);
     c.c2 = (2.0 * kpad + p + 16) * u32 * 1.01;
     c.c4 = 1e-30;
@@ -1283,11 +1290,10 @@ KnnCert knn_cert(int prec, int p, int kpad, const float *cand_d, uint32_t *count
         c.c1 = 2.0 * 3.01 * u2;
         c.c2 += u2;
     }
-    const double k = knn_cert_scale();
-    c.c1 *= k;
-    c.c2 *= k;
-    c.c3 *= k;
-    c.c4 *= k;
+    c.c1 *= scale;
+    c.c2 *= scale;
+    c.c3 *= scale;
+    c.c4 *= scale;
     return c;
 }
 
@@ -1332,7 +1338,10 @@ __device__ inline void knn_certify(const KnnCert &ct, const float *xq, int p, co
     }
     if (!ok) {
         const uint32_t at = atomicAdd(ct.count, 1u);
-        if ((int64_t)at < ct.cap) ct.list[at] = row;
+        if ((int64_t)at < ct.cap) {
+            ct.list[at] = row;
+            if (ct.pos) ct.pos[at] = (uint32_t)i;
+        }
     }
 }
 
@@ -1895,6 +1904,112 @@ void launch_knn_thr_scan(const float *xcc, const float *cn, int64_t nc, int p, c
     const size_t lds = (size_t)kScanQ * ((p + 7) / 8 * 8) * sizeof(float);
     hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)qblocks), dim3(256), lds, s, xcc,
                        cn, nc, p, xqc, qn, m, thr, self ? rows : nullptr, slice, cnt, cand, cap);
+}
+
+// Thresholds of the FP16 threshold test: T as for the FP32 test (the square of the
+// query's current k-th distance, widened by (4p + 64) u32, at least the true k-th
+// squared distance), then s^2 T + E16(a, a + s sqrt(T)) with a = s|q - mean| (the
+// screen's centring and prescale) and E16 the FP16 screen's bound (c1 .. c4, without
+// any test scale), rounded up: every candidate at most as far as the true k-th
+// neighbour has a screen value below it.
+__global__ void k_knn_scan_thr16(const float *dist, const uint32_t *rows, const float *xq_rows, const float *mean,
+                                 const uint32_t *maxbits, int64_t m, int k, int p, double c1, double c2, double c3,
+                                 double c4, float *thr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const double dk = (double)dist[(int64_t)rows[r] * k + (k - 1)];
+    const double T = dk * dk * (1.0 + (4.0 * p + 64.0) * 0x1p-24) + 1e-37;
+    const float *xr = xq_rows + r * p;
+    double qq = 0.0;
+    for (int f = 0; f < p; ++f) {
+        const double v = (double)xr[f] - (double)mean[f];
+        qq += v * v;
+    }
+    const double s = maxbits ? (double)fp16_prescale(maxbits) : 1.0;
+    const double a = s * sqrt(qq) * (1.0 + 1e-6);
+    const double b = a + s * sqrt(T) * (1.0 + 1e-6);
+    const double t = s * s * T * (1.0 + 1e-12) + c1 * a * b + c2 * (a + b) * (a + b) + c3 * (a + b) + c4;
+    float tf = (float)t;
+    if ((double)tf < t) tf = nextafterf(tf, INFINITY);
+    thr[r] = tf;
+}
+
+// The test itself: workgroup = 4 waves x 32 queries; a wave keeps its queries' FP16
+// operands (the screen's B fragments, read at their screen positions) in registers and
+// reads each 64-candidate block's A fragments straight from the fragment-major
+// operands (v_mfma_f32_32x32x16_f16, the screen's own arithmetic); register r of lane
+// l holds candidate 8(r/4) + 4(l/32) + r%4 of a 32-row half against query l % 32.
+// Padding candidates carry an infinite norm (never below a threshold); queries past m
+// have threshold -inf.
+template <int KC>
+__global__ __launch_bounds__(256) void k_knn_thr_mfma16(const u32x4 *__restrict__ xa, const u32x4 *__restrict__ xb,
+                                                        int64_t n_tiles, const uint32_t *__restrict__ pos,
+                                                        const uint32_t *__restrict__ perm_c, int64_t m,
+                                                        const float *__restrict__ thr, int self, int64_t slice_tiles,
+                                                        uint32_t *cnt, uint32_t *cand, int cap) {
+    constexpr int KS = KC / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 31, half = lane >> 5;
+    const int64_t r = (int64_t)blockIdx.y * 128 + wave * 32 + col;
+    const bool real = r < m;
+    const uint32_t qi = real ? pos[r] : 0u;
+    u32x4 b[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+        const u32x4 z = {0u, 0u, 0u, 0u};
+        b[st] = real ? xb[((int64_t)(qi / 64) * KC + 2 * st + half) * 64 + qi % 64] : z;
+    }
+    const float t = real ? thr[r] : -INFINITY;
+    const int64_t t0 = (int64_t)blockIdx.x * slice_tiles;
+    const int64_t t1 = t0 + slice_tiles < n_tiles ? t0 + slice_tiles : n_tiles;
+    for (int64_t tile = t0; tile < t1; ++tile) {
+        const u32x4 *base = xa + tile * KC * 64 + col;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            u32x4 a[KS];
+#pragma unroll
+            for (int st = 0; st < KS; ++st) a[st] = base[(2 * st + half) * 64 + mt * 32];
+            f32x16 acc = {};
+#pragma unroll
+            for (int st = 0; st < KS; ++st) acc = mfma16<1>(a[st], b[st], acc);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t cp = (uint32_t)(tile * 64 + mt * 32 + 8 * (e >> 2) + 4 * half + (e & 3));
+                if (acc[e] <= t && !(self && cp == qi)) {
+                    const uint32_t at = atomicAdd(&cnt[r], 1u);
+                    if ((int64_t)at < cap) cand[r * cap + at] = perm_c[cp];
+                }
+            }
+        }
+    }
+}
+
+void launch_knn_thr16(const void *xa, const void *xb, int kpad, int64_t nc, const KnnOrder &o, const float *xq_rows,
+                      const float *dist, const uint32_t *rows, const uint32_t *pos, int64_t m, int k, int p, int self,
+                      float *thr, uint32_t *cnt, uint32_t *cand, int cap, hipStream_t s) {
+    if (m <= 0) return;
+    const KnnCert c = knn_cert_coefs(kPrecFP16, p, kpad, 1.0);
+    hipLaunchKernelGGL(k_knn_scan_thr16, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, dist, rows, xq_rows, o.mean,
+                       o.maxbits, m, k, p, c.c1, c.c2, c.c3, c.c4, thr);
+    const int64_t n_tiles = knn_rows_padded(nc) / 64;
+    const int64_t qblocks = ceil_div(m, 128);
+    const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
+    const int64_t slice = std::max<int64_t>(1, ceil_div(n_tiles, want));
+    const dim3 grid((unsigned)ceil_div(n_tiles, slice), (unsigned)qblocks);
+    const u32x4 *A = (const u32x4 *)xa, *B = (const u32x4 *)xb;
+    switch (kpad / 8) {
+#define CREST_T16(KCV)                                                                                          \
+    case KCV:                                                                                                   \
+        hipLaunchKernelGGL(k_knn_thr_mfma16<KCV>, grid, dim3(256), 0, s, A, B, n_tiles, pos, o.perm_c, m, thr, \
+                           self, slice, cnt, cand, cap);                                                        \
+        break;
+        CREST_T16(4)
+        CREST_T16(8)
+        CREST_T16(16)
+        CREST_T16(24)
+        CREST_T16(32)
+#undef CREST_T16
+        default: throw std::runtime_error("no FP16 threshold kernel for this width");
+    }
 }
 
 // ---------------------------------------------------------------- large k

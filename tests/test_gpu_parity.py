@@ -450,12 +450,12 @@ def test_knn_certificate_recheck_wide(oracle):
 
 
 @pytest.mark.parametrize("d", [30, 50])
-@pytest.mark.parametrize("scan", ["mfma", "gemm", "valu"])
+@pytest.mark.parametrize("scan", ["mfma", "fp32", "gemm", "valu"])
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
 def test_knn_certificate_forced_recheck(oracle, prec, scan, d, monkeypatch):
     """A bound scaled by 1e9 proves nothing: every query is rechecked (graph and query
-    search) by the threshold scan (its products on FP32 MFMA, by rocBLAS or by the
-    VALU kernel),
+    search) by the threshold scan (after an FP16 screen on its own FP16 operands by
+    default; on FP32 MFMA, by rocBLAS or by the VALU kernel),
     whose answers are exact, so none stays unproven."""
     monkeypatch.setenv("CREST_KNN_CERT_SCALE", "1e9")
     monkeypatch.setenv("CREST_KNN_SCAN", scan)
