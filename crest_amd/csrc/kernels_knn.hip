@@ -1720,15 +1720,14 @@ void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const f
 
 // The threshold test on FP32 MFMA (v_mfma_f32_32x32x2_f32), rows centred as above.
 // Workgroup = 4 waves x 32 queries; each wave keeps its 32 query rows as B operands in
-// registers (lane l: query l % 32, K-step s: element 2s + l / 32), so p <= 96 (two
-// LDS tile buffers of 96 x 65 floats).  The
+// registers (lane l: query l % 32, K-step s: element 2s + l / 32; KS = 16, 26, 32 or
+// 48 K-steps, so p <= 96 with two LDS tile buffers of 2 KS x 65 floats).  The
 // workgroup streams its candidate slice in tiles of 64 rows through two LDS buffers
 // laid out [k][65] (A operand of lane l: candidate l % 32 of a 32-row half, element
 // 2s + l / 32); register r of lane l then holds the dot product of candidate
 // 8(r/4) + 4(l/32) + r%4 of the half with query l % 32, and the key
 // fl(fl(|q'|^2 + |c'|^2) - 2 dot) is tested against that query's threshold.  Rows past
 // nc are zero with norm +inf (never appended); query rows past m have threshold -1.
-constexpr int kMfmaScanMaxP = 96;
 // With a block table (a batch of subsets): blockIdx.y names block y of the table —
 // queries q0 .. q0 + nq - 1 (at most 128, one subset's) against candidates c0 .. c1 - 1
 // of that subset in slices of `slice` (workgroups past the subset's last slice exit).
