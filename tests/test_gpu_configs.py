@@ -14,6 +14,9 @@ against the oracle (tests/test_crest.py:54-118 restated at each configuration):
 * 2,000,000 cells x 2,000 HVGs (seed 3) streamed from host memory in chunks of
   32,768 cells against the same CSR resident on the device and the oracle.
 
+Every kNN search must also leave no query unproven by its certificate (the screen's
+rounding bound, or the exact recheck of the queries it cannot prove).
+
 The oracle is the C restatement of the reference (oracle/) with numpy LAPACK for the
 eigensolve.  Each case prints its name before its GPU work."""
 import os
@@ -111,8 +114,11 @@ def test_c2_operator_path(c2, oracle, prec):
         ca.pp.scale(bf, max_value=10)
         ca.tl.pca(bf, n_comps=50)
         idx, dist = N.knn_graph(bf.obsm["X_pca"], 15)
+        cert = ca.knn_certificate()
     finally:
         ca.set_precision(prev)
+    # every query's set proven against the screen's bound or rechecked exactly
+    assert cert["checked"] == n and cert["uncertified"] == 0, cert
     # HVG: the oracle's statistics under log1pf (the reference's rounding) and under
     # the correctly rounded log1p, each through the pandas seurat restatement
     cmap, gall = np.arange(n, dtype=np.int64), np.arange(G, dtype=np.int32)
@@ -184,6 +190,8 @@ def test_c3_pipeline(c3, oracle, prec_gram, prec_knn):
         assert oracle.principal_angle(out.loadings[:, :r], ref["load"][:, :r]) <= bar
     X = p.x_pca()
     idx, dist = p.knn()
+    cert = ca.knn_certificate()
+    assert cert["checked"] == len(X) and cert["uncertified"] == 0, cert
     q = np.random.default_rng(1).choice(len(X), 4096, replace=False)
     check_knn(oracle, X, idx, dist, q, 15)
 
@@ -216,6 +224,8 @@ def test_c4_batch(c4, oracle):
     X = bp.x_pca()
     idx, dist = bp.knn()
     n_all = int(off[-1])
+    cert = ca.knn_certificate()
+    assert cert["checked"] == n_all and cert["uncertified"] == 0, cert
     gall = np.arange(2000, dtype=np.int32)
     for i in range(256):
         lo, hi = int(off[i]), int(off[i + 1])
@@ -263,6 +273,8 @@ def test_c5_streamed(oracle):
     os_ = ps.run()
     Xs = ps.x_pca()
     i_s, d_s = ps.knn()
+    cert = ca.knn_certificate()
+    assert cert["checked"] == 2_000_000 and cert["uncertified"] == 0, cert
     del ps
     log("2000000 x 2000 resident, fp64 Gram, fp16 screen")
     pr = Pipeline(DeviceCSR.upload(ip, ix, dv, 2000), hvg, n_comps=50, k=15, prec_gram="fp64", prec_knn="fp16")
