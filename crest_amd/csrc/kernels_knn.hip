@@ -1629,6 +1629,7 @@ void launch_knn_place(const uint32_t *ti, const float *td, int64_t m, int k1, in
 // 256th candidate of the slice and keeps the 64 dot products in registers.  The LDS
 // rows take 64 * 4 * (p rounded up to 8) bytes: at most 64 KiB for p <= 256.
 constexpr int kScanQ = 64;
+constexpr int64_t kMaxGridY = 65535;  // the largest gridDim.y
 __global__ __launch_bounds__(256) void k_knn_thr_scan(const float *__restrict__ xcc, const float *__restrict__ cn,
                                                       int64_t nc, int p, const float *__restrict__ xqc,
                                                       const float *__restrict__ qn, int64_t m,
@@ -1710,9 +1711,12 @@ __global__ __launch_bounds__(256) void k_knn_thr_filter(const float *__restrict_
 void launch_knn_thr_filter(const float *dots, int64_t ld, int64_t nc, const float *cn, const float *qn,
                            const float *thr, const uint32_t *rows, int self, int64_t q0, int64_t nrows, uint32_t *cnt,
                            uint32_t *cand, int cap, hipStream_t s) {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_knn_thr_filter, dim3((unsigned)ceil_div(nc, 1024), (unsigned)nrows), dim3(256), 0, s, dots, ld,
-                       nc, cn, qn, thr, self ? rows : nullptr, q0, cnt, cand, cap);
+    // gridDim.y is at most 65,535: rows in launches of that many
+    for (int64_t r0 = 0; r0 < nrows; r0 += kMaxGridY) {
+        const int64_t nr = std::min<int64_t>(kMaxGridY, nrows - r0);
+        hipLaunchKernelGGL(k_knn_thr_filter, dim3((unsigned)ceil_div(nc, 1024), (unsigned)nr), dim3(256), 0, s,
+                           dots + r0 * ld, ld, nc, cn, qn, thr, self ? rows : nullptr, q0 + r0, cnt, cand, cap);
+    }
 }
 
 void launch_knn_scan_thresholds(const float *dist, const uint32_t *rows, const float *xqc, int64_t m, int k, int p,
@@ -1849,17 +1853,21 @@ void launch_knn_thr_mfma(const float *xcc, const float *cn, int64_t nc, int p, c
     const int64_t qblocks = ceil_div(m, 128);
     const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
     const int64_t slice = round_up(std::max<int64_t>(4096, ceil_div(nc, want)), 64);
-    const dim3 grid((unsigned)ceil_div(nc, slice), (unsigned)qblocks);
-    launch_thr_mfma_grid(grid, xcc, cn, nc, p, xqc, qn, m, thr, self ? rows : nullptr, slice, cnt, cand, cap, nullptr,
-                         s);
+    for (int64_t b0 = 0; b0 < qblocks; b0 += kMaxGridY) {  // gridDim.y <= 65,535
+        const int64_t q0 = b0 * 128, mq = std::min<int64_t>(m - q0, kMaxGridY * 128);
+        const dim3 grid((unsigned)ceil_div(nc, slice), (unsigned)ceil_div(mq, 128));
+        launch_thr_mfma_grid(grid, xcc, cn, nc, p, xqc + q0 * p, qn + q0, mq, thr + q0, self ? rows + q0 : nullptr,
+                             slice, cnt + q0, cand + q0 * cap, cap, nullptr, s);
+    }
 }
 
 void launch_knn_thr_mfma_blocks(const float *xcc, const float *cn, int p, const float *xqc, const float *qn,
                                 const float *thr, const uint32_t *rows, const KnnScanBlock *blocks, int n_blocks,
                                 int64_t max_slices, uint32_t *cnt, uint32_t *cand, int cap, hipStream_t s) {
-    if (n_blocks <= 0) return;
-    const dim3 grid((unsigned)max_slices, (unsigned)n_blocks);
-    launch_thr_mfma_grid(grid, xcc, cn, 0, p, xqc, qn, 0, thr, rows, 0, cnt, cand, cap, blocks, s);
+    for (int64_t b0 = 0; b0 < n_blocks; b0 += kMaxGridY) {  // gridDim.y <= 65,535
+        const dim3 grid((unsigned)max_slices, (unsigned)std::min<int64_t>(kMaxGridY, n_blocks - b0));
+        launch_thr_mfma_grid(grid, xcc, cn, 0, p, xqc, qn, 0, thr, rows, 0, cnt, cand, cap, blocks + b0, s);
+    }
 }
 
 // thr[r]: T = the square of output row rows[r]'s current k-th distance (the
@@ -1896,13 +1904,18 @@ void launch_knn_thr_scan(const float *xcc, const float *cn, int64_t nc, int p, c
                          uint32_t *cand, int cap, hipStream_t s) {
     if (m <= 0) return;
     launch_knn_scan_thresholds(dist, rows, xqc, m, k, p, thr, s);
-    // about 2,048 workgroups in all: slices of at least 4,096 candidates
+    // about 2,048 workgroups in all: slices of at least 4,096 candidates; query blocks
+    // in launches of at most 65,535 (the largest gridDim.y)
     const int64_t qblocks = ceil_div(m, kScanQ);
     const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
     const int64_t slice = std::max<int64_t>(4096, ceil_div(nc, want));
     const size_t lds = (size_t)kScanQ * ((p + 7) / 8 * 8) * sizeof(float);
-    hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)qblocks), dim3(256), lds, s, xcc,
-                       cn, nc, p, xqc, qn, m, thr, self ? rows : nullptr, slice, cnt, cand, cap);
+    for (int64_t b0 = 0; b0 < qblocks; b0 += kMaxGridY) {
+        const int64_t q0 = b0 * kScanQ, mq = std::min<int64_t>(m - q0, kMaxGridY * kScanQ);
+        hipLaunchKernelGGL(k_knn_thr_scan, dim3((unsigned)ceil_div(nc, slice), (unsigned)ceil_div(mq, kScanQ)),
+                           dim3(256), lds, s, xcc, cn, nc, p, xqc + q0 * p, qn + q0, mq, thr + q0,
+                           self ? rows + q0 : nullptr, slice, cnt + q0, cand + q0 * cap, cap);
+    }
 }
 
 // Thresholds of the FP16 threshold test: T as for the FP32 test (the square of the
@@ -1993,21 +2006,24 @@ void launch_knn_thr16(const void *xa, const void *xb, int kpad, int64_t nc, cons
     const int64_t qblocks = ceil_div(m, 128);
     const int64_t want = std::max<int64_t>(1, 2048 / qblocks);
     const int64_t slice = std::max<int64_t>(1, ceil_div(n_tiles, want));
-    const dim3 grid((unsigned)ceil_div(n_tiles, slice), (unsigned)qblocks);
     const u32x4 *A = (const u32x4 *)xa, *B = (const u32x4 *)xb;
-    switch (kpad / 8) {
-#define CREST_T16(KCV)                                                                                          \
-    case KCV:                                                                                                   \
-        hipLaunchKernelGGL(k_knn_thr_mfma16<KCV>, grid, dim3(256), 0, s, A, B, n_tiles, pos, o.perm_c, m, thr, \
-                           self, slice, cnt, cand, cap);                                                        \
+    for (int64_t b0 = 0; b0 < qblocks; b0 += kMaxGridY) {  // gridDim.y <= 65,535
+        const int64_t q0 = b0 * 128, mq = std::min<int64_t>(m - q0, kMaxGridY * 128);
+        const dim3 grid((unsigned)ceil_div(n_tiles, slice), (unsigned)ceil_div(mq, 128));
+        switch (kpad / 8) {
+#define CREST_T16(KCV)                                                                                        \
+    case KCV:                                                                                                 \
+        hipLaunchKernelGGL(k_knn_thr_mfma16<KCV>, grid, dim3(256), 0, s, A, B, n_tiles, pos + q0, o.perm_c, mq, \
+                           thr + q0, self, slice, cnt + q0, cand + q0 * cap, cap);                            \
         break;
-        CREST_T16(4)
-        CREST_T16(8)
-        CREST_T16(16)
-        CREST_T16(24)
-        CREST_T16(32)
+            CREST_T16(4)
+            CREST_T16(8)
+            CREST_T16(16)
+            CREST_T16(24)
+            CREST_T16(32)
 #undef CREST_T16
-        default: throw std::runtime_error("no FP16 threshold kernel for this width");
+            default: throw std::runtime_error("no FP16 threshold kernel for this width");
+        }
     }
 }
 
