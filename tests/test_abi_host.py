@@ -34,6 +34,19 @@ def test_no_device_is_an_error_not_a_fallback():
         pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError, match="no HIP device"):
         N.knn_graph(np.zeros((4, 2), np.float32), 2)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        _lib.knn_certificate()
+
+
+def test_screen_bound_needs_no_device():
+    """The certificate's coefficients are host arithmetic: available without a GPU,
+    and an unknown precision is a ValueError."""
+    import ctypes as C
+    coef, kpad = np.zeros(4), C.c_int32(0)
+    _lib.check(_lib.lib().crest_knn_screen_bound(3, 50, coef.ctypes.data, C.byref(kpad)))
+    assert kpad.value == 64 and coef[0] > 0
+    with pytest.raises(ValueError):
+        _lib.check(_lib.lib().crest_knn_screen_bound(7, 50, coef.ctypes.data, C.byref(kpad)))
 
 
 def _ctx(nnz=5):
