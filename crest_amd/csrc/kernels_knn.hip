@@ -973,6 +973,18 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(screen_
             self_mask<L>(c1, cb, half, qb);
         }
         if constexpr (BUF > 0) {
+            // one vote for both query groups: in the common case (nothing below either
+            // lane's threshold) both list updates are skipped by a single branch
+            int32_t m0 = __float_as_int(c0[0]), m1 = __float_as_int(c1[0]);
+#pragma unroll
+            for (int r = 1; r < 16; ++r) {
+                m0 = min(m0, __float_as_int(c0[r]));
+                m1 = min(m1, __float_as_int(c1[r]));
+            }
+            if (__builtin_expect(__builtin_amdgcn_ballot_w64((m0 < __float_as_int(d0[L - 1])) |
+                                                             (m1 < __float_as_int(d1[L - 1]))) == 0,
+                                 1))
+                return;
             topk_update_buf<L, BUF>(d0, i0, c0, cb, half, buf0, cnt0);
             topk_update_buf<L, BUF>(d1, i1, c1, cb, half, buf1, cnt1);
         } else {
